@@ -1,0 +1,952 @@
+/*
+ * ORACLE — test infrastructure only (see alpaca_oracle.h).  CPU restatement
+ * of ALPACA's single-phase FV right-hand side, the SSP-RK3 stage update, the
+ * same-level halo fill with zero-gradient / periodic sides and the CFL time
+ * step.  Each function cites the reference file:line it restates.  Build with
+ * -ffp-contract=off: the reference x86 build has no FMA.
+ */
+#include "alpaca_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NEMAX 5
+#define TC ORA_TC
+#define IC ORA_IC
+#define HS ORA_HS
+
+enum { MASS = 0, ENERGY = 1, MOMX = 2 };
+
+/* std::min / std::max semantics: min(a,b) = (b < a) ? b : a. */
+static inline double smin(double a, double b) { return (b < a) ? b : a; }
+static inline double smax(double a, double b) { return (a < b) ? b : a; }
+
+/* ConsistencyManagedSum, full-symmetry form (mathematical_functions.h:79-91). */
+static double cms3(double a, double b, double c) {
+  double const s1 = (a + b) + c;
+  double const s2 = (a + c) + b;
+  double const s3 = (b + c) + a;
+  double mn = s1, mx = s1;
+  mn = smin(mn, s2);
+  mn = smin(mn, s3);
+  mx = smax(mx, s2);
+  mx = smax(mx, s3);
+  return 0.5 * (mn + mx);
+}
+
+/* DimensionAwareConsistencyManagedSum (mathematical_functions.h:100-108). */
+static inline double dams(int dim, double a, double b, double c) {
+  if (dim == 2)
+    return a + b;
+  return cms3(a, b, c);
+}
+
+static inline int signum(double x) { return (0.0 < x) - (x < 0.0); }
+
+/* ---------------------------------------------------------------- EOS ---- */
+/* GenericStiffenedGas::CalculatePressure<false> (generic_stiffened_gas.h:40-58). */
+static inline double eos_pressure(int dim, double gamma, double b, double rho,
+                                  double mx, double my, double mz, double e) {
+  double const q = 0.5 * dams(dim, mx * mx, my * my, mz * mz);
+  return -gamma * b + (gamma - 1.0) * (e - q / rho);
+}
+/* StiffenedGas::ComputeEnthalpy (stiffened_gas.cpp:80-87). */
+static inline double eos_enthalpy(int dim, double gamma, double b, double rho,
+                                  double mx, double my, double mz, double e) {
+  return (e + eos_pressure(dim, gamma, b, rho, mx, my, mz, e)) / rho;
+}
+/* CalculateSpeedOfSound<false> (generic_stiffened_gas.h:84-97). */
+static inline double eos_sos(double gamma, double b, double rho, double p) {
+  return sqrt(gamma * (p + b) / rho);
+}
+/* CalculateEnergy (generic_stiffened_gas.h:61-71). */
+static inline double eos_energy(int dim, double gamma, double b, double rho,
+                                double u, double v, double w, double p) {
+  return (p + gamma * b) / (gamma - 1.0) +
+         (0.5 * dams(dim, u * u, v * v, w * w) * rho);
+}
+/* StiffenedGas::ComputePsi (stiffened_gas.cpp:127-130). */
+static inline double eos_psi(double gamma, double b, double p, double one_rho) {
+  return (p + gamma * b) * one_rho;
+}
+
+void ora_eos(int dim, double gamma, double b, const double *cons,
+             const double *prim, double *out) {
+  out[0] = eos_pressure(dim, gamma, b, cons[0], cons[1], cons[2], cons[3],
+                        cons[4]);
+  out[1] = eos_enthalpy(dim, gamma, b, cons[0], cons[1], cons[2], cons[3],
+                        cons[4]);
+  out[2] = eos_sos(gamma, b, prim[0], prim[4]);
+  out[3] = eos_energy(dim, gamma, b, prim[0], prim[1], prim[2], prim[3],
+                      prim[4]);
+  out[4] = eos_psi(gamma, b, prim[4], 1.0 / prim[0]);
+}
+
+/* Primitive state of one cell: rho, p, velocity[3] (prime_state_handler.h:
+ * 279-370; velocity = m * (1/rho), pressure from the EOS, absent momenta 0). */
+typedef struct {
+  double rho, p, vel[3];
+} prim_t;
+
+static inline void cons_to_prim(int dim, double gamma, double b,
+                                const double *c, prim_t *o) {
+  o->rho = c[MASS];
+  double const one_density = 1.0 / c[MASS];
+  o->vel[0] = c[MOMX] * one_density;
+  o->vel[1] = c[MOMX + 1] * one_density;
+  o->vel[2] = dim == 3 ? c[MOMX + 2] * one_density : 0.0;
+  o->p = eos_pressure(dim, gamma, b, c[MASS], c[MOMX], c[MOMX + 1],
+                      dim == 3 ? c[MOMX + 2] : 0.0, c[ENERGY]);
+}
+
+/* ------------------------------------------------------------- WENO5 ---- */
+/* WENO5::ApplyImplementation (weno5.h:88-158), v1..v5 already ordered. */
+static double weno5_core(double v1, double v2, double v3, double v4,
+                         double v5) {
+  double const c_s1 = 13.0 / 12.0, c_s2 = 0.25, eps = 1.0e-6;
+  double const s11 = 1.0 * v1 + -2.0 * v2 + 1.0 * v3;
+  double const s12 = 1.0 * v1 + -4.0 * v2 + 3.0 * v3;
+  double const s1 = c_s1 * s11 * s11 + c_s2 * s12 * s12;
+  double const s21 = 1.0 * v2 + -2.0 * v3 + 1.0 * v4;
+  double const s22 = 1.0 * v2 + -1.0 * v4;
+  double const s2 = c_s1 * s21 * s21 + c_s2 * s22 * s22;
+  double const s31 = 1.0 * v3 + -2.0 * v4 + 1.0 * v5;
+  double const s32 = 3.0 * v3 + -4.0 * v4 + 1.0 * v5;
+  double const s3 = c_s1 * s31 * s31 + c_s2 * s32 * s32;
+  double const a1 = 0.1 / ((s1 + eps) * (s1 + eps));
+  double const a2 = 0.6 / ((s2 + eps) * (s2 + eps));
+  double const a3 = 0.3 / ((s3 + eps) * (s3 + eps));
+  double const one_a_sum = 1.0 / (a1 + a2 + a3);
+  double const w1 = a1 * one_a_sum;
+  double const w2 = a2 * one_a_sum;
+  double const w3 = a3 * one_a_sum;
+  return w1 * ((2.0 / 6.0) * v1 + (-7.0 / 6.0) * v2 + (11.0 / 6.0) * v3) +
+         w2 * ((-1.0 / 6.0) * v2 + (5.0 / 6.0) * v3 + (2.0 / 6.0) * v4) +
+         w3 * ((2.0 / 6.0) * v3 + (5.0 / 6.0) * v4 + (-1.0 / 6.0) * v5);
+}
+/* UpwindLeft = {offset 0, sign +1}, UpwindRight = {1, -1} (apply_wrapper.h:
+ * 48-58) on the 6-value stencil with downstream size 2. */
+static inline double weno_left(const double *a) {
+  return weno5_core(a[0], a[1], a[2], a[3], a[4]);
+}
+static inline double weno_right(const double *a) {
+  return weno5_core(a[5], a[4], a[3], a[2], a[1]);
+}
+
+void ora_weno5(const double *v, double *out) {
+  out[0] = weno_left(v);
+  out[1] = weno_right(v);
+}
+
+/* ---------------------------------------------------- Roe eigensystem ---- */
+/* Minor directions of each principal direction (eigendecomposition.h:97-130). */
+static const int kMinor[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+
+/* Symmetry-preserving projection order (state_reconstruction.h:37-57). */
+static const int kSeq2[2][4] = {{2, 3, 0, 1}, {2, 3, 0, 1}};
+static const int kSeq3[3][5] = {
+    {3, 4, 2, 0, 1}, {4, 2, 3, 0, 1}, {2, 3, 4, 0, 1}};
+
+static inline const int *sum_sequence(int dim, int dir) {
+  return dim == 2 ? kSeq2[dir] : kSeq3[dir];
+}
+
+typedef struct {
+  double L[NEMAX][NEMAX];
+  double R[NEMAX][NEMAX];
+  double lam[NEMAX];
+} eig_t;
+
+/* EigenDecomposition::ComputeRoeEigendecomposition for one face
+ * (eigendecomposition.h:238-466).  t = cell i, n = cell i+1 along dir; the
+ * velocities and pressure are the cells' prime states. */
+static int roe_eigensystem(int dim, int dir, double gamma, double b,
+                           const double *ct, const prim_t *pt,
+                           const double *cn, const prim_t *pn, eig_t *ev) {
+  int const ne = dim + 2;
+  memset(ev, 0, sizeof(*ev));
+  if (ct[MASS] <= 0.0 || cn[MASS] <= 0.0)
+    return 0;
+  int const pr = dir, m1 = kMinor[dir][0], m2 = kMinor[dir][1];
+  int const ev_p = dim; /* eigenvector index of the principal momentum */
+  int const Mp = MOMX + pr, M1 = MOMX + m1, M2 = MOMX + m2;
+
+  double const rho_t = ct[MASS];
+  double const one_rho_t = 1.0 / ct[MASS];
+  double const u_t = pt->vel[pr];
+  double const v_t = pt->vel[m1];
+  double const w_t = dim == 3 ? pt->vel[m2] : 0.0;
+  double const p_t = pt->p;
+  double const psi_t = eos_psi(gamma, b, p_t, one_rho_t);
+
+  double const rho_n = cn[MASS];
+  double const one_rho_n = 1.0 / cn[MASS];
+  double const u_n = pn->vel[pr];
+  double const v_n = pn->vel[m1];
+  double const w_n = dim == 3 ? pn->vel[m2] : 0.0;
+  double const p_n = pn->p;
+  double const psi_n = eos_psi(gamma, b, p_n, one_rho_n);
+
+  double const sq_t = sqrt(rho_t);
+  double const sq_n = sqrt(rho_n);
+  double const rho_div = 1.0 / (sq_t + sq_n);
+  double const rho_roe = sq_t * sq_n;
+  double const one_rho_roe = 1.0 / rho_roe;
+
+  double const u = ((u_t * sq_t) + (u_n * sq_n)) * rho_div;
+  double const v = ((v_t * sq_t) + (v_n * sq_n)) * rho_div;
+  double const w = ((w_t * sq_t) + (w_n * sq_n)) * rho_div;
+  double const psi = ((psi_t * sq_t) + (psi_n * sq_n)) * rho_div;
+  double const g = gamma - 1.0;
+  double const mz_t = dim == 3 ? ct[MOMX + 2] : 0.0;
+  double const mz_n = dim == 3 ? cn[MOMX + 2] : 0.0;
+  double const H =
+      (eos_enthalpy(dim, gamma, b, rho_t, ct[MOMX], ct[MOMX + 1], mz_t,
+                    ct[ENERGY]) *
+           sq_t +
+       eos_enthalpy(dim, gamma, b, rho_n, cn[MOMX], cn[MOMX + 1], mz_n,
+                    cn[ENERGY]) *
+           sq_n) *
+      rho_div;
+
+  double const q2 = dams(dim, u * u, v * v, w * w);
+  double const tmp =
+      dams(dim, (u_t - u_n) * (u_t - u_n), (v_t - v_n) * (v_t - v_n),
+           (w_t - w_n) * (w_t - w_n));
+  double const pod = ((p_t * one_rho_t) * sq_t + (p_n * one_rho_n) * sq_n) *
+                         rho_div +
+                     0.5 * rho_roe * (rho_div * rho_div) * tmp;
+  double const cc = psi + g * pod;
+  double const one_cc = 1.0 / cc;
+  double const c = sqrt(cc);
+
+  /* Left eigenvectors (eigendecomposition.h:316-359). */
+  ev->L[0][MASS] = 0.5 * one_cc * (g * q2 - g * H + (u + c) * c);
+  ev->L[0][Mp] = 0.5 * one_cc * (-u * g - c);
+  ev->L[0][M1] = 0.5 * one_cc * (-v * g);
+  if (dim == 3)
+    ev->L[0][M2] = 0.5 * one_cc * (-w * g);
+  ev->L[0][ENERGY] = 0.5 * one_cc * g;
+
+  ev->L[1][MASS] = v * one_rho_roe;
+  ev->L[1][Mp] = 0.0;
+  ev->L[1][M1] = -one_rho_roe;
+  if (dim == 3)
+    ev->L[1][M2] = 0.0;
+  ev->L[1][ENERGY] = 0.0;
+
+  if (dim == 3) {
+    ev->L[2][MASS] = -w * one_rho_roe;
+    ev->L[2][Mp] = 0.0;
+    ev->L[2][M1] = 0.0;
+    ev->L[2][M2] = one_rho_roe;
+    ev->L[2][ENERGY] = 0.0;
+  }
+
+  ev->L[ev_p][MASS] = one_cc * (H - q2);
+  ev->L[ev_p][Mp] = one_cc * u;
+  ev->L[ev_p][M1] = one_cc * v;
+  if (dim == 3)
+    ev->L[ev_p][M2] = one_cc * w;
+  ev->L[ev_p][ENERGY] = -one_cc;
+
+  ev->L[ne - 1][MASS] = 0.5 * one_cc * (g * q2 - g * H - (u - c) * c);
+  ev->L[ne - 1][Mp] = 0.5 * one_cc * (-u * g + c);
+  ev->L[ne - 1][M1] = 0.5 * one_cc * (-v * g);
+  if (dim == 3)
+    ev->L[ne - 1][M2] = 0.5 * one_cc * (-w * g);
+  ev->L[ne - 1][ENERGY] = 0.5 * one_cc * g;
+
+  /* Right eigenvectors (eigendecomposition.h:366-409). */
+  ev->R[MASS][0] = 1.0;
+  ev->R[MASS][1] = 0.0;
+  if (dim == 3)
+    ev->R[MASS][2] = 0.0;
+  ev->R[MASS][ev_p] = g;
+  ev->R[MASS][ne - 1] = 1.0;
+
+  ev->R[Mp][0] = u - c;
+  ev->R[Mp][1] = 0.0;
+  if (dim == 3)
+    ev->R[Mp][2] = 0.0;
+  ev->R[Mp][ev_p] = g * u;
+  ev->R[Mp][ne - 1] = u + c;
+
+  ev->R[M1][0] = v;
+  ev->R[M1][1] = -rho_roe;
+  if (dim == 3)
+    ev->R[M1][2] = 0.0;
+  ev->R[M1][ev_p] = g * v;
+  ev->R[M1][ne - 1] = v;
+
+  if (dim == 3) {
+    ev->R[M2][0] = w;
+    ev->R[M2][1] = 0.0;
+    ev->R[M2][2] = rho_roe;
+    ev->R[M2][ev_p] = g * w;
+    ev->R[M2][ne - 1] = w;
+  }
+
+  ev->R[ENERGY][0] = H - u * c;
+  ev->R[ENERGY][1] = -v * rho_roe;
+  if (dim == 3)
+    ev->R[ENERGY][2] = rho_roe * w;
+  ev->R[ENERGY][ev_p] = g * H - c * c;
+  ev->R[ENERGY][ne - 1] = H + u * c;
+
+  /* Roe flux-function eigenvalues (eigendecomposition.h:416-421). */
+  ev->lam[0] = fabs(u - c);
+  ev->lam[ne - 1] = fabs(u + c);
+  for (int l = 1; l < ne - 1; ++l)
+    ev->lam[l] = fabs(u);
+  return 1;
+}
+
+int ora_roe_eigensystem(int dim, int dir, double gamma, double b,
+                        const double *cons_t, const double *cons_n, double *L,
+                        double *R, double *lambda) {
+  int const ne = dim + 2;
+  prim_t pt, pn;
+  cons_to_prim(dim, gamma, b, cons_t, &pt);
+  cons_to_prim(dim, gamma, b, cons_n, &pn);
+  eig_t ev;
+  int const ok =
+      roe_eigensystem(dim, dir, gamma, b, cons_t, &pt, cons_n, &pn, &ev);
+  for (int r = 0; r < ne; ++r) {
+    for (int c = 0; c < ne; ++c) {
+      L[r * ne + c] = ev.L[r][c];
+      R[r * ne + c] = ev.R[r][c];
+    }
+    lambda[r] = ev.lam[r];
+  }
+  return ok;
+}
+
+/* TransformToPhysicalSpace (eigendecomposition.h:144-165). */
+static void to_physical(int ne, const double *ch, const double R[][NEMAX],
+                        double *out) {
+  for (int l = 0; l < ne; ++l) {
+    double s = 0.0;
+    for (int n = 1; n < ne - 1; ++n)
+      s += ch[n] * R[l][n];
+    s += (ch[0] * R[l][0] + ch[ne - 1] * R[l][ne - 1]);
+    out[l] = s;
+  }
+}
+
+/* ------------------------------------------------------------- HLLC ---- */
+/* CalculateSignalSpeed, Einfeldt branch (hll_signal_speed_calculator.h:
+ * 77-101). */
+static inline void einfeldt(double rl, double rr, double ul, double ur,
+                            double cl, double cr, double *sl, double *sr) {
+  double const sql = sqrt(rl);
+  double const sqr = sqrt(rr);
+  double const rho_div = 1.0 / (sql + sqr);
+  double const u_roe = ((ul * sql) + (ur * sqr)) * rho_div;
+  double const c_e = sqrt(((cl * cl * sql) + (cr * cr * sqr)) * rho_div +
+                          (0.5 * sql * sqr * rho_div * rho_div) * (ur - ul) *
+                              (ur - ul));
+  *sl = smin(u_roe - c_e, ul - cl);
+  *sr = smax(u_roe + c_e, ur + cr);
+}
+
+/* HllcRiemannSolver::SolveRiemannProblemImplementation (hllc_riemann_solver.h:
+ * 52-198), including the clamped signal speed in the energy star state. */
+static void hllc_flux(int dim, int dir, double gamma, double b,
+                      const double *cl, const double *cr, const prim_t *pl,
+                      const prim_t *pr, double *f) {
+  int const ne = dim + 2;
+  int const Mp = MOMX + dir;
+  double qsl[NEMAX], qsr[NEMAX], fl[NEMAX], fr[NEMAX];
+  double const p_l = pl->p, p_r = pr->p;
+  double const one_rho_l = 1.0 / cl[MASS];
+  double const one_rho_r = 1.0 / cr[MASS];
+  double const u_l = pl->vel[dir], u_r = pr->vel[dir];
+  double const c_l = eos_sos(gamma, b, cl[MASS], p_l);
+  double const c_r = eos_sos(gamma, b, cr[MASS], p_r);
+  double sls, srs;
+  einfeldt(cl[MASS], cr[MASS], u_l, u_r, c_l, c_r, &sls, &srs);
+  double const sc =
+      ((p_r - p_l) + (cl[Mp] * (sls - u_l) - cr[Mp] * (srs - u_r))) /
+      (cl[MASS] * (sls - u_l) - cr[MASS] * (srs - u_r));
+  double const sl = smin(sls, 0.0);
+  double const sr = smax(srs, 0.0);
+  double const chi_l = (sls - u_l) / (sls - sc);
+  double const chi_r = (srs - u_r) / (srs - sc);
+
+  qsl[MASS] = cl[MASS] * chi_l;
+  qsl[Mp] = cl[MASS] * chi_l * sc;
+  qsl[ENERGY] = cl[MASS] * chi_l *
+                (cl[ENERGY] * one_rho_l +
+                 (sc - u_l) * (sc + p_l / (cl[MASS] * (sl - u_l))));
+  qsr[MASS] = cr[MASS] * chi_r;
+  qsr[Mp] = cr[MASS] * chi_r * sc;
+  qsr[ENERGY] = cr[MASS] * chi_r *
+                (cr[ENERGY] * one_rho_r +
+                 (sc - u_r) * (sc + p_r / (cr[MASS] * (sr - u_r))));
+
+  fl[MASS] = cl[Mp];
+  fl[Mp] = ((cl[Mp] * cl[Mp]) * one_rho_l) + p_l;
+  fl[ENERGY] = u_l * (cl[ENERGY] + p_l);
+  fr[MASS] = cr[Mp];
+  fr[Mp] = ((cr[Mp] * cr[Mp]) * one_rho_r) + p_r;
+  fr[ENERGY] = u_r * (cr[ENERGY] + p_r);
+
+  for (int d = 0; d < dim - 1; ++d) {
+    int const Mm = MOMX + kMinor[dir][d];
+    qsl[Mm] = chi_l * cl[Mm];
+    qsr[Mm] = chi_r * cr[Mm];
+    fl[Mm] = u_l * cl[Mm];
+    fr[Mm] = u_r * cr[Mm];
+  }
+  int const sg = signum(sc);
+  for (int n = 0; n < ne; ++n) {
+    double const fsl = fl[n] + (sl * (qsl[n] - cl[n]));
+    double const fsr = fr[n] + (sr * (qsr[n] - cr[n]));
+    f[n] = (0.5 * (1.0 + sg) * fsl + 0.5 * (1.0 - sg) * fsr);
+  }
+}
+
+void ora_hllc(int dim, int dir, double gamma, double b, const double *cl,
+              const double *cr, double *out) {
+  prim_t pl, pr;
+  cons_to_prim(dim, gamma, b, cl, &pl);
+  cons_to_prim(dim, gamma, b, cr, &pr);
+  hllc_flux(dim, dir, gamma, b, cl, cr, &pl, &pr, out);
+}
+
+/* ------------------------------------------------------ face fluxes ---- */
+typedef struct {
+  int dim, ne;
+  int tz;              /* TCZ */
+  ptrdiff_t stride[3]; /* element strides of i, j, k */
+  ptrdiff_t field;     /* elements per field */
+} layout_t;
+
+static layout_t make_layout(int dim) {
+  layout_t L;
+  L.dim = dim;
+  L.ne = dim + 2;
+  L.tz = dim == 3 ? TC : 1;
+  L.stride[0] = (ptrdiff_t)TC * L.tz;
+  L.stride[1] = L.tz;
+  L.stride[2] = 1;
+  L.field = (ptrdiff_t)TC * TC * L.tz;
+  return L;
+}
+
+static inline void load_cell(const layout_t *L, const double *blk,
+                             ptrdiff_t off, double *c) {
+  for (int e = 0; e < L->ne; ++e)
+    c[e] = blk[e * L->field + off];
+}
+
+/* Flux of one face (cell `off`, face +1/2 along dir) for either family.
+ * FV: FiniteVolumeScheme::ComputeFluxes (finite_volume_scheme.cpp:160-251)
+ *     with CharacteristicStateReconstruction (characteristic_state_
+ *     reconstruction.h:60-134).
+ * Roe: FluxSplittingScheme::ComputeFluxes (flux_splitting_scheme.cpp:185-315)
+ *     with ComputeAdvection (:42-76).
+ * Returns 0 if the face is skipped (flux left at zero). */
+static int face_flux(const layout_t *L, int flux, int dir, double gamma,
+                     double b, const double *blk, ptrdiff_t off, double *f) {
+  int const dim = L->dim, ne = L->ne;
+  ptrdiff_t const s = L->stride[dir];
+  double U[6][NEMAX];
+  prim_t P[6];
+  for (int m = 0; m < 6; ++m) {
+    load_cell(L, blk, off + (m - 2) * s, U[m]);
+    cons_to_prim(dim, gamma, b, U[m], &P[m]);
+  }
+  eig_t ev;
+  roe_eigensystem(dim, dir, gamma, b, U[2], &P[2], U[3], &P[3], &ev);
+  const int *seq = sum_sequence(dim, dir);
+  for (int e = 0; e < ne; ++e)
+    f[e] = 0.0;
+
+  if (flux == ORA_FLUX_HLLC) {
+    double minus[NEMAX], plus[NEMAX];
+    for (int n = 0; n < ne; ++n) {
+      double uc[6];
+      for (int m = 0; m < 6; ++m) {
+        uc[m] = 0.0;
+        for (int q = 0; q < ne; ++q) {
+          int const l = seq[q];
+          uc[m] += ev.L[n][l] * U[m][l];
+        }
+      }
+      minus[n] = weno_left(uc);
+      plus[n] = weno_right(uc);
+    }
+    double cl[NEMAX], cr[NEMAX];
+    to_physical(ne, minus, (const double(*)[NEMAX])ev.R, cl);
+    to_physical(ne, plus, (const double(*)[NEMAX])ev.R, cr);
+    if (cl[MASS] <= DBL_EPSILON || cr[MASS] <= DBL_EPSILON)
+      return 0;
+    prim_t pl, pr;
+    cons_to_prim(dim, gamma, b, cl, &pl);
+    cons_to_prim(dim, gamma, b, cr, &pr);
+    if (pl.p <= -b || pr.p <= -b)
+      return 0;
+    double hf[NEMAX];
+    hllc_flux(dim, dir, gamma, b, cl, cr, &pl, &pr, hf);
+    for (int e = 0; e < ne; ++e)
+      f[e] += hf[e];
+    return 1;
+  }
+
+  /* Roe flux splitting. */
+  double adv[6][NEMAX];
+  for (int m = 0; m < 6; ++m) {
+    double const rho = P[m].rho, p = P[m].p, ud = P[m].vel[dir];
+    adv[m][MASS] = rho * ud;
+    adv[m][ENERGY] = (U[m][ENERGY] + p) * ud;
+    for (int d = 0; d < dim; ++d)
+      adv[m][MOMX + d] = rho * (ud * P[m].vel[d]);
+    adv[m][MOMX + dir] += p;
+  }
+  double chf[NEMAX] = {0.0};
+  for (int n = 0; n < ne; ++n) {
+    double pos[6], neg[6];
+    for (int m = 0; m < 6; ++m) {
+      pos[m] = 0.0;
+      neg[m] = 0.0;
+      for (int q = 0; q < ne; ++q) {
+        int const l = seq[q];
+        double const uct = U[m][l] * ev.L[n][l];
+        double const act = adv[m][l] * ev.L[n][l];
+        pos[m] += (act + ev.lam[n] * uct);
+        neg[m] += (act - ev.lam[n] * uct);
+      }
+    }
+    chf[n] = 0.5 * (weno_left(pos) + weno_right(neg));
+  }
+  double phys[NEMAX];
+  to_physical(ne, chf, (const double(*)[NEMAX])ev.R, phys);
+  for (int e = 0; e < ne; ++e)
+    f[e] += phys[e];
+  return 1;
+}
+
+/* Face-flux arrays [NE][IC+1][IC+1][ICZ+1] as the reference's UpdateFluxes
+ * holds them: the face i+1/2 of cell i (halo index) sits at i - (HS-1). */
+static void block_face_fluxes(int dim, int flux, double gamma, double b,
+                              const double *blk, double *fx, double *fy,
+                              double *fz) {
+  layout_t const L = make_layout(dim);
+  int const icz = dim == 3 ? IC : 1;
+  size_t const n1 = IC + 1, nz = (size_t)icz + 1;
+  size_t const per_field = n1 * n1 * nz;
+  double *out[3] = {fx, fy, fz};
+  for (int d = 0; d < 3; ++d)
+    if (out[d])
+      memset(out[d], 0, sizeof(double) * (size_t)L.ne * per_field);
+  for (int dir = 0; dir < dim; ++dir) {
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+      if (a >= dim) {
+        lo[a] = 0;
+        hi[a] = 0;
+      } else {
+        lo[a] = (a == dir) ? HS - 1 : HS;
+        hi[a] = HS + IC - 1;
+      }
+    }
+    /* Offsets from halo indices to face-array indices
+     * (finite_volume_scheme.cpp:176-180): HS-1 for active axes, -1 for z in 2D. */
+    int const oz = dim == 3 ? HS - 1 : -1;
+    for (int i = lo[0]; i <= hi[0]; ++i)
+      for (int j = lo[1]; j <= hi[1]; ++j)
+        for (int k = lo[2]; k <= hi[2]; ++k) {
+          ptrdiff_t const off =
+              i * L.stride[0] + j * L.stride[1] + k * L.stride[2];
+          double f[NEMAX];
+          face_flux(&L, flux, dir, gamma, b, blk, off, f);
+          size_t const fi = (size_t)(i - (HS - 1)), fj = (size_t)(j - (HS - 1)),
+                       fk = (size_t)(k - oz);
+          for (int e = 0; e < L.ne; ++e)
+            out[dir][(((size_t)e * n1 + fi) * n1 + fj) * nz + fk] += f[e];
+        }
+  }
+}
+
+void ora_block_fluxes(int dim, int flux, double gamma, double b,
+                      const double *avg, double cell_size, double *out) {
+  (void)cell_size;
+  int const icz = dim == 3 ? IC : 1;
+  size_t const n = (size_t)(dim + 2) * (IC + 1) * (IC + 1) * (icz + 1);
+  block_face_fluxes(dim, flux, gamma, b, avg, out, out + n, out + 2 * n);
+}
+
+/* SpaceSolver::UpdateFluxes divergence (space_solver.cpp:131-153): RHS starts
+ * at zero, volume forces are zero (sources off).  out: [NE][IC][IC][ICZ]. */
+static void block_rhs(int dim, int flux, double gamma, double b,
+                      const double *blk, double cell_size, double *out,
+                      double *scratch) {
+  int const ne = dim + 2, icz = dim == 3 ? IC : 1;
+  size_t const n1 = IC + 1, nz = (size_t)icz + 1;
+  size_t const nf = (size_t)ne * n1 * n1 * nz;
+  double *fx = scratch, *fy = scratch + nf, *fz = scratch + 2 * nf;
+  block_face_fluxes(dim, flux, gamma, b, blk, fx, fy, fz);
+  double const one_cell_size = 1.0 / cell_size;
+#define FI(a, e, i, j, k) a[(((size_t)(e)*n1 + (i)) * n1 + (j)) * nz + (k)]
+  for (int e = 0; e < ne; ++e)
+    for (int i = 0; i < IC; ++i)
+      for (int j = 0; j < IC; ++j)
+        for (int k = 0; k < icz; ++k) {
+          double const s =
+              dams(dim, FI(fx, e, i, j + 1, k + 1) - FI(fx, e, i + 1, j + 1, k + 1),
+                   FI(fy, e, i + 1, j, k + 1) - FI(fy, e, i + 1, j + 1, k + 1),
+                   FI(fz, e, i + 1, j + 1, k) - FI(fz, e, i + 1, j + 1, k + 1));
+          out[(((size_t)e * IC + i) * IC + j) * icz + k] =
+              0.0 + (0.0 + s * one_cell_size);
+        }
+#undef FI
+}
+
+static size_t rhs_scratch_doubles(int dim) {
+  int const icz = dim == 3 ? IC : 1;
+  return 3 * (size_t)(dim + 2) * (IC + 1) * (IC + 1) * (icz + 1);
+}
+
+void ora_block_rhs(int dim, int flux, double gamma, double b,
+                   const double *avg, double cell_size, double *out) {
+  double *scratch = (double *)malloc(sizeof(double) * rhs_scratch_doubles(dim));
+  block_rhs(dim, flux, gamma, b, avg, cell_size, out, scratch);
+  free(scratch);
+}
+
+/* ---------------------------------------------------------- full run ---- */
+typedef struct {
+  ora_grid g;
+  layout_t L;
+  int nblk;
+  size_t bsize; /* doubles per haloed block */
+  double *avg, *rhs, *init;
+} run_t;
+
+typedef void (*block_fn)(run_t *r, int lo, int hi, void *arg);
+typedef struct {
+  run_t *r;
+  block_fn fn;
+  int lo, hi;
+  void *arg;
+} job_t;
+
+static void *job_main(void *p) {
+  job_t *j = (job_t *)p;
+  j->fn(j->r, j->lo, j->hi, j->arg);
+  return NULL;
+}
+
+static void parallel_blocks(run_t *r, int threads, block_fn fn, void *arg) {
+  int const n = r->nblk;
+  if (threads <= 1 || n <= 1) {
+    fn(r, 0, n, arg);
+    return;
+  }
+  if (threads > n)
+    threads = n;
+  pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+  job_t *jobs = (job_t *)malloc(sizeof(job_t) * (size_t)threads);
+  for (int t = 0; t < threads; ++t) {
+    jobs[t].r = r;
+    jobs[t].fn = fn;
+    jobs[t].lo = (int)((long long)n * t / threads);
+    jobs[t].hi = (int)((long long)n * (t + 1) / threads);
+    jobs[t].arg = arg;
+    pthread_create(&tid[t], NULL, job_main, &jobs[t]);
+  }
+  for (int t = 0; t < threads; ++t)
+    pthread_join(tid[t], NULL);
+  free(tid);
+  free(jobs);
+}
+
+static inline void block_coords(const run_t *r, int b, int *c) {
+  c[0] = b / (r->g.nb[1] * r->g.nb[2]);
+  c[1] = (b / r->g.nb[2]) % r->g.nb[1];
+  c[2] = b % r->g.nb[2];
+}
+
+/* Same-level halo copy (internal_halo_manager.cpp:439-466): every halo cell of
+ * every block takes the value of the internal cell that owns its position,
+ * wrapped across periodic sides; positions outside a zero-gradient side are
+ * left to the external BC below. */
+static void halo_copy(run_t *r, int lo, int hi, void *arg) {
+  (void)arg;
+  layout_t const *L = &r->L;
+  int const dim = L->dim;
+  for (int b = lo; b < hi; ++b) {
+    int bc[3];
+    block_coords(r, b, bc);
+    double *dst = r->rhs + (size_t)b * r->bsize;
+    for (int i = 0; i < TC; ++i)
+      for (int j = 0; j < (dim >= 2 ? TC : 1); ++j)
+        for (int k = 0; k < L->tz; ++k) {
+          int const hx = i < HS || i >= HS + IC;
+          int const hy = j < HS || j >= HS + IC;
+          int const hz = dim == 3 && (k < HS || k >= HS + IC);
+          if (!(hx || hy || hz))
+            continue;
+          int idx[3] = {i - HS, j - HS, k - HS};
+          int blk[3] = {bc[0], bc[1], bc[2]};
+          int outside = 0;
+          for (int d = 0; d < dim; ++d) {
+            int gc = blk[d] * IC + idx[d];
+            int const n = r->g.nb[d] * IC;
+            if (gc < 0 || gc >= n) {
+              if (!r->g.periodic[d]) {
+                outside = 1;
+                break;
+              }
+              gc = (gc % n + n) % n;
+            }
+            blk[d] = gc / IC;
+            idx[d] = gc % IC;
+          }
+          if (outside)
+            continue;
+          int const src_b = (blk[0] * r->g.nb[1] + blk[1]) * r->g.nb[2] +
+                            (dim == 3 ? blk[2] : 0);
+          const double *src = r->rhs + (size_t)src_b * r->bsize;
+          ptrdiff_t const so = (idx[0] + HS) * L->stride[0] +
+                               (idx[1] + HS) * L->stride[1] +
+                               (dim == 3 ? (idx[2] + HS) : 0);
+          ptrdiff_t const doff =
+              i * L->stride[0] + j * L->stride[1] + k * L->stride[2];
+          for (int e = 0; e < L->ne; ++e)
+            dst[e * L->field + doff] = src[e * L->field + so];
+        }
+  }
+}
+
+/* ZeroGradientBoundaryCondition (zero_gradient_boundary_condition.h:45-57,
+ * boundary_constants.h:212-253): the halo slab over the full transverse
+ * extent takes the first/last internal layer.  Order East, West, North,
+ * South, Top, Bottom. */
+static void zero_gradient(run_t *r, int lo, int hi, void *arg) {
+  (void)arg;
+  layout_t const *L = &r->L;
+  int const dim = L->dim;
+  for (int b = lo; b < hi; ++b) {
+    int bc[3];
+    block_coords(r, b, bc);
+    double *v = r->rhs + (size_t)b * r->bsize;
+    for (int d = 0; d < dim; ++d) {
+      if (r->g.periodic[d])
+        continue;
+      for (int side = 1; side >= 0; --side) { /* high side first */
+        if (side == 1 && bc[d] != r->g.nb[d] - 1)
+          continue;
+        if (side == 0 && bc[d] != 0)
+          continue;
+        int const src_layer = side ? HS + IC - 1 : HS;
+        int const h0 = side ? HS + IC : 0;
+        int ext[3] = {TC, dim >= 2 ? TC : 1, L->tz};
+        for (int a = 0; a < ext[0]; ++a)
+          for (int c = 0; c < ext[1]; ++c)
+            for (int z = 0; z < ext[2]; ++z) {
+              int idx[3] = {a, c, z};
+              if (idx[d] < h0 || idx[d] >= h0 + HS)
+                continue;
+              int sidx[3] = {a, c, z};
+              sidx[d] = src_layer;
+              ptrdiff_t const doff = idx[0] * L->stride[0] +
+                                     idx[1] * L->stride[1] + idx[2];
+              ptrdiff_t const soff = sidx[0] * L->stride[0] +
+                                     sidx[1] * L->stride[1] + sidx[2];
+              for (int e = 0; e < L->ne; ++e)
+                v[e * L->field + doff] = v[e * L->field + soff];
+            }
+      }
+    }
+  }
+}
+
+typedef struct {
+  int stage;
+  double dt;
+  double *maxs; /* per block signal-speed maximum */
+} stage_arg_t;
+
+/* ComputeTimestepSize inner loop (modular_algorithm_assembler.cpp:1840-1869). */
+static void block_signal(run_t *r, int lo, int hi, void *arg) {
+  stage_arg_t *sa = (stage_arg_t *)arg;
+  layout_t const *L = &r->L;
+  int const dim = L->dim;
+  for (int b = lo; b < hi; ++b) {
+    const double *a = r->avg + (size_t)b * r->bsize;
+    double s = 0.0;
+    for (int i = HS; i < HS + IC; ++i)
+      for (int j = HS; j < HS + IC; ++j)
+        for (int k = (dim == 3 ? HS : 0); k < (dim == 3 ? HS + IC : 1); ++k) {
+          double c[NEMAX];
+          load_cell(L, a, i * L->stride[0] + j * L->stride[1] + k, c);
+          prim_t p;
+          cons_to_prim(dim, r->g.gamma, r->g.b, c, &p);
+          double const cs = eos_sos(r->g.gamma, r->g.b, p.rho, p.p);
+          double vps[3] = {0.0, 0.0, 0.0};
+          for (int d = 0; d < dim; ++d)
+            vps[d] = fabs(p.vel[d]) + cs;
+          double const sum =
+              dim == 2 ? vps[0] + vps[1] : cms3(vps[0], vps[1], vps[2]);
+          s = smax(s, sum);
+        }
+    sa->maxs[b] = s;
+  }
+}
+
+/* One RK stage on blocks [lo, hi): FillInitialBuffer, UpdateFluxes,
+ * PrepareBufferForIntegration, IntegrateNode (time_integrator.h:98-113,
+ * 312-340, 369-457; runge_kutta_3_TVD.h:39-59). */
+static void block_stage(run_t *r, int lo, int hi, void *arg) {
+  stage_arg_t *sa = (stage_arg_t *)arg;
+  layout_t const *L = &r->L;
+  int const dim = L->dim, ne = L->ne, icz = dim == 3 ? IC : 1;
+  static const double kConsMul[3] = {1.0, 0.25, 2.0 / 3.0};
+  static const double kBufMul[2][2] = {{0.25, 0.75}, {2.0 / 3.0, 1.0 / 3.0}};
+  double *scratch = (double *)malloc(sizeof(double) * rhs_scratch_doubles(dim));
+  double *rhs_int =
+      (double *)malloc(sizeof(double) * (size_t)ne * IC * IC * (size_t)icz);
+  double const timestep = 0.0 + sa->dt; /* accumulate over one micro step */
+  double const cdt = kConsMul[sa->stage] * timestep;
+  for (int b = lo; b < hi; ++b) {
+    double *a = r->avg + (size_t)b * r->bsize;
+    double *u0 = r->init + (size_t)b * r->bsize;
+    double *out = r->rhs + (size_t)b * r->bsize;
+    if (sa->stage == 0)
+      memcpy(u0, a, sizeof(double) * r->bsize);
+    block_rhs(dim, r->g.flux, r->g.gamma, r->g.b, a, r->g.cell_size, rhs_int,
+              scratch);
+    if (sa->stage != 0) {
+      double const m0 = kBufMul[sa->stage - 1][0], m1 = kBufMul[sa->stage - 1][1];
+      for (size_t x = 0; x < r->bsize; ++x)
+        a[x] = m0 * a[x] + m1 * u0[x];
+    }
+    for (int e = 0; e < ne; ++e)
+      for (int i = 0; i < IC; ++i)
+        for (int j = 0; j < IC; ++j)
+          for (int k = 0; k < icz; ++k) {
+            ptrdiff_t const off = e * L->field + (i + HS) * L->stride[0] +
+                                  (j + HS) * L->stride[1] +
+                                  (dim == 3 ? k + HS : 0);
+            out[off] =
+                a[off] + cdt * rhs_int[(((size_t)e * IC + i) * IC + j) * icz + k];
+          }
+  }
+  free(scratch);
+  free(rhs_int);
+}
+
+static void halo_and_swap(run_t *r, int threads) {
+  parallel_blocks(r, threads, halo_copy, NULL);
+  parallel_blocks(r, threads, zero_gradient, NULL);
+  double *t = r->avg;
+  r->avg = r->rhs;
+  r->rhs = t;
+}
+
+int ora_run(const ora_grid *g, double cfl, double t_end, int max_steps,
+            int threads, double *state, double *dts, double *t_final) {
+  run_t r;
+  r.g = *g;
+  if (g->dim == 2) {
+    r.g.nb[2] = 1;
+    r.g.periodic[2] = 0;
+  }
+  r.L = make_layout(g->dim);
+  r.nblk = r.g.nb[0] * r.g.nb[1] * r.g.nb[2];
+  r.bsize = (size_t)r.L.ne * (size_t)r.L.field;
+  size_t const total = (size_t)r.nblk * r.bsize;
+  r.avg = (double *)calloc(total, sizeof(double));
+  r.rhs = (double *)calloc(total, sizeof(double));
+  r.init = (double *)calloc(total, sizeof(double));
+  int const dim = g->dim, ne = r.L.ne, icz = dim == 3 ? IC : 1;
+  size_t const NX = (size_t)r.g.nb[0] * IC, NY = (size_t)r.g.nb[1] * IC,
+               NZ = (size_t)r.g.nb[2] * icz;
+  /* ImposeInitialCondition into the RHS buffer, halo update, swap
+   * (modular_algorithm_assembler.cpp:397-517). */
+  for (int b = 0; b < r.nblk; ++b) {
+    int bc[3];
+    block_coords(&r, b, bc);
+    double *o = r.rhs + (size_t)b * r.bsize;
+    for (int e = 0; e < ne; ++e)
+      for (int i = 0; i < IC; ++i)
+        for (int j = 0; j < IC; ++j)
+          for (int k = 0; k < icz; ++k)
+            o[e * r.L.field + (i + HS) * r.L.stride[0] + (j + HS) * r.L.stride[1] +
+              (dim == 3 ? k + HS : 0)] =
+                state[(((size_t)e * NX + (size_t)bc[0] * IC + i) * NY +
+                       (size_t)bc[1] * IC + j) *
+                          NZ +
+                      (size_t)bc[2] * icz + k];
+  }
+  halo_and_swap(&r, threads);
+
+  double *maxs = (double *)malloc(sizeof(double) * (size_t)r.nblk);
+  double current = 0.0;
+  int steps = 0;
+  while (current < t_end && steps < max_steps) {
+    stage_arg_t sa = {0, 0.0, maxs};
+    parallel_blocks(&r, threads, block_signal, &sa);
+    double s = 0.0;
+    for (int b = 0; b < r.nblk; ++b)
+      s = smax(s, maxs[b]);
+    /* dt = CFL / (S / dx), end-time clamp (modular_algorithm_assembler.cpp:
+     * 1932-1982). */
+    double const rate = s / g->cell_size;
+    double dt = rate == 0.0 ? DBL_MAX : cfl / rate;
+    double const run_time = current;
+    if (run_time + dt > t_end)
+      dt = smax(t_end - run_time, 0.0);
+    if (dts)
+      dts[steps] = dt;
+    for (int stage = 0; stage < 3; ++stage) {
+      stage_arg_t st = {stage, dt, maxs};
+      parallel_blocks(&r, threads, block_stage, &st);
+      halo_and_swap(&r, threads);
+    }
+    current += 0.0 + dt; /* FinishMacroTimestep (time_integrator.h:244-248) */
+    ++steps;
+    if (dt < DBL_EPSILON && dt > 0.0)
+      break;
+  }
+  for (int b = 0; b < r.nblk; ++b) {
+    int bc[3];
+    block_coords(&r, b, bc);
+    const double *a = r.avg + (size_t)b * r.bsize;
+    for (int e = 0; e < ne; ++e)
+      for (int i = 0; i < IC; ++i)
+        for (int j = 0; j < IC; ++j)
+          for (int k = 0; k < icz; ++k)
+            state[(((size_t)e * NX + (size_t)bc[0] * IC + i) * NY +
+                   (size_t)bc[1] * IC + j) *
+                      NZ +
+                  (size_t)bc[2] * icz + k] =
+                a[e * r.L.field + (i + HS) * r.L.stride[0] +
+                  (j + HS) * r.L.stride[1] + (dim == 3 ? k + HS : 0)];
+  }
+  if (t_final)
+    *t_final = current;
+  free(maxs);
+  free(r.avg);
+  free(r.rhs);
+  free(r.init);
+  return steps;
+}
+
+void ora_domain_sums(int dim, const int *ncell, const double *state,
+                     double *out) {
+  size_t const n = (size_t)ncell[0] * (size_t)ncell[1] *
+                   (size_t)(dim == 3 ? ncell[2] : 1);
+  for (int e = 0; e < dim + 2; ++e) {
+    double s = 0.0;
+    for (size_t x = 0; x < n; ++x)
+      s += state[(size_t)e * n + x];
+    out[e] = s;
+  }
+}

@@ -1,0 +1,75 @@
+/*
+ * ORACLE — test infrastructure only.  Plain-C restatement of ALPACA's
+ * single-phase finite-volume right-hand side on uniform level-0 blocks, used
+ * as the checker for the HIP path (tests/, __graft_entry__.smoke() and the
+ * "port" CPU-baseline leg of bench.py).  The product never links this.
+ *
+ * Every function follows the reference file:line cited at its definition in
+ * alpaca_oracle.c, with the same operation order (compiled with
+ * -ffp-contract=off, so a*b+c is never fused).
+ *
+ * Storage: one haloed block is [NE][TC][TC][TCZ] doubles, k fastest, with
+ * NE = dim + 2 equations in the order Mass, Energy, MomX, MomY, (MomZ),
+ * TC = 24 and TCZ = 24 in 3D, 1 in 2D (field_buffer.h:39, field_setup.h:101).
+ */
+#ifndef ALPACA_ORACLE_H
+#define ALPACA_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORA_IC = 16, ORA_HS = 4, ORA_TC = 24 };
+enum { ORA_FLUX_HLLC = 0, ORA_FLUX_ROE = 1 };
+
+typedef struct {
+  int dim;         /* 2 or 3 */
+  int nb[3];       /* blocks per direction (nb[2] = 1 in 2D) */
+  int periodic[3]; /* 1: periodic in that direction, 0: zero-gradient */
+  int flux;        /* ORA_FLUX_HLLC or ORA_FLUX_ROE */
+  double gamma;    /* stiffened-gas gamma */
+  double b;        /* stiffened-gas background pressure (0: ideal gas) */
+  double cell_size;
+} ora_grid;
+
+/* WENO5-JS on six values: out[0] upwind-left, out[1] upwind-right. */
+void ora_weno5(const double *v, double *out);
+
+/* Stiffened-gas EOS, same argument/result layout as ref_eos. */
+void ora_eos(int dim, double gamma, double b, const double *cons,
+             const double *prim, double *out);
+
+/* Roe eigensystem at the face between cells t (left) and n (right) along dir;
+ * cons_* are NE conservatives; out L[NE][NE], R[NE][NE], lambda[NE].
+ * Returns 0 if the face is skipped (a density <= 0, all outputs zero). */
+int ora_roe_eigensystem(int dim, int dir, double gamma, double b,
+                        const double *cons_t, const double *cons_n, double *L,
+                        double *R, double *lambda);
+
+/* HLLC (Einfeldt) flux along dir for conservative face states. */
+void ora_hllc(int dim, int dir, double gamma, double b, const double *cl,
+              const double *cr, double *out);
+
+/* Face fluxes of one haloed block: out = three [NE][17][17][17 or 2] arrays. */
+void ora_block_fluxes(int dim, int flux, double gamma, double b,
+                      const double *avg, double cell_size, double *out);
+
+/* Right-hand side of one haloed block on its internal cells [NE][16][16][16|1]. */
+void ora_block_rhs(int dim, int flux, double gamma, double b,
+                   const double *avg, double cell_size, double *out);
+
+/* Full uniform run, same contract as ref_run: state is [NE][NX][NY][NZ]
+ * internal conservatives of the domain, overwritten with the final state. */
+int ora_run(const ora_grid *g, double cfl, double t_end, int max_steps,
+            int threads, double *state, double *dts, double *t_final);
+
+/* Conservation sums over a domain state [NE][NX][NY][NZ]: out[e] = sum. */
+void ora_domain_sums(int dim, const int *ncell, const double *state,
+                     double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
