@@ -1,0 +1,585 @@
+// gfx950 kernels and the C ABI (include/alpaca_gpu.h) for ALPACA's uniform
+// single-phase finite-volume right-hand side.
+//
+// Kernels (one launch each, all on the caller's stream):
+//   face_flux_kernel<DIM,DIR,FLUX,R>  one thread per cell face; six-cell
+//       stencil read straight from the haloed block (coalesced along k);
+//       Roe eigensystem + characteristic WENO5 + HLLC or Roe splitting.
+//   combine_kernel<DIM>  one thread per internal cell: flux divergence with
+//       the reference's ConsistencyManagedSum, times 1/dx, fused with the
+//       SSP-RK3 combination and the non-finite check.
+//   halo_kernel<DIM>  face halo slabs from neighbours / zero-gradient.
+//   signal_kernel<DIM> + dt_kernel  CFL reduction and time step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/alpaca_gpu.h"
+#include "fv_device.h"
+
+namespace alpaca {
+namespace {
+
+constexpr int kThreads = 256;
+
+// Faces per (block, direction): 17 along the direction, 16 across.
+template <int DIM> constexpr int faces_per_dir() {
+  return (kIC + 1) * kIC * (DIM == 3 ? kIC : 1);
+}
+
+// Logical workgroup index with consecutive logical groups kept on one XCD:
+// the hardware deals workgroups round-robin over the 8 XCDs, so group g runs
+// on XCD g % 8.  Remapping makes each XCD sweep a contiguous range of blocks,
+// which keeps a block's stencil data in one XCD's L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned g, unsigned n) {
+  constexpr unsigned kXcd = 8;
+  unsigned const per = n / kXcd;
+  unsigned const full = per * kXcd;
+  if (g >= full) return g;
+  return (g % kXcd) * per + g / kXcd;
+}
+
+struct CaseArgs {
+  int nb[3];
+  int bc[3];
+  double gamma, b, cell_size;
+};
+
+// ---------------------------------------------------------------- faces ---
+// Output layout per (block, dir): [NE][n0][n1][n2] with n_dir = 17 and the
+// other extents 16 (1 for z in 2D): face index a along DIR is the face
+// between halo cells a+3 and a+4.
+template <int DIM, int DIR, int FLUX, typename R>
+__global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__restrict__ in,
+                                                             double *__restrict__ flux_out,
+                                                             int nblocks, double gamma,
+                                                             double b) {
+  using D = Dims<DIM>;
+  constexpr int NE = D::NE;
+  constexpr int NF = faces_per_dir<DIM>();
+  constexpr int n0 = DIR == 0 ? kIC + 1 : kIC;
+  constexpr int n1 = DIR == 1 ? kIC + 1 : kIC;
+  constexpr int n2 = DIM == 3 ? (DIR == 2 ? kIC + 1 : kIC) : 1;
+  constexpr int groups_per_block = (NF + kThreads - 1) / kThreads;
+  unsigned const g = xcd_remap(blockIdx.x, gridDim.x);
+  int const blk = g / groups_per_block;
+  int const f = (g % groups_per_block) * kThreads + threadIdx.x;
+  if (blk >= nblocks || f >= NF) return;
+  int const a2 = f % n2;
+  int const a1 = (f / n2) % n1;
+  int const a0 = f / (n2 * n1);
+  // Halo index of the cell left of the face.
+  int const i = a0 + (DIR == 0 ? kHS - 1 : kHS);
+  int const j = a1 + (DIR == 1 ? kHS - 1 : kHS);
+  int const k = DIM == 3 ? a2 + (DIR == 2 ? kHS - 1 : kHS) : 0;
+  constexpr int s = axis_stride<DIM, DIR>();
+  const double *base = in + (size_t)blk * D::BLOCK + i * D::SX + j * D::SY + k;
+  R U[6][NE];
+#pragma unroll
+  for (int m = 0; m < 6; ++m)
+#pragma unroll
+    for (int e = 0; e < NE; ++e) U[m][e] = R(base[e * D::FIELD + (m - 2) * s]);
+  R fl[NE];
+  face_flux<DIM, DIR, FLUX, R>(R(gamma), R(b), U, fl);
+  double *o = flux_out + ((size_t)blk * DIM + DIR) * NE * NF + f;
+#pragma unroll
+  for (int e = 0; e < NE; ++e) o[e * NF] = double(fl[e]);
+}
+
+// ------------------------------------------------------------- combine ---
+struct StageCoef {
+  double m0, m1, c;
+};
+__host__ __device__ inline StageCoef stage_coef(int stage) {
+  // RungeKutta3TVD buffer and time-step multipliers (runge_kutta_3_TVD.h:39-59).
+  if (stage == 0) return {1.0, 0.0, 1.0};
+  if (stage == 1) return {0.25, 0.75, 0.25};
+  return {2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0};
+}
+
+// Flux divergence (space_solver.cpp:131-153) fused with PrepareBufferFor-
+// Integration and IntegrateConservatives (time_integrator.h:98-113, 434-457).
+// MODE 0: RK stage; MODE 1: write the right-hand side itself.
+template <int DIM, int MODE>
+__global__ __launch_bounds__(kThreads) void combine_kernel(
+    const double *__restrict__ flux, const double *__restrict__ avg, const double *init,
+    double *out, int nblocks, int stage, double inv_dx, alpaca_case_state *state) {
+  using D = Dims<DIM>;
+  constexpr int NE = D::NE;
+  constexpr int NF = faces_per_dir<DIM>();
+  constexpr int ICZ = D::ICZ;
+  constexpr int cells = D::CELLS;
+  constexpr int groups_per_block = (cells + kThreads - 1) / kThreads;
+  unsigned const g = xcd_remap(blockIdx.x, gridDim.x);
+  int const blk = g / groups_per_block;
+  int const c = (g % groups_per_block) * kThreads + threadIdx.x;
+  if (blk >= nblocks || c >= cells) return;
+  int const kk = c % ICZ;
+  int const jj = (c / ICZ) % kIC;
+  int const ii = c / (ICZ * kIC);
+  const double *fx = flux + ((size_t)blk * DIM + 0) * NE * NF;
+  const double *fy = flux + ((size_t)blk * DIM + 1) * NE * NF;
+  const double *fz = DIM == 3 ? flux + ((size_t)blk * DIM + 2) * NE * NF : nullptr;
+  // Face a along a direction lies between internal cells a-1 and a.
+  int const x_lo = (ii * kIC + jj) * ICZ + kk;           // fx [17][16][16|1]
+  int const x_hi = ((ii + 1) * kIC + jj) * ICZ + kk;
+  int const y_lo = (ii * (kIC + 1) + jj) * ICZ + kk;     // fy [16][17][16|1]
+  int const y_hi = (ii * (kIC + 1) + jj + 1) * ICZ + kk;
+  int const z_lo = (ii * kIC + jj) * (kIC + 1) + kk;     // fz [16][16][17]
+  int const z_hi = z_lo + 1;
+  size_t const cell = (size_t)blk * D::BLOCK + (ii + kHS) * D::SX + (jj + kHS) * D::SY +
+                      (DIM == 3 ? kk + kHS : 0);
+  double dt = 0.0;
+  StageCoef co{1.0, 0.0, 1.0};
+  if constexpr (MODE == 0) {
+    dt = state->dt;
+    co = stage_coef(stage);
+  }
+  double const timestep = 0.0 + dt;
+  double const cdt = co.c * timestep;
+  bool bad = false;
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    double const dx_ = fx[e * NF + x_lo] - fx[e * NF + x_hi];
+    double const dy_ = fy[e * NF + y_lo] - fy[e * NF + y_hi];
+    double dz_ = 0.0;
+    if constexpr (DIM == 3) dz_ = fz[e * NF + z_lo] - fz[e * NF + z_hi];
+    double const rhs = 0.0 + (0.0 + dams<DIM>(dx_, dy_, dz_) * inv_dx);
+    size_t const idx = cell + (size_t)e * D::FIELD;
+    if constexpr (MODE == 0) {
+      double const u = avg[idx];
+      double a = u;
+      if (stage == 0) {
+        if (init) const_cast<double *>(init)[idx] = u;
+      } else {
+        a = co.m0 * u + co.m1 * init[idx];
+      }
+      double const v = a + cdt * rhs;
+      out[idx] = v;
+      bad |= !__builtin_isfinite(v);
+    } else {
+      out[idx] = rhs;
+    }
+  }
+  if constexpr (MODE == 0) {
+    if (bad) atomicOr(&state->flags, 1);
+  }
+}
+
+// ---------------------------------------------------------------- halos ---
+// One thread per (block, halo slab cell): 2*DIM slabs of HS x 16 x 16(|1).
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void halo_kernel(double *blocks, int nblocks, CaseArgs ca) {
+  using D = Dims<DIM>;
+  constexpr int NE = D::NE;
+  constexpr int ICZ = D::ICZ;
+  constexpr int slab = kHS * kIC * ICZ;
+  constexpr int per_block = 2 * DIM * slab;
+  size_t const t = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= (size_t)nblocks * per_block) return;
+  int const blk = t / per_block;
+  int const r = t % per_block;
+  int const face = r / slab;  // 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z
+  int const q = r % slab;
+  int const d = face / 2;
+  int const hi = face % 2;
+  // Position within the slab: h across the slab (0..3), then the two
+  // transverse internal indices.
+  int const h = q / (kIC * ICZ);
+  int const t1 = (q / ICZ) % kIC;
+  int const t2 = q % ICZ;
+  int bc3[3] = {blk / (ca.nb[1] * ca.nb[2]), (blk / ca.nb[2]) % ca.nb[1], blk % ca.nb[2]};
+  // Halo cell (halo indices) and its source.
+  int dst[3], src[3];
+  int tr0 = d == 0 ? 1 : 0;
+  int tr1 = d == 2 ? 1 : 2;
+  dst[d] = hi ? kHS + kIC + h : h;
+  dst[tr0] = t1 + kHS;
+  dst[tr1] = DIM == 3 ? t2 + kHS : 0;
+  int sblk = blk;
+  src[0] = dst[0];
+  src[1] = dst[1];
+  src[2] = dst[2];
+  bool const at_edge = hi ? (bc3[d] == ca.nb[d] - 1) : (bc3[d] == 0);
+  if (at_edge && ca.bc[d] == ALPACA_BC_ZERO_GRADIENT) {
+    src[d] = hi ? kHS + kIC - 1 : kHS;
+  } else {
+    int nb3[3] = {bc3[0], bc3[1], bc3[2]};
+    nb3[d] = hi ? (bc3[d] + 1 == ca.nb[d] ? 0 : bc3[d] + 1) : (bc3[d] == 0 ? ca.nb[d] - 1 : bc3[d] - 1);
+    sblk = (nb3[0] * ca.nb[1] + nb3[1]) * ca.nb[2] + nb3[2];
+    src[d] = hi ? kHS + h : kIC + h;  // neighbour's first / last internal cells
+  }
+  size_t const di = (size_t)blk * D::BLOCK + dst[0] * D::SX + dst[1] * D::SY + dst[2];
+  size_t const si = (size_t)sblk * D::BLOCK + src[0] * D::SX + src[1] * D::SY + src[2];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) blocks[di + (size_t)e * D::FIELD] = blocks[si + (size_t)e * D::FIELD];
+}
+
+// ----------------------------------------------------------- time step ---
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = smax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Per block: max over internal cells of sum_d(|u_d| + c)
+// (modular_algorithm_assembler.cpp:1840-1869), then an atomic max on the
+// IEEE bits (non-negative doubles order like their bit patterns).
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void signal_kernel(const double *__restrict__ in, int nblocks,
+                                                          double gamma, double b,
+                                                          alpaca_case_state *state) {
+  using D = Dims<DIM>;
+  constexpr int NE = D::NE;
+  constexpr int ICZ = D::ICZ;
+  __shared__ double part[kThreads / 64];
+  int const blk = blockIdx.x;
+  if (blk >= nblocks) return;
+  double s = 0.0;
+  for (int c = threadIdx.x; c < D::CELLS; c += kThreads) {
+    int const kk = c % ICZ, jj = (c / ICZ) % kIC, ii = c / (ICZ * kIC);
+    const double *p = in + (size_t)blk * D::BLOCK + (ii + kHS) * D::SX + (jj + kHS) * D::SY +
+                      (DIM == 3 ? kk + kHS : 0);
+    double u[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) u[e] = p[e * D::FIELD];
+    Prim<double> pr;
+    cons_to_prim<DIM>(gamma, b, u, pr);
+    double const cs = eos_sos(gamma, b, pr.rho, pr.p);
+    double sum;
+    if constexpr (DIM == 2)
+      sum = (__builtin_fabs(pr.vel[0]) + cs) + (__builtin_fabs(pr.vel[1]) + cs);
+    else
+      sum = cms3(__builtin_fabs(pr.vel[0]) + cs, __builtin_fabs(pr.vel[1]) + cs,
+                 __builtin_fabs(pr.vel[2]) + cs);
+    s = smax(s, sum);
+  }
+  s = wave_max(s);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = part[0];
+#pragma unroll
+    for (int w = 1; w < kThreads / 64; ++w) m = smax(m, part[w]);
+    atomicMax(reinterpret_cast<unsigned long long *>(&state->smax),
+              (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// dt = CFL / (S / dx), end-time clamp, minimum-step health check
+// (modular_algorithm_assembler.cpp:1932-1982; ComputeLoop :137-140).
+__global__ void dt_kernel(alpaca_case_state *state, double cell_size) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double const s = __longlong_as_double((long long)state->smax);
+  double const rate = s / cell_size;
+  double dt = rate == 0.0 ? 1.7976931348623157e308 : state->cfl / rate;
+  if (state->time + dt > state->t_end) dt = smax(state->t_end - state->time, 0.0);
+  state->dt = dt;
+  state->smax = 0;
+  if (dt < 2.220446049250313080847263336181640625e-16 && dt > 0.0) state->flags |= 2;
+}
+
+// FinishMacroTimestep (time_integrator.h:244-248).
+__global__ void finish_step_kernel(alpaca_case_state *state) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  state->time += 0.0 + state->dt;
+  state->steps += 1;
+}
+
+__global__ void state_init_kernel(alpaca_case_state *state, double time, double t_end, double cfl) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  state->time = time;
+  state->t_end = t_end;
+  state->cfl = cfl;
+  state->dt = 0.0;
+  state->smax = 0;
+  state->steps = 0;
+  state->flags = 0;
+}
+
+// Reference face-array layout for the per-block convective-flux entry:
+// three arrays [NE][17][17][17|2]; face (a0,a1,a2) of direction DIR maps to
+// (i - 3, j - 3, k - 3) with k - (-1) in 2D (finite_volume_scheme.cpp:176-180).
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void scatter_ref_fluxes(const double *__restrict__ flux,
+                                                               double *out, int nblocks) {
+  using D = Dims<DIM>;
+  constexpr int NE = D::NE;
+  constexpr int NF = faces_per_dir<DIM>();
+  constexpr int nz = DIM == 3 ? kIC + 1 : 2;
+  constexpr int ref_per_dir = NE * (kIC + 1) * (kIC + 1) * nz;
+  size_t const t = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  size_t const total = (size_t)nblocks * DIM * NF;
+  if (t >= total) return;
+  int const f = t % NF;
+  int const dir = (t / NF) % DIM;
+  int const blk = t / ((size_t)NF * DIM);
+  int const n0 = dir == 0 ? kIC + 1 : kIC;
+  int const n1 = dir == 1 ? kIC + 1 : kIC;
+  int const n2 = DIM == 3 ? (dir == 2 ? kIC + 1 : kIC) : 1;
+  int const a2 = f % n2, a1 = (f / n2) % n1, a0 = f / (n2 * n1);
+  (void)n0;
+  int const ri = a0 + (dir == 0 ? 0 : 1);
+  int const rj = a1 + (dir == 1 ? 0 : 1);
+  int const rk = DIM == 3 ? a2 + (dir == 2 ? 0 : 1) : 1;
+  const double *src = flux + ((size_t)blk * DIM + dir) * NE * NF + f;
+  double *dst = out + ((size_t)blk * 3 + dir) * ref_per_dir;
+#pragma unroll
+  for (int e = 0; e < NE; ++e)
+    dst[((e * (kIC + 1) + ri) * (kIC + 1) + rj) * nz + rk] += src[e * NF];
+}
+
+// --------------------------------------------------------------- host ---
+struct Workspace {
+  double *ptr = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mutex;
+std::vector<Workspace> g_ws;  // per device
+
+int hip_status(hipError_t e) { return e == hipSuccess ? ALPACA_OK : ALPACA_ERR_HIP; }
+
+int workspace(size_t bytes, double **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return ALPACA_ERR_HIP;
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
+  Workspace &w = g_ws[dev];
+  if (w.bytes < bytes) {
+    if (w.ptr) {
+      if (hipDeviceSynchronize() != hipSuccess) return ALPACA_ERR_HIP;
+      if (hipFree(w.ptr) != hipSuccess) return ALPACA_ERR_HIP;
+      w.ptr = nullptr;
+      w.bytes = 0;
+    }
+    if (hipMalloc(&w.ptr, bytes) != hipSuccess) return ALPACA_ERR_HIP;
+    w.bytes = bytes;
+  }
+  *out = w.ptr;
+  return ALPACA_OK;
+}
+
+int validate(const alpaca_case_desc *d) {
+  if (!d) return ALPACA_ERR_INVALID;
+  if (d->dim != 2 && d->dim != 3) return ALPACA_ERR_INVALID;
+  for (int a = 0; a < d->dim; ++a)
+    if (d->nb[a] < 1 || (d->bc[a] != ALPACA_BC_ZERO_GRADIENT && d->bc[a] != ALPACA_BC_PERIODIC))
+      return ALPACA_ERR_INVALID;
+  if (d->dim == 2 && d->nb[2] != 1) return ALPACA_ERR_INVALID;
+  if (d->flux != ALPACA_FLUX_HLLC && d->flux != ALPACA_FLUX_ROE) return ALPACA_ERR_INVALID;
+  if (d->precision != ALPACA_FP64 && d->precision != ALPACA_MIXED) return ALPACA_ERR_INVALID;
+  if (!(d->cell_size > 0.0) || !(d->gamma > 1.0)) return ALPACA_ERR_INVALID;
+  return ALPACA_OK;
+}
+
+int case_blocks(const alpaca_case_desc *d) { return d->nb[0] * d->nb[1] * (d->dim == 3 ? d->nb[2] : 1); }
+
+template <int DIM>
+size_t flux_doubles(int nblocks) {
+  return (size_t)nblocks * DIM * Dims<DIM>::NE * faces_per_dir<DIM>();
+}
+
+template <int DIM, int DIR, int FLUX>
+void launch_faces_dir(const alpaca_case_desc *d, int nblocks, const double *in, double *fl,
+                      hipStream_t s) {
+  constexpr int NF = faces_per_dir<DIM>();
+  constexpr int gpb = (NF + kThreads - 1) / kThreads;
+  dim3 grid((unsigned)(nblocks * gpb));
+  if (d->precision == ALPACA_MIXED)
+    hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, float>), grid, dim3(kThreads), 0, s, in,
+                       fl, nblocks, d->gamma, d->background_pressure);
+  else
+    hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, double>), grid, dim3(kThreads), 0, s, in,
+                       fl, nblocks, d->gamma, d->background_pressure);
+}
+
+template <int DIM, int FLUX>
+void launch_faces_flux(const alpaca_case_desc *d, int nblocks, const double *in, double *fl,
+                       hipStream_t s) {
+  launch_faces_dir<DIM, 0, FLUX>(d, nblocks, in, fl, s);
+  launch_faces_dir<DIM, 1, FLUX>(d, nblocks, in, fl, s);
+  if constexpr (DIM == 3) launch_faces_dir<DIM, 2, FLUX>(d, nblocks, in, fl, s);
+}
+
+template <int DIM>
+void launch_faces(const alpaca_case_desc *d, int nblocks, const double *in, double *fl,
+                  hipStream_t s) {
+  if (d->flux == ALPACA_FLUX_HLLC)
+    launch_faces_flux<DIM, ALPACA_FLUX_HLLC>(d, nblocks, in, fl, s);
+  else
+    launch_faces_flux<DIM, ALPACA_FLUX_ROE>(d, nblocks, in, fl, s);
+}
+
+template <int DIM>
+int stage_impl(const alpaca_case_desc *d, int nblocks, int stage, int mode, const double *avg,
+               double *init, double *out, alpaca_case_state *state, hipStream_t s) {
+  double *fl = nullptr;
+  int st = workspace(flux_doubles<DIM>(nblocks) * sizeof(double), &fl);
+  if (st != ALPACA_OK) return st;
+  launch_faces<DIM>(d, nblocks, avg, fl, s);
+  constexpr int cells = Dims<DIM>::CELLS;
+  constexpr int gpb = (cells + kThreads - 1) / kThreads;
+  dim3 grid((unsigned)(nblocks * gpb));
+  double const inv_dx = 1.0 / d->cell_size;
+  if (mode == 0) {
+    double *ini = (init && init != avg) ? init : nullptr;
+    if (stage > 0) ini = init;
+    hipLaunchKernelGGL((combine_kernel<DIM, 0>), grid, dim3(kThreads), 0, s, fl, avg, ini, out,
+                       nblocks, stage, inv_dx, state);
+  } else {
+    hipLaunchKernelGGL((combine_kernel<DIM, 1>), grid, dim3(kThreads), 0, s, fl, avg, nullptr,
+                       out, nblocks, 0, inv_dx, state);
+  }
+  return hip_status(hipGetLastError());
+}
+
+template <int DIM>
+int halo_impl(const alpaca_case_desc *d, double *blocks, hipStream_t s) {
+  int const nblocks = case_blocks(d);
+  CaseArgs ca{{d->nb[0], d->nb[1], DIM == 3 ? d->nb[2] : 1},
+              {d->bc[0], d->bc[1], DIM == 3 ? d->bc[2] : 0},
+              d->gamma, d->background_pressure, d->cell_size};
+  size_t const total = (size_t)nblocks * 2 * DIM * kHS * kIC * Dims<DIM>::ICZ;
+  dim3 grid((unsigned)((total + kThreads - 1) / kThreads));
+  hipLaunchKernelGGL((halo_kernel<DIM>), grid, dim3(kThreads), 0, s, blocks, nblocks, ca);
+  return hip_status(hipGetLastError());
+}
+
+template <int DIM>
+int timestep_impl(const alpaca_case_desc *d, const double *avg, alpaca_case_state *state,
+                  hipStream_t s) {
+  int const nblocks = case_blocks(d);
+  hipLaunchKernelGGL((signal_kernel<DIM>), dim3(nblocks), dim3(kThreads), 0, s, avg, nblocks,
+                     d->gamma, d->background_pressure, state);
+  hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(64), 0, s, state, d->cell_size);
+  return hip_status(hipGetLastError());
+}
+
+}  // namespace
+}  // namespace alpaca
+
+using namespace alpaca;
+
+extern "C" {
+
+const char *alpaca_status_string(int status) {
+  switch (status) {
+    case ALPACA_OK: return "ok";
+    case ALPACA_ERR_INVALID: return "invalid argument";
+    case ALPACA_ERR_HIP: return "HIP runtime error";
+    case ALPACA_ERR_NONFINITE: return "non-finite state";
+    case ALPACA_ERR_UNSUPPORTED: return "unsupported configuration";
+    case ALPACA_ERR_CAPACITY: return "capacity exhausted";
+    default: return "unknown status";
+  }
+}
+
+int alpaca_abi_version(void) { return ALPACA_GPU_ABI_VERSION; }
+
+size_t alpaca_block_doubles(int dim) {
+  if (dim == 2) return Dims<2>::BLOCK;
+  if (dim == 3) return Dims<3>::BLOCK;
+  return 0;
+}
+
+int alpaca_convective_fluxes(const alpaca_case_desc *desc, int nblocks, const double *d_avg,
+                             double *d_fluxes, void *stream) {
+  int st = validate(desc);
+  if (st != ALPACA_OK) return st;
+  if (nblocks < 1 || !d_avg || !d_fluxes) return ALPACA_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  double *fl = nullptr;
+  if (desc->dim == 2) {
+    if ((st = workspace(flux_doubles<2>(nblocks) * sizeof(double), &fl)) != ALPACA_OK) return st;
+    launch_faces<2>(desc, nblocks, d_avg, fl, s);
+    size_t total = (size_t)nblocks * 2 * faces_per_dir<2>();
+    hipLaunchKernelGGL(scatter_ref_fluxes<2>, dim3((unsigned)((total + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, s, fl, d_fluxes, nblocks);
+  } else {
+    if ((st = workspace(flux_doubles<3>(nblocks) * sizeof(double), &fl)) != ALPACA_OK) return st;
+    launch_faces<3>(desc, nblocks, d_avg, fl, s);
+    size_t total = (size_t)nblocks * 3 * faces_per_dir<3>();
+    hipLaunchKernelGGL(scatter_ref_fluxes<3>, dim3((unsigned)((total + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, s, fl, d_fluxes, nblocks);
+  }
+  return hip_status(hipGetLastError());
+}
+
+int alpaca_right_hand_side(const alpaca_case_desc *desc, int nblocks, const double *d_avg,
+                           double *d_rhs, void *stream) {
+  int st = validate(desc);
+  if (st != ALPACA_OK) return st;
+  if (nblocks < 1 || !d_avg || !d_rhs || d_avg == d_rhs) return ALPACA_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  size_t const bytes = alpaca_block_doubles(desc->dim) * sizeof(double) * (size_t)nblocks;
+  if (hipMemsetAsync(d_rhs, 0, bytes, s) != hipSuccess) return ALPACA_ERR_HIP;
+  if (desc->dim == 2) return stage_impl<2>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s);
+  return stage_impl<3>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s);
+}
+
+int alpaca_rk3_stage(const alpaca_case_desc *desc, int stage, const double *d_avg, double *d_init,
+                     double *d_out, alpaca_case_state *d_state, void *stream) {
+  int st = validate(desc);
+  if (st != ALPACA_OK) return st;
+  if (stage < 0 || stage > 2 || !d_avg || !d_out || !d_state || d_out == d_avg)
+    return ALPACA_ERR_INVALID;
+  if (stage > 0 && !d_init) return ALPACA_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int const nb = case_blocks(desc);
+  if (desc->dim == 2) return stage_impl<2>(desc, nb, stage, 0, d_avg, d_init, d_out, d_state, s);
+  return stage_impl<3>(desc, nb, stage, 0, d_avg, d_init, d_out, d_state, s);
+}
+
+int alpaca_fill_halos(const alpaca_case_desc *desc, double *d_blocks, void *stream) {
+  int st = validate(desc);
+  if (st != ALPACA_OK) return st;
+  if (!d_blocks) return ALPACA_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (desc->dim == 2) return halo_impl<2>(desc, d_blocks, s);
+  return halo_impl<3>(desc, d_blocks, s);
+}
+
+int alpaca_timestep(const alpaca_case_desc *desc, const double *d_avg, alpaca_case_state *d_state,
+                    void *stream) {
+  int st = validate(desc);
+  if (st != ALPACA_OK) return st;
+  if (!d_avg || !d_state) return ALPACA_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (desc->dim == 2) return timestep_impl<2>(desc, d_avg, d_state, s);
+  return timestep_impl<3>(desc, d_avg, d_state, s);
+}
+
+int alpaca_case_state_init(alpaca_case_state *d_state, double time, double t_end, double cfl,
+                           void *stream) {
+  if (!d_state) return ALPACA_ERR_INVALID;
+  hipLaunchKernelGGL(state_init_kernel, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
+                     d_state, time, t_end, cfl);
+  return hip_status(hipGetLastError());
+}
+
+int alpaca_advance(const alpaca_case_desc *desc, double *d_u, double *d_s1, double *d_s2,
+                   alpaca_case_state *d_state, void *stream) {
+  int st = validate(desc);
+  if (st != ALPACA_OK) return st;
+  if (!d_u || !d_s1 || !d_s2 || !d_state || d_u == d_s1 || d_u == d_s2 || d_s1 == d_s2)
+    return ALPACA_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((st = alpaca_timestep(desc, d_u, d_state, stream)) != ALPACA_OK) return st;
+  // Stage 0: u1 = u + dt L(u) -> s1 (u stays as the initial buffer).
+  if ((st = alpaca_rk3_stage(desc, 0, d_u, nullptr, d_s1, d_state, stream)) != ALPACA_OK) return st;
+  if ((st = alpaca_fill_halos(desc, d_s1, stream)) != ALPACA_OK) return st;
+  // Stage 1: u2 = 3/4 u + 1/4 u1 + 1/4 dt L(u1) -> s2.
+  if ((st = alpaca_rk3_stage(desc, 1, d_s1, d_u, d_s2, d_state, stream)) != ALPACA_OK) return st;
+  if ((st = alpaca_fill_halos(desc, d_s2, stream)) != ALPACA_OK) return st;
+  // Stage 2: u^(n+1) = 1/3 u + 2/3 u2 + 2/3 dt L(u2) -> u (in place).
+  if ((st = alpaca_rk3_stage(desc, 2, d_s2, d_u, d_u, d_state, stream)) != ALPACA_OK) return st;
+  if ((st = alpaca_fill_halos(desc, d_u, stream)) != ALPACA_OK) return st;
+  hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, s, d_state);
+  return hip_status(hipGetLastError());
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+"""Python mirror of the C ABI in include/alpaca_gpu.h (ctypes).
+
+PyTorch only provides device memory and the stream; every computation is a
+kernel of ``libalpaca_hip.so``.  Loading fails loudly when the library is not
+built: there is no CPU fallback on this path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import cases
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libalpaca_hip.so")
+
+FLUX_HLLC, FLUX_ROE = 0, 1
+FP64, MIXED = 0, 1
+BC_ZERO_GRADIENT, BC_PERIODIC = 0, 1
+IC, HS, TC = 16, 4, 24
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "non-finite state",
+          4: "unsupported configuration", 5: "capacity exhausted"}
+
+
+class CaseDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("nb", ctypes.c_int * 3), ("bc", ctypes.c_int * 3),
+                ("flux", ctypes.c_int), ("precision", ctypes.c_int), ("gamma", ctypes.c_double),
+                ("background_pressure", ctypes.c_double), ("cell_size", ctypes.c_double)]
+
+
+class CaseState(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("t_end", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("dt", ctypes.c_double), ("smax", ctypes.c_uint64), ("steps", ctypes.c_int32),
+                ("flags", ctypes.c_int32)]
+
+
+class AlpacaError(RuntimeError):
+    pass
+
+
+_lib = None
+
+_SIGNATURES = {
+    "alpaca_status_string": ([ctypes.c_int], ctypes.c_char_p),
+    "alpaca_abi_version": ([], ctypes.c_int),
+    "alpaca_block_doubles": ([ctypes.c_int], ctypes.c_size_t),
+    "alpaca_convective_fluxes": ([ctypes.POINTER(CaseDesc), ctypes.c_int, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_right_hand_side": ([ctypes.POINTER(CaseDesc), ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_rk3_stage": ([ctypes.POINTER(CaseDesc), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_fill_halos": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_void_p],
+                          ctypes.c_int),
+    "alpaca_timestep": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_void_p,
+                         ctypes.c_void_p], ctypes.c_int),
+    "alpaca_case_state_init": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_advance": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_void_p,
+                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+}
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libalpaca_hip.so and declare the C ABI; raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise AlpacaError(f"{path} is not built; run __graft_entry__.build() first")
+    lib = ctypes.CDLL(path)
+    for name, (argtypes, restype) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = restype
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def check(status: int, what: str = ""):
+    if status != 0:
+        raise AlpacaError(f"{what}: {STATUS.get(status, status)}")
+
+
+def make_desc(dim, nb, bc, flux=FLUX_HLLC, precision=FP64, gamma=1.4, b=0.0, cell_size=None,
+              length=1.0):
+    nb3 = list(nb) + [1] * (3 - len(nb))
+    bc3 = list(bc) + [0] * (3 - len(bc))
+    if dim == 2:
+        nb3[2] = 1
+        bc3[2] = 0
+    if cell_size is None:
+        cell_size = length / (nb3[0] * IC)
+    return CaseDesc(dim, (ctypes.c_int * 3)(*nb3), (ctypes.c_int * 3)(*bc3), flux, precision,
+                    gamma, b, cell_size)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_ptr(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def block_doubles(dim):
+    return NE(dim) * TC ** dim
+
+
+def NE(dim):
+    return dim + 2
+
+
+def to_blocks(torch_global, dim, nb):
+    """Domain array [NE][NX][NY][NZ] (device) -> haloed blocks (device), halos zero."""
+    torch = _torch()
+    ne = NE(dim)
+    if dim == 2:
+        g = torch_global.reshape(ne, nb[0], IC, nb[1], IC)
+        g = g.permute(1, 3, 0, 2, 4)
+        blocks = torch.zeros((nb[0], nb[1], ne, TC, TC), dtype=torch.float64,
+                             device=torch_global.device)
+        blocks[..., HS:HS + IC, HS:HS + IC] = g
+        return blocks.reshape(nb[0] * nb[1], ne, TC, TC)
+    g = torch_global.reshape(ne, nb[0], IC, nb[1], IC, nb[2], IC)
+    g = g.permute(1, 3, 5, 0, 2, 4, 6)
+    blocks = torch.zeros((nb[0], nb[1], nb[2], ne, TC, TC, TC), dtype=torch.float64,
+                         device=torch_global.device)
+    blocks[..., HS:HS + IC, HS:HS + IC, HS:HS + IC] = g
+    return blocks.reshape(nb[0] * nb[1] * nb[2], ne, TC, TC, TC)
+
+
+def from_blocks(blocks, dim, nb):
+    """Haloed blocks (device) -> domain array [NE][NX][NY][NZ] (device)."""
+    ne = NE(dim)
+    if dim == 2:
+        b = blocks.reshape(nb[0], nb[1], ne, TC, TC)[..., HS:HS + IC, HS:HS + IC]
+        return b.permute(2, 0, 3, 1, 4).reshape(ne, nb[0] * IC, nb[1] * IC, 1)
+    b = blocks.reshape(nb[0], nb[1], nb[2], ne, TC, TC, TC)[..., HS:HS + IC, HS:HS + IC,
+                                                            HS:HS + IC]
+    return b.permute(3, 0, 4, 1, 5, 2, 6).reshape(ne, nb[0] * IC, nb[1] * IC, nb[2] * IC)
+
+
+class GpuCase:
+    """One uniform case resident in HBM: state u (+ two stage buffers) and
+    its device record.  Mirrors the reference's ModularAlgorithmAssembler
+    ComputeLoop/Advance for a single level (modular_algorithm_assembler.cpp:
+    97-195, 543-910)."""
+
+    def __init__(self, desc: CaseDesc, state0: np.ndarray, cfl: float, t_end: float = 1e300,
+                 device="cuda"):
+        torch = _torch()
+        self.lib = load_library()
+        self.desc = desc
+        self.dim = desc.dim
+        self.nb = [desc.nb[0], desc.nb[1], desc.nb[2] if desc.dim == 3 else 1]
+        self.nblocks = self.nb[0] * self.nb[1] * self.nb[2]
+        g = torch.as_tensor(np.ascontiguousarray(state0), device=device)
+        self.u = to_blocks(g, self.dim, self.nb).contiguous()
+        self.s1 = torch.zeros_like(self.u)
+        self.s2 = torch.zeros_like(self.u)
+        self.state = torch.zeros(ctypes.sizeof(CaseState), dtype=torch.uint8, device=device)
+        self.cfl = cfl
+        self.t_end = t_end
+        st = self._stream()
+        check(self.lib.alpaca_fill_halos(ctypes.byref(desc), ctypes.c_void_p(self.u.data_ptr()), st),
+              "fill_halos")
+        check(self.lib.alpaca_case_state_init(ctypes.c_void_p(self.state.data_ptr()), 0.0, t_end,
+                                              cfl, st), "case_state_init")
+
+    def _stream(self):
+        return _stream_ptr()
+
+    def advance(self, nsteps: int = 1):
+        st = self._stream()
+        d = ctypes.byref(self.desc)
+        u, s1, s2 = (ctypes.c_void_p(t.data_ptr()) for t in (self.u, self.s1, self.s2))
+        stp = ctypes.c_void_p(self.state.data_ptr())
+        for _ in range(nsteps):
+            check(self.lib.alpaca_advance(d, u, s1, s2, stp, st), "advance")
+
+    def record(self) -> CaseState:
+        raw = self.state.cpu().numpy().tobytes()
+        return CaseState.from_buffer_copy(raw)
+
+    def run(self, max_steps: int, sync_every: int = 1):
+        """ComputeLoop: advance while time < t_end (read back each step)."""
+        steps = 0
+        dts = []
+        while steps < max_steps:
+            rec = self.record()
+            if not rec.time < self.t_end:
+                break
+            self.advance(1)
+            steps += 1
+            rec = self.record()
+            dts.append(rec.dt)
+            if rec.flags & 1:
+                raise AlpacaError("non-finite state")
+            if rec.flags & 2:
+                break
+        return steps, dts
+
+    def download(self) -> np.ndarray:
+        return from_blocks(self.u, self.dim, self.nb).cpu().numpy()

@@ -1,0 +1,149 @@
+/*
+ * alpaca_gpu.h — C ABI of the MI355X (gfx950) implementation of ALPACA's
+ * single-phase finite-volume right-hand side on uniform level-0 leaf blocks.
+ *
+ * Plain pointers and sizes only.  Every device pointer is caller-owned HBM;
+ * every call is asynchronous on the given HIP stream (NULL = default stream)
+ * unless stated otherwise, and returns an alpaca_status code (no C++
+ * exceptions cross this boundary; the reference has no error returns on this
+ * path, invalid faces are skipped, FP faults trap — here non-finite states
+ * raise ALPACA_ERR_NONFINITE through the case record).
+ *
+ * Block storage follows the reference Block/FieldBuffer contract
+ * (block.h:54-161, field_buffer.h:39): one haloed block is
+ * [NE][24][24][24] doubles in 3D, [NE][24][24] in 2D (k fastest), NE = dim+2
+ * equations in the order Mass, Energy, MomX, MomY, (MomZ)
+ * (field_setup.h:101-131).  A case is nb[0]*nb[1]*nb[2] consecutive blocks,
+ * block (bx,by,bz) at index (bx*nb[1] + by)*nb[2] + bz.
+ */
+#ifndef ALPACA_GPU_H
+#define ALPACA_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALPACA_GPU_ABI_VERSION 1
+
+typedef enum {
+  ALPACA_OK = 0,
+  ALPACA_ERR_INVALID = 1,     /* bad argument (null pointer, size, stage...) */
+  ALPACA_ERR_HIP = 2,         /* a HIP runtime call failed */
+  ALPACA_ERR_NONFINITE = 3,   /* a case produced a non-finite state */
+  ALPACA_ERR_UNSUPPORTED = 4, /* configuration not built */
+  ALPACA_ERR_CAPACITY = 5     /* serving slots / memory budget exhausted */
+} alpaca_status;
+
+/* Convective flux family: ConvectiveTermSolvers::FiniteVolume with
+ * RiemannSolvers::Hllc + SignalSpeed::Einfeldt, or
+ * ConvectiveTermSolvers::FluxSplitting with FluxSplitting::Roe
+ * (riemann_solver_settings.h:29-54). */
+enum { ALPACA_FLUX_HLLC = 0, ALPACA_FLUX_ROE = 1 };
+
+/* Arithmetic of the reconstruction and Riemann part.  FP64 everywhere, or
+ * mixed: FP32 reconstruction/fluxes, FP64 RK accumulation, pressure recovery
+ * and sums. */
+enum { ALPACA_FP64 = 0, ALPACA_MIXED = 1 };
+
+/* Side condition per direction: zero-gradient (ZeroGradientBoundaryCondition,
+ * zero_gradient_boundary_condition.h:45-84) or periodic (the neighbour wraps,
+ * id_periodic_information.cpp:325). */
+enum { ALPACA_BC_ZERO_GRADIENT = 0, ALPACA_BC_PERIODIC = 1 };
+
+typedef struct alpaca_case_desc {
+  int dim;            /* 2 or 3 */
+  int nb[3];          /* leaf blocks per direction; nb[2] = 1 in 2D */
+  int bc[3];          /* ALPACA_BC_* per direction (both sides alike) */
+  int flux;           /* ALPACA_FLUX_* */
+  int precision;      /* ALPACA_FP64 / ALPACA_MIXED */
+  double gamma;       /* stiffened-gas gamma (StiffenedGas, stiffened_gas.cpp:41) */
+  double background_pressure; /* stiffened-gas B; 0 = ideal gas */
+  double cell_size;   /* dx of the level-0 cells (Node::GetCellSize) */
+} alpaca_case_desc;
+
+/* Device record of a running case (one per case, caller-allocated device
+ * memory of sizeof(alpaca_case_state)).  Written by alpaca_timestep and the
+ * stage kernels; read back by the host only at case completion. */
+typedef struct alpaca_case_state {
+  double time;     /* current simulation time (TimeIntegrator::CurrentRunTime) */
+  double t_end;    /* end time (limit_end_time clamp) */
+  double cfl;      /* CFL number */
+  double dt;       /* time step of the running macro step */
+  uint64_t smax;   /* max over internal cells of sum_d(|u_d|+c), as IEEE bits */
+  int32_t steps;   /* completed macro steps */
+  int32_t flags;   /* bit 0: non-finite state seen; bit 1: dt unhealthy */
+} alpaca_case_state;
+
+const char *alpaca_status_string(int status);
+int alpaca_abi_version(void);
+
+/* Doubles per haloed block (NE * 24^dim). */
+size_t alpaca_block_doubles(int dim);
+
+/* (i) Per-block convective fluxes.
+ * Replaces ConvectiveTermSolver<D>::UpdateConvectiveFluxes
+ * (convective_term_solver.h:67-81) for the selected family.
+ * d_avg: nblocks haloed Average buffers (halos filled).  Prime states are
+ * recovered from d_avg on the fly (the reference reads them from the block's
+ * PrimeStates buffer, which ObtainPrimeStatesFromConservatives fills from the
+ * same Average values).  d_fluxes: per block three arrays x, y, z of
+ * [NE][17][17][17] (3D) or [NE][17][17][2] (2D) doubles, indexed as the
+ * reference's face arrays; fluxes are ACCUMULATED (+=) like the reference. */
+int alpaca_convective_fluxes(const alpaca_case_desc *desc, int nblocks,
+                             const double *d_avg, double *d_fluxes,
+                             void *stream);
+
+/* (ii) Per-block right-hand side: SpaceSolver::UpdateFluxes
+ * (space_solver.cpp:50-153) with source terms off.  Writes the internal cells
+ * of the nblocks haloed blocks d_rhs (halo cells are set to zero). */
+int alpaca_right_hand_side(const alpaca_case_desc *desc, int nblocks,
+                           const double *d_avg, double *d_rhs, void *stream);
+
+/* (iii) One SSP-RK3 stage on every block of a case: FillInitialBuffer,
+ * UpdateFluxes, PrepareBufferForIntegration and IntegrateNode
+ * (time_integrator.h:98-113, 312-457; runge_kutta_3_TVD.h:39-59):
+ *   out = (m0*avg + m1*init) + (c_s*dt)*RHS(avg)  on internal cells,
+ * with (m0, m1, c_s) = (1, 0, 1), (1/4, 3/4, 1/4), (2/3, 1/3, 2/3).
+ * Stage 0 copies d_avg into d_init when d_init is non-null and differs from
+ * d_avg; later stages read d_init.  d_out must not alias d_avg; it may alias
+ * d_init (each cell reads its own init value before writing).  The time step
+ * is read from the case's device record (d_state->dt). */
+int alpaca_rk3_stage(const alpaca_case_desc *desc, int stage,
+                     const double *d_avg, double *d_init, double *d_out,
+                     alpaca_case_state *d_state, void *stream);
+
+/* (iv) Same-level halo fill of a case: each block's face halo slabs take the
+ * neighbour's internal cells (periodic sides wrap), zero-gradient sides copy
+ * the first/last internal layer (internal_halo_manager.cpp:439-466,
+ * boundary_constants.h:212-253).  Edge and corner halo cells, which no
+ * direction-split stencil reads, are not written. */
+int alpaca_fill_halos(const alpaca_case_desc *desc, double *d_blocks,
+                      void *stream);
+
+/* (v) CFL time step of a case: ComputeTimestepSize
+ * (modular_algorithm_assembler.cpp:1777-1997) for inviscid flow.
+ * S = max over internal cells of sum_d(|u_d| + c); dt = cfl / (S / dx),
+ * clamped to t_end - time.  Writes d_state->dt (and smax). */
+int alpaca_timestep(const alpaca_case_desc *desc, const double *d_avg,
+                    alpaca_case_state *d_state, void *stream);
+
+/* Initialise a device case record (time, t_end, cfl; counters zeroed). */
+int alpaca_case_state_init(alpaca_case_state *d_state, double time,
+                           double t_end, double cfl, void *stream);
+
+/* One full macro step of a case, as ModularAlgorithmAssembler::Advance does
+ * on a uniform level (modular_algorithm_assembler.cpp:543-910): time step,
+ * then three stages each followed by the halo fill.  d_u holds u^n (halos
+ * filled) and receives u^(n+1) in place; d_s1 and d_s2 are scratch blocks of
+ * the same size: stage 0 writes u1 to d_s1, stage 1 writes u2 to d_s2 (with
+ * d_u as the initial buffer), stage 2 writes u^(n+1) over d_u. */
+int alpaca_advance(const alpaca_case_desc *desc, double *d_u, double *d_s1,
+                   double *d_s2, alpaca_case_state *d_state, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ALPACA_GPU_H */

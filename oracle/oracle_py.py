@@ -1,0 +1,163 @@
+"""ORACLE — test infrastructure only.  ctypes access to the CPU checkers:
+
+* ``liboracle.so``: the C restatement (oracle/alpaca_oracle.c);
+* ``_ref/libalpaca_ref{2,3}d.so``: the harness linking the reference's own
+  translation units (built from the reference tree where it is present).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = {2: os.path.join(HERE, "_ref", "libalpaca_ref2d.so"),
+           3: os.path.join(HERE, "_ref", "libalpaca_ref3d.so")}
+
+_P = ctypes.POINTER(ctypes.c_double)
+
+
+class OraGrid(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("nb", ctypes.c_int * 3), ("periodic", ctypes.c_int * 3),
+                ("flux", ctypes.c_int), ("gamma", ctypes.c_double), ("b", ctypes.c_double),
+                ("cell_size", ctypes.c_double)]
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_P)
+
+
+_oracle = None
+_refs = {}
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        _oracle = ctypes.CDLL(ORACLE_LIB)
+    return _oracle
+
+
+def ref_available(dim: int) -> bool:
+    return os.path.exists(REF_LIB[dim])
+
+
+def ref(dim: int):
+    if dim not in _refs:
+        if not ref_available(dim):
+            raise RuntimeError(f"{REF_LIB[dim]} is not built (make -C oracle ref)")
+        _refs[dim] = ctypes.CDLL(REF_LIB[dim])
+    return _refs[dim]
+
+
+def nflux(dim):
+    icz = 16 if dim == 3 else 1
+    return 3 * (dim + 2) * 17 * 17 * (icz + 1)
+
+
+def nrhs(dim):
+    icz = 16 if dim == 3 else 1
+    return (dim + 2) * 16 * 16 * icz
+
+
+def weno5(v, which="oracle"):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.zeros(2)
+    if which == "oracle":
+        oracle().ora_weno5(_dp(v), _dp(out))
+    else:
+        ref(2).ref_weno5(_dp(v), _dp(out))
+    return out
+
+
+def eos(dim, gamma, b, cons5, prim5, which="oracle"):
+    c = np.ascontiguousarray(cons5, dtype=np.float64)
+    p = np.ascontiguousarray(prim5, dtype=np.float64)
+    out = np.zeros(5)
+    if which == "oracle":
+        oracle().ora_eos(dim, ctypes.c_double(gamma), ctypes.c_double(b), _dp(c), _dp(p), _dp(out))
+    else:
+        ref(dim).ref_eos(ctypes.c_double(gamma), ctypes.c_double(b), _dp(c), _dp(p), _dp(out))
+    return out
+
+
+def hllc(dim, direction, gamma, b, cl, cr, which="oracle"):
+    cl = np.ascontiguousarray(cl, dtype=np.float64)
+    cr = np.ascontiguousarray(cr, dtype=np.float64)
+    out = np.zeros(dim + 2)
+    if which == "oracle":
+        oracle().ora_hllc(dim, direction, ctypes.c_double(gamma), ctypes.c_double(b), _dp(cl),
+                          _dp(cr), _dp(out))
+    else:
+        ref(dim).ref_hllc(direction, ctypes.c_double(gamma), ctypes.c_double(b), _dp(cl), _dp(cr),
+                          _dp(out))
+    return out
+
+
+def roe_eigensystem(dim, direction, gamma, b, ct, cn):
+    ne = dim + 2
+    ct = np.ascontiguousarray(ct, dtype=np.float64)
+    cn = np.ascontiguousarray(cn, dtype=np.float64)
+    L = np.zeros((ne, ne))
+    R = np.zeros((ne, ne))
+    lam = np.zeros(ne)
+    ok = oracle().ora_roe_eigensystem(dim, direction, ctypes.c_double(gamma), ctypes.c_double(b),
+                                      _dp(ct), _dp(cn), _dp(L), _dp(R), _dp(lam))
+    return ok, L, R, lam
+
+
+def block_fluxes(dim, flux, gamma, b, blk, cell_size, which="oracle"):
+    blk = np.ascontiguousarray(blk, dtype=np.float64)
+    out = np.zeros(nflux(dim))
+    if which == "oracle":
+        oracle().ora_block_fluxes(dim, flux, ctypes.c_double(gamma), ctypes.c_double(b), _dp(blk),
+                                  ctypes.c_double(cell_size), _dp(out))
+    else:
+        ref(dim).ref_block_fluxes(flux, ctypes.c_double(gamma), ctypes.c_double(b), _dp(blk),
+                                  ctypes.c_double(cell_size), _dp(out))
+    icz = 16 if dim == 3 else 1
+    return out.reshape(3, dim + 2, 17, 17, icz + 1)
+
+
+def block_rhs(dim, flux, gamma, b, blk, cell_size, which="oracle"):
+    blk = np.ascontiguousarray(blk, dtype=np.float64)
+    out = np.zeros(nrhs(dim))
+    if which == "oracle":
+        oracle().ora_block_rhs(dim, flux, ctypes.c_double(gamma), ctypes.c_double(b), _dp(blk),
+                               ctypes.c_double(cell_size), _dp(out))
+    else:
+        ref(dim).ref_block_rhs(flux, ctypes.c_double(gamma), ctypes.c_double(b), _dp(blk),
+                               ctypes.c_double(cell_size), _dp(out))
+    icz = 16 if dim == 3 else 1
+    return out.reshape(dim + 2, 16, 16, icz)
+
+
+def run(dim, nb, periodic, flux, cfl, t_end, max_steps, state, gamma=1.4, b=0.0,
+        cell_size=None, threads=8, which="oracle"):
+    """Advance a domain state [NE][NX][NY][NZ]; returns (state, dts, t_final)."""
+    nb3 = list(nb) + [1] * (3 - len(nb))
+    per3 = list(periodic) + [0] * (3 - len(periodic))
+    if cell_size is None:
+        cell_size = 1.0 / (nb3[0] * 16)
+    st = np.ascontiguousarray(state, dtype=np.float64).copy()
+    dts = np.zeros(max(max_steps, 1))
+    tf = ctypes.c_double()
+    if which == "oracle":
+        g = OraGrid(dim, (ctypes.c_int * 3)(*nb3), (ctypes.c_int * 3)(*per3), flux, gamma, b,
+                    cell_size)
+        n = oracle().ora_run(ctypes.byref(g), ctypes.c_double(cfl), ctypes.c_double(t_end),
+                             max_steps, threads, _dp(st), _dp(dts), ctypes.byref(tf))
+    else:
+        n = ref(dim).ref_run(flux, ctypes.c_double(gamma), ctypes.c_double(b),
+                             (ctypes.c_int * 3)(*nb3), (ctypes.c_int * 3)(*per3),
+                             ctypes.c_double(cell_size), ctypes.c_double(cfl),
+                             ctypes.c_double(t_end), max_steps, threads, _dp(st), _dp(dts),
+                             ctypes.byref(tf))
+    return st, dts[:n], tf.value
