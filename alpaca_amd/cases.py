@@ -169,3 +169,54 @@ def random_block(dim: int, seed: int, gamma: float = 1.4, smooth: bool = False):
     if dim == 2:
         cons = cons[..., None]
     return np.ascontiguousarray(cons)
+
+
+def blobs_shock_3d_torch(nb: int, seed: int, gamma: float = 1.4, nblobs: int = 32,
+                         shock_x: float = 0.1, device="cuda", chunk: int = 32):
+    """blobs_shock_3d evaluated with torch on a device (same definition; used
+    where the domain is too large for a host-side numpy evaluation)."""
+    import torch
+
+    n = nb * IC
+    dx = 1.0 / n
+    c1 = (torch.arange(n, dtype=torch.float64, device=device) + 0.5) * dx
+    rng = SplitMix64(0xB10B, seed)
+    blobs = []
+    for _ in range(nblobs):
+        cx, cy, cz = rng.uniform(0, 1), rng.uniform(0, 1), rng.uniform(0, 1)
+        r = rng.uniform(0.02, 0.08)
+        a = rng.uniform(0.2, 2.0)
+        blobs.append((cx, cy, cz, r, a))
+    m = 2.0
+    rho_post = (gamma + 1.0) * m * m / ((gamma - 1.0) * m * m + 2.0)
+    p_post = 1.0 + 2.0 * gamma / (gamma + 1.0) * (m * m - 1.0)
+    u_post = m * math.sqrt(gamma) * (1.0 - 1.0 / rho_post)
+    cons = torch.empty((5, n, n, n), dtype=torch.float64, device=device)
+    for i0 in range(0, n, chunk):
+        xs = c1[i0:i0 + chunk]
+        X, Y, Z = torch.meshgrid(xs, c1, c1, indexing="ij")
+        add = torch.zeros_like(X)
+        for cx, cy, cz, r, a in blobs:
+            dxp = (X - cx).abs()
+            dxp = torch.minimum(dxp, 1.0 - dxp)
+            dyp = (Y - cy).abs()
+            dyp = torch.minimum(dyp, 1.0 - dyp)
+            dzp = (Z - cz).abs()
+            dzp = torch.minimum(dzp, 1.0 - dzp)
+            add += a * torch.exp(-(dxp * dxp + dyp * dyp + dzp * dzp) / (2.0 * r * r))
+        post = X < shock_x
+        rho = torch.where(post, rho_post + add, 1.0 + add)
+        p = torch.where(post, p_post + add, 1.0 + add)
+        u = torch.where(post, torch.full_like(X, u_post), torch.zeros_like(X))
+        z = torch.zeros_like(X)
+        w = z
+        s1 = (u * u + z * z) + w * w
+        s2 = (u * u + w * w) + z * z
+        s3 = (z * z + w * w) + u * u
+        q = 0.5 * (torch.minimum(torch.minimum(s1, s2), s3) + torch.maximum(torch.maximum(s1, s2), s3))
+        cons[0, i0:i0 + chunk] = rho
+        cons[1, i0:i0 + chunk] = p / (gamma - 1.0) + (0.5 * q * rho)
+        cons[2, i0:i0 + chunk] = u * rho
+        cons[3, i0:i0 + chunk] = z
+        cons[4, i0:i0 + chunk] = z
+    return cons

@@ -334,6 +334,38 @@ __global__ __launch_bounds__(kThreads) void scatter_ref_fluxes(const double *__r
 }
 
 // --------------------------------------------------------------- host ---
+// Optional kernel timing: HIP events recorded on the launch stream around
+// each launch of a kernel class, summed on read.
+struct ProfileSlot {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  double units = 0.0;  // algorithmic work units (faces or cells) launched
+};
+std::mutex g_prof_mutex;
+bool g_prof_on = false;
+ProfileSlot g_prof[ALPACA_KERNEL_COUNT];
+
+struct ProfileScope {
+  int kernel;
+  hipStream_t stream;
+  hipEvent_t start = nullptr, stop = nullptr;
+  ProfileScope(int k, hipStream_t s, double units) : kernel(k), stream(s) {
+    if (!g_prof_on) return;
+    if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) {
+      start = stop = nullptr;
+      return;
+    }
+    (void)hipEventRecord(start, stream);
+    std::lock_guard<std::mutex> lock(g_prof_mutex);
+    g_prof[kernel].units += units;
+  }
+  ~ProfileScope() {
+    if (!start) return;
+    (void)hipEventRecord(stop, stream);
+    std::lock_guard<std::mutex> lock(g_prof_mutex);
+    g_prof[kernel].events.emplace_back(start, stop);
+  }
+};
+
 struct Workspace {
   double *ptr = nullptr;
   size_t bytes = 0;
@@ -389,6 +421,7 @@ void launch_faces_dir(const alpaca_case_desc *d, int nblocks, const double *in, 
   constexpr int NF = faces_per_dir<DIM>();
   constexpr int gpb = (NF + kThreads - 1) / kThreads;
   dim3 grid((unsigned)(nblocks * gpb));
+  ProfileScope prof(ALPACA_KERNEL_FACE_FLUX, s, double(NF) * nblocks);
   if (d->precision == ALPACA_MIXED)
     hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, float>), grid, dim3(kThreads), 0, s, in,
                        fl, nblocks, d->gamma, d->background_pressure);
@@ -425,6 +458,7 @@ int stage_impl(const alpaca_case_desc *d, int nblocks, int stage, int mode, cons
   constexpr int gpb = (cells + kThreads - 1) / kThreads;
   dim3 grid((unsigned)(nblocks * gpb));
   double const inv_dx = 1.0 / d->cell_size;
+  ProfileScope prof(ALPACA_KERNEL_COMBINE, s, double(cells) * nblocks);
   if (mode == 0) {
     double *ini = (init && init != avg) ? init : nullptr;
     if (stage > 0) ini = init;
@@ -445,6 +479,7 @@ int halo_impl(const alpaca_case_desc *d, double *blocks, hipStream_t s) {
               d->gamma, d->background_pressure, d->cell_size};
   size_t const total = (size_t)nblocks * 2 * DIM * kHS * kIC * Dims<DIM>::ICZ;
   dim3 grid((unsigned)((total + kThreads - 1) / kThreads));
+  ProfileScope prof(ALPACA_KERNEL_HALO, s, double(total));
   hipLaunchKernelGGL((halo_kernel<DIM>), grid, dim3(kThreads), 0, s, blocks, nblocks, ca);
   return hip_status(hipGetLastError());
 }
@@ -453,6 +488,7 @@ template <int DIM>
 int timestep_impl(const alpaca_case_desc *d, const double *avg, alpaca_case_state *state,
                   hipStream_t s) {
   int const nblocks = case_blocks(d);
+  ProfileScope prof(ALPACA_KERNEL_TIMESTEP, s, double(Dims<DIM>::CELLS) * nblocks);
   hipLaunchKernelGGL((signal_kernel<DIM>), dim3(nblocks), dim3(kThreads), 0, s, avg, nblocks,
                      d->gamma, d->background_pressure, state);
   hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(64), 0, s, state, d->cell_size);
@@ -479,6 +515,39 @@ const char *alpaca_status_string(int status) {
 }
 
 int alpaca_abi_version(void) { return ALPACA_GPU_ABI_VERSION; }
+
+int alpaca_profile_enable(int on) {
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
+  g_prof_on = on != 0;
+  return ALPACA_OK;
+}
+
+int alpaca_profile_read(int kernel, int reset, double *total_ms, long long *launches,
+                        double *units) {
+  if (kernel < 0 || kernel >= ALPACA_KERNEL_COUNT || !total_ms || !launches || !units)
+    return ALPACA_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
+  ProfileSlot &p = g_prof[kernel];
+  double ms = 0.0;
+  for (auto &ev : p.events) {
+    if (hipEventSynchronize(ev.second) != hipSuccess) return ALPACA_ERR_HIP;
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, ev.first, ev.second) != hipSuccess) return ALPACA_ERR_HIP;
+    ms += t;
+  }
+  *total_ms = ms;
+  *launches = (long long)p.events.size();
+  *units = p.units;
+  if (reset) {
+    for (auto &ev : p.events) {
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+    p.events.clear();
+    p.units = 0.0;
+  }
+  return ALPACA_OK;
+}
 
 size_t alpaca_block_doubles(int dim) {
   if (dim == 2) return Dims<2>::BLOCK;

@@ -21,6 +21,8 @@ FP64, MIXED = 0, 1
 BC_ZERO_GRADIENT, BC_PERIODIC = 0, 1
 IC, HS, TC = 16, 4, 24
 
+KERNEL_FACE_FLUX, KERNEL_COMBINE, KERNEL_HALO, KERNEL_TIMESTEP = 0, 1, 2, 3
+
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "non-finite state",
           4: "unsupported configuration", 5: "capacity exhausted"}
 
@@ -59,6 +61,10 @@ _SIGNATURES = {
                          ctypes.c_void_p], ctypes.c_int),
     "alpaca_case_state_init": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_profile_enable": ([ctypes.c_int], ctypes.c_int),
+    "alpaca_profile_read": ([ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double)],
+                            ctypes.c_int),
     "alpaca_advance": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_void_p,
                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
 }
@@ -165,7 +171,10 @@ class GpuCase:
         self.dim = desc.dim
         self.nb = [desc.nb[0], desc.nb[1], desc.nb[2] if desc.dim == 3 else 1]
         self.nblocks = self.nb[0] * self.nb[1] * self.nb[2]
-        g = torch.as_tensor(np.ascontiguousarray(state0), device=device)
+        if torch.is_tensor(state0):
+            g = state0.to(device=device, dtype=torch.float64).contiguous()
+        else:
+            g = torch.as_tensor(np.ascontiguousarray(state0), device=device)
         self.u = to_blocks(g, self.dim, self.nb).contiguous()
         self.s1 = torch.zeros_like(self.u)
         self.s2 = torch.zeros_like(self.u)
@@ -213,3 +222,17 @@ class GpuCase:
 
     def download(self) -> np.ndarray:
         return from_blocks(self.u, self.dim, self.nb).cpu().numpy()
+
+
+def profile_enable(on: bool = True):
+    check(load_library().alpaca_profile_enable(1 if on else 0), "profile_enable")
+
+
+def profile_read(kernel: int, reset: bool = True):
+    """(total_ms, launches, units) of a kernel class since the last reset."""
+    ms = ctypes.c_double()
+    n = ctypes.c_longlong()
+    u = ctypes.c_double()
+    check(load_library().alpaca_profile_read(kernel, 1 if reset else 0, ctypes.byref(ms),
+                                             ctypes.byref(n), ctypes.byref(u)), "profile_read")
+    return ms.value, n.value, u.value

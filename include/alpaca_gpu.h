@@ -80,6 +80,25 @@ typedef struct alpaca_case_state {
 const char *alpaca_status_string(int status);
 int alpaca_abi_version(void);
 
+/* Kernel classes for the timing hooks below. */
+enum {
+  ALPACA_KERNEL_FACE_FLUX = 0, /* reconstruction + Riemann / splitting fluxes */
+  ALPACA_KERNEL_COMBINE = 1,   /* flux divergence + RK stage combination */
+  ALPACA_KERNEL_HALO = 2,      /* same-level halo fill + boundary conditions */
+  ALPACA_KERNEL_TIMESTEP = 3,  /* CFL reduction */
+  ALPACA_KERNEL_COUNT = 4
+};
+
+/* Measurement hooks (no counterpart in the reference, which times macro steps
+ * with MPI_Wtime): when enabled, every launch of a kernel class is bracketed
+ * by HIP events on its launch stream.  alpaca_profile_read synchronises on
+ * the recorded events and returns the summed milliseconds, the number of
+ * launches and the algorithmic work units launched (faces for FACE_FLUX,
+ * cells for COMBINE and TIMESTEP, halo cells for HALO); reset != 0 clears. */
+int alpaca_profile_enable(int on);
+int alpaca_profile_read(int kernel, int reset, double *total_ms, long long *launches,
+                        double *units);
+
 /* Doubles per haloed block (NE * 24^dim). */
 size_t alpaca_block_doubles(int dim);
 
