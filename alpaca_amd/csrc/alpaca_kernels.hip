@@ -20,6 +20,7 @@
 
 #include "../../include/alpaca_gpu.h"
 #include "fv_device.h"
+#include "stage_fused.h"
 
 namespace alpaca {
 namespace {
@@ -29,18 +30,6 @@ constexpr int kThreads = 256;
 // Faces per (block, direction): 17 along the direction, 16 across.
 template <int DIM> constexpr int faces_per_dir() {
   return (kIC + 1) * kIC * (DIM == 3 ? kIC : 1);
-}
-
-// Logical workgroup index with consecutive logical groups kept on one XCD:
-// the hardware deals workgroups round-robin over the 8 XCDs, so group g runs
-// on XCD g % 8.  Remapping makes each XCD sweep a contiguous range of blocks,
-// which keeps a block's stencil data in one XCD's L2.
-__device__ __forceinline__ unsigned xcd_remap(unsigned g, unsigned n) {
-  constexpr unsigned kXcd = 8;
-  unsigned const per = n / kXcd;
-  unsigned const full = per * kXcd;
-  if (g >= full) return g;
-  return (g % kXcd) * per + g / kXcd;
 }
 
 struct CaseArgs {
@@ -65,7 +54,7 @@ __global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__res
   constexpr int n1 = DIR == 1 ? kIC + 1 : kIC;
   constexpr int n2 = DIM == 3 ? (DIR == 2 ? kIC + 1 : kIC) : 1;
   constexpr int groups_per_block = (NF + kThreads - 1) / kThreads;
-  unsigned const g = xcd_remap(blockIdx.x, gridDim.x);
+  unsigned const g = xcd_remap_group(blockIdx.x, gridDim.x);
   int const blk = g / groups_per_block;
   int const f = (g % groups_per_block) * kThreads + threadIdx.x;
   if (blk >= nblocks || f >= NF) return;
@@ -114,7 +103,7 @@ __global__ __launch_bounds__(kThreads) void combine_kernel(
   constexpr int ICZ = D::ICZ;
   constexpr int cells = D::CELLS;
   constexpr int groups_per_block = (cells + kThreads - 1) / kThreads;
-  unsigned const g = xcd_remap(blockIdx.x, gridDim.x);
+  unsigned const g = xcd_remap_group(blockIdx.x, gridDim.x);
   int const blk = g / groups_per_block;
   int const c = (g % groups_per_block) * kThreads + threadIdx.x;
   if (blk >= nblocks || c >= cells) return;
@@ -471,6 +460,59 @@ int stage_impl(const alpaca_case_desc *d, int nblocks, int stage, int mode, cons
   return hip_status(hipGetLastError());
 }
 
+// Tile shape of the fused stage kernel: 8x8x8 cells and 256 threads in 3D,
+// the whole 16x16 block and 128 threads in 2D.
+template <int DIM, int FLUX, typename R>
+void launch_stage_fused(const alpaca_case_desc *d, int stage, const double *avg,
+                        const double *init, double *out, double *init_copy,
+                        alpaca_case_state *state, int signal, hipStream_t s) {
+  int const nblocks = case_blocks(d);
+  StageArgs a;
+  a.nb[0] = d->nb[0];
+  a.nb[1] = d->nb[1];
+  a.nb[2] = DIM == 3 ? d->nb[2] : 1;
+  a.bc[0] = d->bc[0];
+  a.bc[1] = d->bc[1];
+  a.bc[2] = DIM == 3 ? d->bc[2] : 0;
+  a.gamma = d->gamma;
+  a.b = d->background_pressure;
+  a.inv_dx = 1.0 / d->cell_size;
+  a.nblocks = nblocks;
+  a.stage = stage;
+  a.signal = signal;
+  a.init_copy = init_copy;
+  if constexpr (DIM == 3) {
+    constexpr int TX = 8, TY = 8, TZ = 8, NT = 256;
+    constexpr int tiles = (kIC / TX) * (kIC / TY) * (kIC / TZ);
+    ProfileScope prof(ALPACA_KERNEL_FACE_FLUX, s, double(faces_per_dir<3>()) * 3 * nblocks);
+    hipLaunchKernelGGL((stage_kernel<3, FLUX, R, TX, TY, TZ, NT>), dim3(nblocks * tiles), dim3(NT),
+                       0, s, avg, init, out, state, a);
+  } else {
+    constexpr int TX = 16, TY = 16, TZ = 1, NT = 128;
+    ProfileScope prof(ALPACA_KERNEL_FACE_FLUX, s, double(faces_per_dir<2>()) * 2 * nblocks);
+    hipLaunchKernelGGL((stage_kernel<2, FLUX, R, TX, TY, TZ, NT>), dim3(nblocks), dim3(NT), 0, s,
+                       avg, init, out, state, a);
+  }
+}
+
+template <int DIM>
+void stage_fused(const alpaca_case_desc *d, int stage, const double *avg, const double *init,
+                 double *out, double *init_copy, alpaca_case_state *state, int signal,
+                 hipStream_t s) {
+  bool const mixed = d->precision == ALPACA_MIXED;
+  if (d->flux == ALPACA_FLUX_HLLC) {
+    if (mixed)
+      launch_stage_fused<DIM, ALPACA_FLUX_HLLC, float>(d, stage, avg, init, out, init_copy, state, signal, s);
+    else
+      launch_stage_fused<DIM, ALPACA_FLUX_HLLC, double>(d, stage, avg, init, out, init_copy, state, signal, s);
+  } else {
+    if (mixed)
+      launch_stage_fused<DIM, ALPACA_FLUX_ROE, float>(d, stage, avg, init, out, init_copy, state, signal, s);
+    else
+      launch_stage_fused<DIM, ALPACA_FLUX_ROE, double>(d, stage, avg, init, out, init_copy, state, signal, s);
+  }
+}
+
 template <int DIM>
 int halo_impl(const alpaca_case_desc *d, double *blocks, hipStream_t s) {
   int const nblocks = case_blocks(d);
@@ -598,9 +640,12 @@ int alpaca_rk3_stage(const alpaca_case_desc *desc, int stage, const double *d_av
     return ALPACA_ERR_INVALID;
   if (stage > 0 && !d_init) return ALPACA_ERR_INVALID;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  int const nb = case_blocks(desc);
-  if (desc->dim == 2) return stage_impl<2>(desc, nb, stage, 0, d_avg, d_init, d_out, d_state, s);
-  return stage_impl<3>(desc, nb, stage, 0, d_avg, d_init, d_out, d_state, s);
+  double *copy = (stage == 0 && d_init && d_init != d_avg) ? d_init : nullptr;
+  if (desc->dim == 2)
+    stage_fused<2>(desc, stage, d_avg, d_init, d_out, copy, d_state, 0, s);
+  else
+    stage_fused<3>(desc, stage, d_avg, d_init, d_out, copy, d_state, 0, s);
+  return hip_status(hipGetLastError());
 }
 
 int alpaca_fill_halos(const alpaca_case_desc *desc, double *d_blocks, void *stream) {
@@ -631,23 +676,30 @@ int alpaca_case_state_init(alpaca_case_state *d_state, double time, double t_end
 }
 
 int alpaca_advance(const alpaca_case_desc *desc, double *d_u, double *d_s1, double *d_s2,
-                   alpaca_case_state *d_state, void *stream) {
+                   alpaca_case_state *d_state, int nsteps, void *stream) {
   int st = validate(desc);
   if (st != ALPACA_OK) return st;
-  if (!d_u || !d_s1 || !d_s2 || !d_state || d_u == d_s1 || d_u == d_s2 || d_s1 == d_s2)
+  if (!d_u || !d_s1 || !d_s2 || !d_state || d_u == d_s1 || d_u == d_s2 || d_s1 == d_s2 ||
+      nsteps < 0)
     return ALPACA_ERR_INVALID;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if ((st = alpaca_timestep(desc, d_u, d_state, stream)) != ALPACA_OK) return st;
-  // Stage 0: u1 = u + dt L(u) -> s1 (u stays as the initial buffer).
-  if ((st = alpaca_rk3_stage(desc, 0, d_u, nullptr, d_s1, d_state, stream)) != ALPACA_OK) return st;
-  if ((st = alpaca_fill_halos(desc, d_s1, stream)) != ALPACA_OK) return st;
-  // Stage 1: u2 = 3/4 u + 1/4 u1 + 1/4 dt L(u1) -> s2.
-  if ((st = alpaca_rk3_stage(desc, 1, d_s1, d_u, d_s2, d_state, stream)) != ALPACA_OK) return st;
-  if ((st = alpaca_fill_halos(desc, d_s2, stream)) != ALPACA_OK) return st;
-  // Stage 2: u^(n+1) = 1/3 u + 2/3 u2 + 2/3 dt L(u2) -> u (in place).
-  if ((st = alpaca_rk3_stage(desc, 2, d_s2, d_u, d_u, d_state, stream)) != ALPACA_OK) return st;
-  if ((st = alpaca_fill_halos(desc, d_u, stream)) != ALPACA_OK) return st;
-  hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, s, d_state);
+  // The first step reduces the CFL signal of d_u; later steps take it from the
+  // last stage of the step before.
+  if (nsteps > 0 && (st = alpaca_timestep(desc, d_u, d_state, stream)) != ALPACA_OK) return st;
+  for (int n = 0; n < nsteps; ++n) {
+    if (n > 0) hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(64), 0, s, d_state, desc->cell_size);
+    bool const signal = n + 1 < nsteps;
+    if (desc->dim == 2) {
+      stage_fused<2>(desc, 0, d_u, nullptr, d_s1, nullptr, d_state, 0, s);
+      stage_fused<2>(desc, 1, d_s1, d_u, d_s2, nullptr, d_state, 0, s);
+      stage_fused<2>(desc, 2, d_s2, d_u, d_u, nullptr, d_state, signal, s);
+    } else {
+      stage_fused<3>(desc, 0, d_u, nullptr, d_s1, nullptr, d_state, 0, s);
+      stage_fused<3>(desc, 1, d_s1, d_u, d_s2, nullptr, d_state, 0, s);
+      stage_fused<3>(desc, 2, d_s2, d_u, d_u, nullptr, d_state, signal, s);
+    }
+    hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, s, d_state);
+  }
   return hip_status(hipGetLastError());
 }
 

@@ -66,7 +66,8 @@ _SIGNATURES = {
                              ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double)],
                             ctypes.c_int),
     "alpaca_advance": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_void_p,
-                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
+                       ctypes.c_int),
 }
 
 
@@ -195,8 +196,7 @@ class GpuCase:
         d = ctypes.byref(self.desc)
         u, s1, s2 = (ctypes.c_void_p(t.data_ptr()) for t in (self.u, self.s1, self.s2))
         stp = ctypes.c_void_p(self.state.data_ptr())
-        for _ in range(nsteps):
-            check(self.lib.alpaca_advance(d, u, s1, s2, stp, st), "advance")
+        check(self.lib.alpaca_advance(d, u, s1, s2, stp, int(nsteps), st), "advance")
 
     def record(self) -> CaseState:
         raw = self.state.cpu().numpy().tobytes()

@@ -124,7 +124,9 @@ int alpaca_right_hand_side(const alpaca_case_desc *desc, int nblocks,
 /* (iii) One SSP-RK3 stage on every block of a case: FillInitialBuffer,
  * UpdateFluxes, PrepareBufferForIntegration and IntegrateNode
  * (time_integrator.h:98-113, 312-457; runge_kutta_3_TVD.h:39-59):
- *   out = (m0*avg + m1*init) + (c_s*dt)*RHS(avg)  on internal cells,
+ *   out = (m0*avg + m1*init) + (c_s*dt)*RHS(avg)  on internal cells, which
+ * are also written into the face halo slabs that read them (the same-level
+ * halo fill of the output, so no alpaca_fill_halos call is needed after it),
  * with (m0, m1, c_s) = (1, 0, 1), (1/4, 3/4, 1/4), (2/3, 1/3, 2/3).
  * Stage 0 copies d_avg into d_init when d_init is non-null and differs from
  * d_avg; later stages read d_init.  d_out must not alias d_avg; it may alias
@@ -153,14 +155,17 @@ int alpaca_timestep(const alpaca_case_desc *desc, const double *d_avg,
 int alpaca_case_state_init(alpaca_case_state *d_state, double time,
                            double t_end, double cfl, void *stream);
 
-/* One full macro step of a case, as ModularAlgorithmAssembler::Advance does
- * on a uniform level (modular_algorithm_assembler.cpp:543-910): time step,
- * then three stages each followed by the halo fill.  d_u holds u^n (halos
- * filled) and receives u^(n+1) in place; d_s1 and d_s2 are scratch blocks of
- * the same size: stage 0 writes u1 to d_s1, stage 1 writes u2 to d_s2 (with
- * d_u as the initial buffer), stage 2 writes u^(n+1) over d_u. */
+/* nsteps full macro steps of a case, as ModularAlgorithmAssembler::Advance
+ * does on a uniform level (modular_algorithm_assembler.cpp:543-910): time
+ * step, then three stages whose kernels also fill the face halo slabs of
+ * their output.  d_u holds u^n (halos filled) and receives u^(n+1) in place;
+ * d_s1 and d_s2 are scratch blocks of the same size: stage 0 writes u1 to
+ * d_s1, stage 1 writes u2 to d_s2 (d_u is the initial buffer), stage 2 writes
+ * u^(n+1) over d_u.  The CFL signal of each new state is reduced in the last
+ * stage, so only the first of the nsteps runs a separate reduction. */
 int alpaca_advance(const alpaca_case_desc *desc, double *d_u, double *d_s1,
-                   double *d_s2, alpaca_case_state *d_state, void *stream);
+                   double *d_s2, alpaca_case_state *d_state, int nsteps,
+                   void *stream);
 
 #ifdef __cplusplus
 }
