@@ -20,7 +20,6 @@
 
 #include "../../include/alpaca_gpu.h"
 #include "fv_device.h"
-#include "stage_fused.h"
 
 namespace alpaca {
 namespace {
@@ -42,7 +41,27 @@ struct CaseArgs {
 // Output layout per (block, dir): [NE][n0][n1][n2] with n_dir = 17 and the
 // other extents 16 (1 for z in 2D): face index a along DIR is the face
 // between halo cells a+3 and a+4.
-template <int DIM, int DIR, int FLUX, typename R>
+// Arithmetic modes: 0 = FP64 without contraction (bit-exact with the
+// reference), 1 = FP64 with FMA contraction, 2 = mixed (FP32 faces).
+enum : int { kModeExact = 0, kModeFma = 1, kModeMixed = 2 };
+
+template <int DIM, int DIR, int FLUX, int MODE>
+__device__ __forceinline__ void mode_stencil_flux(const double *bb, int i, int j, int k,
+                                                  double gamma, double b, double *out) {
+  constexpr int NE = Dims<DIM>::NE;
+  if constexpr (MODE == kModeExact) {
+    exact::stencil_flux<DIM, DIR, FLUX, double>(bb, i, j, k, gamma, b, out);
+  } else if constexpr (MODE == kModeFma) {
+    fast::stencil_flux<DIM, DIR, FLUX, double>(bb, i, j, k, gamma, b, out);
+  } else {
+    float f[NE];
+    fast::stencil_flux<DIM, DIR, FLUX, float>(bb, i, j, k, gamma, b, f);
+#pragma unroll
+    for (int e = 0; e < NE; ++e) out[e] = double(f[e]);
+  }
+}
+
+template <int DIM, int DIR, int FLUX, int MODE>
 __global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__restrict__ in,
                                                              double *__restrict__ flux_out,
                                                              int nblocks, double gamma,
@@ -50,7 +69,6 @@ __global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__res
   using D = Dims<DIM>;
   constexpr int NE = D::NE;
   constexpr int NF = faces_per_dir<DIM>();
-  constexpr int n0 = DIR == 0 ? kIC + 1 : kIC;
   constexpr int n1 = DIR == 1 ? kIC + 1 : kIC;
   constexpr int n2 = DIM == 3 ? (DIR == 2 ? kIC + 1 : kIC) : 1;
   constexpr int groups_per_block = (NF + kThreads - 1) / kThreads;
@@ -65,18 +83,11 @@ __global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__res
   int const i = a0 + (DIR == 0 ? kHS - 1 : kHS);
   int const j = a1 + (DIR == 1 ? kHS - 1 : kHS);
   int const k = DIM == 3 ? a2 + (DIR == 2 ? kHS - 1 : kHS) : 0;
-  constexpr int s = axis_stride<DIM, DIR>();
-  const double *base = in + (size_t)blk * D::BLOCK + i * D::SX + j * D::SY + k;
-  R U[6][NE];
-#pragma unroll
-  for (int m = 0; m < 6; ++m)
-#pragma unroll
-    for (int e = 0; e < NE; ++e) U[m][e] = R(base[e * D::FIELD + (m - 2) * s]);
-  R fl[NE];
-  face_flux<DIM, DIR, FLUX, R>(R(gamma), R(b), U, fl);
+  double fl[NE];
+  mode_stencil_flux<DIM, DIR, FLUX, MODE>(in + (size_t)blk * D::BLOCK, i, j, k, gamma, b, fl);
   double *o = flux_out + ((size_t)blk * DIM + DIR) * NE * NF + f;
 #pragma unroll
-  for (int e = 0; e < NE; ++e) o[e * NF] = double(fl[e]);
+  for (int e = 0; e < NE; ++e) o[e * NF] = fl[e];
 }
 
 // ------------------------------------------------------------- combine ---
@@ -137,7 +148,7 @@ __global__ __launch_bounds__(kThreads) void combine_kernel(
     double const dy_ = fy[e * NF + y_lo] - fy[e * NF + y_hi];
     double dz_ = 0.0;
     if constexpr (DIM == 3) dz_ = fz[e * NF + z_lo] - fz[e * NF + z_hi];
-    double const rhs = 0.0 + (0.0 + dams<DIM>(dx_, dy_, dz_) * inv_dx);
+    double const rhs = 0.0 + (0.0 + exact::dams<DIM>(dx_, dy_, dz_) * inv_dx);
     size_t const idx = cell + (size_t)e * D::FIELD;
     if constexpr (MODE == 0) {
       double const u = avg[idx];
@@ -209,11 +220,6 @@ __global__ __launch_bounds__(kThreads) void halo_kernel(double *blocks, int nblo
 }
 
 // ----------------------------------------------------------- time step ---
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = smax(v, __shfl_xor(v, o, 64));
-  return v;
-}
 
 // Per block: max over internal cells of sum_d(|u_d| + c)
 // (modular_algorithm_assembler.cpp:1840-1869), then an atomic max on the
@@ -236,24 +242,24 @@ __global__ __launch_bounds__(kThreads) void signal_kernel(const double *__restri
     double u[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) u[e] = p[e * D::FIELD];
-    Prim<double> pr;
-    cons_to_prim<DIM>(gamma, b, u, pr);
-    double const cs = eos_sos(gamma, b, pr.rho, pr.p);
+    exact::Prim<double> pr;
+    exact::cons_to_prim<DIM>(gamma, b, u, pr);
+    double const cs = exact::eos_sos(gamma, b, pr.rho, pr.p);
     double sum;
     if constexpr (DIM == 2)
       sum = (__builtin_fabs(pr.vel[0]) + cs) + (__builtin_fabs(pr.vel[1]) + cs);
     else
-      sum = cms3(__builtin_fabs(pr.vel[0]) + cs, __builtin_fabs(pr.vel[1]) + cs,
+      sum = exact::cms3(__builtin_fabs(pr.vel[0]) + cs, __builtin_fabs(pr.vel[1]) + cs,
                  __builtin_fabs(pr.vel[2]) + cs);
-    s = smax(s, sum);
+    s = exact::smax(s, sum);
   }
-  s = wave_max(s);
+  s = exact::wave_max64(s);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
     double m = part[0];
 #pragma unroll
-    for (int w = 1; w < kThreads / 64; ++w) m = smax(m, part[w]);
+    for (int w = 1; w < kThreads / 64; ++w) m = exact::smax(m, part[w]);
     atomicMax(reinterpret_cast<unsigned long long *>(&state->smax),
               (unsigned long long)__double_as_longlong(m));
   }
@@ -266,7 +272,7 @@ __global__ void dt_kernel(alpaca_case_state *state, double cell_size) {
   double const s = __longlong_as_double((long long)state->smax);
   double const rate = s / cell_size;
   double dt = rate == 0.0 ? 1.7976931348623157e308 : state->cfl / rate;
-  if (state->time + dt > state->t_end) dt = smax(state->t_end - state->time, 0.0);
+  if (state->time + dt > state->t_end) dt = exact::smax(state->t_end - state->time, 0.0);
   state->dt = dt;
   state->smax = 0;
   if (dt < 2.220446049250313080847263336181640625e-16 && dt > 0.0) state->flags |= 2;
@@ -392,7 +398,9 @@ int validate(const alpaca_case_desc *d) {
       return ALPACA_ERR_INVALID;
   if (d->dim == 2 && d->nb[2] != 1) return ALPACA_ERR_INVALID;
   if (d->flux != ALPACA_FLUX_HLLC && d->flux != ALPACA_FLUX_ROE) return ALPACA_ERR_INVALID;
-  if (d->precision != ALPACA_FP64 && d->precision != ALPACA_MIXED) return ALPACA_ERR_INVALID;
+  if (d->precision != ALPACA_FP64 && d->precision != ALPACA_MIXED &&
+      d->precision != ALPACA_FP64_FMA)
+    return ALPACA_ERR_INVALID;
   if (!(d->cell_size > 0.0) || !(d->gamma > 1.0)) return ALPACA_ERR_INVALID;
   return ALPACA_OK;
 }
@@ -412,11 +420,14 @@ void launch_faces_dir(const alpaca_case_desc *d, int nblocks, const double *in, 
   dim3 grid((unsigned)(nblocks * gpb));
   ProfileScope prof(ALPACA_KERNEL_FACE_FLUX, s, double(NF) * nblocks);
   if (d->precision == ALPACA_MIXED)
-    hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, float>), grid, dim3(kThreads), 0, s, in,
-                       fl, nblocks, d->gamma, d->background_pressure);
+    hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, kModeMixed>), grid, dim3(kThreads), 0, s,
+                       in, fl, nblocks, d->gamma, d->background_pressure);
+  else if (d->precision == ALPACA_FP64_FMA)
+    hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, kModeFma>), grid, dim3(kThreads), 0, s,
+                       in, fl, nblocks, d->gamma, d->background_pressure);
   else
-    hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, double>), grid, dim3(kThreads), 0, s, in,
-                       fl, nblocks, d->gamma, d->background_pressure);
+    hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, kModeExact>), grid, dim3(kThreads), 0, s,
+                       in, fl, nblocks, d->gamma, d->background_pressure);
 }
 
 template <int DIM, int FLUX>
@@ -462,7 +473,24 @@ int stage_impl(const alpaca_case_desc *d, int nblocks, int stage, int mode, cons
 
 // Tile shape of the fused stage kernel: 8x8x8 cells and 256 threads in 3D,
 // the whole 16x16 block and 128 threads in 2D.
-template <int DIM, int FLUX, typename R>
+// Fused stage kernel of an arithmetic mode.
+template <int DIM, int FLUX, int MODE, int TX, int TY, int TZ, int NT>
+void launch_stage_kernel(dim3 grid, hipStream_t s, const double *avg, const double *init,
+                         double *out, alpaca_case_state *state, const StageArgs &a) {
+  if constexpr (MODE == kModeExact)
+    hipLaunchKernelGGL((exact::stage_kernel<DIM, FLUX, double, TX, TY, TZ, NT>), grid, dim3(NT), 0,
+                       s, avg, init, out, state, a);
+  else if constexpr (MODE == kModeFma)
+    hipLaunchKernelGGL((fast::stage_kernel<DIM, FLUX, double, TX, TY, TZ, NT>), grid, dim3(NT), 0,
+                       s, avg, init, out, state, a);
+  else
+    hipLaunchKernelGGL((fast::stage_kernel<DIM, FLUX, float, TX, TY, TZ, NT>), grid, dim3(NT), 0,
+                       s, avg, init, out, state, a);
+}
+
+// Tile shape of the fused stage kernel: 8x8x8 cells and 256 threads in 3D,
+// the whole 16x16 block and 128 threads in 2D.
+template <int DIM, int FLUX, int MODE>
 void launch_stage_fused(const alpaca_case_desc *d, int stage, const double *avg,
                         const double *init, double *out, double *init_copy,
                         alpaca_case_state *state, int signal, hipStream_t s) {
@@ -481,17 +509,14 @@ void launch_stage_fused(const alpaca_case_desc *d, int stage, const double *avg,
   a.stage = stage;
   a.signal = signal;
   a.init_copy = init_copy;
+  ProfileScope prof(ALPACA_KERNEL_STAGE, s, double(Dims<DIM>::CELLS) * nblocks);
   if constexpr (DIM == 3) {
     constexpr int TX = 8, TY = 8, TZ = 8, NT = 256;
     constexpr int tiles = (kIC / TX) * (kIC / TY) * (kIC / TZ);
-    ProfileScope prof(ALPACA_KERNEL_FACE_FLUX, s, double(faces_per_dir<3>()) * 3 * nblocks);
-    hipLaunchKernelGGL((stage_kernel<3, FLUX, R, TX, TY, TZ, NT>), dim3(nblocks * tiles), dim3(NT),
-                       0, s, avg, init, out, state, a);
+    launch_stage_kernel<3, FLUX, MODE, TX, TY, TZ, NT>(dim3(nblocks * tiles), s, avg, init, out,
+                                                       state, a);
   } else {
-    constexpr int TX = 16, TY = 16, TZ = 1, NT = 128;
-    ProfileScope prof(ALPACA_KERNEL_FACE_FLUX, s, double(faces_per_dir<2>()) * 2 * nblocks);
-    hipLaunchKernelGGL((stage_kernel<2, FLUX, R, TX, TY, TZ, NT>), dim3(nblocks), dim3(NT), 0, s,
-                       avg, init, out, state, a);
+    launch_stage_kernel<2, FLUX, MODE, 16, 16, 1, 128>(dim3(nblocks), s, avg, init, out, state, a);
   }
 }
 
@@ -499,17 +524,21 @@ template <int DIM>
 void stage_fused(const alpaca_case_desc *d, int stage, const double *avg, const double *init,
                  double *out, double *init_copy, alpaca_case_state *state, int signal,
                  hipStream_t s) {
-  bool const mixed = d->precision == ALPACA_MIXED;
+  int const p = d->precision;
   if (d->flux == ALPACA_FLUX_HLLC) {
-    if (mixed)
-      launch_stage_fused<DIM, ALPACA_FLUX_HLLC, float>(d, stage, avg, init, out, init_copy, state, signal, s);
+    if (p == ALPACA_MIXED)
+      launch_stage_fused<DIM, ALPACA_FLUX_HLLC, kModeMixed>(d, stage, avg, init, out, init_copy, state, signal, s);
+    else if (p == ALPACA_FP64_FMA)
+      launch_stage_fused<DIM, ALPACA_FLUX_HLLC, kModeFma>(d, stage, avg, init, out, init_copy, state, signal, s);
     else
-      launch_stage_fused<DIM, ALPACA_FLUX_HLLC, double>(d, stage, avg, init, out, init_copy, state, signal, s);
+      launch_stage_fused<DIM, ALPACA_FLUX_HLLC, kModeExact>(d, stage, avg, init, out, init_copy, state, signal, s);
   } else {
-    if (mixed)
-      launch_stage_fused<DIM, ALPACA_FLUX_ROE, float>(d, stage, avg, init, out, init_copy, state, signal, s);
+    if (p == ALPACA_MIXED)
+      launch_stage_fused<DIM, ALPACA_FLUX_ROE, kModeMixed>(d, stage, avg, init, out, init_copy, state, signal, s);
+    else if (p == ALPACA_FP64_FMA)
+      launch_stage_fused<DIM, ALPACA_FLUX_ROE, kModeFma>(d, stage, avg, init, out, init_copy, state, signal, s);
     else
-      launch_stage_fused<DIM, ALPACA_FLUX_ROE, double>(d, stage, avg, init, out, init_copy, state, signal, s);
+      launch_stage_fused<DIM, ALPACA_FLUX_ROE, kModeExact>(d, stage, avg, init, out, init_copy, state, signal, s);
   }
 }
 

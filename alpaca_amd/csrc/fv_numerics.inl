@@ -1,0 +1,468 @@
+// Face numerics, included into alpaca::exact and alpaca::fast (see
+// fv_device.h).  Real type R is double (FP64) or float (mixed form).  For
+// R = double every expression keeps the reference's operation order, so the
+// exact namespace reproduces it bit for bit.  Citations name the reference
+// file:line each function follows.
+
+template <typename R> __device__ __forceinline__ R smin(R a, R b) { return (b < a) ? b : a; }
+template <typename R> __device__ __forceinline__ R smax(R a, R b) { return (a < b) ? b : a; }
+
+// Correctly rounded division/square root in FP64; hardware reciprocal and
+// square root (about 1 ulp) in the FP32 part of the mixed form.
+template <typename R> __device__ __forceinline__ R fdiv(R a, R b) { return a / b; }
+template <> __device__ __forceinline__ float fdiv<float>(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
+}
+template <typename R> __device__ __forceinline__ R dsqrt(R x) { return __builtin_sqrt(x); }
+template <> __device__ __forceinline__ float dsqrt<float>(float x) {
+  return __builtin_amdgcn_sqrtf(x);
+}
+
+// ConsistencyManagedSum, full symmetry (mathematical_functions.h:79-91).
+template <typename R> __device__ __forceinline__ R cms3(R a, R b, R c) {
+  R const s1 = (a + b) + c;
+  R const s2 = (a + c) + b;
+  R const s3 = (b + c) + a;
+  R mn = smin(smin(s1, s2), s3);
+  R mx = smax(smax(s1, s2), s3);
+  return R(0.5) * (mn + mx);
+}
+
+// DimensionAwareConsistencyManagedSum (mathematical_functions.h:100-108).
+template <int DIM, typename R> __device__ __forceinline__ R dams(R a, R b, R c) {
+  if constexpr (DIM == 2) {
+    (void)c;
+    return a + b;
+  } else {
+    return cms3(a, b, c);
+  }
+}
+
+// CalculatePressure<false> (generic_stiffened_gas.h:40-58).  The recovery
+// p = -gamma*B + (gamma-1)(E - |m|^2/(2 rho)) always runs in FP64: the
+// difference cancels at high Mach number.
+template <int DIM>
+__device__ __forceinline__ double eos_pressure64(double gamma, double b, double rho, double mx,
+                                                 double my, double mz, double e) {
+  double const q = 0.5 * dams<DIM>(mx * mx, my * my, mz * mz);
+  return -gamma * b + (gamma - 1.0) * (e - q / rho);
+}
+template <int DIM, typename R>
+__device__ __forceinline__ R eos_pressure(R gamma, R b, R rho, R mx, R my, R mz, R e) {
+  return R(eos_pressure64<DIM>(double(gamma), double(b), double(rho), double(mx), double(my),
+                               double(mz), double(e)));
+}
+
+// CalculateSpeedOfSound<false> (generic_stiffened_gas.h:84-97).
+template <typename R> __device__ __forceinline__ R eos_sos(R gamma, R b, R rho, R p) {
+  return dsqrt(fdiv(gamma * (p + b), rho));
+}
+
+// ComputePsi (stiffened_gas.cpp:127-130).
+template <typename R> __device__ __forceinline__ R eos_psi(R gamma, R b, R p, R one_rho) {
+  return (p + gamma * b) * one_rho;
+}
+
+template <typename R> struct Prim {
+  R rho, p, vel[3];
+};
+
+// ConservativesToPrimeStates (prime_state_handler.h:279-370).
+template <int DIM, typename R>
+__device__ __forceinline__ void cons_to_prim(R gamma, R b, const R *c, Prim<R> &o) {
+  o.rho = c[kMass];
+  R const one_density = fdiv(R(1.0), c[kMass]);
+  o.vel[0] = c[kMomX] * one_density;
+  o.vel[1] = c[kMomX + 1] * one_density;
+  if constexpr (DIM == 3)
+    o.vel[2] = c[kMomX + 2] * one_density;
+  else
+    o.vel[2] = R(0.0);
+  R const mz = DIM == 3 ? c[kMomX + DIM - 1] : R(0.0);
+  o.p = eos_pressure<DIM>(gamma, b, c[kMass], c[kMomX], c[kMomX + 1], mz, c[kEnergy]);
+}
+
+// Per-cell quantities the Roe average needs, from the cell's FP64
+// conservatives: 1/rho, prime velocities and pressure
+// (prime_state_handler.h:279-370), psi (stiffened_gas.cpp:127-130) and the
+// enthalpy with its EOS-recomputed pressure (stiffened_gas.cpp:80-87), as
+// eigendecomposition.h:266-283 reads them.
+template <typename R> struct CellQ {
+  R rho, one_rho, vel[3], p, psi, H;
+};
+
+template <int DIM, typename R>
+__device__ __forceinline__ void cell_quantities(double gamma, double b, const double *c,
+                                                CellQ<R> &q) {
+  Prim<double> p;
+  cons_to_prim<DIM, double>(gamma, b, c, p);
+  double const one_rho = 1.0 / c[kMass];
+  double const mz = DIM == 3 ? c[kMomX + DIM - 1] : 0.0;
+  double const H =
+      (c[kEnergy] + eos_pressure64<DIM>(gamma, b, c[kMass], c[kMomX], c[kMomX + 1], mz,
+                                        c[kEnergy])) /
+      c[kMass];
+  q.rho = R(c[kMass]);
+  q.one_rho = R(one_rho);
+  q.vel[0] = R(p.vel[0]);
+  q.vel[1] = R(p.vel[1]);
+  q.vel[2] = R(p.vel[2]);
+  q.p = R(p.p);
+  q.psi = R(eos_psi<double>(gamma, b, p.p, one_rho));
+  q.H = R(H);
+}
+
+// WENO5::ApplyImplementation (weno5.h:88-158) on ordered v1..v5.
+template <typename R> __device__ __forceinline__ R weno5(R v1, R v2, R v3, R v4, R v5) {
+  R const c_s1 = R(13.0 / 12.0), c_s2 = R(0.25), eps = R(1.0e-6);
+  R const s11 = R(1.0) * v1 + R(-2.0) * v2 + R(1.0) * v3;
+  R const s12 = R(1.0) * v1 + R(-4.0) * v2 + R(3.0) * v3;
+  R const s1 = c_s1 * s11 * s11 + c_s2 * s12 * s12;
+  R const s21 = R(1.0) * v2 + R(-2.0) * v3 + R(1.0) * v4;
+  R const s22 = R(1.0) * v2 + R(-1.0) * v4;
+  R const s2 = c_s1 * s21 * s21 + c_s2 * s22 * s22;
+  R const s31 = R(1.0) * v3 + R(-2.0) * v4 + R(1.0) * v5;
+  R const s32 = R(3.0) * v3 + R(-4.0) * v4 + R(1.0) * v5;
+  R const s3 = c_s1 * s31 * s31 + c_s2 * s32 * s32;
+  R const a1 = fdiv(R(0.1), (s1 + eps) * (s1 + eps));
+  R const a2 = fdiv(R(0.6), (s2 + eps) * (s2 + eps));
+  R const a3 = fdiv(R(0.3), (s3 + eps) * (s3 + eps));
+  R const one_a_sum = fdiv(R(1.0), a1 + a2 + a3);
+  R const w1 = a1 * one_a_sum;
+  R const w2 = a2 * one_a_sum;
+  R const w3 = a3 * one_a_sum;
+  return w1 * (R(2.0 / 6.0) * v1 + R(-7.0 / 6.0) * v2 + R(11.0 / 6.0) * v3) +
+         w2 * (R(-1.0 / 6.0) * v2 + R(5.0 / 6.0) * v3 + R(2.0 / 6.0) * v4) +
+         w3 * (R(2.0 / 6.0) * v3 + R(5.0 / 6.0) * v4 + R(-1.0 / 6.0) * v5);
+}
+// UpwindLeft {0,+1} and UpwindRight {1,-1} (apply_wrapper.h:48-58).
+template <typename R> __device__ __forceinline__ R weno_left(const R *a) {
+  return weno5(a[0], a[1], a[2], a[3], a[4]);
+}
+template <typename R> __device__ __forceinline__ R weno_right(const R *a) {
+  return weno5(a[5], a[4], a[3], a[2], a[1]);
+}
+
+template <int NE, typename R> struct Eig {
+  R L[NE][NE];
+  R Rm[NE][NE];
+  R lam[NE];
+};
+
+// ComputeRoeEigendecomposition for one face (eigendecomposition.h:238-466).
+// t = cell i, n = cell i+1 along DIR.  Returns false (all zero) when a
+// density is not positive.
+template <int DIM, int DIR, typename R>
+__device__ __forceinline__ bool roe_eigensystem(R gamma, const CellQ<R> &t, const CellQ<R> &n,
+                                                Eig<DIM + 2, R> &ev) {
+  constexpr int NE = DIM + 2;
+  constexpr int m1 = Minor<DIR>::m1, m2 = Minor<DIR>::m2;
+  constexpr int evp = DIM;
+  constexpr int Mp = kMomX + DIR, M1 = kMomX + m1, M2 = kMomX + m2;
+#pragma unroll
+  for (int r = 0; r < NE; ++r) {
+#pragma unroll
+    for (int c = 0; c < NE; ++c) {
+      ev.L[r][c] = R(0.0);
+      ev.Rm[r][c] = R(0.0);
+    }
+    ev.lam[r] = R(0.0);
+  }
+  if (t.rho <= R(0.0) || n.rho <= R(0.0)) return false;
+
+  R const u_t = t.vel[DIR];
+  R const v_t = t.vel[m1];
+  R const w_t = DIM == 3 ? t.vel[m2] : R(0.0);
+  R const u_n = n.vel[DIR];
+  R const v_n = n.vel[m1];
+  R const w_n = DIM == 3 ? n.vel[m2] : R(0.0);
+
+  R const sq_t = dsqrt(t.rho);
+  R const sq_n = dsqrt(n.rho);
+  R const rho_div = fdiv(R(1.0), sq_t + sq_n);
+  R const rho_roe = sq_t * sq_n;
+  R const one_rho_roe = fdiv(R(1.0), rho_roe);
+
+  R const u = ((u_t * sq_t) + (u_n * sq_n)) * rho_div;
+  R const v = ((v_t * sq_t) + (v_n * sq_n)) * rho_div;
+  R const w = ((w_t * sq_t) + (w_n * sq_n)) * rho_div;
+  R const psi = ((t.psi * sq_t) + (n.psi * sq_n)) * rho_div;
+  R const g = gamma - R(1.0);
+  R const H = (t.H * sq_t + n.H * sq_n) * rho_div;
+
+  R const q2 = dams<DIM>(u * u, v * v, w * w);
+  R const tmp = dams<DIM>((u_t - u_n) * (u_t - u_n), (v_t - v_n) * (v_t - v_n),
+                          (w_t - w_n) * (w_t - w_n));
+  R const pod = ((t.p * t.one_rho) * sq_t + (n.p * n.one_rho) * sq_n) * rho_div +
+                R(0.5) * rho_roe * (rho_div * rho_div) * tmp;
+  R const cc = psi + g * pod;
+  R const one_cc = fdiv(R(1.0), cc);
+  R const c = dsqrt(cc);
+
+  ev.L[0][kMass] = R(0.5) * one_cc * (g * q2 - g * H + (u + c) * c);
+  ev.L[0][Mp] = R(0.5) * one_cc * (-u * g - c);
+  ev.L[0][M1] = R(0.5) * one_cc * (-v * g);
+  if constexpr (DIM == 3) ev.L[0][M2] = R(0.5) * one_cc * (-w * g);
+  ev.L[0][kEnergy] = R(0.5) * one_cc * g;
+
+  ev.L[1][kMass] = v * one_rho_roe;
+  ev.L[1][M1] = -one_rho_roe;
+
+  if constexpr (DIM == 3) {
+    ev.L[2][kMass] = -w * one_rho_roe;
+    ev.L[2][M2] = one_rho_roe;
+  }
+
+  ev.L[evp][kMass] = one_cc * (H - q2);
+  ev.L[evp][Mp] = one_cc * u;
+  ev.L[evp][M1] = one_cc * v;
+  if constexpr (DIM == 3) ev.L[evp][M2] = one_cc * w;
+  ev.L[evp][kEnergy] = -one_cc;
+
+  ev.L[NE - 1][kMass] = R(0.5) * one_cc * (g * q2 - g * H - (u - c) * c);
+  ev.L[NE - 1][Mp] = R(0.5) * one_cc * (-u * g + c);
+  ev.L[NE - 1][M1] = R(0.5) * one_cc * (-v * g);
+  if constexpr (DIM == 3) ev.L[NE - 1][M2] = R(0.5) * one_cc * (-w * g);
+  ev.L[NE - 1][kEnergy] = R(0.5) * one_cc * g;
+
+  ev.Rm[kMass][0] = R(1.0);
+  ev.Rm[kMass][evp] = g;
+  ev.Rm[kMass][NE - 1] = R(1.0);
+
+  ev.Rm[Mp][0] = u - c;
+  ev.Rm[Mp][evp] = g * u;
+  ev.Rm[Mp][NE - 1] = u + c;
+
+  ev.Rm[M1][0] = v;
+  ev.Rm[M1][1] = -rho_roe;
+  ev.Rm[M1][evp] = g * v;
+  ev.Rm[M1][NE - 1] = v;
+
+  if constexpr (DIM == 3) {
+    ev.Rm[M2][0] = w;
+    ev.Rm[M2][2] = rho_roe;
+    ev.Rm[M2][evp] = g * w;
+    ev.Rm[M2][NE - 1] = w;
+  }
+
+  ev.Rm[kEnergy][0] = H - u * c;
+  ev.Rm[kEnergy][1] = -v * rho_roe;
+  if constexpr (DIM == 3) ev.Rm[kEnergy][2] = rho_roe * w;
+  ev.Rm[kEnergy][evp] = g * H - c * c;
+  ev.Rm[kEnergy][NE - 1] = H + u * c;
+
+  ev.lam[0] = __builtin_fabs(u - c);
+  ev.lam[NE - 1] = __builtin_fabs(u + c);
+#pragma unroll
+  for (int l = 1; l < NE - 1; ++l) ev.lam[l] = __builtin_fabs(u);
+  return true;
+}
+
+// TransformToPhysicalSpace (eigendecomposition.h:144-165).
+template <int NE, typename R>
+__device__ __forceinline__ void to_physical(const R *ch, const R (&Rm)[NE][NE], R *out) {
+#pragma unroll
+  for (int l = 0; l < NE; ++l) {
+    R s = R(0.0);
+#pragma unroll
+    for (int n = 1; n < NE - 1; ++n) s += ch[n] * Rm[l][n];
+    s += (ch[0] * Rm[l][0] + ch[NE - 1] * Rm[l][NE - 1]);
+    out[l] = s;
+  }
+}
+
+// CalculateSignalSpeed, Einfeldt (hll_signal_speed_calculator.h:77-101).
+template <typename R>
+__device__ __forceinline__ void einfeldt(R rl, R rr, R ul, R ur, R cl, R cr, R &sl, R &sr) {
+  R const sql = dsqrt(rl);
+  R const sqr = dsqrt(rr);
+  R const rho_div = fdiv(R(1.0), sql + sqr);
+  R const u_roe = ((ul * sql) + (ur * sqr)) * rho_div;
+  R const c_e = dsqrt(((cl * cl * sql) + (cr * cr * sqr)) * rho_div +
+                      (R(0.5) * sql * sqr * rho_div * rho_div) * (ur - ul) * (ur - ul));
+  sl = smin(u_roe - c_e, ul - cl);
+  sr = smax(u_roe + c_e, ur + cr);
+}
+
+// HllcRiemannSolver::SolveRiemannProblemImplementation (hllc_riemann_solver.h:
+// 52-198), including the clamped wave speeds in the energy star states.
+template <int DIM, int DIR, typename R>
+__device__ __forceinline__ void hllc(R gamma, R b, const R *cl, const R *cr, const Prim<R> &pl,
+                                     const Prim<R> &pr, R *f) {
+  constexpr int NE = DIM + 2;
+  constexpr int Mp = kMomX + DIR;
+  R qsl[NE], qsr[NE], fl[NE], fr[NE];
+  R const p_l = pl.p, p_r = pr.p;
+  R const one_rho_l = fdiv(R(1.0), cl[kMass]);
+  R const one_rho_r = fdiv(R(1.0), cr[kMass]);
+  R const u_l = pl.vel[DIR], u_r = pr.vel[DIR];
+  R const c_l = eos_sos(gamma, b, cl[kMass], p_l);
+  R const c_r = eos_sos(gamma, b, cr[kMass], p_r);
+  R sls, srs;
+  einfeldt(cl[kMass], cr[kMass], u_l, u_r, c_l, c_r, sls, srs);
+  R const sc = fdiv((p_r - p_l) + (cl[Mp] * (sls - u_l) - cr[Mp] * (srs - u_r)),
+                    cl[kMass] * (sls - u_l) - cr[kMass] * (srs - u_r));
+  R const sl = smin(sls, R(0.0));
+  R const sr = smax(srs, R(0.0));
+  R const chi_l = fdiv(sls - u_l, sls - sc);
+  R const chi_r = fdiv(srs - u_r, srs - sc);
+
+  qsl[kMass] = cl[kMass] * chi_l;
+  qsl[Mp] = cl[kMass] * chi_l * sc;
+  qsl[kEnergy] = cl[kMass] * chi_l *
+                 (cl[kEnergy] * one_rho_l + (sc - u_l) * (sc + fdiv(p_l, cl[kMass] * (sl - u_l))));
+  qsr[kMass] = cr[kMass] * chi_r;
+  qsr[Mp] = cr[kMass] * chi_r * sc;
+  qsr[kEnergy] = cr[kMass] * chi_r *
+                 (cr[kEnergy] * one_rho_r + (sc - u_r) * (sc + fdiv(p_r, cr[kMass] * (sr - u_r))));
+
+  fl[kMass] = cl[Mp];
+  fl[Mp] = ((cl[Mp] * cl[Mp]) * one_rho_l) + p_l;
+  fl[kEnergy] = u_l * (cl[kEnergy] + p_l);
+  fr[kMass] = cr[Mp];
+  fr[Mp] = ((cr[Mp] * cr[Mp]) * one_rho_r) + p_r;
+  fr[kEnergy] = u_r * (cr[kEnergy] + p_r);
+
+#pragma unroll
+  for (int d = 0; d < DIM - 1; ++d) {
+    int const Mm = kMomX + (d == 0 ? Minor<DIR>::m1 : Minor<DIR>::m2);
+    qsl[Mm] = chi_l * cl[Mm];
+    qsr[Mm] = chi_r * cr[Mm];
+    fl[Mm] = u_l * cl[Mm];
+    fr[Mm] = u_r * cr[Mm];
+  }
+  int const sg = (R(0.0) < sc) - (sc < R(0.0));
+  R const wl = R(0.5) * (R(1.0) + R(sg));
+  R const wr = R(0.5) * (R(1.0) - R(sg));
+#pragma unroll
+  for (int n = 0; n < NE; ++n) {
+    R const fsl = fl[n] + (sl * (qsl[n] - cl[n]));
+    R const fsr = fr[n] + (sr * (qsr[n] - cr[n]));
+    f[n] = (wl * fsl + wr * fsr);
+  }
+}
+
+// One face flux from the six-cell stencil Ud[m] (FP64 conservatives of cells
+// i-2..i+3 along DIR); reconstruction and Riemann arithmetic in R.
+// FV:  FiniteVolumeScheme::ComputeFluxes (finite_volume_scheme.cpp:160-251)
+//      + CharacteristicStateReconstruction (characteristic_state_reconstruction.h:60-134).
+// Roe: FluxSplittingScheme::ComputeFluxes + ComputeAdvection
+//      (flux_splitting_scheme.cpp:42-76, 185-315).
+// Writes f (zero when the reference skips the face).
+template <int DIM, int DIR, int FLUX, typename R>
+__device__ __forceinline__ void face_flux(double gamma64, double b64,
+                                          const double (&Ud)[6][DIM + 2], R *f) {
+  constexpr int NE = DIM + 2;
+  R const gamma = R(gamma64), b = R(b64);
+  R U[6][NE];
+#pragma unroll
+  for (int m = 0; m < 6; ++m)
+#pragma unroll
+    for (int e = 0; e < NE; ++e) U[m][e] = R(Ud[m][e]);
+  Eig<NE, R> ev;
+#pragma unroll
+  for (int e = 0; e < NE; ++e) f[e] = R(0.0);
+  if constexpr (FLUX == 0) {
+    CellQ<R> qt, qn;
+    cell_quantities<DIM, R>(gamma64, b64, Ud[2], qt);
+    cell_quantities<DIM, R>(gamma64, b64, Ud[3], qn);
+    roe_eigensystem<DIM, DIR>(gamma, qt, qn, ev);
+    R minus[NE], plus[NE];
+#pragma unroll
+    for (int n = 0; n < NE; ++n) {
+      R uc[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        R s = R(0.0);
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+          int const l = seq<DIM, DIR>(q);
+          s += ev.L[n][l] * U[m][l];
+        }
+        uc[m] = s;
+      }
+      minus[n] = weno_left(uc);
+      plus[n] = weno_right(uc);
+    }
+    R cl[NE], cr[NE];
+    to_physical<NE>(minus, ev.Rm, cl);
+    to_physical<NE>(plus, ev.Rm, cr);
+    R const eps = R(2.220446049250313080847263336181640625e-16);  // DBL_EPSILON
+    if (cl[kMass] <= eps || cr[kMass] <= eps) return;
+    Prim<R> pl, pr;
+    cons_to_prim<DIM>(gamma, b, cl, pl);
+    cons_to_prim<DIM>(gamma, b, cr, pr);
+    if (pl.p <= -b || pr.p <= -b) return;
+    R hf[NE];
+    hllc<DIM, DIR>(gamma, b, cl, cr, pl, pr, hf);
+#pragma unroll
+    for (int e = 0; e < NE; ++e) f[e] += hf[e];
+  } else {
+    Prim<R> P[6];
+    CellQ<R> qt, qn;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      Prim<double> pd;
+      cons_to_prim<DIM, double>(gamma64, b64, Ud[m], pd);
+      P[m].rho = R(pd.rho);
+      P[m].p = R(pd.p);
+      P[m].vel[0] = R(pd.vel[0]);
+      P[m].vel[1] = R(pd.vel[1]);
+      P[m].vel[2] = R(pd.vel[2]);
+    }
+    cell_quantities<DIM, R>(gamma64, b64, Ud[2], qt);
+    cell_quantities<DIM, R>(gamma64, b64, Ud[3], qn);
+    roe_eigensystem<DIM, DIR>(gamma, qt, qn, ev);
+    R adv[6][NE];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      R const rho = P[m].rho, p = P[m].p, ud = P[m].vel[DIR];
+      adv[m][kMass] = rho * ud;
+      adv[m][kEnergy] = (U[m][kEnergy] + p) * ud;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) adv[m][kMomX + d] = rho * (ud * P[m].vel[d]);
+      adv[m][kMomX + DIR] += p;
+    }
+    R chf[NE];
+#pragma unroll
+    for (int n = 0; n < NE; ++n) {
+      R pos[6], neg[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        R sp = R(0.0), sn = R(0.0);
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+          int const l = seq<DIM, DIR>(q);
+          R const uct = U[m][l] * ev.L[n][l];
+          R const act = adv[m][l] * ev.L[n][l];
+          sp += (act + ev.lam[n] * uct);
+          sn += (act - ev.lam[n] * uct);
+        }
+        pos[m] = sp;
+        neg[m] = sn;
+      }
+      chf[n] = R(0.5) * (weno_left(pos) + weno_right(neg));
+    }
+    R phys[NE];
+    to_physical<NE>(chf, ev.Rm, phys);
+#pragma unroll
+    for (int e = 0; e < NE; ++e) f[e] += phys[e];
+  }
+}
+
+// Loads the six-cell stencil around the face (i,j,k)+1/2 along DIR (halo
+// indices of the cell left of the face) and computes its flux.
+template <int DIM, int DIR, int FLUX, typename R>
+__device__ __forceinline__ void stencil_flux(const double *__restrict__ blk, int i, int j, int k,
+                                             double gamma, double b, R *f) {
+  using D = Dims<DIM>;
+  constexpr int NE = D::NE;
+  constexpr int s = axis_stride<DIM, DIR>();
+  const double *base = blk + i * D::SX + j * D::SY + k;
+  double U[6][NE];
+#pragma unroll
+  for (int m = 0; m < 6; ++m)
+#pragma unroll
+    for (int e = 0; e < NE; ++e) U[m][e] = base[e * D::FIELD + (m - 2) * s];
+  face_flux<DIM, DIR, FLUX, R>(gamma, b, U, f);
+}

@@ -1,72 +1,24 @@
-// Fused SSP-RK3 stage kernel for gfx950.
+// Fused SSP-RK3 stage kernel, included into alpaca::exact and alpaca::fast
+// (see fv_device.h).
 //
 // One workgroup per (leaf block, tile of TX x TY x TZ internal cells):
 //   1. every face of the tile (all directions) is computed from the haloed
 //      Average buffer — a flat loop whose per-direction ranges are padded to
 //      64 so each wavefront works on a single direction — and its flux is
-//      kept in LDS;
+//      kept in LDS (in R: FP32 in the mixed form);
 //   2. after one barrier each thread forms the flux divergence of its cells
-//      with the reference's ConsistencyManagedSum (space_solver.cpp:131-153),
-//      applies the RK combination (time_integrator.h:98-113, 434-457), writes
-//      the new state and pushes it into the face halo slabs of the
-//      neighbouring blocks (internal_halo_manager.cpp:439-466) or, on a
+//      in FP64 with the reference's ConsistencyManagedSum (space_solver.cpp:
+//      131-153), applies the RK combination in FP64 (time_integrator.h:98-113,
+//      434-457), writes the new state and pushes it into the face halo slabs
+//      of the neighbouring blocks (internal_halo_manager.cpp:439-466) or, on a
 //      zero-gradient side, into its own halo (boundary_constants.h:212-253);
 //   3. on the last stage it also reduces the CFL signal of the new state
 //      (modular_algorithm_assembler.cpp:1840-1869) for the next time step.
-// The face numerics are fv_device.h's, so results equal the unfused kernels
-// bit for bit.
-#pragma once
-
-#include "fv_device.h"
-
-namespace alpaca {
-
-constexpr int round_up64(int n) { return (n + 63) / 64 * 64; }
-
-// Logical workgroup index with consecutive logical groups kept on one XCD:
-// the hardware deals workgroups round-robin over the 8 XCDs, so group g runs
-// on XCD g % 8.  Remapping makes each XCD sweep a contiguous range of blocks,
-// which keeps a block's stencil data in one XCD's L2.
-__device__ __forceinline__ unsigned xcd_remap_group(unsigned g, unsigned n) {
-  constexpr unsigned kXcd = 8;
-  unsigned const per = n / kXcd;
-  unsigned const full = per * kXcd;
-  if (g >= full) return g;
-  return (g % kXcd) * per + g / kXcd;
-}
-
-struct StageArgs {
-  int nb[3];
-  int bc[3];
-  double gamma, b;
-  double inv_dx;
-  int nblocks;
-  int stage;
-  int signal;  // reduce the CFL signal of the new state into state->smax
-  double *init_copy;  // stage 0: also copy the Average buffer here (or null)
-};
 
 __device__ __forceinline__ double wave_max64(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = smax(v, __shfl_xor(v, o, 64));
   return v;
-}
-
-// Loads the six-cell stencil around the face (i,j,k)+1/2 along DIR (halo
-// indices of the cell left of the face) and computes its flux.
-template <int DIM, int DIR, int FLUX, typename R>
-__device__ __forceinline__ void stencil_flux(const double *__restrict__ blk, int i, int j, int k,
-                                             R gamma, R b, R *f) {
-  using D = Dims<DIM>;
-  constexpr int NE = D::NE;
-  constexpr int s = axis_stride<DIM, DIR>();
-  const double *base = blk + i * D::SX + j * D::SY + k;
-  R U[6][NE];
-#pragma unroll
-  for (int m = 0; m < 6; ++m)
-#pragma unroll
-    for (int e = 0; e < NE; ++e) U[m][e] = R(base[e * D::FIELD + (m - 2) * s]);
-  face_flux<DIM, DIR, FLUX, R>(gamma, b, U, f);
 }
 
 // Writes value v of field e at internal cell c3 of block blk into the halo
@@ -128,10 +80,10 @@ __global__ __launch_bounds__(NT) void stage_kernel(const double *__restrict__ av
   constexpr int PX = round_up64(NFX), PY = round_up64(NFY), PZ = round_up64(NFZ);
   constexpr int CELLS = TX * TY * TZe;
   static_assert(NT % 64 == 0, "whole wavefronts");
-  __shared__ double lds[NE * (NFX + NFY + NFZ)];
-  double *FX = lds;
-  double *FY = lds + NE * NFX;
-  double *FZ = FY + NE * NFY;
+  __shared__ R lds[NE * (NFX + NFY + NFZ)];
+  R *FX = lds;
+  R *FY = lds + NE * NFX;
+  R *FZ = FY + NE * NFY;
 
   unsigned const g = xcd_remap_group(blockIdx.x, gridDim.x);
   int const blk = g / TILES;
@@ -141,7 +93,6 @@ __global__ __launch_bounds__(NT) void stage_kernel(const double *__restrict__ av
   int const y0 = ((tile / NTZ) % NTY) * TY;
   int const x0 = (tile / (NTZ * NTY)) * TX;
   const double *bb = avg + (size_t)blk * D::BLOCK;
-  R const gamma = R(a.gamma), b = R(a.b);
   int const zh = DIM == 3 ? kHS : 0;  // halo offset along z
 
   // 1. Face fluxes of the tile into LDS.
@@ -150,26 +101,26 @@ __global__ __launch_bounds__(NT) void stage_kernel(const double *__restrict__ av
     if (f < PX) {
       if (f < NFX) {
         int const k = f % TZe, j = (f / TZe) % TY, ia = f / (TZe * TY);
-        stencil_flux<DIM, 0, FLUX, R>(bb, x0 + ia + kHS - 1, y0 + j + kHS, z0 + k + zh, gamma, b, fl);
+        stencil_flux<DIM, 0, FLUX, R>(bb, x0 + ia + kHS - 1, y0 + j + kHS, z0 + k + zh, a.gamma, a.b, fl);
 #pragma unroll
-        for (int e = 0; e < NE; ++e) FX[e * NFX + f] = double(fl[e]);
+        for (int e = 0; e < NE; ++e) FX[e * NFX + f] = fl[e];
       }
     } else if (f < PX + PY) {
       int const r = f - PX;
       if (r < NFY) {
         int const k = r % TZe, ja = (r / TZe) % (TY + 1), i = r / (TZe * (TY + 1));
-        stencil_flux<DIM, 1, FLUX, R>(bb, x0 + i + kHS, y0 + ja + kHS - 1, z0 + k + zh, gamma, b, fl);
+        stencil_flux<DIM, 1, FLUX, R>(bb, x0 + i + kHS, y0 + ja + kHS - 1, z0 + k + zh, a.gamma, a.b, fl);
 #pragma unroll
-        for (int e = 0; e < NE; ++e) FY[e * NFY + r] = double(fl[e]);
+        for (int e = 0; e < NE; ++e) FY[e * NFY + r] = fl[e];
       }
     } else {
       if constexpr (DIM == 3) {
         int const r = f - PX - PY;
         if (r < NFZ) {
           int const ka = r % (TZ + 1), j = (r / (TZ + 1)) % TY, i = r / ((TZ + 1) * TY);
-          stencil_flux<DIM, 2, FLUX, R>(bb, x0 + i + kHS, y0 + j + kHS, z0 + ka + kHS - 1, gamma, b, fl);
+          stencil_flux<DIM, 2, FLUX, R>(bb, x0 + i + kHS, y0 + j + kHS, z0 + ka + kHS - 1, a.gamma, a.b, fl);
 #pragma unroll
-          for (int e = 0; e < NE; ++e) FZ[e * NFZ + r] = double(fl[e]);
+          for (int e = 0; e < NE; ++e) FZ[e * NFZ + r] = fl[e];
         }
       }
     }
@@ -194,14 +145,14 @@ __global__ __launch_bounds__(NT) void stage_kernel(const double *__restrict__ av
     double v[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
-      double const dX = FX[e * NFX + (ii * TY + jj) * TZe + kk] -
-                        FX[e * NFX + ((ii + 1) * TY + jj) * TZe + kk];
-      double const dY = FY[e * NFY + (ii * (TY + 1) + jj) * TZe + kk] -
-                        FY[e * NFY + (ii * (TY + 1) + jj + 1) * TZe + kk];
+      double const dX = double(FX[e * NFX + (ii * TY + jj) * TZe + kk]) -
+                        double(FX[e * NFX + ((ii + 1) * TY + jj) * TZe + kk]);
+      double const dY = double(FY[e * NFY + (ii * (TY + 1) + jj) * TZe + kk]) -
+                        double(FY[e * NFY + (ii * (TY + 1) + jj + 1) * TZe + kk]);
       double dZ = 0.0;
       if constexpr (DIM == 3)
-        dZ = FZ[e * NFZ + (ii * TY + jj) * (TZ + 1) + kk] -
-             FZ[e * NFZ + (ii * TY + jj) * (TZ + 1) + kk + 1];
+        dZ = double(FZ[e * NFZ + (ii * TY + jj) * (TZ + 1) + kk]) -
+             double(FZ[e * NFZ + (ii * TY + jj) * (TZ + 1) + kk + 1]);
       double const rhs = 0.0 + (0.0 + dams<DIM>(dX, dY, dZ) * a.inv_dx);
       size_t const idx = cell + (size_t)e * D::FIELD;
       double const u = avg[idx];
@@ -234,4 +185,3 @@ __global__ __launch_bounds__(NT) void stage_kernel(const double *__restrict__ av
   if (bad) atomicOr(&state->flags, 1);
 }
 
-}  // namespace alpaca

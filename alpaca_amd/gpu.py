@@ -17,11 +17,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libalpaca_hip.so")
 
 FLUX_HLLC, FLUX_ROE = 0, 1
-FP64, MIXED = 0, 1
+FP64, MIXED, FP64_FMA = 0, 1, 2
 BC_ZERO_GRADIENT, BC_PERIODIC = 0, 1
 IC, HS, TC = 16, 4, 24
 
-KERNEL_FACE_FLUX, KERNEL_COMBINE, KERNEL_HALO, KERNEL_TIMESTEP = 0, 1, 2, 3
+KERNEL_FACE_FLUX, KERNEL_COMBINE, KERNEL_HALO, KERNEL_TIMESTEP, KERNEL_STAGE = 0, 1, 2, 3, 4
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "non-finite state",
           4: "unsupported configuration", 5: "capacity exhausted"}

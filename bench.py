@@ -32,7 +32,10 @@ METRIC = ("cell-updates/s WENO5-HLLC-RK3, FP64 vs mixed at stated L1 err; "
 # Algorithmic FLOP per cell face and per cell-update, counted from the
 # reference code (+ - * / sqrt = 1; min/max/abs/compare = 0).
 FLOP_PER_FACE = {(3, 0): 1476.0, (3, 1): 1996.0, (2, 0): 1102.0, (2, 1): 1405.0}
+# Per cell and RK stage: D faces + divergence + RK combination + cell primes.
+FLOP_PER_CELL_STAGE = {(3, 0): 4549.0, (3, 1): 6109.0, (2, 0): 2257.0, (2, 1): 2863.0}
 FLOP_PER_CELL_UPDATE = {(3, 0): 13.7e3, (3, 1): 18.3e3, (2, 0): 6.8e3, (2, 1): 8.6e3}
+PRECISIONS = {"fp64": 0, "mixed": 1, "fp64-fma": 2}
 PEAK_FP64 = 78.6e12   # MI355X FP64 vector/matrix, FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X FP32 vector (packed), FLOP/s
 PEAK_HBM = 8.0e12     # bytes/s
@@ -82,42 +85,22 @@ def load_traffic(kernel_tag: str):
         return None
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--flux", choices=["hllc", "roe"], default="hllc")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--blocks", type=int, default=16, help="leaf blocks per side (16 -> 256^3)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding of the block grid is a design note, not a run mode.",
-              file=sys.stderr)
-        sys.exit(2)
-
+def run_mode(args, flux, precision, nb, keep_state=False):
+    """Time args.steps macro steps of one arithmetic mode; returns a dict."""
     import torch
 
     from alpaca_amd import cases, gpu
 
-    torch.cuda.set_device(0)
-    flux = 0 if args.flux == "hllc" else 1
-    prec = gpu.FP64 if args.precision == "fp64" else gpu.MIXED
-    nb = args.blocks
-    n = nb * 16
+    prec = PRECISIONS[precision]
     desc = gpu.make_desc(3, [nb, nb, nb], [gpu.BC_PERIODIC] * 3, flux=flux, precision=prec)
     ic = cases.blobs_shock_3d_torch(nb, seed=0)
     case = gpu.GpuCase(desc, ic, cfl=0.5)
     del ic
     torch.cuda.synchronize()
-
     case.advance(args.warmup)
     torch.cuda.synchronize()
     gpu.profile_enable(True)
-    for k in range(gpu.KERNEL_TIMESTEP + 1):
+    for k in range(gpu.KERNEL_STAGE + 1):
         gpu.profile_read(k, reset=True)
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
@@ -130,22 +113,70 @@ def main():
     gpu.profile_enable(False)
     rec = case.record()
     if rec.flags & 1:
-        raise SystemExit("bench: non-finite state")
+        raise SystemExit(f"bench: non-finite state ({precision})")
+    cells = (nb * 16) ** 3
+    stage_ms, stage_launches, stage_cells = gpu.profile_read(gpu.KERNEL_STAGE)
+    ts_ms, ts_launches, _ = gpu.profile_read(gpu.KERNEL_TIMESTEP)
+    out = {
+        "value": cells * args.steps / elapsed,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "stage_launches": stage_launches,
+        "stage_avg_ms": stage_ms / max(stage_launches, 1),
+        "cells_per_launch": stage_cells / max(stage_launches, 1),
+        "timestep_ms_total": ts_ms,
+        "time": rec.time,
+        "steps_total": rec.steps,
+    }
+    if keep_state:
+        out["rho"] = gpu.from_blocks(case.u, 3, case.nb)[0].clone()
+        out["sums"] = gpu.from_blocks(case.u, 3, case.nb).sum(dim=(1, 2, 3)).cpu().tolist()
+    del case
+    torch.cuda.empty_cache()
+    return out
 
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--flux", choices=["hllc", "roe"], default="hllc")
+    ap.add_argument("--precision", choices=sorted(PRECISIONS), default="fp64")
+    ap.add_argument("--blocks", type=int, default=16, help="leaf blocks per side (16 -> 256^3)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-modes", action="store_true",
+                    help="time only --precision (skip the other arithmetic modes)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding of the block grid is a design note, not a run mode.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    torch.cuda.set_device(0)
+    flux = 0 if args.flux == "hllc" else 1
+    nb = args.blocks
+    n = nb * 16
     cells = n ** 3
-    value = cells * args.steps / elapsed
-    face_ms, face_launches, faces = gpu.profile_read(gpu.KERNEL_FACE_FLUX)
-    parts = {}
-    for name, k in (("combine", gpu.KERNEL_COMBINE), ("halo", gpu.KERNEL_HALO),
-                    ("timestep", gpu.KERNEL_TIMESTEP)):
-        ms, cnt, _ = gpu.profile_read(k)
-        parts[name] = ms / max(cnt, 1)
-    avg_launch_s = face_ms / 1e3 / max(face_launches, 1)
-    faces_per_launch = faces / max(face_launches, 1)
-    flop_face = FLOP_PER_FACE[(3, flux)]
-    achieved = faces_per_launch * flop_face / avg_launch_s
-    peak = PEAK_FP64 if prec == gpu.FP64 else PEAK_FP32
-    tag = f"face_flux_3d_{args.flux}_{args.precision}"
+    modes = [args.precision] if args.no_modes else ["fp64", "fp64-fma", "mixed"]
+    if args.precision not in modes:
+        modes.insert(0, args.precision)
+    results = {}
+    for m in modes:
+        results[m] = run_mode(args, flux, m, nb, keep_state=not args.no_modes)
+    head = results[args.precision]
+    value = head["value"]
+
+    flop_cell_stage = FLOP_PER_CELL_STAGE[(3, flux)]
+    avg_launch_s = head["stage_avg_ms"] / 1e3
+    achieved = head["cells_per_launch"] * flop_cell_stage / avg_launch_s
+    peak = PEAK_FP32 if args.precision == "mixed" else PEAK_FP64
+    tag = f"stage_3d_{args.flux}_{args.precision}"
+    variant = {"fp64": "exact::", "fp64-fma": "fast::", "mixed": "fast::"}[args.precision]
+    rtype = "float" if args.precision == "mixed" else "double"
     roofline = {
         "bound": "mfma",
         "achieved": achieved / 1e12,
@@ -153,13 +184,12 @@ def main():
         "unit": "TFLOP/s",
         "frac": achieved / peak,
         "traffic": load_traffic(tag),
-        "kernel": f"face_flux_kernel<3,DIR,{'HLLC' if flux == 0 else 'Roe'}>",
-        "note": (f"compute-bound FP64 VALU stencil (no contraction maps to MFMA); "
-                 f"{flop_face:.0f} algorithmic FLOP/face x {faces_per_launch:.0f} faces/launch, "
-                 f"avg launch {avg_launch_s * 1e3:.3f} ms; whole step "
-                 f"{value * FLOP_PER_CELL_UPDATE[(3, flux)] / peak:.3f} of peak; "
-                 f"per-launch ms combine {parts['combine']:.3f} halo {parts['halo']:.3f} "
-                 f"timestep {parts['timestep']:.3f}"),
+        "kernel": f"{variant}stage_kernel<3,{'HLLC' if flux == 0 else 'Roe'},{rtype},8,8,8,256>",
+        "note": (f"compute-bound VALU stencil (no contraction maps to MFMA; peak is the "
+                 f"{'FP32' if args.precision == 'mixed' else 'FP64'} vector figure); "
+                 f"{flop_cell_stage:.0f} algorithmic FLOP per cell and RK stage x "
+                 f"{head['cells_per_launch']:.0f} cells per launch, avg launch "
+                 f"{head['stage_avg_ms']:.3f} ms over {head['stage_launches']} launches"),
     }
     line = {
         "metric": METRIC,
@@ -168,16 +198,16 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == gpu.FP64 else "f32+f64",
+        "dtype": "f64" if args.precision != "mixed" else "f32+f64",
         "data": "synthetic",
         "config": {
             "workload": (f"3D {n}^3 periodic blob+Mach-2 shock, WENO5-JS char + "
                          f"{'HLLC(Einfeldt)' if flux == 0 else 'Roe flux splitting'} + SSP-RK3, "
-                         f"CFL 0.5, {args.precision.upper()}"),
+                         f"CFL 0.5, {args.precision}"),
             "cells": cells,
             "leaf_blocks": nb ** 3,
             "block": "16^3 internal + 4 halo",
@@ -186,6 +216,24 @@ def main():
         },
         "roofline": roofline,
     }
+    if len(results) > 1:
+        summary = {}
+        for m, r in results.items():
+            summary[m] = {"cell_updates_per_s": r["value"], "ms_per_step": r["ms_per_step"],
+                          "stage_avg_ms": r["stage_avg_ms"]}
+        ref = results.get("fp64")
+        if ref is not None and "rho" in ref:
+            for m, r in results.items():
+                if m == "fp64" or "rho" not in r:
+                    continue
+                d = (r["rho"] - ref["rho"]).abs()
+                summary[m]["l1_rho_err_vs_fp64"] = float(d.sum() / ref["rho"].abs().sum())
+                summary[m]["linf_rho_err_vs_fp64"] = float(d.max() / ref["rho"].abs().max())
+                summary[m]["mass_drift"] = abs(r["sums"][0] - ref["sums"][0]) / abs(ref["sums"][0])
+                summary[m]["energy_drift"] = abs(r["sums"][1] - ref["sums"][1]) / abs(ref["sums"][1])
+                summary[m]["speedup_vs_fp64"] = r["value"] / ref["value"]
+            summary["compared_after_steps"] = args.warmup + args.steps
+        line["modes"] = summary
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(flux, cpu_threads())
     print(json.dumps(line), flush=True)

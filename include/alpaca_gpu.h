@@ -43,10 +43,15 @@ typedef enum {
  * (riemann_solver_settings.h:29-54). */
 enum { ALPACA_FLUX_HLLC = 0, ALPACA_FLUX_ROE = 1 };
 
-/* Arithmetic of the reconstruction and Riemann part.  FP64 everywhere, or
- * mixed: FP32 reconstruction/fluxes, FP64 RK accumulation, pressure recovery
- * and sums. */
-enum { ALPACA_FP64 = 0, ALPACA_MIXED = 1 };
+/* Arithmetic of the reconstruction and Riemann part:
+ *   ALPACA_FP64      FP64, a*b+c never fused: reproduces the reference's x86
+ *                    build (no FMA) bit for bit;
+ *   ALPACA_FP64_FMA  FP64 with fused multiply-add contraction;
+ *   ALPACA_MIXED     FP32 reconstruction and Riemann fluxes (hardware
+ *                    reciprocal and square root), FP64 pressure recovery
+ *                    p = (gamma-1)(E - |m|^2/(2 rho)), FP64 flux divergence,
+ *                    RK accumulation and sums. */
+enum { ALPACA_FP64 = 0, ALPACA_MIXED = 1, ALPACA_FP64_FMA = 2 };
 
 /* Side condition per direction: zero-gradient (ZeroGradientBoundaryCondition,
  * zero_gradient_boundary_condition.h:45-84) or periodic (the neighbour wraps,
@@ -86,7 +91,8 @@ enum {
   ALPACA_KERNEL_COMBINE = 1,   /* flux divergence + RK stage combination */
   ALPACA_KERNEL_HALO = 2,      /* same-level halo fill + boundary conditions */
   ALPACA_KERNEL_TIMESTEP = 3,  /* CFL reduction */
-  ALPACA_KERNEL_COUNT = 4
+  ALPACA_KERNEL_STAGE = 4,     /* fused stage: faces + divergence + RK + halo */
+  ALPACA_KERNEL_COUNT = 5
 };
 
 /* Measurement hooks (no counterpart in the reference, which times macro steps
@@ -94,7 +100,8 @@ enum {
  * by HIP events on its launch stream.  alpaca_profile_read synchronises on
  * the recorded events and returns the summed milliseconds, the number of
  * launches and the algorithmic work units launched (faces for FACE_FLUX,
- * cells for COMBINE and TIMESTEP, halo cells for HALO); reset != 0 clears. */
+ * cells for COMBINE, TIMESTEP and STAGE, halo cells for HALO); reset != 0
+ * clears. */
 int alpaca_profile_enable(int on);
 int alpaca_profile_read(int kernel, int reset, double *total_ms, long long *launches,
                         double *units);

@@ -222,3 +222,42 @@ def test_conservation_256_periodic_100_steps(torch, flux):
     drift = np.abs(s1 - s0) / scale
     print(f"  drift per field {drift}")
     assert np.all(drift <= 1e-13)
+
+
+def test_fp64_fma_within_bound(torch):
+    """FP64 with FMA contraction: not bit-exact, but within the FP64 bound
+    (max error <= 1e-10 of the field maximum) after 100 steps, 3D periodic."""
+    print("[fp64-fma 3d]", flush=True)
+    nb = [2, 2, 2]
+    state0 = cases.blobs_shock_3d(2, seed=1)
+    for flux in (gpu.FLUX_HLLC, gpu.FLUX_ROE):
+        desc = gpu.make_desc(3, nb, [1, 1, 1], flux=flux, precision=gpu.FP64_FMA)
+        case = gpu.GpuCase(desc, state0, cfl=0.5)
+        case.advance(100)
+        got = case.download()
+        want, _, _ = oracle_py.run(3, nb, [1, 1, 1], flux, 0.5, 1e300, 100, state0)
+        err = _maxrel(got, want)
+        print(f"  flux={flux} max rel err={err:.3e}")
+        assert err <= REL_TOL
+
+
+MIXED_L1_RHO_TOL = 1e-6  # L1 density error of the mixed form vs the FP64 oracle
+
+
+@pytest.mark.parametrize("flux", [gpu.FLUX_HLLC, gpu.FLUX_ROE])
+def test_mixed_density_error(torch, flux):
+    """Mixed form (FP32 faces, FP64 pressure recovery and RK): relative L1
+    density error vs the FP64 oracle after 100 steps, and mass drift."""
+    print(f"[mixed 3d] flux={flux}", flush=True)
+    nb = [2, 2, 2]
+    state0 = cases.blobs_shock_3d(2, seed=1)
+    desc = gpu.make_desc(3, nb, [1, 1, 1], flux=flux, precision=gpu.MIXED)
+    case = gpu.GpuCase(desc, state0, cfl=0.5)
+    case.advance(100)
+    got = case.download()
+    want, _, _ = oracle_py.run(3, nb, [1, 1, 1], flux, 0.5, 1e300, 100, state0)
+    l1 = float(np.abs(got[0] - want[0]).sum() / np.abs(want[0]).sum())
+    mass = abs(got[0].sum() - state0[0].sum()) / state0[0].sum()
+    print(f"  L1 rho err={l1:.3e} mass drift={mass:.3e}")
+    assert l1 <= MIXED_L1_RHO_TOL
+    assert mass <= 1e-13
