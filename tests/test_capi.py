@@ -1,0 +1,62 @@
+"""The C ABI library loads on a host without a GPU and exports every entry
+point declared in include/alpaca_gpu.h (no compute calls here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from alpaca_amd import gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "alpaca_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(alpaca_\w+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ("alpaca_convective_fluxes", "alpaca_right_hand_side", "alpaca_rk3_stage",
+                 "alpaca_fill_halos", "alpaca_timestep", "alpaca_advance"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(gpu.LIB_PATH):
+        pytest.fail("libalpaca_hip.so is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(gpu.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_header():
+    assert set(declared_functions()) <= set(gpu.exported_symbols())
+
+
+def test_host_only_entry_points():
+    lib = gpu.load_library()
+    assert lib.alpaca_abi_version() == 1
+    assert lib.alpaca_block_doubles(2) == 4 * 24 * 24
+    assert lib.alpaca_block_doubles(3) == 5 * 24 ** 3
+    assert lib.alpaca_block_doubles(4) == 0
+    assert lib.alpaca_status_string(3) == b"non-finite state"
+
+
+def test_invalid_descriptors_are_rejected_before_any_launch():
+    lib = gpu.load_library()
+    bad = gpu.make_desc(2, [2, 2], [0, 0])
+    bad.dim = 4
+    assert lib.alpaca_fill_halos(ctypes.byref(bad), ctypes.c_void_p(1), None) == 1
+    bad = gpu.make_desc(3, [2, 2, 2], [1, 1, 1])
+    bad.bc[1] = 7
+    assert lib.alpaca_timestep(ctypes.byref(bad), ctypes.c_void_p(1), ctypes.c_void_p(1),
+                               None) == 1
+    good = gpu.make_desc(3, [2, 2, 2], [1, 1, 1])
+    assert lib.alpaca_rk3_stage(ctypes.byref(good), 3, ctypes.c_void_p(1), None,
+                                ctypes.c_void_p(2), ctypes.c_void_p(3), None) == 1
+    assert lib.alpaca_advance(ctypes.byref(good), ctypes.c_void_p(1), ctypes.c_void_p(1),
+                              ctypes.c_void_p(2), ctypes.c_void_p(3), 1, None) == 1
