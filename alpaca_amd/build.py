@@ -56,6 +56,16 @@ def build_hip(verbose=False):
     return HIP_LIB
 
 
+def build_host_selftest(verbose=False):
+    """Command-line driver of the C++ host adapters (host code only)."""
+    src = os.path.join(CSRC, "host", "selftest.cpp")
+    out = os.path.join(PKG, "alpaca_selftest")
+    _run([HIPCC, "-O2", "-std=c++17", "-Wno-unused-command-line-argument", "-o", out + ".tmp",
+          src, "-L" + PKG, "-lalpaca_hip", "-Wl,-rpath,$ORIGIN"], verbose=verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle(verbose=False):
     make = shutil.which("make") or "make"
     _run([make, "-C", os.path.join(ROOT, "oracle"), "-j8"], verbose=verbose)
@@ -66,6 +76,7 @@ def build_oracle(verbose=False):
 
 def build_all(verbose=False):
     build_hip(verbose)
+    build_host_selftest(verbose)
     build_oracle(verbose)
 
 

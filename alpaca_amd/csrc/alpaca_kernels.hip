@@ -696,6 +696,13 @@ int alpaca_timestep(const alpaca_case_desc *desc, const double *d_avg, alpaca_ca
   return timestep_impl<3>(desc, d_avg, d_state, s);
 }
 
+int alpaca_finish_step(alpaca_case_state *d_state, void *stream) {
+  if (!d_state) return ALPACA_ERR_INVALID;
+  hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
+                     d_state);
+  return hip_status(hipGetLastError());
+}
+
 int alpaca_case_state_init(alpaca_case_state *d_state, double time, double t_end, double cfl,
                            void *stream) {
   if (!d_state) return ALPACA_ERR_INVALID;

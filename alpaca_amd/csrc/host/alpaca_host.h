@@ -1,0 +1,337 @@
+// Host C++ adapters over the C ABI (include/alpaca_gpu.h) that keep the call
+// shapes of ALPACA's convective-term solver and Runge-Kutta integrator, so a
+// CPU call site can switch to the gfx950 path without changing its
+// arguments.
+//
+//   GpuConvectiveTermSolver<DIM>  ConvectiveTermSolver<D>::UpdateConvectiveFluxes
+//                                 (convective_term_solver.h:67-81)
+//   GpuRungeKutta3TVD<DIM>        TimeIntegrator<RungeKutta3TVD> stage calls
+//                                 (time_integrator.h:240-526)
+//   GpuUniformSolver<DIM>         ModularAlgorithmAssembler::ComputeLoop /
+//                                 Advance on a uniform level 0
+//                                 (modular_algorithm_assembler.cpp:97-195, 543-910)
+//
+// Errors: the C ABI returns status codes; these adapters turn a non-OK status
+// into AlpacaGpuError (the reference path has no error returns and traps FP
+// faults instead).  Device memory is owned by the adapters (RAII).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../../include/alpaca_gpu.h"
+
+namespace alpaca_host {
+
+class AlpacaGpuError : public std::runtime_error {
+ public:
+  AlpacaGpuError(int status, std::string const &what)
+      : std::runtime_error(what + ": " + alpaca_status_string(status)), status_(status) {}
+  int status() const { return status_; }
+
+ private:
+  int status_;
+};
+
+inline void Check(int status, char const *what) {
+  if (status != ALPACA_OK) throw AlpacaGpuError(status, what);
+}
+inline void CheckHip(hipError_t e, char const *what) {
+  if (e != hipSuccess) throw AlpacaGpuError(ALPACA_ERR_HIP, std::string(what) + " (" + hipGetErrorString(e) + ")");
+}
+
+// Compile-time block geometry of the reference (compile_time_constants.h:39-44).
+template <int DIM> struct Geometry {
+  static constexpr int IC = 16, HS = 4, TC = 24;
+  static constexpr int NE = DIM + 2;                 // MF::ANOE()
+  static constexpr int TCZ = DIM == 3 ? TC : 1;      // CC::TCZ()
+  static constexpr int ICZ = DIM == 3 ? IC : 1;      // CC::ICZ()
+  static constexpr std::size_t kBlockDoubles = std::size_t(NE) * TC * TC * TCZ;
+  using Conservatives = double[NE][TC][TC][TCZ];           // FieldBuffer layout
+  using FaceFluxes = double[NE][IC + 1][IC + 1][ICZ + 1];  // face arrays
+  using VolumeForces = double[NE][IC][IC][ICZ];
+};
+
+// Device buffer with RAII.
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  explicit DeviceBuffer(std::size_t bytes) { Allocate(bytes); }
+  DeviceBuffer(DeviceBuffer const &) = delete;
+  DeviceBuffer &operator=(DeviceBuffer const &) = delete;
+  DeviceBuffer(DeviceBuffer &&o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) {
+    o.ptr_ = nullptr;
+    o.bytes_ = 0;
+  }
+  DeviceBuffer &operator=(DeviceBuffer &&o) noexcept {
+    std::swap(ptr_, o.ptr_);
+    std::swap(bytes_, o.bytes_);
+    return *this;
+  }
+  ~DeviceBuffer() {
+    if (ptr_) (void)hipFree(ptr_);
+  }
+  void Allocate(std::size_t bytes) {
+    if (bytes <= bytes_) return;
+    if (ptr_) CheckHip(hipFree(ptr_), "hipFree");
+    ptr_ = nullptr;
+    CheckHip(hipMalloc(&ptr_, bytes), "hipMalloc");
+    bytes_ = bytes;
+  }
+  template <typename T = void> T *get() const { return static_cast<T *>(ptr_); }
+  std::size_t bytes() const { return bytes_; }
+
+ private:
+  void *ptr_ = nullptr;
+  std::size_t bytes_ = 0;
+};
+
+inline alpaca_case_desc MakeDesc(int dim, int const nb[3], int const bc[3], int flux,
+                                 int precision, double gamma, double background_pressure,
+                                 double cell_size) {
+  alpaca_case_desc d{};
+  d.dim = dim;
+  for (int a = 0; a < 3; ++a) {
+    d.nb[a] = (a < dim) ? nb[a] : 1;
+    d.bc[a] = (a < dim) ? bc[a] : ALPACA_BC_ZERO_GRADIENT;
+  }
+  d.flux = flux;
+  d.precision = precision;
+  d.gamma = gamma;
+  d.background_pressure = background_pressure;
+  d.cell_size = cell_size;
+  return d;
+}
+
+// ConvectiveTermSolver<D>::UpdateConvectiveFluxes with the reference's
+// argument meaning: the block's Average conservatives (halos filled; prime
+// states are recovered from them), the cell size, and the three face-flux
+// arrays plus volume forces, which are ACCUMULATED into (+=) as
+// finite_volume_scheme.cpp:244-246 / flux_splitting_scheme.cpp:309-311 do.
+// Volume forces stay untouched (no convective volume term for these
+// equations).  The flux family is fixed at construction, as the reference's
+// Concretize<ConvectiveTermSolvers> selection fixes it at compile time.
+template <int DIM> class GpuConvectiveTermSolver {
+  using G = Geometry<DIM>;
+
+ public:
+  GpuConvectiveTermSolver(int flux_family, double gamma, double background_pressure,
+                          int precision = ALPACA_FP64, hipStream_t stream = nullptr)
+      : stream_(stream) {
+    int const nb[3] = {1, 1, 1};
+    int const bc[3] = {ALPACA_BC_ZERO_GRADIENT, ALPACA_BC_ZERO_GRADIENT,
+                       ALPACA_BC_ZERO_GRADIENT};
+    desc_ = MakeDesc(DIM, nb, bc, flux_family, precision, gamma, background_pressure, 1.0);
+    block_.Allocate(sizeof(typename G::Conservatives));
+    fluxes_.Allocate(3 * sizeof(typename G::FaceFluxes));
+  }
+
+  void UpdateConvectiveFluxes(typename G::Conservatives const &average, double cell_size,
+                              typename G::FaceFluxes &fluxes_x, typename G::FaceFluxes &fluxes_y,
+                              typename G::FaceFluxes &fluxes_z,
+                              typename G::VolumeForces &volume_forces) const {
+    (void)volume_forces;
+    alpaca_case_desc d = desc_;
+    d.cell_size = cell_size;
+    std::size_t const fb = sizeof(typename G::FaceFluxes);
+    CheckHip(hipMemcpyAsync(block_.get(), &average[0][0][0][0], sizeof(average),
+                            hipMemcpyHostToDevice, stream_), "upload block");
+    // Accumulate into the caller's arrays: start the device copy from them.
+    CheckHip(hipMemcpyAsync(fluxes_.get<char>(), &fluxes_x[0][0][0][0], fb,
+                            hipMemcpyHostToDevice, stream_), "upload fluxes");
+    CheckHip(hipMemcpyAsync(fluxes_.get<char>() + fb, &fluxes_y[0][0][0][0], fb,
+                            hipMemcpyHostToDevice, stream_), "upload fluxes");
+    CheckHip(hipMemcpyAsync(fluxes_.get<char>() + 2 * fb, &fluxes_z[0][0][0][0], fb,
+                            hipMemcpyHostToDevice, stream_), "upload fluxes");
+    Check(alpaca_convective_fluxes(&d, 1, block_.get<double>(), fluxes_.get<double>(), stream_),
+          "alpaca_convective_fluxes");
+    CheckHip(hipMemcpyAsync(&fluxes_x[0][0][0][0], fluxes_.get<char>(), fb,
+                            hipMemcpyDeviceToHost, stream_), "download fluxes");
+    CheckHip(hipMemcpyAsync(&fluxes_y[0][0][0][0], fluxes_.get<char>() + fb, fb,
+                            hipMemcpyDeviceToHost, stream_), "download fluxes");
+    CheckHip(hipMemcpyAsync(&fluxes_z[0][0][0][0], fluxes_.get<char>() + 2 * fb, fb,
+                            hipMemcpyDeviceToHost, stream_), "download fluxes");
+    CheckHip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  }
+
+ private:
+  alpaca_case_desc desc_;
+  hipStream_t stream_;
+  mutable DeviceBuffer block_, fluxes_;
+};
+
+// Device-resident leaf blocks of one uniform level-0 case: the Average,
+// RightHandSide and Initial conservative buffers of every Block
+// (block.h:57-60), block (bx,by,bz) at index (bx*nb[1]+by)*nb[2]+bz.
+template <int DIM> class DeviceBlocks {
+  using G = Geometry<DIM>;
+
+ public:
+  DeviceBlocks(alpaca_case_desc const &desc, hipStream_t stream = nullptr)
+      : desc_(desc), stream_(stream) {
+    nblocks_ = desc.nb[0] * desc.nb[1] * (DIM == 3 ? desc.nb[2] : 1);
+    std::size_t const bytes = sizeof(double) * G::kBlockDoubles * nblocks_;
+    average_.Allocate(bytes);
+    rhs_.Allocate(bytes);
+    initial_.Allocate(bytes);
+    state_.Allocate(sizeof(alpaca_case_state));
+    CheckHip(hipMemsetAsync(average_.get(), 0, bytes, stream_), "memset");
+    CheckHip(hipMemsetAsync(rhs_.get(), 0, bytes, stream_), "memset");
+    CheckHip(hipMemsetAsync(initial_.get(), 0, bytes, stream_), "memset");
+  }
+  int NumberOfBlocks() const { return nblocks_; }
+  std::size_t BlockDoubles() const { return G::kBlockDoubles; }
+  double *Average() const { return average_.get<double>(); }
+  double *RightHandSide() const { return rhs_.get<double>(); }
+  double *Initial() const { return initial_.get<double>(); }
+  alpaca_case_state *State() const { return state_.get<alpaca_case_state>(); }
+  alpaca_case_desc const &Desc() const { return desc_; }
+  hipStream_t Stream() const { return stream_; }
+  // SwapConservativeBuffers<RightHandSide, Average> (time_integrator.h:515-526).
+  void SwapRightHandSideAndAverage() { std::swap(average_, rhs_); }
+
+  // Upload/download the internal cells of a domain array [NE][NX][NY][NZ]
+  // (k fastest) through host-side block staging; halos are then filled.
+  void UploadDomain(std::vector<double> const &domain) {
+    std::vector<double> host(G::kBlockDoubles * nblocks_, 0.0);
+    ForEachInternal([&](std::size_t b, std::size_t g) { host[b] = domain[g]; });
+    CheckHip(hipMemcpyAsync(Average(), host.data(), host.size() * sizeof(double),
+                            hipMemcpyHostToDevice, stream_), "upload domain");
+    Check(alpaca_fill_halos(&desc_, Average(), stream_), "alpaca_fill_halos");
+    CheckHip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  }
+  std::vector<double> DownloadDomain() const {
+    std::vector<double> host(G::kBlockDoubles * nblocks_);
+    CheckHip(hipMemcpyAsync(host.data(), Average(), host.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, stream_), "download domain");
+    CheckHip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    std::vector<double> domain(DomainCells() * G::NE);
+    const_cast<DeviceBlocks *>(this)->ForEachInternal(
+        [&](std::size_t b, std::size_t g) { domain[g] = host[b]; });
+    return domain;
+  }
+  std::size_t DomainCells() const {
+    return std::size_t(desc_.nb[0]) * G::IC * desc_.nb[1] * G::IC * (DIM == 3 ? desc_.nb[2] * G::IC : 1);
+  }
+  alpaca_case_state ReadState() const {
+    alpaca_case_state s{};
+    CheckHip(hipMemcpyAsync(&s, State(), sizeof(s), hipMemcpyDeviceToHost, stream_), "read state");
+    CheckHip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    return s;
+  }
+
+ private:
+  template <typename F> void ForEachInternal(F const &f) {
+    std::size_t const NX = std::size_t(desc_.nb[0]) * G::IC, NY = std::size_t(desc_.nb[1]) * G::IC,
+                      NZ = DIM == 3 ? std::size_t(desc_.nb[2]) * G::IC : 1;
+    int const nbz = DIM == 3 ? desc_.nb[2] : 1;
+    for (int bx = 0; bx < desc_.nb[0]; ++bx)
+      for (int by = 0; by < desc_.nb[1]; ++by)
+        for (int bz = 0; bz < nbz; ++bz) {
+          std::size_t const b = (std::size_t(bx) * desc_.nb[1] + by) * nbz + bz;
+          for (int e = 0; e < G::NE; ++e)
+            for (int i = 0; i < G::IC; ++i)
+              for (int j = 0; j < G::IC; ++j)
+                for (int k = 0; k < G::ICZ; ++k) {
+                  std::size_t const bi =
+                      b * G::kBlockDoubles +
+                      ((std::size_t(e) * G::TC + i + G::HS) * G::TC + j + G::HS) * G::TCZ +
+                      (DIM == 3 ? k + G::HS : 0);
+                  std::size_t const gi =
+                      ((std::size_t(e) * NX + bx * G::IC + i) * NY + by * G::IC + j) * NZ +
+                      (DIM == 3 ? std::size_t(bz) * G::IC + k : 0);
+                  f(bi, gi);
+                }
+        }
+  }
+  alpaca_case_desc desc_;
+  hipStream_t stream_;
+  int nblocks_ = 0;
+  DeviceBuffer average_, rhs_, initial_, state_;
+};
+
+// TimeIntegrator<RungeKutta3TVD> over device blocks (time_integrator.h:
+// 240-526, runge_kutta_3_TVD.h:39-59).  The reference's per-node calls
+// FillInitialBuffer, UpdateFluxes, PrepareBufferForIntegration and
+// IntegrateNode of one stage become one fused device stage; the time step
+// is the reference's micro time step (AppendMicroTimestep) held in the
+// case's device record.
+template <int DIM> class GpuRungeKutta3TVD {
+ public:
+  static constexpr unsigned int number_of_stages_ = 3;
+  constexpr unsigned int NumberOfStages() const { return number_of_stages_; }
+  constexpr bool IsLastStage(unsigned int stage) const { return stage == number_of_stages_ - 1; }
+
+  // One stage: out(RHS buffer) = (m0 Average + m1 Initial) + c_s dt RHS(Average),
+  // including the halo fill of the RHS buffer.
+  void IntegrateStage(DeviceBlocks<DIM> &blocks, unsigned int stage) const {
+    if (stage >= number_of_stages_)
+      throw std::invalid_argument("Stage is too large for Runge-Kutta-3 integration");
+    Check(alpaca_rk3_stage(&blocks.Desc(), int(stage), blocks.Average(), blocks.Initial(),
+                           blocks.RightHandSide(), blocks.State(), blocks.Stream()),
+          "alpaca_rk3_stage");
+  }
+  void SwapBuffersForNextStage(DeviceBlocks<DIM> &blocks) const {
+    blocks.SwapRightHandSideAndAverage();
+  }
+};
+
+// Uniform level-0 ComputeLoop / Advance (modular_algorithm_assembler.cpp:
+// 97-195, 543-910) with the end-time clamp and the minimum-step health check.
+template <int DIM> class GpuUniformSolver {
+ public:
+  GpuUniformSolver(alpaca_case_desc const &desc, double cfl, double end_time,
+                   hipStream_t stream = nullptr)
+      : blocks_(desc, stream), cfl_(cfl), end_time_(end_time) {}
+
+  DeviceBlocks<DIM> &Blocks() { return blocks_; }
+
+  void Initialize(std::vector<double> const &domain_conservatives) {
+    blocks_.UploadDomain(domain_conservatives);
+    Check(alpaca_case_state_init(blocks_.State(), 0.0, end_time_, cfl_, blocks_.Stream()),
+          "alpaca_case_state_init");
+  }
+
+  // One macro step: ComputeTimestepSize, then the three stages each followed
+  // by the buffer swap (the stage kernels fill the halos of their output).
+  void Advance() {
+    Check(alpaca_timestep(&blocks_.Desc(), blocks_.Average(), blocks_.State(), blocks_.Stream()),
+          "alpaca_timestep");
+    for (unsigned int stage = 0; stage < integrator_.NumberOfStages(); ++stage) {
+      integrator_.IntegrateStage(blocks_, stage);
+      integrator_.SwapBuffersForNextStage(blocks_);
+    }
+    ++steps_;
+  }
+
+  // ComputeLoop: advance while time < end_time and the time step is healthy;
+  // returns the number of macro steps.  Reads the device record each step.
+  int ComputeLoop(int max_steps) {
+    int n = 0;
+    for (;;) {
+      alpaca_case_state s = blocks_.ReadState();
+      if (s.flags & 1) throw AlpacaGpuError(ALPACA_ERR_NONFINITE, "ComputeLoop");
+      if (!(s.time < end_time_) || (s.flags & 2) || n >= max_steps) break;
+      Advance();
+      // TimeIntegrator::FinishMacroTimestep (time_integrator.h:244-248).
+      Check(alpaca_finish_step(blocks_.State(), blocks_.Stream()), "alpaca_finish_step");
+      ++n;
+    }
+    return n;
+  }
+
+  std::vector<double> State() const { return blocks_.DownloadDomain(); }
+
+ private:
+  DeviceBlocks<DIM> blocks_;
+  GpuRungeKutta3TVD<DIM> integrator_;
+  double cfl_, end_time_;
+  int steps_ = 0;
+};
+
+}  // namespace alpaca_host

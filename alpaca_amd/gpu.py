@@ -59,6 +59,7 @@ _SIGNATURES = {
                           ctypes.c_int),
     "alpaca_timestep": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_void_p,
                          ctypes.c_void_p], ctypes.c_int),
+    "alpaca_finish_step": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "alpaca_case_state_init": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_void_p], ctypes.c_int),
     "alpaca_profile_enable": ([ctypes.c_int], ctypes.c_int),

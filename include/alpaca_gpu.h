@@ -158,6 +158,11 @@ int alpaca_fill_halos(const alpaca_case_desc *desc, double *d_blocks,
 int alpaca_timestep(const alpaca_case_desc *desc, const double *d_avg,
                     alpaca_case_state *d_state, void *stream);
 
+/* TimeIntegrator::FinishMacroTimestep (time_integrator.h:244-248) on the
+ * device record: time += dt, steps += 1.  alpaca_advance does this itself;
+ * callers that drive the stages one by one call it after the last stage. */
+int alpaca_finish_step(alpaca_case_state *d_state, void *stream);
+
 /* Initialise a device case record (time, t_end, cfl; counters zeroed). */
 int alpaca_case_state_init(alpaca_case_state *d_state, double time,
                            double t_end, double cfl, void *stream);

@@ -1,0 +1,76 @@
+"""The C++ host adapters (alpaca_amd/csrc/host/alpaca_host.h), exercised
+through their command-line driver, against the CPU oracle: the
+ConvectiveTermSolver-shaped per-block call and the uniform-level
+ComputeLoop/Advance built on the RungeKutta3TVD-shaped stage calls."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from alpaca_amd import cases, gpu
+import oracle_py
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "alpaca_amd", "alpaca_selftest")
+
+
+def _header(dim, nb, bc, flux, precision, max_steps, gamma, b, cell_size, cfl, end_time):
+    nb3 = list(nb) + [1] * (3 - len(nb))
+    bc3 = list(bc) + [0] * (3 - len(bc))
+    return struct.pack("<10i5d", dim, *nb3, *bc3, flux, precision, max_steps, gamma, b,
+                       cell_size, cfl, end_time)
+
+
+def _run(mode, payload, tmp_path):
+    if not os.path.exists(EXE):
+        pytest.fail("alpaca_selftest is not built (run __graft_entry__.build())")
+    inp = tmp_path / "in.bin"
+    out = tmp_path / "out.bin"
+    inp.write_bytes(payload)
+    res = subprocess.run([EXE, mode, str(inp), str(out)], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stderr
+    return out.read_bytes()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("flux", [0, 1])
+def test_convective_term_solver_adapter(tmp_path, dim, flux):
+    print(f"[host fluxes] dim={dim} flux={flux}", flush=True)
+    blk = cases.random_block(dim, 21 + flux)
+    raw = _run("fluxes", _header(dim, [1, 1, 1], [0, 0, 0], flux, 0, 0, 1.4, 0.0, 1.0 / 64,
+                                 0.0, 0.0) + blk.tobytes(), tmp_path)
+    got = np.frombuffer(raw, dtype=np.float64)
+    want = oracle_py.block_fluxes(dim, flux, 1.4, 0.0, blk, 1.0 / 64).ravel()
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("flux", [0, 1])
+def test_uniform_solver_adapter_2d(tmp_path, flux):
+    print(f"[host solver 2d] flux={flux}", flush=True)
+    nb = [4, 3]
+    st0 = cases.four_quadrant_2d(4, 3, seed=9)
+    raw = _run("run", _header(2, nb, [0, 0], flux, 0, 10000, 1.4, 0.0, 1.0 / 64, 0.6, 0.05) +
+               st0.tobytes(), tmp_path)
+    steps, t = struct.unpack("<id", raw[:12])
+    got = np.frombuffer(raw[12:], dtype=np.float64).reshape(st0.shape)
+    want, dts, tf = oracle_py.run(2, nb, [0, 0], flux, 0.6, 0.05, 10000, st0,
+                                  cell_size=1.0 / 64)
+    assert steps == len(dts) and t == tf
+    assert np.array_equal(got, want)
+
+
+def test_uniform_solver_adapter_3d_periodic(tmp_path):
+    print("[host solver 3d]", flush=True)
+    st0 = cases.blobs_shock_3d(2, seed=6)
+    raw = _run("run", _header(3, [2, 2, 2], [1, 1, 1], 0, 0, 10, 1.4, 0.0, 1.0 / 32, 0.5,
+                              1e300) + st0.tobytes(), tmp_path)
+    steps, _ = struct.unpack("<id", raw[:12])
+    got = np.frombuffer(raw[12:], dtype=np.float64).reshape(st0.shape)
+    want, _, _ = oracle_py.run(3, [2, 2, 2], [1, 1, 1], 0, 0.5, 1e300, 10, st0)
+    assert steps == 10
+    assert np.array_equal(got, want)
