@@ -28,15 +28,13 @@ HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-fPIC",
-    "-shared",
     "-std=c++17",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
 
-HIP_SOURCES = ["alpaca_kernels.hip"]
-
+HIP_SOURCES = ["alpaca_kernels.hip", "ensemble.hip"]
 
 def _run(cmd, cwd=None, verbose=False):
     if verbose:
@@ -49,9 +47,20 @@ def _run(cmd, cwd=None, verbose=False):
 
 
 def build_hip(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    """Compile every HIP translation unit (in parallel) and link the library."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    jobs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        jobs.append(([HIPCC, *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)], obj))
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        list(ex.map(lambda j: _run(j[0], verbose=verbose), jobs))
     tmp = HIP_LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-o", tmp, *srcs], verbose=verbose)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *[o for _, o in jobs]],
+         verbose=verbose)
     os.replace(tmp, HIP_LIB)
     return HIP_LIB
 

@@ -91,6 +91,50 @@ struct StageArgs {
   double *init_copy;  // stage 0: also copy the Average buffer here (or null)
 };
 
+// One tile of work for the fused stage: the case's buffers (block 0 of the
+// case at pool index `base`), its block grid and side conditions, and the
+// stage.  Shared by the single-case and the batched (persistent) kernels.
+struct TileJob {
+  const double *avg;
+  const double *init;
+  double *out;
+  alpaca_case_state *state;
+  double *init_copy;
+  int base;
+  int nb[3];
+  int bc[3];
+  double gamma, b, inv_dx;
+  int stage;
+  int signal;
+};
+
+// Device description of a case held in an ensemble pool.
+struct CaseSlot {
+  int nb[3];
+  int bc[3];
+  int base;      // pool index of the case's first block
+  int nblocks;
+  double gamma, b, inv_dx, cell_size;
+};
+
+// Ensemble case-record flags (alpaca_case_state::flags).
+enum : int {
+  kFlagNonFinite = 1,   // a non-finite value was written
+  kFlagUnhealthy = 2,   // time step below DBL_EPSILON
+  kFlagDone = 4,        // reached t_end
+  kFlagCancelled = 8,   // cancelled by the host
+  kFlagFresh = 16,      // dt already computed for the next step
+  kFlagActive = 32      // slot holds a case
+};
+__host__ __device__ constexpr int kFlagsStop() {
+  return kFlagNonFinite | kFlagUnhealthy | kFlagDone | kFlagCancelled;
+}
+
+struct WorkItem {
+  int case_id;
+  int block;  // block index within the case
+};
+
 namespace exact {
 #pragma clang fp contract(off)
 #include "fv_numerics.inl"

@@ -21,22 +21,23 @@ __device__ __forceinline__ double wave_max64(double v) {
   return v;
 }
 
-// Writes value v of field e at internal cell c3 of block blk into the halo
-// slabs that read it: the neighbours' slabs across each near face, or the
-// block's own slab on a zero-gradient domain side.
+// Writes value v of field e at internal cell c3 of local block bc3 of the
+// job's case into the halo slabs that read it: the neighbours' slabs across
+// each near face, or the block's own slab on a zero-gradient domain side.
 template <int DIM>
-__device__ __forceinline__ void push_halo(double *__restrict__ out, int blk, const int *bc3,
-                                          const int *c3, const StageArgs &a, int e, double v) {
+__device__ __forceinline__ void push_halo(const TileJob &j, int blk, const int *bc3, const int *c3,
+                                          int e, double v) {
   using D = Dims<DIM>;
+  double *__restrict__ out = j.out;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     int const c = c3[d];
     int pos[3] = {c3[0] + kHS, c3[1] + kHS, DIM == 3 ? c3[2] + kHS : 0};
     if (c < kHS) {
-      if (bc3[d] > 0 || a.bc[d] == ALPACA_BC_PERIODIC) {
+      if (bc3[d] > 0 || j.bc[d] == ALPACA_BC_PERIODIC) {
         int nb3[3] = {bc3[0], bc3[1], bc3[2]};
-        nb3[d] = bc3[d] > 0 ? bc3[d] - 1 : a.nb[d] - 1;
-        int const nbk = (nb3[0] * a.nb[1] + nb3[1]) * a.nb[2] + nb3[2];
+        nb3[d] = bc3[d] > 0 ? bc3[d] - 1 : j.nb[d] - 1;
+        int const nbk = j.base + (nb3[0] * j.nb[1] + nb3[1]) * j.nb[2] + nb3[2];
         pos[d] = c + kHS + kIC;
         out[(size_t)nbk * D::BLOCK + e * D::FIELD + pos[0] * D::SX + pos[1] * D::SY + pos[2]] = v;
       } else if (c == 0) {
@@ -48,10 +49,10 @@ __device__ __forceinline__ void push_halo(double *__restrict__ out, int blk, con
       }
     }
     if (c >= kIC - kHS) {
-      if (bc3[d] < a.nb[d] - 1 || a.bc[d] == ALPACA_BC_PERIODIC) {
+      if (bc3[d] < j.nb[d] - 1 || j.bc[d] == ALPACA_BC_PERIODIC) {
         int nb3[3] = {bc3[0], bc3[1], bc3[2]};
-        nb3[d] = bc3[d] < a.nb[d] - 1 ? bc3[d] + 1 : 0;
-        int const nbk = (nb3[0] * a.nb[1] + nb3[1]) * a.nb[2] + nb3[2];
+        nb3[d] = bc3[d] < j.nb[d] - 1 ? bc3[d] + 1 : 0;
+        int const nbk = j.base + (nb3[0] * j.nb[1] + nb3[1]) * j.nb[2] + nb3[2];
         pos[d] = c - kIC + kHS;
         out[(size_t)nbk * D::BLOCK + e * D::FIELD + pos[0] * D::SX + pos[1] * D::SY + pos[2]] = v;
       } else if (c == kIC - 1) {
@@ -65,34 +66,37 @@ __device__ __forceinline__ void push_halo(double *__restrict__ out, int blk, con
   }
 }
 
+template <int DIM, int TX, int TY, int TZ> struct TileShape {
+  static constexpr int TZe = DIM == 3 ? TZ : 1;
+  static constexpr int NTX = kIC / TX, NTY = kIC / TY, NTZ = DIM == 3 ? kIC / TZ : 1;
+  static constexpr int TILES = NTX * NTY * NTZ;
+  static constexpr int NFX = (TX + 1) * TY * TZe;
+  static constexpr int NFY = TX * (TY + 1) * TZe;
+  static constexpr int NFZ = DIM == 3 ? TX * TY * (TZ + 1) : 0;
+  static constexpr int PX = round_up64(NFX), PY = round_up64(NFY), PZ = round_up64(NFZ);
+  static constexpr int CELLS = TX * TY * TZe;
+  static constexpr int LDS = (DIM + 2) * (NFX + NFY + NFZ);  // reals
+};
+
+// One tile of one block: faces into LDS, barrier, divergence + RK + halo
+// push (+ CFL signal).  blk_local is the block's index within its case.
+// All threads of the workgroup must call it (it contains a barrier).
 template <int DIM, int FLUX, typename R, int TX, int TY, int TZ, int NT>
-__global__ __launch_bounds__(NT) void stage_kernel(const double *__restrict__ avg,
-                                                   const double *init, double *out,
-                                                   alpaca_case_state *state, StageArgs a) {
+__device__ __forceinline__ void stage_tile(const TileJob &j, int blk_local, int tile, R *lds) {
   using D = Dims<DIM>;
+  using T = TileShape<DIM, TX, TY, TZ>;
   constexpr int NE = D::NE;
-  constexpr int TZe = DIM == 3 ? TZ : 1;
-  constexpr int NTX = kIC / TX, NTY = kIC / TY, NTZ = DIM == 3 ? kIC / TZ : 1;
-  constexpr int TILES = NTX * NTY * NTZ;
-  constexpr int NFX = (TX + 1) * TY * TZe;
-  constexpr int NFY = TX * (TY + 1) * TZe;
-  constexpr int NFZ = DIM == 3 ? TX * TY * (TZ + 1) : 0;
-  constexpr int PX = round_up64(NFX), PY = round_up64(NFY), PZ = round_up64(NFZ);
-  constexpr int CELLS = TX * TY * TZe;
-  static_assert(NT % 64 == 0, "whole wavefronts");
-  __shared__ R lds[NE * (NFX + NFY + NFZ)];
+  constexpr int TZe = T::TZe, NTY = T::NTY, NTZ = T::NTZ;
+  constexpr int NFX = T::NFX, NFY = T::NFY, NFZ = T::NFZ;
+  constexpr int PX = T::PX, PY = T::PY, PZ = T::PZ;
   R *FX = lds;
   R *FY = lds + NE * NFX;
   R *FZ = FY + NE * NFY;
-
-  unsigned const g = xcd_remap_group(blockIdx.x, gridDim.x);
-  int const blk = g / TILES;
-  if (blk >= a.nblocks) return;  // uniform over the workgroup
-  int const tile = g % TILES;
+  int const blk = j.base + blk_local;
   int const z0 = DIM == 3 ? (tile % NTZ) * TZ : 0;
   int const y0 = ((tile / NTZ) % NTY) * TY;
   int const x0 = (tile / (NTZ * NTY)) * TX;
-  const double *bb = avg + (size_t)blk * D::BLOCK;
+  const double *bb = j.avg + (size_t)blk * D::BLOCK;
   int const zh = DIM == 3 ? kHS : 0;  // halo offset along z
 
   // 1. Face fluxes of the tile into LDS.
@@ -100,8 +104,9 @@ __global__ __launch_bounds__(NT) void stage_kernel(const double *__restrict__ av
     R fl[NE];
     if (f < PX) {
       if (f < NFX) {
-        int const k = f % TZe, j = (f / TZe) % TY, ia = f / (TZe * TY);
-        stencil_flux<DIM, 0, FLUX, R>(bb, x0 + ia + kHS - 1, y0 + j + kHS, z0 + k + zh, a.gamma, a.b, fl);
+        int const k = f % TZe, jj = (f / TZe) % TY, ia = f / (TZe * TY);
+        stencil_flux<DIM, 0, FLUX, R>(bb, x0 + ia + kHS - 1, y0 + jj + kHS, z0 + k + zh, j.gamma,
+                                      j.b, fl);
 #pragma unroll
         for (int e = 0; e < NE; ++e) FX[e * NFX + f] = fl[e];
       }
@@ -109,7 +114,8 @@ __global__ __launch_bounds__(NT) void stage_kernel(const double *__restrict__ av
       int const r = f - PX;
       if (r < NFY) {
         int const k = r % TZe, ja = (r / TZe) % (TY + 1), i = r / (TZe * (TY + 1));
-        stencil_flux<DIM, 1, FLUX, R>(bb, x0 + i + kHS, y0 + ja + kHS - 1, z0 + k + zh, a.gamma, a.b, fl);
+        stencil_flux<DIM, 1, FLUX, R>(bb, x0 + i + kHS, y0 + ja + kHS - 1, z0 + k + zh, j.gamma,
+                                      j.b, fl);
 #pragma unroll
         for (int e = 0; e < NE; ++e) FY[e * NFY + r] = fl[e];
       }
@@ -117,8 +123,9 @@ __global__ __launch_bounds__(NT) void stage_kernel(const double *__restrict__ av
       if constexpr (DIM == 3) {
         int const r = f - PX - PY;
         if (r < NFZ) {
-          int const ka = r % (TZ + 1), j = (r / (TZ + 1)) % TY, i = r / ((TZ + 1) * TY);
-          stencil_flux<DIM, 2, FLUX, R>(bb, x0 + i + kHS, y0 + j + kHS, z0 + ka + kHS - 1, a.gamma, a.b, fl);
+          int const ka = r % (TZ + 1), jj = (r / (TZ + 1)) % TY, i = r / ((TZ + 1) * TY);
+          stencil_flux<DIM, 2, FLUX, R>(bb, x0 + i + kHS, y0 + jj + kHS, z0 + ka + kHS - 1,
+                                        j.gamma, j.b, fl);
 #pragma unroll
           for (int e = 0; e < NE; ++e) FZ[e * NFZ + r] = fl[e];
         }
@@ -128,16 +135,17 @@ __global__ __launch_bounds__(NT) void stage_kernel(const double *__restrict__ av
   __syncthreads();
 
   // 2. Divergence + RK combination + halo push (+ CFL signal).
-  double const dt = state->dt;
-  int const stage = a.stage;
+  double const dt = j.state->dt;
+  int const stage = j.stage;
   double const m0 = stage == 0 ? 1.0 : (stage == 1 ? 0.25 : 2.0 / 3.0);
   double const m1 = stage == 0 ? 0.0 : (stage == 1 ? 0.75 : 1.0 / 3.0);
   double const cs = stage == 0 ? 1.0 : (stage == 1 ? 0.25 : 2.0 / 3.0);
   double const cdt = cs * (0.0 + dt);
-  int const bc3[3] = {blk / (a.nb[1] * a.nb[2]), (blk / a.nb[2]) % a.nb[1], blk % a.nb[2]};
+  int const bc3[3] = {blk_local / (j.nb[1] * j.nb[2]), (blk_local / j.nb[2]) % j.nb[1],
+                      blk_local % j.nb[2]};
   double sig = 0.0;
   bool bad = false;
-  for (int c = threadIdx.x; c < CELLS; c += NT) {
+  for (int c = threadIdx.x; c < T::CELLS; c += NT) {
     int const kk = c % TZe, jj = (c / TZe) % TY, ii = c / (TZe * TY);
     int const c3[3] = {x0 + ii, y0 + jj, z0 + kk};
     size_t const cell = (size_t)blk * D::BLOCK + (c3[0] + kHS) * D::SX + (c3[1] + kHS) * D::SY +
@@ -153,35 +161,121 @@ __global__ __launch_bounds__(NT) void stage_kernel(const double *__restrict__ av
       if constexpr (DIM == 3)
         dZ = double(FZ[e * NFZ + (ii * TY + jj) * (TZ + 1) + kk]) -
              double(FZ[e * NFZ + (ii * TY + jj) * (TZ + 1) + kk + 1]);
-      double const rhs = 0.0 + (0.0 + dams<DIM>(dX, dY, dZ) * a.inv_dx);
+      double const rhs = 0.0 + (0.0 + dams<DIM>(dX, dY, dZ) * j.inv_dx);
       size_t const idx = cell + (size_t)e * D::FIELD;
-      double const u = avg[idx];
-      if (stage == 0 && a.init_copy) a.init_copy[idx] = u;
-      double const base = stage == 0 ? u : m0 * u + m1 * init[idx];
+      double const u = j.avg[idx];
+      if (stage == 0 && j.init_copy) j.init_copy[idx] = u;
+      double const base = stage == 0 ? u : m0 * u + m1 * j.init[idx];
       v[e] = base + cdt * rhs;
-      out[idx] = v[e];
+      j.out[idx] = v[e];
       bad |= !__builtin_isfinite(v[e]);
-      push_halo<DIM>(out, blk, bc3, c3, a, e, v[e]);
+      push_halo<DIM>(j, blk, bc3, c3, e, v[e]);
     }
-    if (a.signal) {
-      Prim<double> p;
-      cons_to_prim<DIM>(a.gamma, a.b, v, p);
-      double const cso = eos_sos(a.gamma, a.b, p.rho, p.p);
+    if (j.signal) {
+      // The CFL signal is FP64 without contraction in every mode, so a step's
+      // dt does not depend on which kernel reduced the signal.
+      exact::Prim<double> p;
+      exact::cons_to_prim<DIM>(j.gamma, j.b, v, p);
+      double const cso = exact::eos_sos(j.gamma, j.b, p.rho, p.p);
       double sum;
       if constexpr (DIM == 2)
         sum = (__builtin_fabs(p.vel[0]) + cso) + (__builtin_fabs(p.vel[1]) + cso);
       else
-        sum = cms3(__builtin_fabs(p.vel[0]) + cso, __builtin_fabs(p.vel[1]) + cso,
-                   __builtin_fabs(p.vel[2]) + cso);
+        sum = exact::cms3(__builtin_fabs(p.vel[0]) + cso, __builtin_fabs(p.vel[1]) + cso,
+                          __builtin_fabs(p.vel[2]) + cso);
       sig = smax(sig, sum);
     }
   }
-  if (a.signal) {
+  if (j.signal) {
     sig = wave_max64(sig);
     if ((threadIdx.x & 63) == 0)
-      atomicMax(reinterpret_cast<unsigned long long *>(&state->smax),
+      atomicMax(reinterpret_cast<unsigned long long *>(&j.state->smax),
                 (unsigned long long)__double_as_longlong(sig));
   }
-  if (bad) atomicOr(&state->flags, 1);
+  if (bad) atomicOr(&j.state->flags, kFlagNonFinite);
 }
 
+// Single case: one workgroup per (block, tile).
+template <int DIM, int FLUX, typename R, int TX, int TY, int TZ, int NT>
+__global__ __launch_bounds__(NT) void stage_kernel(const double *__restrict__ avg,
+                                                   const double *init, double *out,
+                                                   alpaca_case_state *state, StageArgs a) {
+  using T = TileShape<DIM, TX, TY, TZ>;
+  static_assert(NT % 64 == 0, "whole wavefronts");
+  __shared__ R lds[T::LDS];
+  unsigned const g = xcd_remap_group(blockIdx.x, gridDim.x);
+  int const blk = g / T::TILES;
+  if (blk >= a.nblocks) return;  // uniform over the workgroup
+  TileJob j;
+  j.avg = avg;
+  j.init = init;
+  j.out = out;
+  j.state = state;
+  j.init_copy = a.init_copy;
+  j.base = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    j.nb[d] = a.nb[d];
+    j.bc[d] = a.bc[d];
+  }
+  j.gamma = a.gamma;
+  j.b = a.b;
+  j.inv_dx = a.inv_dx;
+  j.stage = a.stage;
+  j.signal = a.signal;
+  stage_tile<DIM, FLUX, R, TX, TY, TZ, NT>(j, blk, g % T::TILES, lds);
+}
+
+// Every tile of every listed block of every case of an ensemble, one
+// persistent kernel per RK stage: the grid is sized to the device and each
+// workgroup walks the work list (read from device memory, so a captured
+// graph of the step stays valid when the list changes).  Work units are
+// split into 8 contiguous ranges, one per XCD (group g runs on XCD g % 8),
+// so a case's blocks share an L2.  Units of stopped cases are skipped.
+template <int DIM, int FLUX, typename R, int TX, int TY, int TZ, int NT>
+__global__ __launch_bounds__(NT) void batch_stage_kernel(
+    const double *__restrict__ avg, const double *init, double *out, const CaseSlot *slots,
+    alpaca_case_state *states, const WorkItem *items, const int *n_items, int stage) {
+  using T = TileShape<DIM, TX, TY, TZ>;
+  __shared__ R lds[T::LDS];
+  __shared__ int skip;
+  constexpr unsigned kXcd = 8;
+  unsigned const units = unsigned(*n_items) * T::TILES;
+  unsigned const groups = gridDim.x;
+  unsigned const xcd = blockIdx.x % kXcd;
+  unsigned const per_xcd = groups / kXcd;  // grid is a multiple of 8
+  unsigned const m = blockIdx.x / kXcd;
+  unsigned const lo = (unsigned)((unsigned long long)units * xcd / kXcd);
+  unsigned const hi = (unsigned)((unsigned long long)units * (xcd + 1) / kXcd);
+  for (unsigned u = lo + m; u < hi; u += per_xcd) {
+    WorkItem const it = items[u / T::TILES];
+    alpaca_case_state *st = states + it.case_id;
+    // Another workgroup may flag the case during this stage: decide once per
+    // workgroup so every thread takes the same branch around the barriers.
+    if (threadIdx.x == 0) skip = (__atomic_load_n(&st->flags, __ATOMIC_RELAXED) & kFlagsStop()) != 0;
+    __syncthreads();
+    bool const stop = skip;
+    __syncthreads();
+    if (stop) continue;
+    CaseSlot const &cs = slots[it.case_id];
+    TileJob j;
+    j.avg = avg;
+    j.init = init;
+    j.out = out;
+    j.state = st;
+    j.init_copy = nullptr;
+    j.base = cs.base;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      j.nb[d] = cs.nb[d];
+      j.bc[d] = cs.bc[d];
+    }
+    j.gamma = cs.gamma;
+    j.b = cs.b;
+    j.inv_dx = cs.inv_dx;
+    j.stage = stage;
+    j.signal = stage == 2;
+    stage_tile<DIM, FLUX, R, TX, TY, TZ, NT>(j, it.block, u % T::TILES, lds);
+    __syncthreads();  // LDS is reused by the next unit
+  }
+}

@@ -66,6 +66,27 @@ _SIGNATURES = {
     "alpaca_profile_read": ([ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                              ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double)],
                             ctypes.c_int),
+    "alpaca_ensemble_bytes": ([ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_size_t),
+    "alpaca_ensemble_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint,
+                                ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "alpaca_ensemble_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "alpaca_ensemble_add_case": ([ctypes.c_void_p, ctypes.POINTER(CaseDesc), ctypes.c_void_p,
+                                  ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int)],
+                                 ctypes.c_int),
+    "alpaca_ensemble_remove_case": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "alpaca_ensemble_cancel_case": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "alpaca_ensemble_set_tuning": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "alpaca_ensemble_step": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "alpaca_ensemble_read_states": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
+                                    ctypes.c_int),
+    "alpaca_ensemble_case_blocks": ([ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_void_p),
+                                     ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "alpaca_ensemble_copy_case": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
+                                  ctypes.c_int),
+    "alpaca_ensemble_capacity": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+                                 ctypes.c_int),
     "alpaca_advance": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_void_p,
                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
                        ctypes.c_int),

@@ -179,6 +179,67 @@ int alpaca_advance(const alpaca_case_desc *desc, double *d_u, double *d_s1,
                    double *d_s2, alpaca_case_state *d_state, int nsteps,
                    void *stream);
 
+/* ------------------------------------------------------------ ensembles --
+ * Many independent uniform cases of one dimension resident in one HBM block
+ * pool (no counterpart in the reference, which runs one simulation per MPI
+ * job).  Each case keeps its own time step, end time and stop flags; one
+ * alpaca_ensemble_step advances every running case by one macro step with
+ * one persistent kernel per RK stage and per enabled class (flux family x
+ * arithmetic), replayed from a captured hipGraph.  A case's results do not
+ * depend on what else runs: per-case reductions only, fixed arithmetic order.
+ * The ensemble owns its device memory (allocated at create).
+ *
+ * Class bit for (flux, precision): 1 << (3*flux + m), m = 0 for ALPACA_FP64,
+ * 1 for ALPACA_FP64_FMA, 2 for ALPACA_MIXED.
+ * Case record flags: bit 0 non-finite state, bit 1 unhealthy time step,
+ * bit 2 t_end reached, bit 3 cancelled, bit 5 slot in use; a case with any of
+ * bits 0-3 set no longer advances. */
+typedef struct alpaca_ensemble alpaca_ensemble;
+
+#define ALPACA_CLASS_BIT(flux, m) (1u << (3 * (flux) + (m)))
+enum {
+  ALPACA_CASE_NONFINITE = 1,
+  ALPACA_CASE_UNHEALTHY = 2,
+  ALPACA_CASE_DONE = 4,
+  ALPACA_CASE_CANCELLED = 8,
+  ALPACA_CASE_ACTIVE = 32
+};
+
+/* HBM bytes an ensemble of this capacity allocates. */
+size_t alpaca_ensemble_bytes(int dim, int max_blocks, int max_cases);
+int alpaca_ensemble_create(int dim, int max_blocks, int max_cases, unsigned class_mask,
+                           void *stream, alpaca_ensemble **out);
+int alpaca_ensemble_destroy(alpaca_ensemble *e);
+/* Admits a case: copies its nb[0]*nb[1]*nb[2] haloed blocks (internal cells
+ * are used) from d_blocks into the pool, fills its halos and computes its
+ * first time step; it advances from the next alpaca_ensemble_step.
+ * ALPACA_ERR_CAPACITY when no contiguous block range or case slot is free. */
+int alpaca_ensemble_add_case(alpaca_ensemble *e, const alpaca_case_desc *desc,
+                             const double *d_blocks, double t_end, double cfl,
+                             int *case_id);
+/* Releases a case's slot and blocks (read its blocks first). */
+int alpaca_ensemble_remove_case(alpaca_ensemble *e, int case_id);
+/* Stops a case at the next stage boundary (the ABORTFILE analogue,
+ * input_output_manager.h:197-199). */
+int alpaca_ensemble_cancel_case(alpaca_ensemble *e, int case_id);
+/* Tile shape (3D: 0 = 8x8x8/256 threads, 1 = 16x8x8/512; 2D: 0 = 16x16/128,
+ * 1 = 16x16/256, 2 = 16x8/128) and persistent workgroups per CU (0 = the
+ * occupancy default) of the stage kernels. */
+int alpaca_ensemble_set_tuning(alpaca_ensemble *e, int tile_config, int groups_per_cu);
+/* Advances every running case by nsteps macro steps (asynchronous). */
+int alpaca_ensemble_step(alpaca_ensemble *e, int nsteps);
+/* Synchronous copy of the first n case records to host memory. */
+int alpaca_ensemble_read_states(alpaca_ensemble *e, alpaca_case_state *host_out, int n);
+/* Device pointer to a case's current state blocks and their count. */
+int alpaca_ensemble_case_blocks(alpaca_ensemble *e, int case_id, double **d_blocks,
+                                int *nblocks);
+/* Stream-ordered copy of a case's current state blocks into d_dst (caller
+ * memory of nblocks * alpaca_block_doubles(dim) doubles). */
+int alpaca_ensemble_copy_case(alpaca_ensemble *e, int case_id, double *d_dst);
+/* Free pool blocks, the largest free contiguous range and free case slots. */
+int alpaca_ensemble_capacity(alpaca_ensemble *e, int *free_blocks, int *largest_range,
+                             int *free_slots);
+
 #ifdef __cplusplus
 }
 #endif
