@@ -253,6 +253,73 @@ def run_batched(args):
     print(json.dumps(line), flush=True)
 
 
+def run_accuracy(args):
+    """Mixed vs FP64 on the 256^3 HLLC case: relative L1 / Linf density error
+    and mass / energy drift of the mixed form against the FP64 path after each
+    checkpoint step count."""
+    import numpy as np
+    import torch
+
+    from alpaca_amd import cases, gpu
+
+    nb = args.blocks
+    checkpoints = sorted({int(c) for c in args.checkpoints.split(",")})
+    runs = {}
+    for name, prec in (("fp64", gpu.FP64), ("mixed", gpu.MIXED)):
+        desc = gpu.make_desc(3, [nb] * 3, [gpu.BC_PERIODIC] * 3, flux=gpu.FLUX_HLLC,
+                             precision=prec)
+        case = gpu.GpuCase(desc, cases.blobs_shock_3d_torch(nb, seed=0), cfl=0.5)
+        g0 = gpu.from_blocks(case.u, 3, case.nb)
+        sums0 = g0.sum(dim=(1, 2, 3)).cpu().numpy()
+        del g0
+        done = 0
+        snaps = {}
+        for c in checkpoints:
+            case.advance(c - done)
+            done = c
+            g = gpu.from_blocks(case.u, 3, case.nb)
+            snaps[c] = {"rho": g[0].clone(), "sums": g.sum(dim=(1, 2, 3)).cpu().numpy(),
+                        "time": case.record().time}
+            del g
+            print(f"[accuracy] {name} step {c} t={snaps[c]['time']:.6f}", flush=True)
+        runs[name] = (sums0, snaps)
+        del case
+        torch.cuda.empty_cache()
+    rows = []
+    for c in checkpoints:
+        ref = runs["fp64"][1][c]
+        mix = runs["mixed"][1][c]
+        d = (mix["rho"] - ref["rho"]).abs()
+        s0 = runs["fp64"][0]
+        rows.append({
+            "steps": c,
+            "time_fp64": ref["time"], "time_mixed": mix["time"],
+            "l1_rho_err": float(d.sum() / ref["rho"].abs().sum()),
+            "linf_rho_err": float(d.max() / ref["rho"].abs().max()),
+            "mass_drift_mixed": float(abs(mix["sums"][0] - s0[0]) / abs(s0[0])),
+            "energy_drift_mixed": float(abs(mix["sums"][1] - s0[1]) / abs(s0[1])),
+            "mass_drift_fp64": float(abs(ref["sums"][0] - s0[0]) / abs(s0[0])),
+            "energy_drift_fp64": float(abs(ref["sums"][1] - s0[1]) / abs(s0[1])),
+        })
+    print(json.dumps({"workload": f"3D {nb * 16}^3 periodic blob+shock HLLC, mixed vs FP64",
+                      "rows": rows}), flush=True)
+
+
+def run_copy_probe():
+    """A plain device copy of known size, for scaling the HBM counters."""
+    import torch
+
+    n = 1 << 28  # 2 GiB of doubles
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read_per_launch": n * 8,
+                      "copy_probe_bytes_written_per_launch": n * 8}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,7 +331,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-modes", action="store_true",
                     help="time only --precision (skip the other arithmetic modes)")
-    ap.add_argument("--workload", choices=["uniform", "batched", "serving"], default="uniform",
+    ap.add_argument("--workload", choices=["uniform", "batched", "serving", "accuracy"],
+                    default="uniform",
                     help="uniform: the 256^3 case (default); batched: 512 uneven 2D cases; "
                          "serving: Poisson request stream")
     ap.add_argument("--cases", type=int, default=512, help="batched: number of cases")
@@ -272,6 +340,9 @@ def main():
     ap.add_argument("--tune", action="store_true", help="batched: autotune the tile shape")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: requests per second")
     ap.add_argument("--window", type=float, default=2.0, help="serving: arrival window (s)")
+    ap.add_argument("--checkpoints", default="100,1000", help="accuracy: step counts")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="time nothing: run a plain 2 GiB device copy (counter scaling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -283,6 +354,12 @@ def main():
     import torch
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        run_copy_probe()
+        return
+    if args.workload == "accuracy":
+        run_accuracy(args)
+        return
     if args.workload == "batched":
         run_batched(args)
         return
