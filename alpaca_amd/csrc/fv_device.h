@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/alpaca_gpu.h"
 
 namespace alpaca {
@@ -137,12 +139,14 @@ struct WorkItem {
 
 namespace exact {
 #pragma clang fp contract(off)
+constexpr bool kFastArith = false;  // correctly rounded FP64 division and sqrt
 #include "fv_numerics.inl"
 #include "stage_kernel.inl"
 }  // namespace exact
 
 namespace fast {
 #pragma clang fp contract(fast)
+constexpr bool kFastArith = true;  // Newton-Raphson FP64 division and sqrt (<= 1 ulp)
 #include "fv_numerics.inl"
 #include "stage_kernel.inl"
 }  // namespace fast

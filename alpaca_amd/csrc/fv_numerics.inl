@@ -7,15 +7,54 @@
 template <typename R> __device__ __forceinline__ R smin(R a, R b) { return (b < a) ? b : a; }
 template <typename R> __device__ __forceinline__ R smax(R a, R b) { return (a < b) ? b : a; }
 
-// Correctly rounded division/square root in FP64; hardware reciprocal and
-// square root (about 1 ulp) in the FP32 part of the mixed form.
-template <typename R> __device__ __forceinline__ R fdiv(R a, R b) { return a / b; }
-template <> __device__ __forceinline__ float fdiv<float>(float a, float b) {
-  return a * __builtin_amdgcn_rcpf(b);
+// FP64 division a/b by Newton-Raphson on the hardware reciprocal: two
+// iterations on r ~ 1/b (r += r(1 - b r)), then one residual correction of
+// the quotient (q += r(a - b q)); each residual is exact in an FMA, so the
+// result is within 1 ulp of a/b for normal operands.
+__device__ __forceinline__ double nr_div64(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  double const q = a * r;
+  double const rem = __builtin_fma(-b, q, a);
+  return __builtin_fma(rem, r, q);
 }
-template <typename R> __device__ __forceinline__ R dsqrt(R x) { return __builtin_sqrt(x); }
-template <> __device__ __forceinline__ float dsqrt<float>(float x) {
-  return __builtin_amdgcn_sqrtf(x);
+
+// FP64 square root by the coupled Newton iteration for g ~ sqrt(x) and
+// h ~ 1/(2 sqrt(x)) from the hardware reciprocal square root, with a final
+// residual correction (x - g^2 exact in an FMA).  sqrt(0) = 0.
+__device__ __forceinline__ double nr_sqrt64(double x) {
+  double const r = __builtin_amdgcn_rsq(x);
+  double g = x * r;
+  double h = 0.5 * r;
+  double const e = __builtin_fma(-g, h, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  double const d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return x == 0.0 ? 0.0 : g;
+}
+
+// Division and square root: correctly rounded FP64 in the exact namespace;
+// Newton-Raphson FP64 in the fast namespace; hardware reciprocal and square
+// root (about 1 ulp) for the FP32 part of the mixed form.
+template <typename R> __device__ __forceinline__ R fdiv(R a, R b) {
+  if constexpr (std::is_same<R, float>::value)
+    return a * __builtin_amdgcn_rcpf(b);
+  else if constexpr (kFastArith)
+    return nr_div64(a, b);
+  else
+    return a / b;
+}
+template <typename R> __device__ __forceinline__ R dsqrt(R x) {
+  if constexpr (std::is_same<R, float>::value)
+    return __builtin_amdgcn_sqrtf(x);
+  else if constexpr (kFastArith)
+    return nr_sqrt64(x);
+  else
+    return __builtin_sqrt(x);
 }
 
 // ConsistencyManagedSum, full symmetry (mathematical_functions.h:79-91).
@@ -45,7 +84,7 @@ template <int DIM>
 __device__ __forceinline__ double eos_pressure64(double gamma, double b, double rho, double mx,
                                                  double my, double mz, double e) {
   double const q = 0.5 * dams<DIM>(mx * mx, my * my, mz * mz);
-  return -gamma * b + (gamma - 1.0) * (e - q / rho);
+  return -gamma * b + (gamma - 1.0) * (e - fdiv(q, rho));
 }
 template <int DIM, typename R>
 __device__ __forceinline__ R eos_pressure(R gamma, R b, R rho, R mx, R my, R mz, R e) {
@@ -94,14 +133,30 @@ template <typename R> struct CellQ {
 template <int DIM, typename R>
 __device__ __forceinline__ void cell_quantities(double gamma, double b, const double *c,
                                                 CellQ<R> &q) {
+  if constexpr (std::is_same<R, float>::value) {
+    // Mixed form: only the pressure recovery in FP64.
+    double const mz = DIM == 3 ? c[kMomX + DIM - 1] : 0.0;
+    float const p =
+        float(eos_pressure64<DIM>(gamma, b, c[kMass], c[kMomX], c[kMomX + 1], mz, c[kEnergy]));
+    float const rho = float(c[kMass]);
+    float const one_rho = fdiv(1.0f, rho);
+    q.rho = rho;
+    q.one_rho = one_rho;
+    q.vel[0] = float(c[kMomX]) * one_rho;
+    q.vel[1] = float(c[kMomX + 1]) * one_rho;
+    q.vel[2] = DIM == 3 ? float(c[kMomX + 2]) * one_rho : 0.0f;
+    q.p = p;
+    q.psi = eos_psi<float>(float(gamma), float(b), p, one_rho);
+    q.H = (float(c[kEnergy]) + p) * one_rho;
+    return;
+  }
   Prim<double> p;
   cons_to_prim<DIM, double>(gamma, b, c, p);
-  double const one_rho = 1.0 / c[kMass];
+  double const one_rho = fdiv(1.0, c[kMass]);
   double const mz = DIM == 3 ? c[kMomX + DIM - 1] : 0.0;
-  double const H =
-      (c[kEnergy] + eos_pressure64<DIM>(gamma, b, c[kMass], c[kMomX], c[kMomX + 1], mz,
-                                        c[kEnergy])) /
-      c[kMass];
+  double const H = fdiv(c[kEnergy] + eos_pressure64<DIM>(gamma, b, c[kMass], c[kMomX],
+                                                         c[kMomX + 1], mz, c[kEnergy]),
+                        c[kMass]);
   q.rho = R(c[kMass]);
   q.one_rho = R(one_rho);
   q.vel[0] = R(p.vel[0]);
