@@ -102,19 +102,31 @@ def four_quadrant_2d(nbx: int, nby: int, seed: int, gamma: float = 1.4, case: in
     return cons[..., None]
 
 
-def blobs_shock_3d(nb: int, seed: int, gamma: float = 1.4, nblobs: int = 32,
-                   shock_x: float = 0.1, chunk: int = 16):
-    """3D uniform background + seeded Gaussian blobs + planar Mach-2 shock.
+_LN2_HI = 6.93147180369123816490e-01  # ln 2 to 32 bits (low bits zero: k*_LN2_HI is exact)
+_LN2_LO = 1.90821492927058770002e-10  # ln 2 - _LN2_HI
+_INV_LN2 = 1.44269504088896338700e+00
+_EXP_TAYLOR = [1.0 / math.factorial(n) for n in range(14)]
 
-    Background rho = p = 1 at rest.  Each blob adds A exp(-|x-c|^2 / (2 r^2))
-    (periodic distance) to rho and p with c ~ U[0,1]^3, r ~ U[0.02, 0.08],
-    A ~ U[0.2, 2].  For x < shock_x the Rankine-Hugoniot post-shock state of a
-    Mach-2 shock into the background: rho = 8/3, p = 4.5,
-    u = 2 sqrt(gamma) (1 - 3/8).
+
+def det_exp(x):
+    """exp(x) for x <= 0 from IEEE +, -, * and exact scaling only.
+
+    Cody-Waite reduction x = k ln2 + r (|r| <= ln2/2) and the degree-13 Taylor
+    polynomial of exp(r) in Horner form (truncation < 1e-17 relative); every
+    IEEE-754 machine returns the same bits, unlike library exp routines, so a
+    state built from it can be regenerated anywhere and compared by hash.
     """
-    n = nb * IC
-    dx = 1.0 / n
-    c1 = (np.arange(n) + 0.5) * dx
+    x = np.asarray(x, dtype=np.float64)
+    k = np.rint(x * _INV_LN2)
+    r = (x - k * _LN2_HI) - k * _LN2_LO
+    p = np.full_like(r, _EXP_TAYLOR[13])
+    for c in reversed(_EXP_TAYLOR[:13]):
+        p = p * r + c
+    kk = np.clip(k, -1100, 0).astype(np.int64)
+    return np.where(x < -740.0, 0.0, np.ldexp(p, kk))
+
+
+def _blob_params(seed: int, nblobs: int):
     rng = SplitMix64(0xB10B, seed)
     blobs = []
     for _ in range(nblobs):
@@ -122,32 +134,59 @@ def blobs_shock_3d(nb: int, seed: int, gamma: float = 1.4, nblobs: int = 32,
         r = rng.uniform(0.02, 0.08)
         a = rng.uniform(0.2, 2.0)
         blobs.append((cx, cy, cz, r, a))
+    return blobs
+
+
+def _blob_profiles(n: int, blobs):
+    """Per blob the three 1D factors g(x) = exp(-d(x)^2 / (2 r^2)) with d the
+    periodic distance of the cell centre to the blob centre."""
+    c1 = (np.arange(n) + 0.5) * (1.0 / n)
+    out = []
+    for cx, cy, cz, r, a in blobs:
+        g = []
+        for c in (cx, cy, cz):
+            d = np.abs(c1 - c)
+            d = np.minimum(d, 1.0 - d)
+            g.append(det_exp(-(d * d) / (2.0 * r * r)))
+        out.append((a, g[0], g[1], g[2]))
+    return c1, out
+
+
+def _shock_state(gamma: float):
     m = 2.0
     rho_post = (gamma + 1.0) * m * m / ((gamma - 1.0) * m * m + 2.0)
     p_post = 1.0 + 2.0 * gamma / (gamma + 1.0) * (m * m - 1.0)
     u_post = m * math.sqrt(gamma) * (1.0 - 1.0 / rho_post)
+    return rho_post, p_post, u_post
+
+
+def blobs_shock_3d(nb: int, seed: int, gamma: float = 1.4, nblobs: int = 32,
+                   shock_x: float = 0.1, chunk: int = 16):
+    """3D uniform background + seeded Gaussian blobs + planar Mach-2 shock.
+
+    Background rho = p = 1 at rest.  Each blob adds
+    A gx(x) gy(y) gz(z) = A exp(-|x-c|^2 / (2 r^2)) (periodic distance per
+    axis) to rho and p with c ~ U[0,1]^3, r ~ U[0.02, 0.08], A ~ U[0.2, 2].
+    For x < shock_x the Rankine-Hugoniot post-shock state of a Mach-2 shock
+    into the background: rho = 8/3, p = 4.5, u = 2 sqrt(gamma) (1 - 3/8).
+    Only IEEE +, -, *, / and det_exp are used, so the state is the same bits
+    on every machine (and in blobs_shock_3d_torch).
+    """
+    n = nb * IC
+    c1, prof = _blob_profiles(n, _blob_params(seed, nblobs))
+    rho_post, p_post, u_post = _shock_state(gamma)
     cons = np.empty((5, n, n, n), dtype=np.float64)
     for i0 in range(0, n, chunk):
-        xs = c1[i0:i0 + chunk]
-        X, Y, Z = np.meshgrid(xs, c1, c1, indexing="ij")
-        add = np.zeros_like(X)
-        for cx, cy, cz, r, a in blobs:
-            dxp = np.abs(X - cx)
-            dxp = np.minimum(dxp, 1.0 - dxp)
-            dyp = np.abs(Y - cy)
-            dyp = np.minimum(dyp, 1.0 - dyp)
-            dzp = np.abs(Z - cz)
-            dzp = np.minimum(dzp, 1.0 - dzp)
-            add += a * np.exp(-(dxp * dxp + dyp * dyp + dzp * dzp) / (2.0 * r * r))
-        rho = 1.0 + add
-        p = 1.0 + add
-        u = np.zeros_like(X)
-        post = X < shock_x
-        rho = np.where(post, rho_post + add, rho)
-        p = np.where(post, p_post + add, p)
-        u = np.where(post, u_post, u)
-        z = np.zeros_like(X)
-        cons[:, i0:i0 + chunk] = primes_to_conservatives(3, gamma, 0.0, rho, (u, z, z), p)
+        i1 = min(n, i0 + chunk)
+        add = np.zeros((i1 - i0, n, n))
+        for a, gx, gy, gz in prof:
+            add += a * ((gx[i0:i1, None, None] * gy[None, :, None]) * gz[None, None, :])
+        post = np.broadcast_to((c1[i0:i1] < shock_x)[:, None, None], add.shape)
+        rho = np.where(post, rho_post + add, 1.0 + add)
+        p = np.where(post, p_post + add, 1.0 + add)
+        u = np.where(post, u_post, 0.0)
+        z = np.zeros_like(add)
+        cons[:, i0:i1] = primes_to_conservatives(3, gamma, 0.0, rho, (u, z, z), p)
     return cons
 
 
@@ -173,50 +212,39 @@ def random_block(dim: int, seed: int, gamma: float = 1.4, smooth: bool = False):
 
 def blobs_shock_3d_torch(nb: int, seed: int, gamma: float = 1.4, nblobs: int = 32,
                          shock_x: float = 0.1, device="cuda", chunk: int = 32):
-    """blobs_shock_3d evaluated with torch on a device (same definition; used
-    where the domain is too large for a host-side numpy evaluation)."""
+    """blobs_shock_3d evaluated on a device (the 1D factors on the host, the 3D
+    products, sums and conversion as tensor-tensor operations with the same
+    order); used where the domain is too large for a host-side evaluation."""
     import torch
 
     n = nb * IC
-    dx = 1.0 / n
-    c1 = (torch.arange(n, dtype=torch.float64, device=device) + 0.5) * dx
-    rng = SplitMix64(0xB10B, seed)
-    blobs = []
-    for _ in range(nblobs):
-        cx, cy, cz = rng.uniform(0, 1), rng.uniform(0, 1), rng.uniform(0, 1)
-        r = rng.uniform(0.02, 0.08)
-        a = rng.uniform(0.2, 2.0)
-        blobs.append((cx, cy, cz, r, a))
-    m = 2.0
-    rho_post = (gamma + 1.0) * m * m / ((gamma - 1.0) * m * m + 2.0)
-    p_post = 1.0 + 2.0 * gamma / (gamma + 1.0) * (m * m - 1.0)
-    u_post = m * math.sqrt(gamma) * (1.0 - 1.0 / rho_post)
+    c1, prof = _blob_profiles(n, _blob_params(seed, nblobs))
+    rho_post, p_post, u_post = _shock_state(gamma)
+    dev = [(a, torch.from_numpy(gx).to(device), torch.from_numpy(gy).to(device),
+            torch.from_numpy(gz).to(device)) for a, gx, gy, gz in prof]
+    xc = torch.from_numpy(c1).to(device)
     cons = torch.empty((5, n, n, n), dtype=torch.float64, device=device)
+    gm1 = torch.tensor(gamma - 1.0, dtype=torch.float64, device=device)
     for i0 in range(0, n, chunk):
-        xs = c1[i0:i0 + chunk]
-        X, Y, Z = torch.meshgrid(xs, c1, c1, indexing="ij")
-        add = torch.zeros_like(X)
-        for cx, cy, cz, r, a in blobs:
-            dxp = (X - cx).abs()
-            dxp = torch.minimum(dxp, 1.0 - dxp)
-            dyp = (Y - cy).abs()
-            dyp = torch.minimum(dyp, 1.0 - dyp)
-            dzp = (Z - cz).abs()
-            dzp = torch.minimum(dzp, 1.0 - dzp)
-            add += a * torch.exp(-(dxp * dxp + dyp * dyp + dzp * dzp) / (2.0 * r * r))
-        post = X < shock_x
+        i1 = min(n, i0 + chunk)
+        add = torch.zeros((i1 - i0, n, n), dtype=torch.float64, device=device)
+        for a, gx, gy, gz in dev:
+            add += a * ((gx[i0:i1, None, None] * gy[None, :, None]) * gz[None, None, :])
+        post = (xc[i0:i1] < shock_x)[:, None, None].expand_as(add)
         rho = torch.where(post, rho_post + add, 1.0 + add)
         p = torch.where(post, p_post + add, 1.0 + add)
-        u = torch.where(post, torch.full_like(X, u_post), torch.zeros_like(X))
-        z = torch.zeros_like(X)
-        w = z
-        s1 = (u * u + z * z) + w * w
-        s2 = (u * u + w * w) + z * z
-        s3 = (z * z + w * w) + u * u
-        q = 0.5 * (torch.minimum(torch.minimum(s1, s2), s3) + torch.maximum(torch.maximum(s1, s2), s3))
-        cons[0, i0:i0 + chunk] = rho
-        cons[1, i0:i0 + chunk] = p / (gamma - 1.0) + (0.5 * q * rho)
-        cons[2, i0:i0 + chunk] = u * rho
-        cons[3, i0:i0 + chunk] = z
-        cons[4, i0:i0 + chunk] = z
+        u = torch.where(post, torch.full_like(add, u_post), torch.zeros_like(add))
+        z = torch.zeros_like(add)
+        uu = u * u
+        zz = z * z
+        s1 = (uu + zz) + zz
+        s2 = (uu + zz) + zz
+        s3 = (zz + zz) + uu
+        q = 0.5 * (torch.minimum(torch.minimum(s1, s2), s3) +
+                   torch.maximum(torch.maximum(s1, s2), s3))
+        cons[0, i0:i1] = rho
+        cons[1, i0:i1] = p / gm1 + (0.5 * q * rho)
+        cons[2, i0:i1] = u * rho
+        cons[3, i0:i1] = z
+        cons[4, i0:i1] = z
     return cons

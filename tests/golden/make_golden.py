@@ -14,6 +14,17 @@ Outputs (small, committed):
                       SHA-256 of the final state's bytes, per-field L1 sums.
 
     python tests/golden/make_golden.py
+
+With ``--full`` it instead runs the full-size 3D case (256^3 cells = 16^3
+blocks, periodic, blob + Mach-2 shock seed 0, CFL 0.5, 100 steps, FP64) with
+the reference's Roe flux splitting and HLLC schemes on all host threads (about
+half an hour per scheme on 8 cores) and writes
+
+  golden_full_256.json  per scheme: SHA-256 of the initial and final states,
+                        every dt, final time, conservation sums;
+  golden_full_256.npz   the final state at 4096 fixed cells (all equations).
+
+    python tests/golden/make_golden.py --full [--threads N] [--flux roe|hllc]
 """
 from __future__ import annotations
 
@@ -21,6 +32,7 @@ import hashlib
 import json
 import os
 import sys
+import time
 
 import numpy as np
 
@@ -162,5 +174,55 @@ def sod_state():
     return cases.primes_to_conservatives(2, GAMMA, 0.0, rho, (z, z), p)[..., None]
 
 
+FULL_NB = 16
+FULL_STEPS = 100
+FULL_SAMPLES = 4096
+
+
+def full_sample_index(n=FULL_NB * 16, count=FULL_SAMPLES):
+    """Fixed cells (i, j, k) of the 256^3 domain where the final state is kept."""
+    rng = np.random.default_rng(256)
+    return rng.integers(0, n, size=(count, 3))
+
+
+def main_full(threads, fluxes):
+    path_json = os.path.join(HERE, "golden_full_256.json")
+    path_npz = os.path.join(HERE, "golden_full_256.npz")
+    meta = json.load(open(path_json)) if os.path.exists(path_json) else {"runs": {}}
+    arrays = dict(np.load(path_npz)) if os.path.exists(path_npz) else {}
+    idx = full_sample_index()
+    arrays["sample_index"] = idx
+    nb = FULL_NB
+    for flux in fluxes:
+        st0 = cases.blobs_shock_3d(nb, seed=0)
+        t0 = time.time()
+        st, dts, tf = oracle_py.run(3, [nb] * 3, [1, 1, 1], flux, 0.5, 1e300, FULL_STEPS, st0,
+                                    threads=threads, which="ref")
+        wall = time.time() - t0
+        name = "roe" if flux == 1 else "hllc"
+        meta["runs"][name] = {
+            "dim": 3, "nb": [nb] * 3, "periodic": [1, 1, 1], "flux": flux, "cfl": 0.5,
+            "steps": int(len(dts)), "ic": {"kind": "blobs_shock_3d", "seed": 0},
+            "ic_sha256": sha(st0), "sha256": sha(st), "dts": dts.tolist(), "t_final": tf,
+            "sums0": st0.sum(axis=(1, 2, 3)).tolist(), "sums": st.sum(axis=(1, 2, 3)).tolist(),
+            "reference_wall_s": wall, "threads": threads}
+        arrays[f"{name}_final_sample"] = st[:, idx[:, 0], idx[:, 1], idx[:, 2]].T.copy()
+        print(f"{name}: {len(dts)} steps, t={tf:.6e}, {wall:.0f} s on {threads} threads",
+              flush=True)
+        with open(path_json, "w") as fh:
+            json.dump(meta, fh, indent=1)
+        np.savez_compressed(path_npz, **arrays)
+
+
 if __name__ == "__main__":
-    main()
+    import argparse
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--full", action="store_true")
+    ap.add_argument("--threads", type=int, default=os.cpu_count() or 8)
+    ap.add_argument("--flux", choices=["roe", "hllc", "both"], default="both")
+    a = ap.parse_args()
+    if a.full:
+        main_full(a.threads, {"roe": [1], "hllc": [0], "both": [1, 0]}[a.flux])
+    else:
+        main()
