@@ -8,7 +8,7 @@
 //   combine_kernel<DIM>  one thread per internal cell: flux divergence with
 //       the reference's ConsistencyManagedSum, times 1/dx, fused with the
 //       SSP-RK3 combination and the non-finite check.
-//   halo_kernel<DIM>  face halo slabs from neighbours / zero-gradient.
+//   halo_kernel<DIM>  face halo slabs from neighbours / zero-gradient / symmetry.
 //   signal_kernel<DIM> + dt_kernel  CFL reduction and time step.
 #include <hip/hip_runtime.h>
 
@@ -205,8 +205,11 @@ __global__ __launch_bounds__(kThreads) void halo_kernel(double *blocks, int nblo
   src[1] = dst[1];
   src[2] = dst[2];
   bool const at_edge = hi ? (bc3[d] == ca.nb[d] - 1) : (bc3[d] == 0);
+  bool const symmetry = at_edge && ca.bc[d] == ALPACA_BC_SYMMETRY;
   if (at_edge && ca.bc[d] == ALPACA_BC_ZERO_GRADIENT) {
     src[d] = hi ? kHS + kIC - 1 : kHS;
+  } else if (symmetry) {
+    src[d] = (hi ? 2 * (kHS + kIC) - 1 : 2 * kHS - 1) - dst[d];  // mirror about the face
   } else {
     int nb3[3] = {bc3[0], bc3[1], bc3[2]};
     nb3[d] = hi ? (bc3[d] + 1 == ca.nb[d] ? 0 : bc3[d] + 1) : (bc3[d] == 0 ? ca.nb[d] - 1 : bc3[d] - 1);
@@ -216,7 +219,10 @@ __global__ __launch_bounds__(kThreads) void halo_kernel(double *blocks, int nblo
   size_t const di = (size_t)blk * D::BLOCK + dst[0] * D::SX + dst[1] * D::SY + dst[2];
   size_t const si = (size_t)sblk * D::BLOCK + src[0] * D::SX + src[1] * D::SY + src[2];
 #pragma unroll
-  for (int e = 0; e < NE; ++e) blocks[di + (size_t)e * D::FIELD] = blocks[si + (size_t)e * D::FIELD];
+  for (int e = 0; e < NE; ++e) {
+    double const v = blocks[si + (size_t)e * D::FIELD];
+    blocks[di + (size_t)e * D::FIELD] = (symmetry && e == kMomX + d) ? -1.0 * v : v;
+  }
 }
 
 // ----------------------------------------------------------- time step ---
@@ -394,7 +400,8 @@ int validate(const alpaca_case_desc *d) {
   if (!d) return ALPACA_ERR_INVALID;
   if (d->dim != 2 && d->dim != 3) return ALPACA_ERR_INVALID;
   for (int a = 0; a < d->dim; ++a)
-    if (d->nb[a] < 1 || (d->bc[a] != ALPACA_BC_ZERO_GRADIENT && d->bc[a] != ALPACA_BC_PERIODIC))
+    if (d->nb[a] < 1 || (d->bc[a] != ALPACA_BC_ZERO_GRADIENT && d->bc[a] != ALPACA_BC_PERIODIC &&
+                         d->bc[a] != ALPACA_BC_SYMMETRY))
       return ALPACA_ERR_INVALID;
   if (d->dim == 2 && d->nb[2] != 1) return ALPACA_ERR_INVALID;
   if (d->flux != ALPACA_FLUX_HLLC && d->flux != ALPACA_FLUX_ROE) return ALPACA_ERR_INVALID;

@@ -11,7 +11,8 @@
 //      131-153), applies the RK combination in FP64 (time_integrator.h:98-113,
 //      434-457), writes the new state and pushes it into the face halo slabs
 //      of the neighbouring blocks (internal_halo_manager.cpp:439-466) or, on a
-//      zero-gradient side, into its own halo (boundary_constants.h:212-253);
+//      zero-gradient or symmetry side, into its own halo
+//      (boundary_constants.h:169-253);
 //   3. on the last stage it also reduces the CFL signal of the new state
 //      (modular_algorithm_assembler.cpp:1840-1869) for the next time step.
 
@@ -21,9 +22,12 @@ __device__ __forceinline__ double wave_max64(double v) {
   return v;
 }
 
-// Writes value v of field e at internal cell c3 of local block bc3 of the
+// Writes the new value v of equation e of internal cell c3 of block blk of the
 // job's case into the halo slabs that read it: the neighbours' slabs across
-// each near face, or the block's own slab on a zero-gradient domain side.
+// each near face, or the block's own slab on a zero-gradient side (the first
+// or last internal layer fills all four halo layers) or a symmetry side (the
+// cell at internal depth c fills the halo layer at depth c, the face-normal
+// momentum negated).
 template <int DIM>
 __device__ __forceinline__ void push_halo(const TileJob &j, int blk, const int *bc3, const int *c3,
                                           int e, double v) {
@@ -33,6 +37,7 @@ __device__ __forceinline__ void push_halo(const TileJob &j, int blk, const int *
   for (int d = 0; d < DIM; ++d) {
     int const c = c3[d];
     int pos[3] = {c3[0] + kHS, c3[1] + kHS, DIM == 3 ? c3[2] + kHS : 0};
+    double const mirrored = (e == kMomX + d) ? -1.0 * v : v;
     if (c < kHS) {
       if (bc3[d] > 0 || j.bc[d] == ALPACA_BC_PERIODIC) {
         int nb3[3] = {bc3[0], bc3[1], bc3[2]};
@@ -40,6 +45,10 @@ __device__ __forceinline__ void push_halo(const TileJob &j, int blk, const int *
         int const nbk = j.base + (nb3[0] * j.nb[1] + nb3[1]) * j.nb[2] + nb3[2];
         pos[d] = c + kHS + kIC;
         out[(size_t)nbk * D::BLOCK + e * D::FIELD + pos[0] * D::SX + pos[1] * D::SY + pos[2]] = v;
+      } else if (j.bc[d] == ALPACA_BC_SYMMETRY) {
+        pos[d] = kHS - 1 - c;
+        out[(size_t)blk * D::BLOCK + e * D::FIELD + pos[0] * D::SX + pos[1] * D::SY + pos[2]] =
+            mirrored;
       } else if (c == 0) {
 #pragma unroll
         for (int h = 0; h < kHS; ++h) {
@@ -55,6 +64,10 @@ __device__ __forceinline__ void push_halo(const TileJob &j, int blk, const int *
         int const nbk = j.base + (nb3[0] * j.nb[1] + nb3[1]) * j.nb[2] + nb3[2];
         pos[d] = c - kIC + kHS;
         out[(size_t)nbk * D::BLOCK + e * D::FIELD + pos[0] * D::SX + pos[1] * D::SY + pos[2]] = v;
+      } else if (j.bc[d] == ALPACA_BC_SYMMETRY) {
+        pos[d] = kHS + 2 * kIC - 1 - c;
+        out[(size_t)blk * D::BLOCK + e * D::FIELD + pos[0] * D::SX + pos[1] * D::SY + pos[2]] =
+            mirrored;
       } else if (c == kIC - 1) {
 #pragma unroll
         for (int h = 0; h < kHS; ++h) {

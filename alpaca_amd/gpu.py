@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libalpaca_hip.so")
 
 FLUX_HLLC, FLUX_ROE = 0, 1
 FP64, MIXED, FP64_FMA = 0, 1, 2
-BC_ZERO_GRADIENT, BC_PERIODIC = 0, 1
+BC_ZERO_GRADIENT, BC_PERIODIC, BC_SYMMETRY = 0, 1, 2
 IC, HS, TC = 16, 4, 24
 
 KERNEL_FACE_FLUX, KERNEL_COMBINE, KERNEL_HALO, KERNEL_TIMESTEP, KERNEL_STAGE = 0, 1, 2, 3, 4

@@ -54,9 +54,11 @@ enum { ALPACA_FLUX_HLLC = 0, ALPACA_FLUX_ROE = 1 };
 enum { ALPACA_FP64 = 0, ALPACA_MIXED = 1, ALPACA_FP64_FMA = 2 };
 
 /* Side condition per direction: zero-gradient (ZeroGradientBoundaryCondition,
- * zero_gradient_boundary_condition.h:45-84) or periodic (the neighbour wraps,
- * id_periodic_information.cpp:325). */
-enum { ALPACA_BC_ZERO_GRADIENT = 0, ALPACA_BC_PERIODIC = 1 };
+ * zero_gradient_boundary_condition.h:45-84), periodic (the neighbour wraps,
+ * id_periodic_information.cpp:325) or symmetry (SymmetryBoundaryCondition,
+ * symmetry_boundary_condition.h:89-109: the mirrored internal cell, with the
+ * face-normal momentum negated). */
+enum { ALPACA_BC_ZERO_GRADIENT = 0, ALPACA_BC_PERIODIC = 1, ALPACA_BC_SYMMETRY = 2 };
 
 typedef struct alpaca_case_desc {
   int dim;            /* 2 or 3 */

@@ -1,7 +1,7 @@
 /*
  * ORACLE — test infrastructure only (see alpaca_oracle.h).  CPU restatement
  * of ALPACA's single-phase FV right-hand side, the SSP-RK3 stage update, the
- * same-level halo fill with zero-gradient / periodic sides and the CFL time
+ * same-level halo fill with zero-gradient / symmetry / periodic sides and the CFL time
  * step.  Each function cites the reference file:line it restates.  Build with
  * -ffp-contract=off: the reference x86 build has no FMA.
  */
@@ -674,8 +674,8 @@ static inline void block_coords(const run_t *r, int b, int *c) {
 
 /* Same-level halo copy (internal_halo_manager.cpp:439-466): every halo cell of
  * every block takes the value of the internal cell that owns its position,
- * wrapped across periodic sides; positions outside a zero-gradient side are
- * left to the external BC below. */
+ * wrapped across periodic sides; positions outside a zero-gradient or
+ * symmetry side are left to the external BC below. */
 static void halo_copy(run_t *r, int lo, int hi, void *arg) {
   (void)arg;
   layout_t const *L = &r->L;
@@ -699,7 +699,7 @@ static void halo_copy(run_t *r, int lo, int hi, void *arg) {
             int gc = blk[d] * IC + idx[d];
             int const n = r->g.nb[d] * IC;
             if (gc < 0 || gc >= n) {
-              if (!r->g.periodic[d]) {
+              if (r->g.bc[d] != ORA_BC_PERIODIC) {
                 outside = 1;
                 break;
               }
@@ -724,11 +724,15 @@ static void halo_copy(run_t *r, int lo, int hi, void *arg) {
   }
 }
 
-/* ZeroGradientBoundaryCondition (zero_gradient_boundary_condition.h:45-57,
- * boundary_constants.h:212-253): the halo slab over the full transverse
- * extent takes the first/last internal layer.  Order East, West, North,
- * South, Top, Bottom. */
-static void zero_gradient(run_t *r, int lo, int hi, void *arg) {
+/* External boundary conditions on the halo slab over the full transverse
+ * extent, in the order East, West, North, South, Top, Bottom:
+ *  - ZeroGradientBoundaryCondition (zero_gradient_boundary_condition.h:45-57,
+ *    boundary_constants.h:212-253): the first/last internal layer;
+ *  - SymmetryBoundaryCondition (symmetry_boundary_condition.h:89-109,
+ *    boundary_constants.h:169-208, mirror offsets :39-48): the internal cell
+ *    mirrored about the domain face times a sign that is -1 for the
+ *    face-normal momentum only (symmetry_boundary_condition.h:39-53,148-215). */
+static void external_bc(run_t *r, int lo, int hi, void *arg) {
   (void)arg;
   layout_t const *L = &r->L;
   int const dim = L->dim;
@@ -737,7 +741,8 @@ static void zero_gradient(run_t *r, int lo, int hi, void *arg) {
     block_coords(r, b, bc);
     double *v = r->rhs + (size_t)b * r->bsize;
     for (int d = 0; d < dim; ++d) {
-      if (r->g.periodic[d])
+      int const kind = r->g.bc[d];
+      if (kind == ORA_BC_PERIODIC)
         continue;
       for (int side = 1; side >= 0; --side) { /* high side first */
         if (side == 1 && bc[d] != r->g.nb[d] - 1)
@@ -745,6 +750,8 @@ static void zero_gradient(run_t *r, int lo, int hi, void *arg) {
         if (side == 0 && bc[d] != 0)
           continue;
         int const src_layer = side ? HS + IC - 1 : HS;
+        /* mirror: halo index h <-> internal index mirror - h */
+        int const mirror = side ? 2 * (HS + IC) - 1 : 2 * HS - 1;
         int const h0 = side ? HS + IC : 0;
         int ext[3] = {TC, dim >= 2 ? TC : 1, L->tz};
         for (int a = 0; a < ext[0]; ++a)
@@ -754,13 +761,17 @@ static void zero_gradient(run_t *r, int lo, int hi, void *arg) {
               if (idx[d] < h0 || idx[d] >= h0 + HS)
                 continue;
               int sidx[3] = {a, c, z};
-              sidx[d] = src_layer;
+              sidx[d] = kind == ORA_BC_SYMMETRY ? mirror - idx[d] : src_layer;
               ptrdiff_t const doff = idx[0] * L->stride[0] +
                                      idx[1] * L->stride[1] + idx[2];
               ptrdiff_t const soff = sidx[0] * L->stride[0] +
                                      sidx[1] * L->stride[1] + sidx[2];
-              for (int e = 0; e < L->ne; ++e)
-                v[e * L->field + doff] = v[e * L->field + soff];
+              for (int e = 0; e < L->ne; ++e) {
+                double const val = v[e * L->field + soff];
+                /* equation order Mass, Energy, MomX, MomY, MomZ */
+                v[e * L->field + doff] =
+                    (kind == ORA_BC_SYMMETRY && e == 2 + d) ? -1.0 * val : val;
+              }
             }
       }
     }
@@ -844,7 +855,7 @@ static void block_stage(run_t *r, int lo, int hi, void *arg) {
 
 static void halo_and_swap(run_t *r, int threads) {
   parallel_blocks(r, threads, halo_copy, NULL);
-  parallel_blocks(r, threads, zero_gradient, NULL);
+  parallel_blocks(r, threads, external_bc, NULL);
   double *t = r->avg;
   r->avg = r->rhs;
   r->rhs = t;
@@ -856,7 +867,7 @@ int ora_run(const ora_grid *g, double cfl, double t_end, int max_steps,
   r.g = *g;
   if (g->dim == 2) {
     r.g.nb[2] = 1;
-    r.g.periodic[2] = 0;
+    r.g.bc[2] = ORA_BC_ZERO_GRADIENT;
   }
   r.L = make_layout(g->dim);
   r.nblk = r.g.nb[0] * r.g.nb[1] * r.g.nb[2];

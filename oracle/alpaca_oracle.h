@@ -23,11 +23,12 @@ extern "C" {
 
 enum { ORA_IC = 16, ORA_HS = 4, ORA_TC = 24 };
 enum { ORA_FLUX_HLLC = 0, ORA_FLUX_ROE = 1 };
+enum { ORA_BC_ZERO_GRADIENT = 0, ORA_BC_PERIODIC = 1, ORA_BC_SYMMETRY = 2 };
 
 typedef struct {
   int dim;         /* 2 or 3 */
   int nb[3];       /* blocks per direction (nb[2] = 1 in 2D) */
-  int periodic[3]; /* 1: periodic in that direction, 0: zero-gradient */
+  int bc[3];       /* per direction, both sides: ORA_BC_* */
   int flux;        /* ORA_FLUX_HLLC or ORA_FLUX_ROE */
   double gamma;    /* stiffened-gas gamma */
   double b;        /* stiffened-gas background pressure (0: ideal gas) */

@@ -22,7 +22,7 @@ _P = ctypes.POINTER(ctypes.c_double)
 
 
 class OraGrid(ctypes.Structure):
-    _fields_ = [("dim", ctypes.c_int), ("nb", ctypes.c_int * 3), ("periodic", ctypes.c_int * 3),
+    _fields_ = [("dim", ctypes.c_int), ("nb", ctypes.c_int * 3), ("bc", ctypes.c_int * 3),
                 ("flux", ctypes.c_int), ("gamma", ctypes.c_double), ("b", ctypes.c_double),
                 ("cell_size", ctypes.c_double)]
 
@@ -139,11 +139,15 @@ def block_rhs(dim, flux, gamma, b, blk, cell_size, which="oracle"):
     return out.reshape(dim + 2, 16, 16, icz)
 
 
-def run(dim, nb, periodic, flux, cfl, t_end, max_steps, state, gamma=1.4, b=0.0,
+BC_ZERO_GRADIENT, BC_PERIODIC, BC_SYMMETRY = 0, 1, 2
+
+
+def run(dim, nb, bc, flux, cfl, t_end, max_steps, state, gamma=1.4, b=0.0,
         cell_size=None, threads=8, which="oracle"):
-    """Advance a domain state [NE][NX][NY][NZ]; returns (state, dts, t_final)."""
+    """Advance a domain state [NE][NX][NY][NZ]; returns (state, dts, t_final).
+    bc: per direction (both sides) 0 zero-gradient, 1 periodic, 2 symmetry."""
     nb3 = list(nb) + [1] * (3 - len(nb))
-    per3 = list(periodic) + [0] * (3 - len(periodic))
+    per3 = list(bc) + [0] * (3 - len(bc))
     if cell_size is None:
         cell_size = 1.0 / (nb3[0] * 16)
     st = np.ascontiguousarray(state, dtype=np.float64).copy()

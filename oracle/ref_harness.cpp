@@ -11,7 +11,7 @@
 // (FiniteVolumeScheme + HllcRiemannSolver/Einfeldt, FluxSplittingScheme/Roe,
 // EigenDecomposition, CharacteristicStateReconstruction, WENO5), the
 // stiffened-gas EOS, ConservativesToPrimeStates, the RungeKutta3TVD time
-// integrator on reference Nodes/Blocks, ZeroGradientBoundaryCondition and
+// integrator on reference Nodes/Blocks, Zero-gradient/Symmetry boundary conditions and
 // ConsistencyManagedSum.  What this file restates around them (all cited):
 // the SpaceSolver divergence loop (space_solver.cpp:50-153, source terms off),
 // the uniform-level Advance choreography (modular_algorithm_assembler.cpp:
@@ -19,6 +19,7 @@
 // same-level halo copy (internal_halo_manager.cpp:439-466) because those live
 // in TUs that need MPI and the topology manager.
 #include "block_definitions/block.h"
+#include "boundary_condition/symmetry_boundary_condition.h"
 #include "boundary_condition/zero_gradient_boundary_condition.h"
 #include "integrator/runge_kutta_3_TVD.h"
 #include "materials/equations_of_state/stiffened_gas.h"
@@ -210,7 +211,7 @@ template <typename F> void ParallelFor(int n, int threads, F const &fn) {
 
 struct Grid {
   int nb[3];
-  int periodic[3];
+  int bc[3];  // 0 zero-gradient, 1 periodic, 2 symmetry (both sides)
   std::vector<std::unique_ptr<Node>> nodes;
   int NumBlocks() const { return nb[0] * nb[1] * nb[2]; }
   int Index(int bx, int by, int bz) const {
@@ -245,7 +246,7 @@ void HaloCopy(Grid &g, int b_lo, int b_hi) {
             int gcell = blk[d] * ic[d] + idx[d];
             int n = g.nb[d] * ic[d];
             if (gcell < 0 || gcell >= n) {
-              if (!g.periodic[d]) {
+              if (g.bc[d] != 1) {
                 outside = true;
                 break;
               }
@@ -266,9 +267,14 @@ void HaloCopy(Grid &g, int b_lo, int b_hi) {
   }
 }
 
-template <BoundaryLocation LOC> void ZeroGradient(Node &node) {
-  ZeroGradientBoundaryCondition<LOC> bc;
-  bc.UpdateMaterialExternal(node, MaterialFieldType::Conservatives);
+template <BoundaryLocation LOC> void External(Node &node, int kind) {
+  if (kind == 2) {
+    SymmetryBoundaryCondition<LOC> bc;
+    bc.UpdateMaterialExternal(node, MaterialFieldType::Conservatives);
+  } else {
+    ZeroGradientBoundaryCondition<LOC> bc;
+    bc.UpdateMaterialExternal(node, MaterialFieldType::Conservatives);
+  }
 }
 
 // External boundaries in the order East, West, North, South, Top, Bottom.
@@ -278,21 +284,21 @@ void ExternalHalo(Grid &g, int b_lo, int b_hi) {
     int by = (b / g.nb[2]) % g.nb[1];
     int bz = b % g.nb[2];
     Node &node = *g.nodes[b];
-    if (!g.periodic[0] && bx == g.nb[0] - 1)
-      ZeroGradient<BoundaryLocation::East>(node);
-    if (!g.periodic[0] && bx == 0)
-      ZeroGradient<BoundaryLocation::West>(node);
+    if (g.bc[0] != 1 && bx == g.nb[0] - 1)
+      External<BoundaryLocation::East>(node, g.bc[0]);
+    if (g.bc[0] != 1 && bx == 0)
+      External<BoundaryLocation::West>(node, g.bc[0]);
     if constexpr (DIMS >= 2) {
-      if (!g.periodic[1] && by == g.nb[1] - 1)
-        ZeroGradient<BoundaryLocation::North>(node);
-      if (!g.periodic[1] && by == 0)
-        ZeroGradient<BoundaryLocation::South>(node);
+      if (g.bc[1] != 1 && by == g.nb[1] - 1)
+        External<BoundaryLocation::North>(node, g.bc[1]);
+      if (g.bc[1] != 1 && by == 0)
+        External<BoundaryLocation::South>(node, g.bc[1]);
     }
     if constexpr (DIMS == 3) {
-      if (!g.periodic[2] && bz == g.nb[2] - 1)
-        ZeroGradient<BoundaryLocation::Top>(node);
-      if (!g.periodic[2] && bz == 0)
-        ZeroGradient<BoundaryLocation::Bottom>(node);
+      if (g.bc[2] != 1 && bz == g.nb[2] - 1)
+        External<BoundaryLocation::Top>(node, g.bc[2]);
+      if (g.bc[2] != 1 && bz == 0)
+        External<BoundaryLocation::Bottom>(node, g.bc[2]);
     }
   }
 }
@@ -391,18 +397,18 @@ void ref_eos(double gamma, double b, double const *cons, double const *prim,
 
 // Full uniform level-0 run.  state: internal conservatives of the whole domain
 // [NE][NX][NY][NZ] (k fastest, NZ = 1 in 2D), overwritten with the final state.
-// periodic[d] != 0 selects periodic sides in d, otherwise zero-gradient.
+// bc[d] per direction (both sides): 0 zero-gradient, 1 periodic, 2 symmetry.
 // Stops after max_steps macro steps or at t_end (whichever first); the dt of
 // each step is written to dts (capacity max_steps).  Returns the step count.
 int ref_run(int family, double gamma, double b, int const *nb,
-            int const *periodic, double cell_size, double cfl, double t_end,
+            int const *bc, double cell_size, double cfl, double t_end,
             int max_steps, int threads, double *state, double *dts,
             double *t_final) {
   EnsureSetup(gamma, b);
   Grid g;
   for (int d = 0; d < 3; ++d) {
     g.nb[d] = d < DIMS ? nb[d] : 1;
-    g.periodic[d] = d < DIMS ? periodic[d] : 0;
+    g.bc[d] = d < DIMS ? bc[d] : 0;
   }
   int const nblk = g.NumBlocks();
   for (int i = 0; i < nblk; ++i)

@@ -130,7 +130,7 @@ def main():
             st0 = cases.four_quadrant_2d(8, 8, seed=seed)
             st, dts, tf = oracle_py.run(2, [8, 8], [0, 0], flux, 0.6, 0.2, 100000, st0, which="ref")
             runs.append({"name": f"quadrant2d_seed{seed}_flux{flux}", "dim": 2, "nb": [8, 8, 1],
-                         "periodic": [0, 0, 0], "flux": flux, "cfl": 0.6, "t_end": 0.2,
+                         "bc": [0, 0, 0], "flux": flux, "cfl": 0.6, "t_end": 0.2,
                          "max_steps": 100000, "ic": {"kind": "four_quadrant_2d", "seed": seed},
                          "steps": int(len(dts)), "dts": dts.tolist(), "t_final": tf,
                          "sha256": sha(st), "l1": np.abs(st).sum(axis=(1, 2, 3)).tolist()})
@@ -141,7 +141,7 @@ def main():
         st, dts, tf = oracle_py.run(3, [2, 2, 2], [1, 1, 1], flux, 0.5, 1e300, 100, st0,
                                     which="ref")
         runs.append({"name": f"blobs3d_seed1_flux{flux}", "dim": 3, "nb": [2, 2, 2],
-                     "periodic": [1, 1, 1], "flux": flux, "cfl": 0.5, "t_end": 1e300,
+                     "bc": [1, 1, 1], "flux": flux, "cfl": 0.5, "t_end": 1e300,
                      "max_steps": 100, "ic": {"kind": "blobs_shock_3d", "seed": 1},
                      "steps": int(len(dts)), "dts": dts.tolist(), "t_final": tf,
                      "sha256": sha(st), "l1": np.abs(st).sum(axis=(1, 2, 3)).tolist()})
@@ -153,10 +153,30 @@ def main():
         st, dts, tf = oracle_py.run(2, [4, 1], [0, 0], flux, 0.6, 0.2, 100000, st0,
                                     cell_size=1.0 / 64, which="ref")
         runs.append({"name": f"sodx2d_flux{flux}", "dim": 2, "nb": [4, 1, 1],
-                     "periodic": [0, 0, 0], "flux": flux, "cfl": 0.6, "t_end": 0.2,
+                     "bc": [0, 0, 0], "flux": flux, "cfl": 0.6, "t_end": 0.2,
                      "max_steps": 100000, "cell_size": 1.0 / 64, "ic": {"kind": "sod_x"},
                      "steps": int(len(dts)), "dts": dts.tolist(), "t_final": tf,
                      "sha256": sha(st), "l1": np.abs(st).sum(axis=(1, 2, 3)).tolist()})
+    # Symmetry sides (SymmetryBoundaryCondition): 2D four-quadrant with
+    # symmetry in x and y, and a 3D case with symmetry / zero-gradient /
+    # periodic in x / y / z.
+    for flux in (0, 1):
+        st0 = cases.four_quadrant_2d(6, 4, seed=7)
+        st, dts, tf = oracle_py.run(2, [6, 4], [2, 2], flux, 0.6, 0.2, 100000, st0, which="ref")
+        runs.append({"name": f"quadrant2d_symmetry_seed7_flux{flux}", "dim": 2, "nb": [6, 4, 1],
+                     "bc": [2, 2, 0], "flux": flux, "cfl": 0.6, "t_end": 0.2,
+                     "max_steps": 100000, "ic": {"kind": "four_quadrant_2d", "seed": 7},
+                     "steps": int(len(dts)), "dts": dts.tolist(), "t_final": tf,
+                     "sha256": sha(st), "l1": np.abs(st).sum(axis=(1, 2, 3)).tolist()})
+        st0 = cases.blobs_shock_3d(2, seed=5)
+        st, dts, tf = oracle_py.run(3, [2, 2, 2], [2, 0, 1], flux, 0.5, 1e300, 60, st0,
+                                    which="ref")
+        runs.append({"name": f"blobs3d_mixedbc_seed5_flux{flux}", "dim": 3, "nb": [2, 2, 2],
+                     "bc": [2, 0, 1], "flux": flux, "cfl": 0.5, "t_end": 1e300,
+                     "max_steps": 60, "ic": {"kind": "blobs_shock_3d", "seed": 5},
+                     "steps": int(len(dts)), "dts": dts.tolist(), "t_final": tf,
+                     "sha256": sha(st), "l1": np.abs(st).sum(axis=(1, 2, 3)).tolist()})
+        print(runs[-2]["name"], runs[-2]["steps"], runs[-1]["name"], flush=True)
     meta["runs"] = runs
     with open(os.path.join(HERE, "golden_runs.json"), "w") as fh:
         json.dump(meta, fh, indent=1)
@@ -201,7 +221,7 @@ def main_full(threads, fluxes):
         wall = time.time() - t0
         name = "roe" if flux == 1 else "hllc"
         meta["runs"][name] = {
-            "dim": 3, "nb": [nb] * 3, "periodic": [1, 1, 1], "flux": flux, "cfl": 0.5,
+            "dim": 3, "nb": [nb] * 3, "bc": [1, 1, 1], "flux": flux, "cfl": 0.5,
             "steps": int(len(dts)), "ic": {"kind": "blobs_shock_3d", "seed": 0},
             "ic_sha256": sha(st0), "sha256": sha(st), "dts": dts.tolist(), "t_final": tf,
             "sums0": st0.sum(axis=(1, 2, 3)).tolist(), "sums": st.sum(axis=(1, 2, 3)).tolist(),

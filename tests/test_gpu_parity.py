@@ -100,8 +100,9 @@ def test_blobs_shock_3d_periodic_100_steps(torch, flux):
 def _face_slab_reference(blocks, dim, nb, bc):
     """Expected face halo slabs of every block (numpy): neighbour's internal
     cells, wrapped on periodic sides, first/last internal layer on
-    zero-gradient sides (internal_halo_manager.cpp:439-466,
-    boundary_constants.h:212-253)."""
+    zero-gradient sides, mirrored internal cells with the normal momentum
+    negated on symmetry sides (internal_halo_manager.cpp:439-466,
+    boundary_constants.h:169-253, symmetry_boundary_condition.h:89-109)."""
     nbk = blocks.shape[0]
     exp = blocks.copy()
     nb3 = list(nb) + [1] * (3 - len(nb))
@@ -119,6 +120,15 @@ def _face_slab_reference(blocks, dim, nb, bc):
                         sl_src[d] = layer
                         exp[(b, slice(None)) + tuple(sl_dst)] = blocks[(b, slice(None)) + tuple(sl_src)]
                     continue
+                if edge and bc[d] == gpu.BC_SYMMETRY:
+                    for h in range(4):
+                        hidx = (20 + h) if hi else h
+                        sl_dst[d] = hidx
+                        sl_src[d] = (39 if hi else 7) - hidx
+                        for e in range(dim + 2):
+                            sign = -1.0 if e == 2 + d else 1.0
+                            exp[(b, e) + tuple(sl_dst)] = sign * blocks[(b, e) + tuple(sl_src)]
+                    continue
                 n = list(c)
                 n[d] = (c[d] + (1 if hi else -1)) % nb3[d]
                 nbi = (n[0] * nb3[1] + n[1]) * nb3[2] + n[2]
@@ -126,6 +136,28 @@ def _face_slab_reference(blocks, dim, nb, bc):
                 sl_src[d] = slice(4, 8) if hi else slice(16, 20)
                 exp[(b, slice(None)) + tuple(sl_dst)] = blocks[(nbi, slice(None)) + tuple(sl_src)]
     return exp
+
+
+@pytest.mark.parametrize("flux", [gpu.FLUX_HLLC, gpu.FLUX_ROE])
+def test_symmetry_sides_match_oracle(torch, flux):
+    """Symmetry sides: 2D four-quadrant with symmetry in x and y to t = 0.2,
+    and 3D blob + shock with symmetry / zero-gradient / periodic sides for 60
+    steps, both bit-exact with the oracle (itself bit-exact with the
+    reference's SymmetryBoundaryCondition by the golden trajectories)."""
+    print(f"[symmetry] flux={flux}", flush=True)
+    st0 = cases.four_quadrant_2d(6, 4, seed=7)
+    case = gpu.GpuCase(gpu.make_desc(2, [6, 4], [2, 2], flux=flux), st0, cfl=0.6, t_end=0.2)
+    steps, dts = case.run(100000)
+    want, want_dts, _ = oracle_py.run(2, [6, 4], [2, 2], flux, 0.6, 0.2, 100000, st0)
+    assert steps == len(want_dts) and np.array_equal(np.array(dts), want_dts)
+    assert np.array_equal(case.download(), want)
+    st0 = cases.blobs_shock_3d(2, seed=5)
+    case = gpu.GpuCase(gpu.make_desc(3, [2, 2, 2], [2, 0, 1], flux=flux), st0, cfl=0.5)
+    case.advance(60)
+    want, _, _ = oracle_py.run(3, [2, 2, 2], [2, 0, 1], flux, 0.5, 1e300, 60, st0)
+    got = case.download()
+    print(f"  3D max rel err={_maxrel(got, want):.3e}", flush=True)
+    assert np.array_equal(got, want)
 
 
 def _face_slab_mask(dim):
@@ -139,7 +171,8 @@ def _face_slab_mask(dim):
 
 
 @pytest.mark.parametrize("dim,nb,bc", [(2, [3, 2], [0, 1]), (3, [2, 3, 2], [1, 0, 1]),
-                                       (3, [1, 1, 2], [1, 1, 0])])
+                                       (3, [1, 1, 2], [1, 1, 0]), (2, [3, 2], [2, 2]),
+                                       (3, [2, 1, 3], [2, 1, 0]), (3, [1, 2, 2], [0, 2, 2])])
 def test_fill_halos_face_slabs(torch, dim, nb, bc):
     print(f"[halo] dim={dim} nb={nb} bc={bc}", flush=True)
     lib = gpu.load_library()

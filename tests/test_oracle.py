@@ -111,7 +111,7 @@ def test_trajectories_golden(runs):
     and the SHA-256 of the final state."""
     for run in runs["runs"]:
         st0 = _initial_state(run)
-        st, dts, tf = oracle_py.run(run["dim"], run["nb"], run["periodic"], run["flux"],
+        st, dts, tf = oracle_py.run(run["dim"], run["nb"], run["bc"], run["flux"],
                                     run["cfl"], run["t_end"], run["max_steps"], st0,
                                     cell_size=run.get("cell_size"))
         assert len(dts) == run["steps"], run["name"]
@@ -316,3 +316,28 @@ def test_oracle_equals_reference_harness_on_fresh_blocks(dim):
             a = oracle_py.block_rhs(dim, flux, 1.4, 0.0, blk, 1.0 / 64)
             b = oracle_py.block_rhs(dim, flux, 1.4, 0.0, blk, 1.0 / 64, which="ref")
             assert np.array_equal(a, b)
+
+
+def test_symmetry_equals_mirrored_periodic_domain():
+    """Symmetry sides on [0,1] equal the right half of the mirror-extended
+    domain [-1,1] with periodic sides (mirror: rho, p even; u odd in x): the
+    mirror image of every halo cell is then an internal cell.  A wrong mirror
+    offset or a missing sign flip of the normal momentum breaks this."""
+    nbx, nby = 3, 2
+    st = cases.four_quadrant_2d(nbx, nby, seed=11)
+    mirrored = st[:, ::-1, :, :].copy()
+    mirrored[2] *= -1.0  # x momentum is odd under the mirror
+    ext = np.concatenate([mirrored, st], axis=1)
+    a, da, ta = oracle_py.run(2, [nbx, nby], [2, 2], 0, 0.6, 0.15, 10000, st,
+                              cell_size=1.0 / (16 * nbx))
+    b, db, tb = oracle_py.run(2, [2 * nbx, nby], [1, 2], 0, 0.6, 0.15, 10000, ext,
+                              cell_size=1.0 / (16 * nbx))
+    assert len(da) == len(db)
+    assert np.max(np.abs(np.array(da) - np.array(db))) <= 1e-14 * max(da)
+    half = b[:, 16 * nbx:]
+    scale = np.abs(a).reshape(4, -1).max(axis=1)[:, None, None, None]
+    assert np.max(np.abs(half - a) / scale) <= 1e-12
+    # and the left half is the mirror image
+    left = b[:, :16 * nbx][:, ::-1]
+    assert np.max(np.abs(left[[0, 1, 3]] - a[[0, 1, 3]]) / scale[[0, 1, 3]]) <= 1e-12
+    assert np.max(np.abs(-left[2] - a[2]) / scale[2]) <= 1e-12
