@@ -1,0 +1,246 @@
+"""Run ALPACA XML input files on the GPU path (single-phase, uniform level).
+
+Reads the subset of the reference's input format that the single-phase
+uniform path uses (input_output/input_reader/*: ``domain/nodeSize``,
+``domain/nodeRatio``, ``domain/boundaryConditions/material``,
+``domain/initialConditions/material1``, ``materials`` with one stiffened gas,
+``timeControl``), places the initial prime states at the cell centres the way
+PrimeStateInitializer::GetInitialPrimeStates does
+(prime_state_initializer.cpp:59-100: ``origin + (i + 0.5) * cell_size`` with
+the block origin summed over the node id's bits as in
+DomainCoordinatesOfId, id_information.h:118-142), converts them with the
+stiffened-gas energy and advances the case to the end time with the HIP path.
+
+What the input can ask for and this path does not do is refused with an
+error rather than ignored: more than one material, another equation of
+state, gravity or viscous/heat-conduction properties, side conditions other
+than ZeroGradient/Symmetry/Periodic or different conditions on the two sides
+of a direction, non-unit dimensionalization references, a start time other
+than zero.  Multiresolution is replaced by a uniform level: ``level`` L gives
+``nodeRatio * 2^L`` blocks per direction with cell size
+``nodeSize / 16 / 2^L`` (CellSizeOfLevel, id_information.h:73-77).
+
+    python -m alpaca_amd.inputfile SodX.xml --dim 2 --flux hllc --out sod.npz
+"""
+from __future__ import annotations
+
+import argparse
+import dataclasses
+import json
+import sys
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+from . import cases, gpu
+from .expression import Program
+
+IC = 16
+_BC_KINDS = {"zerogradient": gpu.BC_ZERO_GRADIENT, "symmetry": gpu.BC_SYMMETRY,
+             "periodic": gpu.BC_PERIODIC}
+_SIDES = (("west", "east"), ("south", "north"), ("bottom", "top"))
+PRIME_NAMES = ("density", "velocityX", "velocityY", "velocityZ", "pressure")
+
+
+class InputError(ValueError):
+    pass
+
+
+@dataclasses.dataclass
+class InputCase:
+    path: str
+    dim: int
+    node_size: float
+    node_ratio: list
+    bc: list
+    gamma: float
+    background_pressure: float
+    ic_text: str
+    end_time: float
+    cfl: float
+    maximum_level: int
+
+
+def _text(root, path, default=None):
+    el = root.find(path)
+    if el is None or el.text is None:
+        if default is not None:
+            return default
+        raise InputError(f"missing <{path}>")
+    return el.text.strip()
+
+
+def _float(root, path, default=None):
+    return float(_text(root, path, None if default is None else str(default)))
+
+
+def read_input(path: str, dim: int | None = None) -> InputCase:
+    """Parse an input file.  dim is the compile-time dimension of the
+    reference build the file is meant for; when None it is 3 if the node
+    ratio has a z entry above one, else 2."""
+    root = ET.parse(path).getroot()
+    ratio_el = root.find("domain/nodeRatio")
+    if ratio_el is None:
+        raise InputError("missing <domain/nodeRatio>")
+    ratio = [int(_text(root, f"domain/nodeRatio/{a}", "1")) for a in "xyz"]
+    if dim is None:
+        dim = 3 if ratio[2] > 1 else 2
+    if dim not in (2, 3):
+        raise InputError("only 2D and 3D builds are supported")
+    if dim == 2:
+        ratio[2] = 1
+    nmat = int(_text(root, "materials/numberOfMaterials", "1"))
+    if nmat != 1:
+        raise InputError(f"{nmat} materials: only single-phase inputs run on this path")
+    eos_type = _text(root, "materials/material1/equationOfState/type")
+    if eos_type != "StiffenedGas":
+        raise InputError(f"equation of state {eos_type}: only StiffenedGas is implemented")
+    gamma = _float(root, "materials/material1/equationOfState/gamma")
+    bpress = _float(root, "materials/material1/equationOfState/backgroundPressure", 0.0)
+    for prop in ("shearViscosity", "bulkViscosity", "thermalConductivity"):
+        v = root.find(f"materials/material1/properties/{prop}")
+        if v is not None and v.text and float(v.text) != 0.0:
+            raise InputError(f"{prop} = {v.text.strip()}: viscous and heat terms are off on "
+                             "this path (Euler)")
+    grav = root.find("sourceTerms/gravity")
+    if grav is not None:
+        for c in grav:
+            if c.text and float(c.text) != 0.0:
+                raise InputError("gravity source term is off on this path")
+    bc = []
+    for d in range(dim):
+        kinds = []
+        for side in _SIDES[d]:
+            name = _text(root, f"domain/boundaryConditions/material/{side}").lower()
+            if name not in _BC_KINDS:
+                raise InputError(f"{side} boundary {name}: only ZeroGradient, Symmetry and "
+                                 "Periodic are implemented")
+            kinds.append(_BC_KINDS[name])
+        if kinds[0] != kinds[1]:
+            raise InputError(f"direction {'xyz'[d]}: both sides must carry the same condition")
+        bc.append(kinds[0])
+    while len(bc) < 3:
+        bc.append(gpu.BC_ZERO_GRADIENT)
+    dimz = root.find("dimensionalization")
+    if dimz is not None:
+        for c in dimz:
+            if c.text and float(c.text) != 1.0:
+                raise InputError("dimensionalization references other than 1 are not "
+                                 "supported")
+    if _float(root, "timeControl/startTime", 0.0) != 0.0:
+        raise InputError("start time must be 0")
+    return InputCase(
+        path=path, dim=dim, node_size=_float(root, "domain/nodeSize"), node_ratio=ratio, bc=bc,
+        gamma=gamma, background_pressure=bpress,
+        ic_text=_text(root, "domain/initialConditions/material1"),
+        end_time=_float(root, "timeControl/endTime"), cfl=_float(root, "timeControl/CFLNumber"),
+        maximum_level=int(_text(root, "multiResolution/maximumLevel", "0")))
+
+
+def block_origin(index: int, block_size: float) -> float:
+    """Origin of the block with integer index along one axis: the sum of
+    bit_k * block_size * 2^k accumulated from the lowest bit up, the order
+    DomainCoordinatesOfId (id_information.h:118-142) adds them in."""
+    x = 0.0
+    size = block_size
+    while index:
+        x += (index & 1) * size
+        index >>= 1
+        size *= 2.0
+    return x
+
+
+def cell_centres(case: InputCase, level: int = 0):
+    """Per-axis cell-centre coordinates of the uniform level (1D arrays)."""
+    block_size = case.node_size / float(1 << level)
+    cell_size = case.node_size / float(IC) / float(1 << level)
+    axes = []
+    for d in range(3):
+        nb = case.node_ratio[d] << level if d < case.dim else 1
+        if d >= case.dim:
+            axes.append(np.zeros(1))
+            continue
+        c = np.empty(nb * IC)
+        for b in range(nb):
+            o = block_origin(b, block_size)
+            c[b * IC:(b + 1) * IC] = o + (np.arange(IC, dtype=np.float64) + 0.5) * cell_size
+        axes.append(c)
+    return axes, cell_size
+
+
+def initial_state(case: InputCase, level: int = 0):
+    """Conservatives [NE][NX][NY][NZ] of the initial condition."""
+    (xs, ys, zs), _ = cell_centres(case, level)
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    prog = Program(case.ic_text)
+    env = prog.run({"x": X, "y": Y, "z": Z}, list(PRIME_NAMES))
+    rho = env["density"]
+    vel = (env["velocityX"], env["velocityY"], env["velocityZ"])
+    p = env["pressure"]
+    cons = cases.primes_to_conservatives(case.dim, case.gamma, case.background_pressure, rho,
+                                         vel[:case.dim], p)
+    if case.dim == 2:
+        cons = cons[:, :, :, :1]
+    return np.ascontiguousarray(cons)
+
+
+def case_desc(case: InputCase, level: int = 0, flux: int = gpu.FLUX_HLLC,
+              precision: int = gpu.FP64):
+    nb = [r << level if d < case.dim else 1 for d, r in enumerate(case.node_ratio)]
+    _, cell_size = cell_centres(case, level)
+    return gpu.make_desc(case.dim, nb[:case.dim], case.bc[:case.dim], flux=flux,
+                         precision=precision, gamma=case.gamma, b=case.background_pressure,
+                         cell_size=cell_size)
+
+
+def run(case: InputCase, level: int = 0, flux: int = gpu.FLUX_HLLC, precision: int = gpu.FP64,
+        max_steps: int = 10 ** 9):
+    """Advance the case to its end time on the GPU; returns a dict with the
+    final conservatives, the primes, the step count, every dt and the time."""
+    desc = case_desc(case, level, flux, precision)
+    st0 = initial_state(case, level)
+    gc = gpu.GpuCase(desc, st0, cfl=case.cfl, t_end=case.end_time)
+    steps, dts = gc.run(max_steps)
+    rec = gc.record()
+    cons = gc.download()
+    return {"conservatives": cons, "steps": steps, "dts": np.array(dts), "time": rec.time,
+            "flags": rec.flags, **primes(case, cons)}
+
+
+def primes(case: InputCase, cons):
+    """Density, velocity and pressure of a conservative state (the
+    ConservativesToPrimeStates arithmetic: u = m * (1/rho), stiffened-gas
+    pressure)."""
+    rho = cons[0]
+    one = 1.0 / rho
+    vel = [cons[2 + d] * one for d in range(case.dim)]
+    m = [cons[2 + d] * cons[2 + d] for d in range(case.dim)] + [0.0] * (3 - case.dim)
+    q = 0.5 * cases._dams(case.dim, m[0], m[1], m[2])
+    p = -case.gamma * case.background_pressure + (case.gamma - 1.0) * (cons[1] - q / rho)
+    return {"density": rho, "velocity": np.stack(vel), "pressure": p}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("input")
+    ap.add_argument("--dim", type=int, default=None)
+    ap.add_argument("--level", type=int, default=0, help="uniform level (multiresolution off)")
+    ap.add_argument("--flux", choices=["hllc", "roe"], default="hllc")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "fp64-fma"], default="fp64")
+    ap.add_argument("--max-steps", type=int, default=10 ** 9)
+    ap.add_argument("--out", default=None, help="write the final state (.npz)")
+    a = ap.parse_args(argv)
+    case = read_input(a.input, a.dim)
+    prec = {"fp64": gpu.FP64, "mixed": gpu.MIXED, "fp64-fma": gpu.FP64_FMA}[a.precision]
+    res = run(case, a.level, gpu.FLUX_HLLC if a.flux == "hllc" else gpu.FLUX_ROE, prec,
+              a.max_steps)
+    if a.out:
+        np.savez(a.out, **{k: v for k, v in res.items()})
+    print(json.dumps({"input": a.input, "dim": case.dim, "level": a.level,
+                      "blocks": [r << a.level for r in case.node_ratio[:case.dim]],
+                      "steps": res["steps"], "time": res["time"], "flags": res["flags"]}))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
