@@ -72,17 +72,15 @@ def cpu_baseline(flux: int, threads: int):
                       f"{dt:.2f} s wall"}
 
 
-def load_traffic(kernel_tag: str):
+def load_profile(kernel_tag: str):
+    """Counter-derived figures of a kernel (profiles/traffic.json, written from
+    the rocprofv3 --pmc runs the profile summaries in profiles/ describe), or {}."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
-    if not os.path.exists(path):
-        return None
     try:
         with open(path) as fh:
-            data = json.load(fh)
-        entry = data.get(kernel_tag)
-        return None if entry is None else float(entry["hbm_bytes_per_launch"])
-    except (OSError, ValueError, KeyError):
-        return None
+            return json.load(fh).get(kernel_tag) or {}
+    except (OSError, ValueError):
+        return {}
 
 
 def run_mode(args, flux, precision, nb, keep_state=False):
@@ -388,13 +386,14 @@ def main():
     tag = f"stage_3d_{args.flux}_{args.precision}"
     variant = {"fp64": "exact::", "fp64-fma": "fast::", "mixed": "fast::"}[args.precision]
     rtype = "float" if args.precision == "mixed" else "double"
+    prof = load_profile(tag)
     roofline = {
         "bound": "mfma",
         "achieved": achieved / 1e12,
         "peak": peak / 1e12,
         "unit": "TFLOP/s",
         "frac": achieved / peak,
-        "traffic": load_traffic(tag),
+        "traffic": prof.get("hbm_bytes_per_launch"),
         "kernel": f"{variant}stage_kernel<3,{'HLLC' if flux == 0 else 'Roe'},{rtype},8,8,8,256>",
         "note": (f"compute-bound VALU stencil (no contraction maps to MFMA; peak is the "
                  f"{'FP32' if args.precision == 'mixed' else 'FP64'} vector figure); "
@@ -402,6 +401,16 @@ def main():
                  f"{head['cells_per_launch']:.0f} cells per launch, avg launch "
                  f"{head['stage_avg_ms']:.3f} ms over {head['stage_launches']} launches"),
     }
+    if prof.get("hbm_bytes_per_launch"):
+        # HBM bytes per launch (scaled FETCH_SIZE + WRITE_SIZE) over this run's launch time.
+        roofline["hbm_achieved_GBs"] = prof["hbm_bytes_per_launch"] / avg_launch_s / 1e9
+        roofline["hbm_frac"] = roofline["hbm_achieved_GBs"] / 8000.0
+    if prof.get("hw_flop_per_launch"):
+        # Hardware FLOP per launch (SQ_INSTS_VALU_FLOPS_FP64 x 64, FMA = 2) over this run's launch time.
+        roofline["valu_flop_frac"] = prof["hw_flop_per_launch"] / avg_launch_s / peak
+    if prof.get("valu_busy"):
+        roofline["valu_busy"] = prof["valu_busy"]
+        roofline["valu_issue_frac"] = prof.get("valu_issue_frac")
     line = {
         "metric": METRIC,
         "value": value,
