@@ -164,23 +164,6 @@ template <typename R> struct CellQ {
 template <int DIM, typename R>
 __device__ __forceinline__ void cell_quantities(double gamma, double b, const double *c,
                                                 CellQ<R> &q) {
-  if constexpr (std::is_same<R, float>::value) {
-    // Mixed form: only the pressure recovery in FP64.
-    double const mz = DIM == 3 ? c[kMomX + DIM - 1] : 0.0;
-    float const p =
-        float(eos_pressure64<DIM>(gamma, b, c[kMass], c[kMomX], c[kMomX + 1], mz, c[kEnergy]));
-    float const rho = float(c[kMass]);
-    float const one_rho = fdiv(1.0f, rho);
-    q.rho = rho;
-    q.one_rho = one_rho;
-    q.vel[0] = float(c[kMomX]) * one_rho;
-    q.vel[1] = float(c[kMomX + 1]) * one_rho;
-    q.vel[2] = DIM == 3 ? float(c[kMomX + 2]) * one_rho : 0.0f;
-    q.p = p;
-    q.psi = eos_psi<float>(float(gamma), float(b), p, one_rho);
-    q.H = (float(c[kEnergy]) + p) * one_rho;
-    return;
-  }
   Prim<double> p;
   cons_to_prim<DIM, double>(gamma, b, c, p);
   double const one_rho = fdiv(1.0, c[kMass]);
