@@ -37,9 +37,10 @@ __device__ __forceinline__ double nr_sqrt64(double x) {
   return x == 0.0 ? 0.0 : g;
 }
 
-// Division and square root: correctly rounded FP64 in the exact namespace;
-// Newton-Raphson FP64 in the fast namespace; hardware reciprocal and square
-// root (about 1 ulp) for the FP32 part of the mixed form.
+// Division and square root: the compiler's FP64 sequences (what a/b and
+// sqrt(x) compile to) in the exact namespace; Newton-Raphson FP64 in the fast
+// namespace; hardware reciprocal and square root (about 1 ulp) for the FP32
+// part of the mixed form.
 template <typename R> __device__ __forceinline__ R fdiv(R a, R b) {
   if constexpr (std::is_same<R, float>::value)
     return a * __builtin_amdgcn_rcpf(b);
@@ -54,7 +55,7 @@ template <typename R> __device__ __forceinline__ R dsqrt(R x) {
   } else if constexpr (kFastArith) {
     return nr_sqrt64(x);
   } else {
-    // Correctly rounded square root.  For finite x >= 2^-767 (high word in
+    // The compiler's square root.  For finite x >= 2^-767 (high word in
     // [0x10000000, 0x7FF00000)) the hardware sequence scales nothing and its
     // zero/infinity select passes the iterate through, so it reduces to this
     // core (v_rsq, then Goldschmidt refinement of g ~ sqrt(x) and
@@ -451,11 +452,19 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
                                           const double (&Ud)[6][DIM + 2], R *f) {
   constexpr int NE = DIM + 2;
   R const gamma = R(gamma64), b = R(b64);
+  // Mixed form: the stencil enters the FP32 reconstruction as differences
+  // from cell i (taken in FP64), and cell i is added back in FP64 after the
+  // back-transform.  Exact arithmetic gives the same flux: the WENO5
+  // smoothness indicators see only differences and its candidate stencils
+  // sum to one, so WENO(v - c) = WENO(v) - c, and R L = I.  In FP32 it keeps
+  // the rounding error proportional to the local variation instead of the
+  // state's magnitude.
+  constexpr bool kShift = std::is_same<R, float>::value;
   R U[6][NE];
 #pragma unroll
   for (int m = 0; m < 6; ++m)
 #pragma unroll
-    for (int e = 0; e < NE; ++e) U[m][e] = R(Ud[m][e]);
+    for (int e = 0; e < NE; ++e) U[m][e] = kShift ? R(Ud[m][e] - Ud[2][e]) : R(Ud[m][e]);
   Eig<NE, R> ev;
 #pragma unroll
   for (int e = 0; e < NE; ++e) f[e] = R(0.0);
@@ -484,6 +493,13 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
     R cl[NE], cr[NE];
     to_physical<NE>(minus, ev.Rm, cl);
     to_physical<NE>(plus, ev.Rm, cr);
+    if constexpr (kShift) {
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        cl[e] = R(double(cl[e]) + Ud[2][e]);
+        cr[e] = R(double(cr[e]) + Ud[2][e]);
+      }
+    }
     R const eps = R(2.220446049250313080847263336181640625e-16);  // DBL_EPSILON
     if (cl[kMass] <= eps || cr[kMass] <= eps) return;
     Prim<R> pl, pr;
@@ -511,14 +527,38 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
     cell_quantities<DIM, R>(gamma64, b64, Ud[3], qn);
     roe_eigensystem<DIM, DIR>(gamma, qt, qn, ev);
     R adv[6][NE];
+    double adv2[NE];  // advection of cell i (FP64; mixed form only)
+    if constexpr (kShift) {
+      // ComputeAdvection in FP64 for the six cells, then differences from
+      // cell i into FP32 (see above).
+      double a64[6][NE];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      R const rho = P[m].rho, p = P[m].p, ud = P[m].vel[DIR];
-      adv[m][kMass] = rho * ud;
-      adv[m][kEnergy] = (U[m][kEnergy] + p) * ud;
+      for (int m = 0; m < 6; ++m) {
+        Prim<double> pd;
+        cons_to_prim<DIM, double>(gamma64, b64, Ud[m], pd);
+        double const ud = pd.vel[DIR];
+        a64[m][kMass] = pd.rho * ud;
+        a64[m][kEnergy] = (Ud[m][kEnergy] + pd.p) * ud;
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) adv[m][kMomX + d] = rho * (ud * P[m].vel[d]);
-      adv[m][kMomX + DIR] += p;
+        for (int d = 0; d < DIM; ++d) a64[m][kMomX + d] = pd.rho * (ud * pd.vel[d]);
+        a64[m][kMomX + DIR] += pd.p;
+      }
+#pragma unroll
+      for (int e = 0; e < NE; ++e) adv2[e] = a64[2][e];
+#pragma unroll
+      for (int m = 0; m < 6; ++m)
+#pragma unroll
+        for (int e = 0; e < NE; ++e) adv[m][e] = R(a64[m][e] - adv2[e]);
+    } else {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        R const rho = P[m].rho, p = P[m].p, ud = P[m].vel[DIR];
+        adv[m][kMass] = rho * ud;
+        adv[m][kEnergy] = (U[m][kEnergy] + p) * ud;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) adv[m][kMomX + d] = rho * (ud * P[m].vel[d]);
+        adv[m][kMomX + DIR] += p;
+      }
     }
     R chf[NE];
 #pragma unroll
@@ -542,8 +582,14 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
     }
     R phys[NE];
     to_physical<NE>(chf, ev.Rm, phys);
+    if constexpr (kShift) {
+      // R L (adv of cell i) = adv of cell i: the half sums of the shifts.
 #pragma unroll
-    for (int e = 0; e < NE; ++e) f[e] += phys[e];
+      for (int e = 0; e < NE; ++e) f[e] += R(double(phys[e]) + adv2[e]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < NE; ++e) f[e] += phys[e];
+    }
   }
 }
 
