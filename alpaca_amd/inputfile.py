@@ -21,6 +21,7 @@ than zero.  Multiresolution is replaced by a uniform level: ``level`` L gives
 ``nodeSize / 16 / 2^L`` (CellSizeOfLevel, id_information.h:73-77).
 
     python -m alpaca_amd.inputfile SodX.xml --dim 2 --flux hllc --out sod.npz
+    python -m alpaca_amd.inputfile SodX.xml --out sod.h5   # reference HDF5 + XDMF layout
 """
 from __future__ import annotations
 
@@ -228,13 +229,22 @@ def main(argv=None):
     ap.add_argument("--flux", choices=["hllc", "roe"], default="hllc")
     ap.add_argument("--precision", choices=["fp64", "mixed", "fp64-fma"], default="fp64")
     ap.add_argument("--max-steps", type=int, default=10 ** 9)
-    ap.add_argument("--out", default=None, help="write the final state (.npz)")
+    ap.add_argument("--out", default=None,
+                    help="write the final state: .npz (all arrays) or .h5 (the reference's "
+                         "HDF5 + XDMF output layout)")
     a = ap.parse_args(argv)
     case = read_input(a.input, a.dim)
     prec = {"fp64": gpu.FP64, "mixed": gpu.MIXED, "fp64-fma": gpu.FP64_FMA}[a.precision]
     res = run(case, a.level, gpu.FLUX_HLLC if a.flux == "hllc" else gpu.FLUX_ROE, prec,
               a.max_steps)
-    if a.out:
+    if a.out and a.out.endswith(".h5"):
+        from . import h5output
+
+        nb = [r << a.level for r in case.node_ratio]
+        _, cell_size = cell_centres(case, a.level)
+        h5output.write(a.out[:-3], res["time"], case.dim, nb, cell_size, res["density"],
+                       res["velocity"], res["pressure"])
+    elif a.out:
         np.savez(a.out, **{k: v for k, v in res.items()})
     print(json.dumps({"input": a.input, "dim": case.dim, "level": a.level,
                       "blocks": [r << a.level for r in case.node_ratio[:case.dim]],

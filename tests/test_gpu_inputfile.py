@@ -84,3 +84,15 @@ def test_input_file_command_line(torch, tmp_path):
     assert rc == 0
     data = np.load(out)
     assert float(data["time"]) == 0.2 and data["density"].shape == (16, 64, 1)
+    from alpaca_amd import h5output
+
+    try:
+        h5output._lib()
+    except h5output.Hdf5Error:
+        return
+    rc = inputfile.main([os.path.join(INPUTS, "DetailedSod_2D_SodY.xml"), "--flux", "roe",
+                         "--out", str(tmp_path / "sod.h5")])
+    assert rc == 0
+    rho = h5output.read_dataset(str(tmp_path / "sod.h5"), "/cell_data/density")
+    assert rho.shape == (16 * 64, 1, 1)
+    assert np.array_equal(np.sort(rho.ravel()), np.sort(data["density"].ravel()))
