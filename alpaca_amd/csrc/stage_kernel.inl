@@ -253,14 +253,21 @@ __global__ __launch_bounds__(NT) void batch_stage_kernel(
   __shared__ R lds[T::LDS];
   __shared__ int skip;
   constexpr unsigned kXcd = 8;
-  unsigned const units = unsigned(*n_items) * T::TILES;
+  unsigned const n = unsigned(*n_items);
   unsigned const groups = gridDim.x;
+  // Work items are dealt to the 8 XCDs round-robin (item k to XCD k % 8; the
+  // hardware deals workgroups round-robin too, so blockIdx % 8 is the XCD),
+  // all tiles of an item staying on one XCD's L2.  Round-robin rather than
+  // contiguous ranges keeps the XCDs balanced when the active cases cluster
+  // in the list (finished cases are skipped); within an XCD the workgroups
+  // stride over its items' tiles.
   unsigned const xcd = blockIdx.x % kXcd;
   unsigned const per_xcd = groups / kXcd;  // grid is a multiple of 8
   unsigned const m = blockIdx.x / kXcd;
-  unsigned const lo = (unsigned)((unsigned long long)units * xcd / kXcd);
-  unsigned const hi = (unsigned)((unsigned long long)units * (xcd + 1) / kXcd);
-  for (unsigned u = lo + m; u < hi; u += per_xcd) {
+  unsigned const items_here = n > xcd ? (n - xcd + kXcd - 1) / kXcd : 0u;
+  unsigned const units_here = items_here * T::TILES;
+  for (unsigned w = m; w < units_here; w += per_xcd) {
+    unsigned const u = (xcd + kXcd * (w / T::TILES)) * T::TILES + w % T::TILES;
     WorkItem const it = items[u / T::TILES];
     alpaca_case_state *st = states + it.case_id;
     // Another workgroup may flag the case during this stage: decide once per
