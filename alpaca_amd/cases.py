@@ -52,6 +52,19 @@ def _dams(dim, a, b, c):
     return 0.5 * (np.minimum(np.minimum(s1, s2), s3) + np.maximum(np.maximum(s1, s2), s3))
 
 
+def conservatives_to_primes(dim, gamma, b, cons):
+    """(rho, [u, v, (w)], p) of conservatives cons[NE][...] with the
+    ConservativesToPrimeStates arithmetic (prime_state_handler.h:279-370):
+    u = m * (1 / rho), stiffened-gas pressure."""
+    rho = cons[0]
+    one = 1.0 / rho
+    vel = [cons[2 + d] * one for d in range(dim)]
+    m2 = [cons[2 + d] * cons[2 + d] for d in range(dim)] + [0.0] * (3 - dim)
+    q = 0.5 * _dams(dim, m2[0], m2[1], m2[2])
+    p = -gamma * b + (gamma - 1.0) * (cons[1] - q / rho)
+    return rho, vel, p
+
+
 def primes_to_conservatives(dim, gamma, b, rho, vel, p):
     """PrimeStatesToConservatives with the stiffened-gas energy."""
     u = vel[0]

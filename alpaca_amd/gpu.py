@@ -187,7 +187,7 @@ class GpuCase:
     97-195, 543-910)."""
 
     def __init__(self, desc: CaseDesc, state0: np.ndarray, cfl: float, t_end: float = 1e300,
-                 device="cuda"):
+                 device="cuda", time0: float = 0.0):
         torch = _torch()
         self.lib = load_library()
         self.desc = desc
@@ -207,8 +207,8 @@ class GpuCase:
         st = self._stream()
         check(self.lib.alpaca_fill_halos(ctypes.byref(desc), ctypes.c_void_p(self.u.data_ptr()), st),
               "fill_halos")
-        check(self.lib.alpaca_case_state_init(ctypes.c_void_p(self.state.data_ptr()), 0.0, t_end,
-                                              cfl, st), "case_state_init")
+        check(self.lib.alpaca_case_state_init(ctypes.c_void_p(self.state.data_ptr()), time0,
+                                              t_end, cfl, st), "case_state_init")
 
     def _stream(self):
         return _stream_ptr()
@@ -224,8 +224,10 @@ class GpuCase:
         raw = self.state.cpu().numpy().tobytes()
         return CaseState.from_buffer_copy(raw)
 
-    def run(self, max_steps: int, sync_every: int = 1):
-        """ComputeLoop: advance while time < t_end (read back each step)."""
+    def run(self, max_steps: int, sync_every: int = 1, on_step=None):
+        """ComputeLoop: advance while time < t_end (read back each step);
+        on_step(record) runs after each macro step, where ComputeLoop writes
+        restart snapshots and output."""
         steps = 0
         dts = []
         while steps < max_steps:
@@ -238,12 +240,20 @@ class GpuCase:
             dts.append(rec.dt)
             if rec.flags & 1:
                 raise AlpacaError("non-finite state")
+            if on_step is not None:
+                on_step(rec)
             if rec.flags & 2:
                 break
         return steps, dts
 
     def download(self) -> np.ndarray:
         return from_blocks(self.u, self.dim, self.nb).cpu().numpy()
+
+    def download_blocks(self) -> np.ndarray:
+        """The haloed blocks as the device holds them, [n][NE][24][24][TZ]
+        (TZ = 1 in 2D)."""
+        tz = TC if self.dim == 3 else 1
+        return self.u.cpu().numpy().reshape(self.nblocks, NE(self.dim), TC, TC, tz)
 
 
 def profile_enable(on: bool = True):

@@ -22,6 +22,11 @@ than zero.  Multiresolution is replaced by a uniform level: ``level`` L gives
 
     python -m alpaca_amd.inputfile SodX.xml --dim 2 --flux hllc --out sod.npz
     python -m alpaca_amd.inputfile SodX.xml --out sod.h5   # reference HDF5 + XDMF layout
+    python -m alpaca_amd.inputfile SodX.xml --restore SodX/restart/restart_0.100000.h5
+
+Restart snapshots follow the input's ``<restart>`` section (restore mode and
+file, snapshot stamps / wall-clock interval / intervals to keep) in the
+reference's restart file layout (alpaca_amd.restart).
 """
 from __future__ import annotations
 
@@ -60,6 +65,12 @@ class InputCase:
     end_time: float
     cfl: float
     maximum_level: int
+    restore_mode: str = "off"          # restart/restore/mode: off, soft or forced
+    restore_file: str = ""
+    snapshot_stamps: list = dataclasses.field(default_factory=list)
+    snapshot_interval: int = 0         # wall seconds, 0 = no interval snapshots
+    snapshots_to_keep: int = 0
+    time_naming_factor: float = 1.0
 
 
 def _text(root, path, default=None):
@@ -130,12 +141,38 @@ def read_input(path: str, dim: int | None = None) -> InputCase:
                                  "supported")
     if _float(root, "timeControl/startTime", 0.0) != 0.0:
         raise InputError("start time must be 0")
+    end_time = _float(root, "timeControl/endTime")
+    restore_mode = _text(root, "restart/restore/mode", "Off").replace(" ", "").lower()
+    if restore_mode not in ("off", "soft", "forced"):
+        raise InputError(f"restore mode {restore_mode} not known")
+    restore_file = _text(root, "restart/restore/fileName", "") if restore_mode != "off" else ""
+    snap_type = _text(root, "restart/snapshots/type", "Off").replace(" ", "").lower()
+    if snap_type not in ("off", "interval", "stamps", "intervalstamps"):
+        raise InputError(f"snapshot type {snap_type} not known")
+    stamps = []
+    if snap_type in ("stamps", "intervalstamps"):
+        # ComputeSnapshotTimes (instantiation_input_output_manager.cpp:168-230):
+        # <ts1>, <ts2>, ... inside [start, end], then the end time itself
+        k = 1
+        while root.find(f"restart/snapshots/stamps/ts{k}") is not None:
+            stamps.append(_float(root, f"restart/snapshots/stamps/ts{k}"))
+            k += 1
+        stamps = [t for t in stamps if 0.0 <= t <= end_time] + [end_time]
+        if stamps != sorted(stamps):
+            raise InputError("snapshot timestamps must be ascending")
+    interval = keep = 0
+    if snap_type in ("interval", "intervalstamps"):
+        interval = int(_text(root, "restart/snapshots/interval"))
+        keep = int(_text(root, "restart/snapshots/intervalsToKeep"))
     return InputCase(
         path=path, dim=dim, node_size=_float(root, "domain/nodeSize"), node_ratio=ratio, bc=bc,
         gamma=gamma, background_pressure=bpress,
         ic_text=_text(root, "domain/initialConditions/material1"),
-        end_time=_float(root, "timeControl/endTime"), cfl=_float(root, "timeControl/CFLNumber"),
-        maximum_level=int(_text(root, "multiResolution/maximumLevel", "0")))
+        end_time=end_time, cfl=_float(root, "timeControl/CFLNumber"),
+        maximum_level=int(_text(root, "multiResolution/maximumLevel", "0")),
+        restore_mode=restore_mode, restore_file=restore_file, snapshot_stamps=stamps,
+        snapshot_interval=interval, snapshots_to_keep=keep,
+        time_naming_factor=_float(root, "output/timeNamingFactor", 1.0))
 
 
 def block_origin(index: int, block_size: float) -> float:
@@ -194,30 +231,86 @@ def case_desc(case: InputCase, level: int = 0, flux: int = gpu.FLUX_HLLC,
                          cell_size=cell_size)
 
 
+def restore_path(case: InputCase, restore: str | None = None):
+    """The restart file to start from, or None (RestoreSimulationFromSnapshot,
+    input_output_manager.cpp:477-520): an explicit path must exist; from the
+    input file, mode Soft starts fresh when the file is missing and Forced
+    fails."""
+    import os
+
+    if restore:
+        if not os.path.exists(restore):
+            raise InputError(f"restart file {restore} does not exist")
+        return restore
+    if case.restore_mode == "off":
+        return None
+    if os.path.exists(case.restore_file):
+        return case.restore_file
+    if case.restore_mode == "forced":
+        raise InputError("Force restore not possible: restart file "
+                         f"{case.restore_file} does not exist")
+    return None
+
+
 def run(case: InputCase, level: int = 0, flux: int = gpu.FLUX_HLLC, precision: int = gpu.FP64,
-        max_steps: int = 10 ** 9):
+        max_steps: int = 10 ** 9, restart_folder: str | None = None, restore: str | None = None):
     """Advance the case to its end time on the GPU; returns a dict with the
-    final conservatives, the primes, the step count, every dt and the time."""
+    final conservatives, the primes, the step count, every dt, the time and
+    the restart files written.
+
+    Like ModularAlgorithmAssembler::Initialization and ComputeLoop
+    (modular_algorithm_assembler.cpp:97-240): the state comes from a restart
+    file when restore_path() names one (its time becomes the start time),
+    else from the initial condition; with restart_folder set, snapshots are
+    written there after initialisation and after each macro step as the
+    input's <restart><snapshots> asks (restart.SnapshotSchedule), forced
+    when the time step turns unhealthy."""
+    from . import restart as rs
+
     desc = case_desc(case, level, flux, precision)
-    st0 = initial_state(case, level)
-    gc = gpu.GpuCase(desc, st0, cfl=case.cfl, t_end=case.end_time)
-    steps, dts = gc.run(max_steps)
+    path = restore_path(case, restore)
+    if (path or restart_folder) and level != 0:
+        raise InputError("restart files hold level 0 only")
+    time0 = 0.0
+    if path:
+        time0, fdim, fnb, blocks = rs.read(path, case.dim)
+        want = [case.node_ratio[d] for d in range(case.dim)]
+        if list(fnb) != want:
+            raise InputError(f"restart file has {fnb} blocks, the input {want}")
+        st0 = rs.domain_from_blocks(blocks, case.dim, fnb)
+    else:
+        st0 = initial_state(case, level)
+    gc = gpu.GpuCase(desc, st0, cfl=case.cfl, t_end=case.end_time, time0=time0)
+    schedule = None
+    if restart_folder is not None:
+        schedule = rs.SnapshotSchedule(restart_folder, case.snapshot_stamps,
+                                       case.snapshot_interval, case.snapshots_to_keep,
+                                       case.time_naming_factor,
+                                       restored_time=time0 if path else None)
+
+        def write(t, force=False):
+            schedule.maybe_write(t, lambda f: rs.write(
+                f, t, case.dim, desc.nb[:case.dim], gc.download_blocks(), case.gamma,
+                case.background_pressure), force)
+
+        write(time0)
+        on_step = lambda rec: write(rec.time, bool(rec.flags & 2))  # noqa: E731
+    else:
+        on_step = None
+    steps, dts = gc.run(max_steps, on_step=on_step)
     rec = gc.record()
     cons = gc.download()
     return {"conservatives": cons, "steps": steps, "dts": np.array(dts), "time": rec.time,
-            "flags": rec.flags, **primes(case, cons)}
+            "start_time": time0, "flags": rec.flags,
+            "restart_files": list(schedule.written) if schedule else [], **primes(case, cons)}
 
 
 def primes(case: InputCase, cons):
     """Density, velocity and pressure of a conservative state (the
     ConservativesToPrimeStates arithmetic: u = m * (1/rho), stiffened-gas
     pressure)."""
-    rho = cons[0]
-    one = 1.0 / rho
-    vel = [cons[2 + d] * one for d in range(case.dim)]
-    m = [cons[2 + d] * cons[2 + d] for d in range(case.dim)] + [0.0] * (3 - case.dim)
-    q = 0.5 * cases._dams(case.dim, m[0], m[1], m[2])
-    p = -case.gamma * case.background_pressure + (case.gamma - 1.0) * (cons[1] - q / rho)
+    rho, vel, p = cases.conservatives_to_primes(case.dim, case.gamma, case.background_pressure,
+                                                cons)
     return {"density": rho, "velocity": np.stack(vel), "pressure": p}
 
 
@@ -232,11 +325,21 @@ def main(argv=None):
     ap.add_argument("--out", default=None,
                     help="write the final state: .npz (all arrays) or .h5 (the reference's "
                          "HDF5 + XDMF output layout)")
+    ap.add_argument("--restart-dir", default=None,
+                    help="folder for the restart snapshots the input file asks for "
+                         "(default <input name>/restart in the working directory)")
+    ap.add_argument("--restore", default=None,
+                    help="start from this restart file (overrides <restart><restore>)")
     a = ap.parse_args(argv)
     case = read_input(a.input, a.dim)
     prec = {"fp64": gpu.FP64, "mixed": gpu.MIXED, "fp64-fma": gpu.FP64_FMA}[a.precision]
+    folder = a.restart_dir
+    if folder is None and a.level == 0:
+        import os
+
+        folder = os.path.join(os.path.splitext(os.path.basename(a.input))[0], "restart")
     res = run(case, a.level, gpu.FLUX_HLLC if a.flux == "hllc" else gpu.FLUX_ROE, prec,
-              a.max_steps)
+              a.max_steps, restart_folder=folder, restore=a.restore)
     if a.out and a.out.endswith(".h5"):
         from . import h5output
 
@@ -248,7 +351,9 @@ def main(argv=None):
         np.savez(a.out, **{k: v for k, v in res.items()})
     print(json.dumps({"input": a.input, "dim": case.dim, "level": a.level,
                       "blocks": [r << a.level for r in case.node_ratio[:case.dim]],
-                      "steps": res["steps"], "time": res["time"], "flags": res["flags"]}))
+                      "steps": res["steps"], "start_time": res["start_time"],
+                      "time": res["time"], "flags": res["flags"],
+                      "restart_files": res["restart_files"]}))
     return 0
 
 

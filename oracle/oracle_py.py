@@ -57,6 +57,15 @@ def ref(dim: int):
     return _refs[dim]
 
 
+def ref_level0_ids(nb, dim=3):
+    """Level-0 node ids of a block grid nb = (nx, ny, nz) in the order the
+    reference's TopologyManager creates them (its own id arithmetic)."""
+    nb3 = (ctypes.c_int * 3)(*(list(nb) + [1] * (3 - len(nb))))
+    out = np.zeros(nb3[0] * nb3[1] * nb3[2], dtype=np.uint64)
+    ref(dim).ref_level0_ids(nb3, out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
 def nflux(dim):
     icz = 16 if dim == 3 else 1
     return 3 * (dim + 2) * 17 * 17 * (icz + 1)

@@ -522,4 +522,23 @@ int ref_run(int family, double gamma, double b, int const *nb,
   return steps;
 }
 
+// Level-0 node ids of a block grid in the order TopologyManager creates them
+// (topology_manager.cpp:109-140: x fastest, then y, then z, each step taken
+// with the reference's own neighbour-id arithmetic).  out[nx*ny*nz].
+void ref_level0_ids(int const *nb, unsigned long long *out) {
+  std::vector<nid_t> list;
+  nid_t id = IdSeed();
+  for (int i = 0; i < nb[2]; ++i) {
+    for (int j = 0; j < nb[1]; ++j) {
+      for (int k = 0; k < nb[0]; ++k) {
+        list.push_back(id);
+        id = EastNeighborOfNodeWithId(list.back());
+      }
+      id = NorthNeighborOfNodeWithId(list[nb[0] * (j + nb[1] * i)]);
+    }
+    id = TopNeighborOfNodeWithId(list[(nb[1] * nb[0]) * i]);
+  }
+  for (size_t n = 0; n < list.size(); ++n) out[n] = list[n];
+}
+
 } // extern "C"

@@ -25,6 +25,9 @@ half an hour per scheme on 8 cores) and writes
   golden_full_256.npz   the final state at 4096 fixed cells (all equations).
 
     python tests/golden/make_golden.py --full [--threads N] [--flux roe|hllc]
+
+With ``--ids`` it writes level0_ids.json only (level-0 node ids, used to pin
+the restart files' node-id encoding).
 """
 from __future__ import annotations
 
@@ -234,6 +237,26 @@ def main_full(threads, fluxes):
         np.savez_compressed(path_npz, **arrays)
 
 
+ID_SHAPES_FULL = ([1, 1, 1], [4, 1, 1], [3, 5, 1], [2, 3, 4], [5, 4, 3])
+ID_SHAPES_HASHED = ([16, 16, 16], [128, 2, 1], [1, 128, 2], [2, 1, 128], [64, 64, 1])
+
+
+def main_ids():
+    """level0_ids.json: the reference's level-0 node ids (TopologyManager's
+    creation loop over the id arithmetic of id_information.cpp) in creation
+    order, in full for small grids and as SHA-256 of the uint64 array for
+    large ones."""
+    meta = {"full": {}, "sha256": {}}
+    for nb in ID_SHAPES_FULL:
+        meta["full"][",".join(map(str, nb))] = [int(v) for v in oracle_py.ref_level0_ids(nb)]
+    for nb in ID_SHAPES_HASHED:
+        ids = oracle_py.ref_level0_ids(nb)
+        meta["sha256"][",".join(map(str, nb))] = hashlib.sha256(
+            np.ascontiguousarray(ids, dtype=np.uint64).tobytes()).hexdigest()
+    with open(os.path.join(HERE, "level0_ids.json"), "w") as fh:
+        json.dump(meta, fh, indent=1)
+
+
 if __name__ == "__main__":
     import argparse
 
@@ -241,8 +264,11 @@ if __name__ == "__main__":
     ap.add_argument("--full", action="store_true")
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 8)
     ap.add_argument("--flux", choices=["roe", "hllc", "both"], default="both")
+    ap.add_argument("--ids", action="store_true", help="write level0_ids.json only")
     a = ap.parse_args()
-    if a.full:
+    if a.ids:
+        main_ids()
+    elif a.full:
         main_full(a.threads, {"roe": [1], "hllc": [0], "both": [1, 0]}[a.flux])
     else:
         main()
