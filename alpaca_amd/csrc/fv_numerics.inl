@@ -245,11 +245,38 @@ template <int NE, typename R> struct Eig {
   R L[NE][NE];
   R Rm[NE][NE];
   R lam[NE];
+  // Roe-averaged quantities the right eigenvectors are built from (filled by
+  // roe_eigensystem, read by roe_right_eigenvectors).
+  R u, v, w, c, H, g, rho_roe;
+  bool ok;
 };
 
-// ComputeRoeEigendecomposition for one face (eigendecomposition.h:238-466).
-// t = cell i, n = cell i+1 along DIR.  Returns false (all zero) when a
-// density is not positive.
+// Structural zeros of the Roe eigenvectors below: the left-eigenvector rows
+// of the shear waves (row 1, and row 2 in 3D) hold only the mass entry and
+// that wave's transverse momentum entry, and the right-eigenvector columns
+// 1 and 2 only that momentum entry and the energy entry.  The reference
+// multiplies by these zeros too (characteristic_state_reconstruction.h:
+// 85-103, eigendecomposition.h:144-165); leaving out a product 0 * x of a
+// finite x changes at most the sign of a zero partial sum, and no sign of
+// zero reaches a non-zero value downstream (no division by these sums; the
+// flux divergence is formed as 0.0 + (...), which maps -0 to +0), so the
+// state bits are unchanged.
+template <int DIM, int DIR> __device__ __forceinline__ constexpr bool l_nonzero(int n, int l) {
+  if (n == 1) return l == kMass || l == kMomX + Minor<DIR>::m1;
+  if (DIM == 3 && n == 2) return l == kMass || l == kMomX + Minor<DIR>::m2;
+  return true;
+}
+template <int DIM, int DIR> __device__ __forceinline__ constexpr bool r_nonzero(int l, int n) {
+  if (n == 1) return l == kEnergy || l == kMomX + Minor<DIR>::m1;
+  if (DIM == 3 && n == 2) return l == kEnergy || l == kMomX + Minor<DIR>::m2;
+  return true;
+}
+
+// ComputeRoeEigendecomposition for one face (eigendecomposition.h:238-466):
+// the left eigenvectors and eigenvalues here, the right eigenvectors in
+// roe_right_eigenvectors (called once the characteristic fluxes are formed,
+// so they are not held in registers meanwhile).  t = cell i, n = cell i+1
+// along DIR.  Returns false (all zero) when a density is not positive.
 template <int DIM, int DIR, typename R>
 __device__ __forceinline__ bool roe_eigensystem(R gamma, const CellQ<R> &t, const CellQ<R> &n,
                                                 Eig<DIM + 2, R> &ev) {
@@ -260,13 +287,12 @@ __device__ __forceinline__ bool roe_eigensystem(R gamma, const CellQ<R> &t, cons
 #pragma unroll
   for (int r = 0; r < NE; ++r) {
 #pragma unroll
-    for (int c = 0; c < NE; ++c) {
-      ev.L[r][c] = R(0.0);
-      ev.Rm[r][c] = R(0.0);
-    }
+    for (int c = 0; c < NE; ++c) ev.L[r][c] = R(0.0);
     ev.lam[r] = R(0.0);
   }
+  ev.ok = false;
   if (t.rho <= R(0.0) || n.rho <= R(0.0)) return false;
+  ev.ok = true;
 
   R const u_t = t.vel[DIR];
   R const v_t = t.vel[m1];
@@ -323,6 +349,38 @@ __device__ __forceinline__ bool roe_eigensystem(R gamma, const CellQ<R> &t, cons
   if constexpr (DIM == 3) ev.L[NE - 1][M2] = R(0.5) * one_cc * (-w * g);
   ev.L[NE - 1][kEnergy] = R(0.5) * one_cc * g;
 
+  ev.u = u;
+  ev.v = v;
+  ev.w = w;
+  ev.c = c;
+  ev.H = H;
+  ev.g = g;
+  ev.rho_roe = rho_roe;
+
+  ev.lam[0] = __builtin_fabs(u - c);
+  ev.lam[NE - 1] = __builtin_fabs(u + c);
+#pragma unroll
+  for (int l = 1; l < NE - 1; ++l) ev.lam[l] = __builtin_fabs(u);
+  return true;
+}
+
+// The right eigenvectors of roe_eigensystem (eigendecomposition.h:238-466).
+template <int DIM, int DIR, typename R>
+__device__ __forceinline__ void roe_right_eigenvectors(Eig<DIM + 2, R> &ev) {
+  constexpr int NE = DIM + 2;
+  constexpr int m1 = Minor<DIR>::m1, m2 = Minor<DIR>::m2;
+  constexpr int evp = DIM;
+  constexpr int Mp = kMomX + DIR, M1 = kMomX + m1, M2 = kMomX + m2;
+  (void)m2;
+  (void)M2;
+#pragma unroll
+  for (int r = 0; r < NE; ++r)
+#pragma unroll
+    for (int c = 0; c < NE; ++c) ev.Rm[r][c] = R(0.0);
+  if (!ev.ok) return;
+  R const u = ev.u, v = ev.v, w = ev.w, c = ev.c, H = ev.H, g = ev.g, rho_roe = ev.rho_roe;
+  (void)w;
+
   ev.Rm[kMass][0] = R(1.0);
   ev.Rm[kMass][evp] = g;
   ev.Rm[kMass][NE - 1] = R(1.0);
@@ -348,22 +406,19 @@ __device__ __forceinline__ bool roe_eigensystem(R gamma, const CellQ<R> &t, cons
   if constexpr (DIM == 3) ev.Rm[kEnergy][2] = rho_roe * w;
   ev.Rm[kEnergy][evp] = g * H - c * c;
   ev.Rm[kEnergy][NE - 1] = H + u * c;
-
-  ev.lam[0] = __builtin_fabs(u - c);
-  ev.lam[NE - 1] = __builtin_fabs(u + c);
-#pragma unroll
-  for (int l = 1; l < NE - 1; ++l) ev.lam[l] = __builtin_fabs(u);
-  return true;
 }
 
-// TransformToPhysicalSpace (eigendecomposition.h:144-165).
-template <int NE, typename R>
-__device__ __forceinline__ void to_physical(const R *ch, const R (&Rm)[NE][NE], R *out) {
+// TransformToPhysicalSpace (eigendecomposition.h:144-165), structural zeros
+// of the right eigenvectors left out (see r_nonzero).
+template <int DIM, int DIR, typename R>
+__device__ __forceinline__ void to_physical(const R *ch, const R (&Rm)[DIM + 2][DIM + 2], R *out) {
+  constexpr int NE = DIM + 2;
 #pragma unroll
   for (int l = 0; l < NE; ++l) {
     R s = R(0.0);
 #pragma unroll
-    for (int n = 1; n < NE - 1; ++n) s += ch[n] * Rm[l][n];
+    for (int n = 1; n < NE - 1; ++n)
+      if (r_nonzero<DIM, DIR>(l, n)) s += ch[n] * Rm[l][n];
     s += (ch[0] * Rm[l][0] + ch[NE - 1] * Rm[l][NE - 1]);
     out[l] = s;
   }
@@ -440,6 +495,17 @@ __device__ __forceinline__ void hllc(R gamma, R b, const R *cl, const R *cr, con
   }
 }
 
+// ComputeAdvection (flux_splitting_scheme.cpp:42-76) of one cell in FP64.
+template <int DIM, int DIR>
+__device__ __forceinline__ void advection(const double *c, const Prim<double> &pd, double *a) {
+  double const ud = pd.vel[DIR];
+  a[kMass] = pd.rho * ud;
+  a[kEnergy] = (c[kEnergy] + pd.p) * ud;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) a[kMomX + d] = pd.rho * (ud * pd.vel[d]);
+  a[kMomX + DIR] += pd.p;
+}
+
 // One face flux from the six-cell stencil Ud[m] (FP64 conservatives of cells
 // i-2..i+3 along DIR); reconstruction and Riemann arithmetic in R.
 // FV:  FiniteVolumeScheme::ComputeFluxes (finite_volume_scheme.cpp:160-251)
@@ -449,9 +515,15 @@ __device__ __forceinline__ void hllc(R gamma, R b, const R *cl, const R *cr, con
 // Writes f (zero when the reference skips the face).
 template <int DIM, int DIR, int FLUX, typename R>
 __device__ __forceinline__ void face_flux(double gamma64, double b64,
-                                          const double (&Ud)[6][DIM + 2], R *f) {
+                                          const double *__restrict__ base, R *f) {
   constexpr int NE = DIM + 2;
   R const gamma = R(gamma64), b = R(b64);
+  // Cell i-2+m of the stencil: base points at cell i's first field.
+  auto load = [base](int m, double *c) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e)
+      c[e] = base[e * Dims<DIM>::FIELD + (m - 2) * axis_stride<DIM, DIR>()];
+  };
   // Mixed form: the stencil enters the FP32 reconstruction as differences
   // from cell i (taken in FP64), and cell i is added back in FP64 after the
   // back-transform.  Exact arithmetic gives the same flux: the WENO5
@@ -460,15 +532,18 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
   // the rounding error proportional to the local variation instead of the
   // state's magnitude.
   constexpr bool kShift = std::is_same<R, float>::value;
-  R U[6][NE];
-#pragma unroll
-  for (int m = 0; m < 6; ++m)
-#pragma unroll
-    for (int e = 0; e < NE; ++e) U[m][e] = kShift ? R(Ud[m][e] - Ud[2][e]) : R(Ud[m][e]);
   Eig<NE, R> ev;
 #pragma unroll
   for (int e = 0; e < NE; ++e) f[e] = R(0.0);
   if constexpr (FLUX == 0) {
+    double Ud[6][NE];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) load(m, Ud[m]);
+    R U[6][NE];
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+      for (int e = 0; e < NE; ++e) U[m][e] = kShift ? R(Ud[m][e] - Ud[2][e]) : R(Ud[m][e]);
     CellQ<R> qt, qn;
     cell_quantities<DIM, R>(gamma64, b64, Ud[2], qt);
     cell_quantities<DIM, R>(gamma64, b64, Ud[3], qn);
@@ -483,7 +558,7 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
 #pragma unroll
         for (int q = 0; q < NE; ++q) {
           int const l = seq<DIM, DIR>(q);
-          s += ev.L[n][l] * U[m][l];
+          if (l_nonzero<DIM, DIR>(n, l)) s += ev.L[n][l] * U[m][l];
         }
         uc[m] = s;
       }
@@ -491,8 +566,9 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
       plus[n] = weno_right(uc);
     }
     R cl[NE], cr[NE];
-    to_physical<NE>(minus, ev.Rm, cl);
-    to_physical<NE>(plus, ev.Rm, cr);
+    roe_right_eigenvectors<DIM, DIR>(ev);
+    to_physical<DIM, DIR>(minus, ev.Rm, cl);
+    to_physical<DIM, DIR>(plus, ev.Rm, cr);
     if constexpr (kShift) {
 #pragma unroll
       for (int e = 0; e < NE; ++e) {
@@ -511,77 +587,76 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
 #pragma unroll
     for (int e = 0; e < NE; ++e) f[e] += hf[e];
   } else {
-    Prim<R> P[6];
+    double Ui[NE], Un[NE];
+    load(2, Ui);
+    load(3, Un);
     CellQ<R> qt, qn;
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      Prim<double> pd;
-      cons_to_prim<DIM, double>(gamma64, b64, Ud[m], pd);
-      P[m].rho = R(pd.rho);
-      P[m].p = R(pd.p);
-      P[m].vel[0] = R(pd.vel[0]);
-      P[m].vel[1] = R(pd.vel[1]);
-      P[m].vel[2] = R(pd.vel[2]);
-    }
-    cell_quantities<DIM, R>(gamma64, b64, Ud[2], qt);
-    cell_quantities<DIM, R>(gamma64, b64, Ud[3], qn);
+    cell_quantities<DIM, R>(gamma64, b64, Ui, qt);
+    cell_quantities<DIM, R>(gamma64, b64, Un, qn);
     roe_eigensystem<DIM, DIR>(gamma, qt, qn, ev);
-    R adv[6][NE];
+    // Cell by cell (each loaded when it is used): its advection
+    // (ComputeAdvection), then its projections onto every characteristic
+    // field; only the projections stay live.
     double adv2[NE];  // advection of cell i (FP64; mixed form only)
     if constexpr (kShift) {
-      // ComputeAdvection in FP64 for the six cells, then differences from
-      // cell i into FP32 (see above).
-      double a64[6][NE];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        Prim<double> pd;
-        cons_to_prim<DIM, double>(gamma64, b64, Ud[m], pd);
-        double const ud = pd.vel[DIR];
-        a64[m][kMass] = pd.rho * ud;
-        a64[m][kEnergy] = (Ud[m][kEnergy] + pd.p) * ud;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) a64[m][kMomX + d] = pd.rho * (ud * pd.vel[d]);
-        a64[m][kMomX + DIR] += pd.p;
-      }
-#pragma unroll
-      for (int e = 0; e < NE; ++e) adv2[e] = a64[2][e];
-#pragma unroll
-      for (int m = 0; m < 6; ++m)
-#pragma unroll
-        for (int e = 0; e < NE; ++e) adv[m][e] = R(a64[m][e] - adv2[e]);
-    } else {
-#pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        R const rho = P[m].rho, p = P[m].p, ud = P[m].vel[DIR];
-        adv[m][kMass] = rho * ud;
-        adv[m][kEnergy] = (U[m][kEnergy] + p) * ud;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) adv[m][kMomX + d] = rho * (ud * P[m].vel[d]);
-        adv[m][kMomX + DIR] += p;
-      }
+      Prim<double> pd;
+      cons_to_prim<DIM, double>(gamma64, b64, Ui, pd);
+      advection<DIM, DIR>(Ui, pd, adv2);
     }
-    R chf[NE];
+    R pos[NE][6], neg[NE][6];
 #pragma unroll
-    for (int n = 0; n < NE; ++n) {
-      R pos[6], neg[6];
+    for (int m = 0; m < 6; ++m) {
+      double Um[NE];
+      load(m, Um);
+      R Uc[NE];
 #pragma unroll
-      for (int m = 0; m < 6; ++m) {
+      for (int e = 0; e < NE; ++e) Uc[e] = kShift ? R(Um[e] - Ui[e]) : R(Um[e]);
+      R adv[NE];
+      Prim<double> pd;
+      cons_to_prim<DIM, double>(gamma64, b64, Um, pd);
+      if constexpr (kShift) {
+        // ComputeAdvection in FP64, then the difference from cell i into
+        // FP32 (see above).
+        double a64[NE];
+        advection<DIM, DIR>(Um, pd, a64);
+#pragma unroll
+        for (int e = 0; e < NE; ++e) adv[e] = R(a64[e] - adv2[e]);
+      } else {
+        Prim<R> P;
+        P.rho = R(pd.rho);
+        P.p = R(pd.p);
+        P.vel[0] = R(pd.vel[0]);
+        P.vel[1] = R(pd.vel[1]);
+        P.vel[2] = R(pd.vel[2]);
+        R const rho = P.rho, p = P.p, ud = P.vel[DIR];
+        adv[kMass] = rho * ud;
+        adv[kEnergy] = (Uc[kEnergy] + p) * ud;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) adv[kMomX + d] = rho * (ud * P.vel[d]);
+        adv[kMomX + DIR] += p;
+      }
+#pragma unroll
+      for (int n = 0; n < NE; ++n) {
         R sp = R(0.0), sn = R(0.0);
 #pragma unroll
         for (int q = 0; q < NE; ++q) {
           int const l = seq<DIM, DIR>(q);
-          R const uct = U[m][l] * ev.L[n][l];
-          R const act = adv[m][l] * ev.L[n][l];
+          if (!l_nonzero<DIM, DIR>(n, l)) continue;
+          R const uct = Uc[l] * ev.L[n][l];
+          R const act = adv[l] * ev.L[n][l];
           sp += (act + ev.lam[n] * uct);
           sn += (act - ev.lam[n] * uct);
         }
-        pos[m] = sp;
-        neg[m] = sn;
+        pos[n][m] = sp;
+        neg[n][m] = sn;
       }
-      chf[n] = R(0.5) * (weno_left(pos) + weno_right(neg));
     }
+    R chf[NE];
+#pragma unroll
+    for (int n = 0; n < NE; ++n) chf[n] = R(0.5) * (weno_left(pos[n]) + weno_right(neg[n]));
     R phys[NE];
-    to_physical<NE>(chf, ev.Rm, phys);
+    roe_right_eigenvectors<DIM, DIR>(ev);
+    to_physical<DIM, DIR>(chf, ev.Rm, phys);
     if constexpr (kShift) {
       // R L (adv of cell i) = adv of cell i: the half sums of the shifts.
 #pragma unroll
@@ -599,13 +674,5 @@ template <int DIM, int DIR, int FLUX, typename R>
 __device__ __forceinline__ void stencil_flux(const double *__restrict__ blk, int i, int j, int k,
                                              double gamma, double b, R *f) {
   using D = Dims<DIM>;
-  constexpr int NE = D::NE;
-  constexpr int s = axis_stride<DIM, DIR>();
-  const double *base = blk + i * D::SX + j * D::SY + k;
-  double U[6][NE];
-#pragma unroll
-  for (int m = 0; m < 6; ++m)
-#pragma unroll
-    for (int e = 0; e < NE; ++e) U[m][e] = base[e * D::FIELD + (m - 2) * s];
-  face_flux<DIM, DIR, FLUX, R>(gamma, b, U, f);
+  face_flux<DIM, DIR, FLUX, R>(gamma, b, blk + i * D::SX + j * D::SY + k, f);
 }
