@@ -245,11 +245,65 @@ template <int NE, typename R> struct Eig {
   R L[NE][NE];
   R Rm[NE][NE];
   R lam[NE];
-  // Roe-averaged quantities the right eigenvectors are built from (filled by
-  // roe_eigensystem, read by roe_right_eigenvectors).
+  // Roe averages the right eigenvectors are built from when they are built
+  // late (kDeferRight below).
   R u, v, w, c, H, g, rho_roe;
   bool ok;
 };
+
+// The exact FP64 kernels build the right eigenvectors only once the
+// characteristic values are formed, so the 19 entries are not held in
+// registers meanwhile (it saves 20 VGPRs in 3D HLLC); without contraction
+// the position changes no bit.  The contracting kernels build them beside
+// the left eigenvectors: there the basic block they sit in decides which
+// products fuse, and the mixed form's accuracy was measured that way.
+template <typename R>
+constexpr bool kDeferRight = std::is_same<R, double>::value && !kFastArith;
+
+template <int DIM, int DIR, typename R>
+__device__ __forceinline__ void set_right_eigenvectors(Eig<DIM + 2, R> &ev, R u, R v, R w, R c,
+                                                       R H, R g, R rho_roe) {
+  constexpr int NE = DIM + 2;
+  constexpr int m1 = Minor<DIR>::m1, m2 = Minor<DIR>::m2;
+  constexpr int evp = DIM;
+  constexpr int Mp = kMomX + DIR, M1 = kMomX + m1, M2 = kMomX + m2;
+  (void)m2;
+  (void)M2;
+  (void)w;
+  ev.Rm[kMass][0] = R(1.0);
+  ev.Rm[kMass][evp] = g;
+  ev.Rm[kMass][NE - 1] = R(1.0);
+
+  ev.Rm[Mp][0] = u - c;
+  ev.Rm[Mp][evp] = g * u;
+  ev.Rm[Mp][NE - 1] = u + c;
+
+  ev.Rm[M1][0] = v;
+  ev.Rm[M1][1] = -rho_roe;
+  ev.Rm[M1][evp] = g * v;
+  ev.Rm[M1][NE - 1] = v;
+
+  if constexpr (DIM == 3) {
+    ev.Rm[M2][0] = w;
+    ev.Rm[M2][2] = rho_roe;
+    ev.Rm[M2][evp] = g * w;
+    ev.Rm[M2][NE - 1] = w;
+  }
+
+  ev.Rm[kEnergy][0] = H - u * c;
+  ev.Rm[kEnergy][1] = -v * rho_roe;
+  if constexpr (DIM == 3) ev.Rm[kEnergy][2] = rho_roe * w;
+  ev.Rm[kEnergy][evp] = g * H - c * c;
+  ev.Rm[kEnergy][NE - 1] = H + u * c;
+}
+
+// The right eigenvectors of a roe_eigensystem call that deferred them.
+template <int DIM, int DIR, typename R>
+__device__ __forceinline__ void roe_right_eigenvectors(Eig<DIM + 2, R> &ev) {
+  if constexpr (kDeferRight<R>) {
+    if (ev.ok) set_right_eigenvectors<DIM, DIR>(ev, ev.u, ev.v, ev.w, ev.c, ev.H, ev.g, ev.rho_roe);
+  }
+}
 
 // Structural zeros of the Roe eigenvectors below: the left-eigenvector rows
 // of the shear waves (row 1, and row 2 in 3D) hold only the mass entry and
@@ -261,22 +315,26 @@ template <int NE, typename R> struct Eig {
 // zero reaches a non-zero value downstream (no division by these sums; the
 // flux divergence is formed as 0.0 + (...), which maps -0 to +0), so the
 // state bits are unchanged.
-template <int DIM, int DIR> __device__ __forceinline__ constexpr bool l_nonzero(int n, int l) {
+// The FP32 part of the mixed form keeps every product: its contraction
+// pattern (and so its rounding) is the one its accuracy was measured with.
+template <int DIM, int DIR, typename R>
+__device__ __forceinline__ constexpr bool l_nonzero(int n, int l) {
+  if (!std::is_same<R, double>::value) return true;
   if (n == 1) return l == kMass || l == kMomX + Minor<DIR>::m1;
   if (DIM == 3 && n == 2) return l == kMass || l == kMomX + Minor<DIR>::m2;
   return true;
 }
-template <int DIM, int DIR> __device__ __forceinline__ constexpr bool r_nonzero(int l, int n) {
+template <int DIM, int DIR, typename R>
+__device__ __forceinline__ constexpr bool r_nonzero(int l, int n) {
+  if (!std::is_same<R, double>::value) return true;
   if (n == 1) return l == kEnergy || l == kMomX + Minor<DIR>::m1;
   if (DIM == 3 && n == 2) return l == kEnergy || l == kMomX + Minor<DIR>::m2;
   return true;
 }
 
-// ComputeRoeEigendecomposition for one face (eigendecomposition.h:238-466):
-// the left eigenvectors and eigenvalues here, the right eigenvectors in
-// roe_right_eigenvectors (called once the characteristic fluxes are formed,
-// so they are not held in registers meanwhile).  t = cell i, n = cell i+1
-// along DIR.  Returns false (all zero) when a density is not positive.
+// ComputeRoeEigendecomposition for one face (eigendecomposition.h:238-466).
+// t = cell i, n = cell i+1 along DIR.  Returns false (all zero) when a
+// density is not positive.
 template <int DIM, int DIR, typename R>
 __device__ __forceinline__ bool roe_eigensystem(R gamma, const CellQ<R> &t, const CellQ<R> &n,
                                                 Eig<DIM + 2, R> &ev) {
@@ -287,7 +345,10 @@ __device__ __forceinline__ bool roe_eigensystem(R gamma, const CellQ<R> &t, cons
 #pragma unroll
   for (int r = 0; r < NE; ++r) {
 #pragma unroll
-    for (int c = 0; c < NE; ++c) ev.L[r][c] = R(0.0);
+    for (int c = 0; c < NE; ++c) {
+      ev.L[r][c] = R(0.0);
+      ev.Rm[r][c] = R(0.0);
+    }
     ev.lam[r] = R(0.0);
   }
   ev.ok = false;
@@ -349,63 +410,23 @@ __device__ __forceinline__ bool roe_eigensystem(R gamma, const CellQ<R> &t, cons
   if constexpr (DIM == 3) ev.L[NE - 1][M2] = R(0.5) * one_cc * (-w * g);
   ev.L[NE - 1][kEnergy] = R(0.5) * one_cc * g;
 
-  ev.u = u;
-  ev.v = v;
-  ev.w = w;
-  ev.c = c;
-  ev.H = H;
-  ev.g = g;
-  ev.rho_roe = rho_roe;
+  if constexpr (kDeferRight<R>) {
+    ev.u = u;
+    ev.v = v;
+    ev.w = w;
+    ev.c = c;
+    ev.H = H;
+    ev.g = g;
+    ev.rho_roe = rho_roe;
+  } else {
+    set_right_eigenvectors<DIM, DIR>(ev, u, v, w, c, H, g, rho_roe);
+  }
 
   ev.lam[0] = __builtin_fabs(u - c);
   ev.lam[NE - 1] = __builtin_fabs(u + c);
 #pragma unroll
   for (int l = 1; l < NE - 1; ++l) ev.lam[l] = __builtin_fabs(u);
   return true;
-}
-
-// The right eigenvectors of roe_eigensystem (eigendecomposition.h:238-466).
-template <int DIM, int DIR, typename R>
-__device__ __forceinline__ void roe_right_eigenvectors(Eig<DIM + 2, R> &ev) {
-  constexpr int NE = DIM + 2;
-  constexpr int m1 = Minor<DIR>::m1, m2 = Minor<DIR>::m2;
-  constexpr int evp = DIM;
-  constexpr int Mp = kMomX + DIR, M1 = kMomX + m1, M2 = kMomX + m2;
-  (void)m2;
-  (void)M2;
-#pragma unroll
-  for (int r = 0; r < NE; ++r)
-#pragma unroll
-    for (int c = 0; c < NE; ++c) ev.Rm[r][c] = R(0.0);
-  if (!ev.ok) return;
-  R const u = ev.u, v = ev.v, w = ev.w, c = ev.c, H = ev.H, g = ev.g, rho_roe = ev.rho_roe;
-  (void)w;
-
-  ev.Rm[kMass][0] = R(1.0);
-  ev.Rm[kMass][evp] = g;
-  ev.Rm[kMass][NE - 1] = R(1.0);
-
-  ev.Rm[Mp][0] = u - c;
-  ev.Rm[Mp][evp] = g * u;
-  ev.Rm[Mp][NE - 1] = u + c;
-
-  ev.Rm[M1][0] = v;
-  ev.Rm[M1][1] = -rho_roe;
-  ev.Rm[M1][evp] = g * v;
-  ev.Rm[M1][NE - 1] = v;
-
-  if constexpr (DIM == 3) {
-    ev.Rm[M2][0] = w;
-    ev.Rm[M2][2] = rho_roe;
-    ev.Rm[M2][evp] = g * w;
-    ev.Rm[M2][NE - 1] = w;
-  }
-
-  ev.Rm[kEnergy][0] = H - u * c;
-  ev.Rm[kEnergy][1] = -v * rho_roe;
-  if constexpr (DIM == 3) ev.Rm[kEnergy][2] = rho_roe * w;
-  ev.Rm[kEnergy][evp] = g * H - c * c;
-  ev.Rm[kEnergy][NE - 1] = H + u * c;
 }
 
 // TransformToPhysicalSpace (eigendecomposition.h:144-165), structural zeros
@@ -418,7 +439,7 @@ __device__ __forceinline__ void to_physical(const R *ch, const R (&Rm)[DIM + 2][
     R s = R(0.0);
 #pragma unroll
     for (int n = 1; n < NE - 1; ++n)
-      if (r_nonzero<DIM, DIR>(l, n)) s += ch[n] * Rm[l][n];
+      if (r_nonzero<DIM, DIR, R>(l, n)) s += ch[n] * Rm[l][n];
     s += (ch[0] * Rm[l][0] + ch[NE - 1] * Rm[l][NE - 1]);
     out[l] = s;
   }
@@ -558,7 +579,7 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
 #pragma unroll
         for (int q = 0; q < NE; ++q) {
           int const l = seq<DIM, DIR>(q);
-          if (l_nonzero<DIM, DIR>(n, l)) s += ev.L[n][l] * U[m][l];
+          if (l_nonzero<DIM, DIR, R>(n, l)) s += ev.L[n][l] * U[m][l];
         }
         uc[m] = s;
       }
@@ -641,7 +662,7 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
 #pragma unroll
         for (int q = 0; q < NE; ++q) {
           int const l = seq<DIM, DIR>(q);
-          if (!l_nonzero<DIM, DIR>(n, l)) continue;
+          if (!l_nonzero<DIM, DIR, R>(n, l)) continue;
           R const uct = Uc[l] * ev.L[n][l];
           R const act = adv[l] * ev.L[n][l];
           sp += (act + ev.lam[n] * uct);

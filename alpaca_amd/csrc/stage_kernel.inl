@@ -208,9 +208,18 @@ __device__ __forceinline__ void stage_tile(const TileJob &j, int blk_local, int 
   if (bad) atomicOr(&j.state->flags, kFlagNonFinite);
 }
 
+// Waves per SIMD the register allocator is held to in the single-case
+// kernel.  The exact FP64 2D HLLC kernel needs 178 VGPRs unconstrained (2
+// waves); held to 168 it fits 3 waves with 12 spilled registers (2048^2
+// cells: 2.16e9 -> 2.36e9 cell-updates/s).  The persistent batch kernel
+// spills 34-64 registers under the same hold and runs slower with it, so it
+// and every other variant keep the default.
+template <int DIM, int FLUX, typename R>
+constexpr int kMinWaves = (DIM == 2 && FLUX == 0 && !kFastArith && std::is_same<R, double>::value) ? 3 : 1;
+
 // Single case: one workgroup per (block, tile).
 template <int DIM, int FLUX, typename R, int TX, int TY, int TZ, int NT>
-__global__ __launch_bounds__(NT) void stage_kernel(const double *__restrict__ avg,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kMinWaves<DIM, FLUX, R>, 8))) void stage_kernel(const double *__restrict__ avg,
                                                    const double *init, double *out,
                                                    alpaca_case_state *state, StageArgs a) {
   using T = TileShape<DIM, TX, TY, TZ>;
