@@ -225,6 +225,77 @@ __global__ __launch_bounds__(kThreads) void halo_kernel(double *blocks, int nblo
   }
 }
 
+// Halo fill over a neighbour table: every halo cell (faces, edges, corners)
+// of every block.  Its offset o in {-1,0,1}^DIM from the block selects the
+// covering neighbour; a present neighbour supplies its internal cell.  When
+// the offset leaves the domain, the axes whose face neighbour is absent are
+// external: each maps its coordinate into the block as its side condition
+// does (zero gradient: the first/last internal layer; symmetry: the mirror
+// about the face, that axis's momentum negated), and the axes left over take
+// the neighbour across them.  These maps act on different axes and commute,
+// so the result equals the reference's order of internal copies followed by
+// the external conditions over the full transverse extent.  Every source is
+// an internal cell, so no thread reads a cell another writes.
+template <int DIM> __device__ __forceinline__ int offset_slot(const int *o) {
+  return DIM == 3 ? ((o[0] + 1) * 3 + (o[1] + 1)) * 3 + (o[2] + 1) : (o[0] + 1) * 3 + (o[1] + 1);
+}
+
+struct SideBc {
+  int kind[6];  // -x, +x, -y, +y, -z, +z
+};
+
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void halo_table_kernel(double *blocks, int nblocks,
+                                                              const int *__restrict__ nbr,
+                                                              SideBc bc) {
+  using D = Dims<DIM>;
+  constexpr int NE = D::NE;
+  constexpr int SLOTS = DIM == 3 ? 27 : 9;
+  constexpr int CELLS = kTC * kTC * D::TZ;
+  size_t const t = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= (size_t)nblocks * CELLS) return;
+  int const blk = int(t / CELLS);
+  int const r = int(t % CELLS);
+  int c[3] = {r / (kTC * D::TZ), (r / D::TZ) % kTC, DIM == 3 ? r % kTC : 0};
+  int const dst[3] = {c[0], c[1], c[2]};
+  int o[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) o[a] = c[a] < kHS ? -1 : (c[a] >= kHS + kIC ? 1 : 0);
+  if (o[0] == 0 && o[1] == 0 && o[2] == 0) return;  // internal cell
+  const int *tb = nbr + (size_t)blk * SLOTS;
+  int src_blk = tb[offset_slot<DIM>(o)];
+  double sign[NE];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) sign[e] = 1.0;
+  if (src_blk < 0) {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      if (o[a] == 0) continue;
+      int f[3] = {0, 0, 0};
+      f[a] = o[a];
+      if (tb[offset_slot<DIM>(f)] >= 0) continue;  // an internal axis
+      int const kind = bc.kind[2 * a + (o[a] > 0)];
+      if (kind == ALPACA_BC_ZERO_GRADIENT) {
+        c[a] = o[a] < 0 ? kHS : kHS + kIC - 1;
+      } else if (kind == ALPACA_BC_SYMMETRY) {
+        c[a] = (o[a] < 0 ? 2 * kHS - 1 : 2 * (kHS + kIC) - 1) - c[a];
+        sign[kMomX + a] = -1.0 * sign[kMomX + a];
+      } else {
+        return;  // a periodic side must appear in the table
+      }
+      o[a] = 0;
+    }
+    src_blk = (o[0] == 0 && o[1] == 0 && o[2] == 0) ? blk : tb[offset_slot<DIM>(o)];
+    if (src_blk < 0) return;  // no covering neighbour across the remaining axes
+  }
+  if (src_blk >= nblocks) return;
+  size_t const di = (size_t)blk * D::BLOCK + dst[0] * D::SX + dst[1] * D::SY + dst[2];
+  size_t const si = (size_t)src_blk * D::BLOCK + (c[0] - o[0] * kIC) * D::SX +
+                    (c[1] - o[1] * kIC) * D::SY + (c[2] - o[2] * kIC);
+#pragma unroll
+  for (int e = 0; e < NE; ++e) blocks[di + (size_t)e * D::FIELD] = sign[e] * blocks[si + (size_t)e * D::FIELD];
+}
+
 // ----------------------------------------------------------- time step ---
 
 // Per block: max over internal cells of sum_d(|u_d| + c)
@@ -563,6 +634,15 @@ int halo_impl(const alpaca_case_desc *d, double *blocks, hipStream_t s) {
 }
 
 template <int DIM>
+int halo_table_impl(double *blocks, int nblocks, const int *nbr, const SideBc &bc, hipStream_t s) {
+  size_t const total = (size_t)nblocks * kTC * kTC * Dims<DIM>::TZ;
+  dim3 grid((unsigned)((total + kThreads - 1) / kThreads));
+  ProfileScope prof(ALPACA_KERNEL_HALO, s, double(total));
+  hipLaunchKernelGGL((halo_table_kernel<DIM>), grid, dim3(kThreads), 0, s, blocks, nblocks, nbr, bc);
+  return hip_status(hipGetLastError());
+}
+
+template <int DIM>
 int timestep_impl(const alpaca_case_desc *d, const double *avg, alpaca_case_state *state,
                   hipStream_t s) {
   int const nblocks = case_blocks(d);
@@ -691,6 +771,22 @@ int alpaca_fill_halos(const alpaca_case_desc *desc, double *d_blocks, void *stre
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (desc->dim == 2) return halo_impl<2>(desc, d_blocks, s);
   return halo_impl<3>(desc, d_blocks, s);
+}
+
+int alpaca_fill_halos_table(int dim, double *d_blocks, int nblocks, const int *d_neighbors,
+                            const int *side_bc, void *stream) {
+  if ((dim != 2 && dim != 3) || !d_blocks || !d_neighbors || !side_bc || nblocks < 1)
+    return ALPACA_ERR_INVALID;
+  SideBc bc;
+  for (int i = 0; i < 6; ++i) {
+    bc.kind[i] = i < 2 * dim ? side_bc[i] : ALPACA_BC_ZERO_GRADIENT;
+    if (bc.kind[i] != ALPACA_BC_ZERO_GRADIENT && bc.kind[i] != ALPACA_BC_SYMMETRY &&
+        bc.kind[i] != ALPACA_BC_PERIODIC)
+      return ALPACA_ERR_INVALID;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dim == 2) return halo_table_impl<2>(d_blocks, nblocks, d_neighbors, bc, s);
+  return halo_table_impl<3>(d_blocks, nblocks, d_neighbors, bc, s);
 }
 
 int alpaca_timestep(const alpaca_case_desc *desc, const double *d_avg, alpaca_case_state *d_state,

@@ -55,6 +55,8 @@ _SIGNATURES = {
                                 ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "alpaca_rk3_stage": ([ctypes.POINTER(CaseDesc), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_fill_halos_table": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                 ctypes.POINTER(ctypes.c_int), ctypes.c_void_p], ctypes.c_int),
     "alpaca_fill_halos": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_void_p],
                           ctypes.c_int),
     "alpaca_timestep": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_void_p,
@@ -178,6 +180,50 @@ def from_blocks(blocks, dim, nb):
     b = blocks.reshape(nb[0], nb[1], nb[2], ne, TC, TC, TC)[..., HS:HS + IC, HS:HS + IC,
                                                             HS:HS + IC]
     return b.permute(3, 0, 4, 1, 5, 2, 6).reshape(ne, nb[0] * IC, nb[1] * IC, nb[2] * IC)
+
+
+def grid_neighbors(dim, nb, bc):
+    """Neighbour table of a block grid for alpaca_fill_halos_table:
+    int32[nblocks][27] (2D: [nblocks][9]), blocks in this package's order
+    (bx * ny + by) * nz + bz, slot ((dx+1)*3 + (dy+1))*3 + (dz+1) (2D:
+    (dx+1)*3 + (dy+1)); -1 where an offset leaves the domain across a
+    non-periodic side, wrapped across periodic ones (bc per direction)."""
+    nb3 = [int(x) for x in nb] + [1] * (3 - len(nb))
+    if dim == 2:
+        nb3[2] = 1
+    offs = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1)
+            for dz in ((-1, 0, 1) if dim == 3 else (0,))]
+    table = np.full((nb3[0] * nb3[1] * nb3[2], len(offs)), -1, dtype=np.int32)
+    for bx in range(nb3[0]):
+        for by in range(nb3[1]):
+            for bz in range(nb3[2]):
+                me = (bx * nb3[1] + by) * nb3[2] + bz
+                for slot, off in enumerate(offs):
+                    q = [bx + off[0], by + off[1], bz + off[2]]
+                    ok = True
+                    for a in range(dim):
+                        if 0 <= q[a] < nb3[a]:
+                            continue
+                        if bc[a] == BC_PERIODIC:
+                            q[a] %= nb3[a]
+                        else:
+                            ok = False
+                    if ok:
+                        table[me, slot] = (q[0] * nb3[1] + q[1]) * nb3[2] + q[2]
+    return table
+
+
+def fill_halos_table(blocks, dim, neighbors, side_bc, stream=None):
+    """alpaca_fill_halos_table on device tensors: blocks (float64, haloed
+    blocks), neighbors (int32, as grid_neighbors returns); side_bc: the
+    condition of each external side (-x, +x, -y, +y, (-z, +z))."""
+    lib = load_library()
+    bc = (ctypes.c_int * 6)(*(list(side_bc) + [BC_ZERO_GRADIENT] * (6 - len(side_bc))))
+    nblocks = neighbors.shape[0]
+    st = _stream_ptr() if stream is None else stream
+    check(lib.alpaca_fill_halos_table(dim, ctypes.c_void_p(blocks.data_ptr()), nblocks,
+                                      ctypes.c_void_p(neighbors.data_ptr()), bc, st),
+          "fill_halos_table")
 
 
 class GpuCase:

@@ -153,6 +153,26 @@ int alpaca_rk3_stage(const alpaca_case_desc *desc, int stage,
 int alpaca_fill_halos(const alpaca_case_desc *desc, double *d_blocks,
                       void *stream);
 
+/* (iv) over a neighbour table, for block topologies that are not a full
+ * grid: the same-level halo update the reference runs over its tree
+ * (HaloManager::MaterialHaloUpdate, halo_manager.cpp:59-133;
+ * InternalHaloManager::UpdateNoJumpLocal, internal_halo_manager.cpp:439-466;
+ * external conditions boundary_constants.h:103-253).  Every halo cell of the
+ * nblocks haloed blocks in d_blocks -- faces, edges and corners -- takes the
+ * internal cell of the neighbour covering it.  d_neighbors (device,
+ * int32[nblocks][27] in 3D, [nblocks][9] in 2D) holds for each offset
+ * (dx, dy, dz) in {-1, 0, 1}^DIM, at index ((dx+1)*3 + (dy+1))*3 + (dz+1)
+ * (2D: (dx+1)*3 + (dy+1)), the neighbour's block index, or -1 where the
+ * offset leaves the domain; the centre entry is ignored.  Periodic sides
+ * appear as neighbours.  side_bc (host, int[2*dim]: -x, +x, -y, +y, -z, +z)
+ * gives the condition of external sides: ALPACA_BC_ZERO_GRADIENT (the
+ * first/last internal layer) or ALPACA_BC_SYMMETRY (the mirror, normal
+ * momentum negated), applied over the full transverse extent; a halo cell on
+ * a side marked ALPACA_BC_PERIODIC without a neighbour is left unchanged. */
+int alpaca_fill_halos_table(int dim, double *d_blocks, int nblocks,
+                            const int *d_neighbors, const int *side_bc,
+                            void *stream);
+
 /* (v) CFL time step of a case: ComputeTimestepSize
  * (modular_algorithm_assembler.cpp:1777-1997) for inviscid flow.
  * S = max over internal cells of sum_d(|u_d| + c); dt = cfl / (S / dx),

@@ -60,3 +60,36 @@ def test_invalid_descriptors_are_rejected_before_any_launch():
                                 ctypes.c_void_p(2), ctypes.c_void_p(3), None) == 1
     assert lib.alpaca_advance(ctypes.byref(good), ctypes.c_void_p(1), ctypes.c_void_p(1),
                               ctypes.c_void_p(2), ctypes.c_void_p(3), 1, None) == 1
+
+
+def test_grid_neighbour_table():
+    """alpaca_fill_halos_table's table for a grid: offsets reciprocate
+    (B at +o of A means A at -o of B), periodic directions wrap, other
+    directions end in -1, and the centre slot is the block itself."""
+    import numpy as np
+
+    for dim, nb, bc in ((2, [3, 2], [gpu.BC_PERIODIC, gpu.BC_ZERO_GRADIENT]),
+                        (3, [2, 3, 4], [gpu.BC_SYMMETRY, gpu.BC_PERIODIC, gpu.BC_ZERO_GRADIENT])):
+        t = gpu.grid_neighbors(dim, nb, bc)
+        slots = t.shape[1]
+        assert slots == (27 if dim == 3 else 9)
+        centre = slots // 2
+        assert np.array_equal(t[:, centre], np.arange(t.shape[0]))
+        for a in range(t.shape[0]):
+            for s in range(slots):
+                b = t[a, s]
+                if b >= 0:
+                    assert t[b, slots - 1 - s] == a
+        # a block on a non-periodic side has no neighbour beyond it
+        nb3 = list(nb) + [1] * (3 - len(nb))
+        for ax in range(dim):
+            if bc[ax] == gpu.BC_PERIODIC:
+                assert (t >= 0).sum() > 0
+                continue
+            off = [0, 0, 0]
+            off[ax] = -1
+            slot = ((off[0] + 1) * 3 + (off[1] + 1)) * 3 + (off[2] + 1) if dim == 3 else \
+                (off[0] + 1) * 3 + (off[1] + 1)
+            for k in range(t.shape[0]):
+                pos = [k // (nb3[1] * nb3[2]), (k // nb3[2]) % nb3[1], k % nb3[2]]
+                assert (t[k, slot] < 0) == (pos[ax] == 0)
