@@ -10,10 +10,13 @@ whole domain; a cell-update is one internal cell advanced one macro step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Prints one JSON line.  `roofline` covers the dominant kernel (the face-flux
-kernel) timed with HIP events on its launch stream during the timed region;
-`cpu_baseline` times the reference's own numerics (oracle/_ref, threads over
-blocks) on a bounded sample of the same workload on this host's cores.
+Prints one JSON line.  `roofline` covers the dominant kernel (the fused
+stage kernel) timed with HIP events on its launch stream during the timed
+region; `modes` repeats the workload in the FP64-with-FMA and mixed forms
+(with the mixed density error against FP64); `roe_fp64` runs it with Roe
+flux splitting in FP64; `cpu_baseline` times the reference's own numerics
+(oracle/_ref, threads over blocks) on a bounded sample of the same workload
+on this host's cores.
 """
 from __future__ import annotations
 
@@ -50,26 +53,26 @@ def cpu_threads():
 
 def cpu_baseline(flux: int, threads: int):
     """Reference numerics (or the C restatement if the reference harness is
-    not built) on a 128x128x64 periodic sample of the same workload, 2 steps."""
+    not built) on a 128^3 periodic sample of the same workload (the 8^3-block
+    form of the blob + shock state), 2 steps: about 20 core-seconds."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
     from alpaca_amd import cases
 
-    nb = [8, 8, 4]
-    full = cases.blobs_shock_3d(8, seed=0)
-    state = full[:, :, :, :64].copy()
+    nb = [8, 8, 8]
+    state = cases.blobs_shock_3d(8, seed=0)
     steps = 2
     which = "ref" if oracle_py.ref_available(3) else "oracle"
     t0 = time.perf_counter()
     oracle_py.run(3, nb, [1, 1, 1], flux, 0.5, 1e300, steps, state, cell_size=1.0 / 128,
                   threads=threads, which=which)
     dt = time.perf_counter() - t0
-    cells = 128 * 128 * 64
+    cells = 128 ** 3
     return {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": threads,
             "kind": "reference" if which == "ref" else "port",
-            "sample": f"128x128x64 periodic blob+shock slab (8x8x4 blocks), {steps} steps, "
+            "sample": f"128^3 periodic blob+shock (8^3 blocks), {steps} steps, "
                       f"{'HLLC' if flux == 0 else 'Roe'} FP64, {threads} threads over blocks, "
-                      f"{dt:.2f} s wall"}
+                      f"{dt:.2f} s wall ({dt * threads:.0f} core-seconds)"}
 
 
 def load_profile(kernel_tag: str):
@@ -454,6 +457,15 @@ def main():
                 summary[m]["speedup_vs_fp64"] = r["value"] / ref["value"]
             summary["compared_after_steps"] = args.warmup + args.steps
         line["modes"] = summary
+        if flux == 0:
+            # The same workload with Roe flux splitting in FP64: the 100-step
+            # oracle and conservation configuration, reported beside the
+            # HLLC headline.
+            roe = run_mode(args, 1, "fp64", nb, keep_state=False)
+            line["roe_fp64"] = {"cell_updates_per_s": roe["value"], "ms_per_step": roe["ms_per_step"],
+                                "stage_avg_ms": roe["stage_avg_ms"],
+                                "roofline_frac": (roe["cells_per_launch"] * FLOP_PER_CELL_STAGE[(3, 1)]
+                                                  / (roe["stage_avg_ms"] / 1e3) / PEAK_FP64)}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(flux, cpu_threads())
     print(json.dumps(line), flush=True)
