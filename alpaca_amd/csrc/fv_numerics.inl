@@ -251,10 +251,11 @@ template <int NE, typename R> struct Eig {
   bool ok;
 };
 
-// The exact FP64 kernels build the right eigenvectors only once the
-// characteristic values are formed, so the 19 entries are not held in
-// registers meanwhile (it saves 20 VGPRs in 3D HLLC); without contraction
-// the position changes no bit.  The contracting kernels build them beside
+// The exact FP64 finite-volume (HLLC) kernels build the right eigenvectors
+// only once the characteristic values are formed, so the 19 entries are not
+// held in registers meanwhile (it saves 20 VGPRs in 3D); without contraction
+// the position changes no bit.  (Flux splitting builds them with the left
+// ones: deferring them there made the exact kernel slower.)  The contracting kernels build them beside
 // the left eigenvectors: there the basic block they sit in decides which
 // products fuse, and the mixed form's accuracy was measured that way.
 template <typename R>
@@ -298,44 +299,17 @@ __device__ __forceinline__ void set_right_eigenvectors(Eig<DIM + 2, R> &ev, R u,
 }
 
 // The right eigenvectors of a roe_eigensystem call that deferred them.
-template <int DIM, int DIR, typename R>
+template <int DIM, int DIR, typename R, bool kDefer = kDeferRight<R>>
 __device__ __forceinline__ void roe_right_eigenvectors(Eig<DIM + 2, R> &ev) {
-  if constexpr (kDeferRight<R>) {
+  if constexpr (kDefer) {
     if (ev.ok) set_right_eigenvectors<DIM, DIR>(ev, ev.u, ev.v, ev.w, ev.c, ev.H, ev.g, ev.rho_roe);
   }
-}
-
-// Structural zeros of the Roe eigenvectors below: the left-eigenvector rows
-// of the shear waves (row 1, and row 2 in 3D) hold only the mass entry and
-// that wave's transverse momentum entry, and the right-eigenvector columns
-// 1 and 2 only that momentum entry and the energy entry.  The reference
-// multiplies by these zeros too (characteristic_state_reconstruction.h:
-// 85-103, eigendecomposition.h:144-165); leaving out a product 0 * x of a
-// finite x changes at most the sign of a zero partial sum, and no sign of
-// zero reaches a non-zero value downstream (no division by these sums; the
-// flux divergence is formed as 0.0 + (...), which maps -0 to +0), so the
-// state bits are unchanged.
-// The FP32 part of the mixed form keeps every product: its contraction
-// pattern (and so its rounding) is the one its accuracy was measured with.
-template <int DIM, int DIR, typename R>
-__device__ __forceinline__ constexpr bool l_nonzero(int n, int l) {
-  if (!std::is_same<R, double>::value) return true;
-  if (n == 1) return l == kMass || l == kMomX + Minor<DIR>::m1;
-  if (DIM == 3 && n == 2) return l == kMass || l == kMomX + Minor<DIR>::m2;
-  return true;
-}
-template <int DIM, int DIR, typename R>
-__device__ __forceinline__ constexpr bool r_nonzero(int l, int n) {
-  if (!std::is_same<R, double>::value) return true;
-  if (n == 1) return l == kEnergy || l == kMomX + Minor<DIR>::m1;
-  if (DIM == 3 && n == 2) return l == kEnergy || l == kMomX + Minor<DIR>::m2;
-  return true;
 }
 
 // ComputeRoeEigendecomposition for one face (eigendecomposition.h:238-466).
 // t = cell i, n = cell i+1 along DIR.  Returns false (all zero) when a
 // density is not positive.
-template <int DIM, int DIR, typename R>
+template <int DIM, int DIR, typename R, bool kDefer = kDeferRight<R>>
 __device__ __forceinline__ bool roe_eigensystem(R gamma, const CellQ<R> &t, const CellQ<R> &n,
                                                 Eig<DIM + 2, R> &ev) {
   constexpr int NE = DIM + 2;
@@ -410,7 +384,7 @@ __device__ __forceinline__ bool roe_eigensystem(R gamma, const CellQ<R> &t, cons
   if constexpr (DIM == 3) ev.L[NE - 1][M2] = R(0.5) * one_cc * (-w * g);
   ev.L[NE - 1][kEnergy] = R(0.5) * one_cc * g;
 
-  if constexpr (kDeferRight<R>) {
+  if constexpr (kDefer) {
     ev.u = u;
     ev.v = v;
     ev.w = w;
@@ -429,8 +403,9 @@ __device__ __forceinline__ bool roe_eigensystem(R gamma, const CellQ<R> &t, cons
   return true;
 }
 
-// TransformToPhysicalSpace (eigendecomposition.h:144-165), structural zeros
-// of the right eigenvectors left out (see r_nonzero).
+// TransformToPhysicalSpace (eigendecomposition.h:144-165).  Every product is
+// formed, the zero entries' too: 0 * x is NaN for a non-finite x, as in the
+// reference.
 template <int DIM, int DIR, typename R>
 __device__ __forceinline__ void to_physical(const R *ch, const R (&Rm)[DIM + 2][DIM + 2], R *out) {
   constexpr int NE = DIM + 2;
@@ -438,8 +413,7 @@ __device__ __forceinline__ void to_physical(const R *ch, const R (&Rm)[DIM + 2][
   for (int l = 0; l < NE; ++l) {
     R s = R(0.0);
 #pragma unroll
-    for (int n = 1; n < NE - 1; ++n)
-      if (r_nonzero<DIM, DIR, R>(l, n)) s += ch[n] * Rm[l][n];
+    for (int n = 1; n < NE - 1; ++n) s += ch[n] * Rm[l][n];
     s += (ch[0] * Rm[l][0] + ch[NE - 1] * Rm[l][NE - 1]);
     out[l] = s;
   }
@@ -527,6 +501,128 @@ __device__ __forceinline__ void advection(const double *c, const Prim<double> &p
   a[kMomX + DIR] += pd.p;
 }
 
+// FluxSplittingScheme::ComputeFluxes + ComputeAdvection for one face
+// (flux_splitting_scheme.cpp:42-76, 185-315), added into f.  The exact FP64
+// kernel keeps the reference's order -- every stencil cell's advection, then
+// field by field the projections and WENO5 -- while the contracting kernels
+// walk the stencil cell by cell (load, advection, projection onto every
+// field), which keeps fewer values live: 2 waves per SIMD instead of 1 for
+// the FP64-with-FMA kernel, 3 instead of 2 for the mixed one.  Each
+// projection is the same expression either way.
+template <int DIM, int DIR, typename R>
+__device__ __forceinline__ void roe_split_flux(double gamma64, double b64,
+                                               const double *__restrict__ base, R *f) {
+  constexpr int NE = DIM + 2;
+  constexpr bool kShift = std::is_same<R, float>::value;
+  R const gamma = R(gamma64);
+  auto load = [base](int m, double *c) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e)
+      c[e] = base[e * Dims<DIM>::FIELD + (m - 2) * axis_stride<DIM, DIR>()];
+  };
+  Eig<NE, R> ev;
+  CellQ<R> qt, qn;
+  double adv2[NE];  // advection of cell i (FP64; mixed form only)
+  R chf[NE];
+  if constexpr (std::is_same<R, double>::value && !kFastArith) {
+    double U[6][NE], adv[6][NE];
+    Prim<double> P[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) load(m, U[m]);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) cons_to_prim<DIM, double>(gamma64, b64, U[m], P[m]);
+    cell_quantities<DIM, R>(gamma64, b64, U[2], qt);
+    cell_quantities<DIM, R>(gamma64, b64, U[3], qn);
+    roe_eigensystem<DIM, DIR, R, false>(gamma, qt, qn, ev);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) advection<DIM, DIR>(U[m], P[m], adv[m]);
+#pragma unroll
+    for (int n = 0; n < NE; ++n) {
+      R pos[6], neg[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        R sp = R(0.0), sn = R(0.0);
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+          int const l = seq<DIM, DIR>(q);
+          R const uct = U[m][l] * ev.L[n][l];
+          R const act = adv[m][l] * ev.L[n][l];
+          sp += (act + ev.lam[n] * uct);
+          sn += (act - ev.lam[n] * uct);
+        }
+        pos[m] = sp;
+        neg[m] = sn;
+      }
+      chf[n] = R(0.5) * (weno_left(pos) + weno_right(neg));
+    }
+  } else {
+    double Ui[NE], Un[NE];
+    load(2, Ui);
+    load(3, Un);
+    cell_quantities<DIM, R>(gamma64, b64, Ui, qt);
+    cell_quantities<DIM, R>(gamma64, b64, Un, qn);
+    roe_eigensystem<DIM, DIR, R, false>(gamma, qt, qn, ev);
+    if constexpr (kShift) {
+      Prim<double> pd;
+      cons_to_prim<DIM, double>(gamma64, b64, Ui, pd);
+      advection<DIM, DIR>(Ui, pd, adv2);
+    }
+    R pos[NE][6], neg[NE][6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      double Um[NE];
+      load(m, Um);
+      R Uc[NE];
+#pragma unroll
+      for (int e = 0; e < NE; ++e) Uc[e] = kShift ? R(Um[e] - Ui[e]) : R(Um[e]);
+      R adv[NE];
+      Prim<double> pd;
+      cons_to_prim<DIM, double>(gamma64, b64, Um, pd);
+      if constexpr (kShift) {
+        // ComputeAdvection in FP64, then the difference from cell i into
+        // FP32 (see face_flux).
+        double a64[NE];
+        advection<DIM, DIR>(Um, pd, a64);
+#pragma unroll
+        for (int e = 0; e < NE; ++e) adv[e] = R(a64[e] - adv2[e]);
+      } else {
+        R const rho = R(pd.rho), p = R(pd.p), ud = R(pd.vel[DIR]);
+        adv[kMass] = rho * ud;
+        adv[kEnergy] = (Uc[kEnergy] + p) * ud;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) adv[kMomX + d] = rho * (ud * R(pd.vel[d]));
+        adv[kMomX + DIR] += p;
+      }
+#pragma unroll
+      for (int n = 0; n < NE; ++n) {
+        R sp = R(0.0), sn = R(0.0);
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+          int const l = seq<DIM, DIR>(q);
+          R const uct = Uc[l] * ev.L[n][l];
+          R const act = adv[l] * ev.L[n][l];
+          sp += (act + ev.lam[n] * uct);
+          sn += (act - ev.lam[n] * uct);
+        }
+        pos[n][m] = sp;
+        neg[n][m] = sn;
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < NE; ++n) chf[n] = R(0.5) * (weno_left(pos[n]) + weno_right(neg[n]));
+  }
+  R phys[NE];
+  to_physical<DIM, DIR>(chf, ev.Rm, phys);
+  if constexpr (kShift) {
+    // R L (adv of cell i) = adv of cell i: the half sums of the shifts.
+#pragma unroll
+    for (int e = 0; e < NE; ++e) f[e] += R(double(phys[e]) + adv2[e]);
+  } else {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) f[e] += phys[e];
+  }
+}
+
 // One face flux from the six-cell stencil Ud[m] (FP64 conservatives of cells
 // i-2..i+3 along DIR); reconstruction and Riemann arithmetic in R.
 // FV:  FiniteVolumeScheme::ComputeFluxes (finite_volume_scheme.cpp:160-251)
@@ -579,7 +675,7 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
 #pragma unroll
         for (int q = 0; q < NE; ++q) {
           int const l = seq<DIM, DIR>(q);
-          if (l_nonzero<DIM, DIR, R>(n, l)) s += ev.L[n][l] * U[m][l];
+          s += ev.L[n][l] * U[m][l];
         }
         uc[m] = s;
       }
@@ -608,84 +704,7 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
 #pragma unroll
     for (int e = 0; e < NE; ++e) f[e] += hf[e];
   } else {
-    double Ui[NE], Un[NE];
-    load(2, Ui);
-    load(3, Un);
-    CellQ<R> qt, qn;
-    cell_quantities<DIM, R>(gamma64, b64, Ui, qt);
-    cell_quantities<DIM, R>(gamma64, b64, Un, qn);
-    roe_eigensystem<DIM, DIR>(gamma, qt, qn, ev);
-    // Cell by cell (each loaded when it is used): its advection
-    // (ComputeAdvection), then its projections onto every characteristic
-    // field; only the projections stay live.
-    double adv2[NE];  // advection of cell i (FP64; mixed form only)
-    if constexpr (kShift) {
-      Prim<double> pd;
-      cons_to_prim<DIM, double>(gamma64, b64, Ui, pd);
-      advection<DIM, DIR>(Ui, pd, adv2);
-    }
-    R pos[NE][6], neg[NE][6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      double Um[NE];
-      load(m, Um);
-      R Uc[NE];
-#pragma unroll
-      for (int e = 0; e < NE; ++e) Uc[e] = kShift ? R(Um[e] - Ui[e]) : R(Um[e]);
-      R adv[NE];
-      Prim<double> pd;
-      cons_to_prim<DIM, double>(gamma64, b64, Um, pd);
-      if constexpr (kShift) {
-        // ComputeAdvection in FP64, then the difference from cell i into
-        // FP32 (see above).
-        double a64[NE];
-        advection<DIM, DIR>(Um, pd, a64);
-#pragma unroll
-        for (int e = 0; e < NE; ++e) adv[e] = R(a64[e] - adv2[e]);
-      } else {
-        Prim<R> P;
-        P.rho = R(pd.rho);
-        P.p = R(pd.p);
-        P.vel[0] = R(pd.vel[0]);
-        P.vel[1] = R(pd.vel[1]);
-        P.vel[2] = R(pd.vel[2]);
-        R const rho = P.rho, p = P.p, ud = P.vel[DIR];
-        adv[kMass] = rho * ud;
-        adv[kEnergy] = (Uc[kEnergy] + p) * ud;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) adv[kMomX + d] = rho * (ud * P.vel[d]);
-        adv[kMomX + DIR] += p;
-      }
-#pragma unroll
-      for (int n = 0; n < NE; ++n) {
-        R sp = R(0.0), sn = R(0.0);
-#pragma unroll
-        for (int q = 0; q < NE; ++q) {
-          int const l = seq<DIM, DIR>(q);
-          if (!l_nonzero<DIM, DIR, R>(n, l)) continue;
-          R const uct = Uc[l] * ev.L[n][l];
-          R const act = adv[l] * ev.L[n][l];
-          sp += (act + ev.lam[n] * uct);
-          sn += (act - ev.lam[n] * uct);
-        }
-        pos[n][m] = sp;
-        neg[n][m] = sn;
-      }
-    }
-    R chf[NE];
-#pragma unroll
-    for (int n = 0; n < NE; ++n) chf[n] = R(0.5) * (weno_left(pos[n]) + weno_right(neg[n]));
-    R phys[NE];
-    roe_right_eigenvectors<DIM, DIR>(ev);
-    to_physical<DIM, DIR>(chf, ev.Rm, phys);
-    if constexpr (kShift) {
-      // R L (adv of cell i) = adv of cell i: the half sums of the shifts.
-#pragma unroll
-      for (int e = 0; e < NE; ++e) f[e] += R(double(phys[e]) + adv2[e]);
-    } else {
-#pragma unroll
-      for (int e = 0; e < NE; ++e) f[e] += phys[e];
-    }
+    roe_split_flux<DIM, DIR, R>(gamma64, b64, base, f);
   }
 }
 

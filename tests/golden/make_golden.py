@@ -27,7 +27,9 @@ half an hour per scheme on 8 cores) and writes
     python tests/golden/make_golden.py --full [--threads N] [--flux roe|hllc]
 
 With ``--ids`` it writes level0_ids.json only (level-0 node ids, used to pin
-the restart files' node-id encoding).
+the restart files' node-id encoding); with ``--edge`` edge_blocks.json only
+(hashes of the reference's per-block fluxes and right-hand sides on the
+special-branch states of tests/edge_states.py).
 """
 from __future__ import annotations
 
@@ -237,6 +239,29 @@ def main_full(threads, fluxes):
         np.savez_compressed(path_npz, **arrays)
 
 
+def main_edge():
+    """edge_blocks.json: per dimension, flux family and edge kind (see
+    tests/edge_states.py) the SHA-256 of the reference's per-block face
+    fluxes and right-hand side, NaNs made canonical."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    import edge_states
+
+    meta = {}
+    with np.errstate(all="ignore"):
+        for dim in (2, 3):
+            for flux in (0, 1):
+                for kind in edge_states.KINDS:
+                    blk = edge_states.edge_block(dim, kind, seed=3)
+                    fl = oracle_py.block_fluxes(dim, flux, GAMMA, 0.0, blk, 1.0 / 128, which="ref")
+                    rhs = oracle_py.block_rhs(dim, flux, GAMMA, 0.0, blk, 1.0 / 128, which="ref")
+                    meta[f"{dim}d_{'roe' if flux else 'hllc'}_{kind}"] = {
+                        "fluxes_sha256": hashlib.sha256(edge_states.canonical(fl)).hexdigest(),
+                        "rhs_sha256": hashlib.sha256(edge_states.canonical(rhs)).hexdigest(),
+                        "nan_fluxes": int(np.isnan(fl).sum())}
+    with open(os.path.join(HERE, "edge_blocks.json"), "w") as fh:
+        json.dump(meta, fh, indent=1)
+
+
 ID_SHAPES_FULL = ([1, 1, 1], [4, 1, 1], [3, 5, 1], [2, 3, 4], [5, 4, 3])
 ID_SHAPES_HASHED = ([16, 16, 16], [128, 2, 1], [1, 128, 2], [2, 1, 128], [64, 64, 1])
 
@@ -265,9 +290,12 @@ if __name__ == "__main__":
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 8)
     ap.add_argument("--flux", choices=["roe", "hllc", "both"], default="both")
     ap.add_argument("--ids", action="store_true", help="write level0_ids.json only")
+    ap.add_argument("--edge", action="store_true", help="write edge_blocks.json only")
     a = ap.parse_args()
     if a.ids:
         main_ids()
+    elif a.edge:
+        main_edge()
     elif a.full:
         main_full(a.threads, {"roe": [1], "hllc": [0], "both": [1, 0]}[a.flux])
     else:

@@ -294,3 +294,41 @@ def test_mixed_density_error(torch, flux):
     print(f"  L1 rho err={l1:.3e} mass drift={mass:.3e}")
     assert l1 <= MIXED_L1_RHO_TOL
     assert mass <= 1e-13
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("flux", [gpu.FLUX_HLLC, gpu.FLUX_ROE])
+def test_edge_blocks_bitexact(torch, dim, flux):
+    """The special branches (face skips, S* = 0, the eigensystem skip, the
+    scaled square-root and division paths; tests/edge_states.py): the FP64
+    kernels' per-block fluxes and right-hand sides equal the oracle's bit for
+    bit, NaNs in the same places."""
+    import edge_states
+
+    print(f"[edge blocks] dim={dim} flux={flux}", flush=True)
+    lib = gpu.load_library()
+    kinds = edge_states.KINDS
+    host = np.stack([edge_states.edge_block(dim, k, seed=3) for k in kinds])
+    dev = torch.as_tensor(host, device="cuda").contiguous()
+    desc = gpu.make_desc(dim, [1, 1, 1], [0, 0, 0], flux=flux, cell_size=1.0 / 128)
+    nfl = oracle_py.nflux(dim)
+    out = torch.zeros(len(kinds) * nfl, dtype=torch.float64, device="cuda")
+    rhs = torch.zeros_like(dev)
+    s = gpu._stream_ptr()
+    gpu.check(lib.alpaca_convective_fluxes(ctypes.byref(desc), len(kinds),
+                                           ctypes.c_void_p(dev.data_ptr()),
+                                           ctypes.c_void_p(out.data_ptr()), s), "fluxes")
+    gpu.check(lib.alpaca_right_hand_side(ctypes.byref(desc), len(kinds),
+                                         ctypes.c_void_p(dev.data_ptr()),
+                                         ctypes.c_void_p(rhs.data_ptr()), s), "rhs")
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().reshape(len(kinds), -1)
+    got_rhs = gpu.from_blocks(rhs, dim, [len(kinds), 1, 1]).cpu().numpy()
+    icz = 16 if dim == 3 else 1
+    with np.errstate(all="ignore"):
+        for bi, kind in enumerate(kinds):
+            want = oracle_py.block_fluxes(dim, flux, 1.4, 0.0, host[bi], 1.0 / 128).ravel()
+            assert np.array_equal(got[bi], want, equal_nan=True), kind
+            want_rhs = oracle_py.block_rhs(dim, flux, 1.4, 0.0, host[bi], 1.0 / 128)
+            g = got_rhs[:, bi * 16:(bi + 1) * 16, :, :icz]
+            assert np.array_equal(g, want_rhs, equal_nan=True), kind
