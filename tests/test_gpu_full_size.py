@@ -47,14 +47,18 @@ def _fixture(name):
     return runs[name], np.load(SAMPLES)
 
 
+@pytest.mark.parametrize("precision", [gpu.FP64, gpu.FP64_FMA])
 @pytest.mark.parametrize("name", ["roe", "hllc"])
-def test_full_256_matches_reference(torch, name):
+def test_full_256_matches_reference(torch, name, precision):
+    """Both FP64 forms: the exact one (the reference's operation order, no
+    contraction) and the one with FMA contraction are held to the same bound."""
     run, arrays = _fixture(name)
-    print(f"[full 256^3] scheme={name} steps={run['steps']}", flush=True)
+    print(f"[full 256^3] scheme={name} precision={precision} steps={run['steps']}", flush=True)
     nb = run["nb"][0]
     st0 = cases.blobs_shock_3d(nb, seed=run["ic"]["seed"])
     assert _sha(st0) == run["ic_sha256"], "host initial state differs from the fixture's"
-    desc = gpu.make_desc(3, run["nb"], [gpu.BC_PERIODIC] * 3, flux=run["flux"])
+    desc = gpu.make_desc(3, run["nb"], [gpu.BC_PERIODIC] * 3, flux=run["flux"],
+                         precision=precision)
     case = gpu.GpuCase(desc, st0, cfl=run["cfl"])
     steps, dts = case.run(run["steps"])
     assert steps == run["steps"]
