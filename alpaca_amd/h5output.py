@@ -162,8 +162,10 @@ def _block_cell_order(field, dim, nb):
     return f.transpose(0, 2, 4, 5, 3, 1).reshape(-1)     # bx by bz k j i
 
 
-def write(path_without_extension, time, dim, nb, cell_size, density, velocity, pressure):
-    """Write ``<path>.h5`` and ``<path>.xdmf`` for one output time.
+def write(path_without_extension, time, dim, nb, cell_size, density, velocity, pressure,
+          series=None):
+    """Write ``<path>.h5`` and ``<path>.xdmf`` for one output time, and append
+    the time's grid to ``<series>.xdmf`` when a time-series file is given.
     density, pressure: [NX][NY][NZ]; velocity: [dim][NX][NY][NZ]."""
     lib = _lib()
     h5 = path_without_extension + ".h5"
@@ -199,18 +201,37 @@ def write(path_without_extension, time, dim, nb, cell_size, density, velocity, p
         lib.H5Gclose(g)
     finally:
         lib.H5Fclose(fid)
-    short = os.path.basename(h5)
+    block = spatial_xdmf(time, os.path.basename(h5), ncells, coords.shape[0])
+    with open(path_without_extension + ".xdmf", "w") as fh:
+        fh.write(xdmf_header("TimeStep") + block + xdmf_footer())
+    if series is not None:
+        with open(series + ".xdmf", "a") as fh:
+            fh.write(block)
+    return h5
+
+
+def xdmf_header(name):
+    """XdmfUtilities::HeaderInformation (xdmf_utilities.cpp:220-227)."""
+    return ('<?xml version="1.0" ?>\n<!DOCTYPE Xdmf SYSTEM "Xdmf.dtd" []>\n<Xdmf Version="3.0">\n'
+            f' <Domain>\n  <Grid Name="{name}" GridType="Collection" CollectionType="Temporal">\n')
+
+
+def xdmf_footer():
+    """XdmfUtilities::FooterInformation (xdmf_utilities.cpp:234-238)."""
+    return "  </Grid>\n </Domain>\n</Xdmf>"
+
+
+def spatial_xdmf(time, short, ncells, nv):
+    """One time's grid (OutputWriter::XdmfSpatialDataInformation,
+    output_writer.cpp:210-260): time, hexahedron topology, vertex geometry,
+    density / velocity / pressure cell attributes in the HDF5 file `short`."""
     ind = lambda n: " " * n  # noqa: E731
-    nv = coords.shape[0]
 
     def item(name, dims):
         return (f'<DataItem Format="HDF" NumberType="Float" Precision="8" Dimensions="{ncells}'
                 f'{dims}"> {short}:/cell_data/{name} </DataItem>\n')
 
-    xdmf = ('<?xml version="1.0" ?>\n<!DOCTYPE Xdmf SYSTEM "Xdmf.dtd" []>\n<Xdmf Version="3.0">\n'
-            f'{ind(1)}<Domain>\n{ind(2)}<Grid Name="TimeStep" GridType="Collection" '
-            'CollectionType="Temporal">\n'
-            f'{ind(4)}<Grid Name="SpatialData_{time:.6e}" GridType="Uniform">\n'
+    return (f'{ind(4)}<Grid Name="SpatialData_{time:.6e}" GridType="Uniform">\n'
             f'{ind(6)}<Time TimeType="Single" Value="{time:.6e}" />\n\n'
             f'{ind(6)}<Topology TopologyType="Hexahedron" NumberOfElements="{ncells}">\n'
             f'{ind(8)}<DataItem NumberType="Int" Format="HDF" Dimensions="{ncells} 8"> '
@@ -224,10 +245,67 @@ def write(path_without_extension, time, dim, nb, cell_size, density, velocity, p
             f'{ind(8)}{item("velocity", " 3")}{ind(6)}</Attribute>\n'
             f'{ind(6)}<Attribute Name="pressure" AttributeType="Scalar" Center="Cell">\n'
             f'{ind(8)}{item("pressure", "")}{ind(6)}</Attribute>\n'
-            f'{ind(4)}</Grid>\n{ind(2)}</Grid>\n{ind(1)}</Domain>\n</Xdmf>')
-    with open(path_without_extension + ".xdmf", "w") as fh:
-        fh.write(xdmf)
-    return h5
+            f'{ind(4)}</Grid>\n')
+
+
+def output_times(kind, interval, stamps, start, end):
+    """ComputeOutputTimes (instantiation_input_output_manager.cpp:41-160):
+    kind is off, interval, stamps or intervalstamps.  Interval times are
+    start + k * interval (summed one interval at a time) below the end; stamps
+    strictly inside (start, end); then the end time itself."""
+    import math
+
+    if kind == "off":
+        return []
+    times = []
+    if kind in ("interval", "intervalstamps"):
+        n = math.ceil(math.nextafter((end - start) / interval, 0.0))
+        if n != 0:
+            n -= 1
+        t = start
+        for _ in range(n):
+            t += interval
+            times.append(t)
+    if kind in ("stamps", "intervalstamps"):
+        times = sorted(times + [x for x in stamps if start < x < end])
+    times.append(end)
+    return times
+
+
+class OutputSeries:
+    """Standard output during a run (InputOutputManager::WriteFullOutput,
+    input_output_manager.cpp:229-300): data_<t>.h5 / .xdmf in `folder` when
+    the first pending output time is reached (all passed times are consumed)
+    or when forced, each also appended to data_time_series.xdmf, which
+    begin() opens and end() closes."""
+
+    def __init__(self, folder, times, time_naming_factor=1.0, restored_time=None):
+        self.folder = folder
+        self.times = sorted(times)
+        if restored_time is not None:     # RemoveTimeStamps(..., restart time)
+            self.times = [t for t in self.times if t > restored_time]
+        self.factor = time_naming_factor
+        self.series = os.path.join(folder, "data_time_series")
+        self.written = []
+
+    def begin(self):
+        os.makedirs(self.folder, exist_ok=True)
+        with open(self.series + ".xdmf", "w") as fh:
+            fh.write(xdmf_header("TimeSeries"))
+
+    def maybe_write(self, t, write_file, force=False):
+        """write_file(path_without_extension, series) writes one output."""
+        if not (force or (self.times and self.times[0] <= t)):
+            return None
+        self.times = [x for x in self.times if x > t]
+        path = os.path.join(self.folder, "data_%f" % (t * self.factor))
+        write_file(path, self.series)
+        self.written.append(path + ".h5")
+        return path + ".h5"
+
+    def end(self):
+        with open(self.series + ".xdmf", "a") as fh:
+            fh.write(xdmf_footer())
 
 
 def read_dataset(h5, path):

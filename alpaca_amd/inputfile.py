@@ -24,9 +24,12 @@ than zero.  Multiresolution is replaced by a uniform level: ``level`` L gives
     python -m alpaca_amd.inputfile SodX.xml --out sod.h5   # reference HDF5 + XDMF layout
     python -m alpaca_amd.inputfile SodX.xml --restore SodX/restart/restart_0.100000.h5
 
-Restart snapshots follow the input's ``<restart>`` section (restore mode and
-file, snapshot stamps / wall-clock interval / intervals to keep) in the
-reference's restart file layout (alpaca_amd.restart).
+As the reference does, a run writes into a folder named after the input file
+(``--output-folder`` overrides it): the standard output the input's
+``<output><standardOutput>`` asks for in ``domain/`` (data_<t>.h5 + .xdmf in
+the reference's layout and data_time_series.xdmf), and the restart snapshots
+its ``<restart>`` section asks for (restore mode and file, snapshot stamps /
+wall-clock interval / intervals to keep) in ``restart/`` (alpaca_amd.restart).
 """
 from __future__ import annotations
 
@@ -71,6 +74,7 @@ class InputCase:
     snapshot_interval: int = 0         # wall seconds, 0 = no interval snapshots
     snapshots_to_keep: int = 0
     time_naming_factor: float = 1.0
+    output_times: list = dataclasses.field(default_factory=list)  # standard output
 
 
 def _text(root, path, default=None):
@@ -164,6 +168,19 @@ def read_input(path: str, dim: int | None = None) -> InputCase:
     if snap_type in ("interval", "intervalstamps"):
         interval = int(_text(root, "restart/snapshots/interval"))
         keep = int(_text(root, "restart/snapshots/intervalsToKeep"))
+    out_type = _text(root, "output/standardOutput/type", "Off").replace(" ", "").lower()
+    if out_type not in ("off", "interval", "stamps", "intervalstamps"):
+        raise InputError(f"standard output type {out_type} not known")
+    out_stamps = []
+    k = 1
+    while root.find(f"output/standardOutput/stamps/ts{k}") is not None:
+        out_stamps.append(_float(root, f"output/standardOutput/stamps/ts{k}"))
+        k += 1
+    from . import h5output
+
+    out_times = h5output.output_times(
+        out_type, _float(root, "output/standardOutput/interval", 0.0) if out_type in
+        ("interval", "intervalstamps") else 0.0, out_stamps, 0.0, end_time)
     return InputCase(
         path=path, dim=dim, node_size=_float(root, "domain/nodeSize"), node_ratio=ratio, bc=bc,
         gamma=gamma, background_pressure=bpress,
@@ -172,7 +189,7 @@ def read_input(path: str, dim: int | None = None) -> InputCase:
         maximum_level=int(_text(root, "multiResolution/maximumLevel", "0")),
         restore_mode=restore_mode, restore_file=restore_file, snapshot_stamps=stamps,
         snapshot_interval=interval, snapshots_to_keep=keep,
-        time_naming_factor=_float(root, "output/timeNamingFactor", 1.0))
+        time_naming_factor=_float(root, "output/timeNamingFactor", 1.0), output_times=out_times)
 
 
 def block_origin(index: int, block_size: float) -> float:
@@ -253,7 +270,8 @@ def restore_path(case: InputCase, restore: str | None = None):
 
 
 def run(case: InputCase, level: int = 0, flux: int = gpu.FLUX_HLLC, precision: int = gpu.FP64,
-        max_steps: int = 10 ** 9, restart_folder: str | None = None, restore: str | None = None):
+        max_steps: int = 10 ** 9, restart_folder: str | None = None, restore: str | None = None,
+        output_folder: str | None = None):
     """Advance the case to its end time on the GPU; returns a dict with the
     final conservatives, the primes, the step count, every dt, the time and
     the restart files written.
@@ -264,7 +282,11 @@ def run(case: InputCase, level: int = 0, flux: int = gpu.FLUX_HLLC, precision: i
     else from the initial condition; with restart_folder set, snapshots are
     written there after initialisation and after each macro step as the
     input's <restart><snapshots> asks (restart.SnapshotSchedule), forced
-    when the time step turns unhealthy."""
+    when the time step turns unhealthy; with output_folder, the standard
+    output the input's <output><standardOutput> asks is written there
+    (h5output.OutputSeries: density, velocity and pressure in the reference's
+    HDF5 + XDMF layout, forced at the start and when the step turns
+    unhealthy, plus the data_time_series.xdmf file)."""
     from . import restart as rs
 
     desc = case_desc(case, level, flux, precision)
@@ -281,6 +303,24 @@ def run(case: InputCase, level: int = 0, flux: int = gpu.FLUX_HLLC, precision: i
     else:
         st0 = initial_state(case, level)
     gc = gpu.GpuCase(desc, st0, cfl=case.cfl, t_end=case.end_time, time0=time0)
+    outputs = None
+    if output_folder is not None:
+        from . import h5output
+
+        outputs = h5output.OutputSeries(output_folder, case.output_times, case.time_naming_factor,
+                                        restored_time=time0 if path else None)
+        outputs.begin()
+        _, cell_size = cell_centres(case, level)
+        nb_out = [desc.nb[d] if d < case.dim else 1 for d in range(3)]
+
+        def write_output(t, force=False):
+            def one(p, series):
+                pr = primes(case, gc.download())
+                h5output.write(p, t, case.dim, nb_out, cell_size, pr["density"], pr["velocity"],
+                               pr["pressure"], series=series)
+            outputs.maybe_write(t, one, force)
+
+        write_output(time0, force=True)
     schedule = None
     if restart_folder is not None:
         schedule = rs.SnapshotSchedule(restart_folder, case.snapshot_stamps,
@@ -294,15 +334,25 @@ def run(case: InputCase, level: int = 0, flux: int = gpu.FLUX_HLLC, precision: i
                 case.background_pressure), force)
 
         write(time0)
-        on_step = lambda rec: write(rec.time, bool(rec.flags & 2))  # noqa: E731
-    else:
-        on_step = None
-    steps, dts = gc.run(max_steps, on_step=on_step)
+    hooks = []
+    if schedule is not None:   # a restart snapshot comes before the output
+        hooks.append(lambda rec: write(rec.time, bool(rec.flags & 2)))
+    if outputs is not None:
+        hooks.append(lambda rec: write_output(rec.time, bool(rec.flags & 2)))
+
+    def on_step(rec):
+        for h in hooks:
+            h(rec)
+
+    steps, dts = gc.run(max_steps, on_step=on_step if hooks else None)
+    if outputs is not None:
+        outputs.end()
     rec = gc.record()
     cons = gc.download()
     return {"conservatives": cons, "steps": steps, "dts": np.array(dts), "time": rec.time,
             "start_time": time0, "flags": rec.flags,
-            "restart_files": list(schedule.written) if schedule else [], **primes(case, cons)}
+            "restart_files": list(schedule.written) if schedule else [],
+            "output_files": list(outputs.written) if outputs else [], **primes(case, cons)}
 
 
 def primes(case: InputCase, cons):
@@ -325,21 +375,30 @@ def main(argv=None):
     ap.add_argument("--out", default=None,
                     help="write the final state: .npz (all arrays) or .h5 (the reference's "
                          "HDF5 + XDMF output layout)")
-    ap.add_argument("--restart-dir", default=None,
-                    help="folder for the restart snapshots the input file asks for "
-                         "(default <input name>/restart in the working directory)")
+    ap.add_argument("--output-folder", default=None,
+                    help="folder for the standard output (domain/) and restart snapshots "
+                         "(restart/) the input file asks for; default: the input file's name "
+                         "without extension, in the working directory, as the reference does")
     ap.add_argument("--restore", default=None,
                     help="start from this restart file (overrides <restart><restore>)")
     a = ap.parse_args(argv)
     case = read_input(a.input, a.dim)
     prec = {"fp64": gpu.FP64, "mixed": gpu.MIXED, "fp64-fma": gpu.FP64_FMA}[a.precision]
-    folder = a.restart_dir
-    if folder is None and a.level == 0:
-        import os
+    import os
 
-        folder = os.path.join(os.path.splitext(os.path.basename(a.input))[0], "restart")
+    base = a.output_folder or os.path.splitext(os.path.basename(a.input))[0]
+    from . import h5output
+
+    try:
+        h5output._lib()
+        have_hdf5 = True
+    except h5output.Hdf5Error:
+        have_hdf5 = False
     res = run(case, a.level, gpu.FLUX_HLLC if a.flux == "hllc" else gpu.FLUX_ROE, prec,
-              a.max_steps, restart_folder=folder, restore=a.restore)
+              a.max_steps,
+              restart_folder=os.path.join(base, "restart") if a.level == 0 and have_hdf5 else None,
+              restore=a.restore,
+              output_folder=os.path.join(base, "domain") if have_hdf5 else None)
     if a.out and a.out.endswith(".h5"):
         from . import h5output
 
@@ -353,7 +412,8 @@ def main(argv=None):
                       "blocks": [r << a.level for r in case.node_ratio[:case.dim]],
                       "steps": res["steps"], "start_time": res["start_time"],
                       "time": res["time"], "flags": res["flags"],
-                      "restart_files": res["restart_files"]}))
+                      "restart_files": res["restart_files"],
+                      "output_files": len(res["output_files"])}))
     return 0
 
 

@@ -80,7 +80,7 @@ def test_input_file_runs_match_oracle(torch, name, dim, flux):
 def test_input_file_command_line(torch, tmp_path):
     out = tmp_path / "sod.npz"
     rc = inputfile.main([os.path.join(INPUTS, "DetailedSod_2D_SodY.xml"), "--flux", "roe",
-                         "--out", str(out)])
+                         "--out", str(out), "--output-folder", str(tmp_path / "run1")])
     assert rc == 0
     data = np.load(out)
     assert float(data["time"]) == 0.2 and data["density"].shape == (16, 64, 1)
@@ -91,8 +91,37 @@ def test_input_file_command_line(torch, tmp_path):
     except h5output.Hdf5Error:
         return
     rc = inputfile.main([os.path.join(INPUTS, "DetailedSod_2D_SodY.xml"), "--flux", "roe",
-                         "--out", str(tmp_path / "sod.h5")])
+                         "--out", str(tmp_path / "sod.h5"), "--output-folder",
+                         str(tmp_path / "run2")])
     assert rc == 0
     rho = h5output.read_dataset(str(tmp_path / "sod.h5"), "/cell_data/density")
     assert rho.shape == (16 * 64, 1, 1)
     assert np.array_equal(np.sort(rho.ravel()), np.sort(data["density"].ravel()))
+
+
+def test_run_time_output_in_reference_layout(torch, tmp_path):
+    """SodX asks for standard output every 0.05: the run writes the initial
+    state, one file after each output time is passed and the end state, each
+    in the reference's HDF5 + XDMF layout, plus the time-series collection."""
+    import xml.etree.ElementTree as ET
+
+    from alpaca_amd import h5output
+
+    try:
+        h5output._lib()
+    except h5output.Hdf5Error:
+        pytest.skip("libhdf5 not available")
+    case = inputfile.read_input(os.path.join(INPUTS, "DetailedSod_2D_SodX.xml"))
+    assert case.output_times == [0.05, 0.1, 0.05 + 0.05 + 0.05, 0.2]
+    res = inputfile.run(case, 0, gpu.FLUX_ROE, output_folder=str(tmp_path / "domain"))
+    files = [os.path.basename(f) for f in res["output_files"]]
+    print(f"[run-time output] {files}", flush=True)
+    assert len(files) == 5 and files[0] == "data_0.000000.h5" and files[-1] == "data_0.200000.h5"
+    times = [h5output.read_time(f) for f in res["output_files"]]
+    assert times[0] == 0.0 and times[-1] == 0.2 and times == sorted(times)
+    for t, want in zip(times[1:4], (0.05, 0.1, 0.15)):
+        assert want <= t < want + 0.01
+    rho = h5output.read_dataset(res["output_files"][-1], "/cell_data/density")[:, 0, 0]
+    assert np.array_equal(np.sort(rho), np.sort(res["density"].ravel()))
+    root = ET.parse(str(tmp_path / "domain" / "data_time_series.xdmf")).getroot()
+    assert len(root.findall("Domain/Grid/Grid")) == 5

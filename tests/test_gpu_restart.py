@@ -112,7 +112,7 @@ def test_restore_from_input_file_and_command_line(torch, tmp_path):
         tree.getroot().find("restart/restore/mode").text = "Forced"
         tree.getroot().find("restart/restore/fileName").text = os.path.join(snap, mid)
         tree.write(path)
-        rc = inputfile.main([path, "--flux", "roe", "--restart-dir", str(tmp_path / "r2"),
+        rc = inputfile.main([path, "--flux", "roe", "--output-folder", str(tmp_path / "r2"),
                              "--out", str(tmp_path / "b.npz")])
         assert rc == 0
         a, b = np.load(tmp_path / "a.npz"), np.load(tmp_path / "b.npz")

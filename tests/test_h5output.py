@@ -100,3 +100,43 @@ def test_3d_layout_and_xdmf(tmp_path):
     assert names == ["density", "velocity", "pressure"]
     item = grid.find("Attribute/DataItem").text.strip()
     assert item == "out.h5:/cell_data/density"
+
+
+def test_output_times_follow_compute_output_times():
+    """ComputeOutputTimes (instantiation_input_output_manager.cpp:41-160):
+    interval times summed one interval at a time and kept below the end,
+    stamps strictly inside (start, end), the end time always last."""
+    ot = h5output.output_times
+    assert ot("off", 0.05, [], 0.0, 0.2) == []
+    assert ot("interval", 0.05, [], 0.0, 0.2) == [0.05, 0.1, 0.05 + 0.05 + 0.05, 0.2]
+    assert ot("interval", 0.3, [], 0.0, 1.0) == [0.3, 0.6, 0.3 + 0.3 + 0.3, 1.0]
+    assert ot("interval", 5.0, [], 0.0, 1.0) == [1.0]
+    assert ot("stamps", 0.0, [0.5, 0.0, 1.0, 0.25, 2.0], 0.0, 1.0) == [0.25, 0.5, 1.0]
+    assert ot("intervalstamps", 0.4, [0.1], 0.0, 1.0) == [0.1, 0.4, 0.8, 1.0]
+
+
+def test_output_series_schedule_and_time_series_file(tmp_path):
+    rng = np.random.default_rng(1)
+    dim, nb, cs = 2, [2, 1, 1], 1.0 / 32
+    rho = rng.uniform(1, 2, (32, 16, 1))
+    vel = rng.uniform(-1, 1, (2, 32, 16, 1))
+    p = rng.uniform(1, 2, (32, 16, 1))
+    series = h5output.OutputSeries(str(tmp_path / "domain"), [0.05, 0.1, 0.2])
+    series.begin()
+
+    def writer(t):
+        return lambda path, ts: h5output.write(path, t, dim, nb, cs, rho, vel, p, series=ts)
+
+    for t, force in ((0.0, True), (0.03, False), (0.12, False), (0.2, False), (0.21, False)):
+        series.maybe_write(t, writer(t), force)
+    series.end()
+    names = [os.path.basename(f) for f in series.written]
+    # forced at the start; 0.12 consumes 0.05 and 0.1; then the end time
+    assert names == ["data_0.000000.h5", "data_0.120000.h5", "data_0.200000.h5"]
+    root = ET.parse(str(tmp_path / "domain" / "data_time_series.xdmf")).getroot()
+    grids = root.findall("Domain/Grid/Grid")
+    assert [g.find("Time").get("Value") for g in grids] == ["0.000000e+00", "1.200000e-01",
+                                                           "2.000000e-01"]
+    assert grids[1].find("Attribute/DataItem").text.strip() == "data_0.120000.h5:/cell_data/density"
+    # a restart at 0.1 drops the times not after it
+    assert h5output.OutputSeries("x", [0.05, 0.1, 0.2], restored_time=0.1).times == [0.2]
