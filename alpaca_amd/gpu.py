@@ -11,8 +11,6 @@ import os
 
 import numpy as np
 
-from . import cases
-
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libalpaca_hip.so")
 
