@@ -1,0 +1,414 @@
+// Multiresolution operators of ALPACA between parent and child blocks held in
+// one pool of haloed blocks on gfx950: conservative averaging of a child into
+// its parent (Multiresolution::Average, multiresolution.h:94-167), Harten's
+// fifth-order prediction of a parent into a child (Multiresolution::
+// Prediction, multiresolution.cpp:146-292) and the L-infinity wavelet detail
+// with the remeshing decision (ChildNeedsRemeshing<Norm::Linfinity>,
+// multiresolution.cpp:299-323; RemeshingDecision, :408-428).
+//
+// Every expression keeps the reference's operation order and the translation
+// unit is built without contraction, so the results are the reference's bits;
+// std::min/std::max picks (first extreme, NaN comparisons false) are spelled
+// out so that non-finite inputs propagate as they do there.
+//
+// Work decomposition.  A link names (child block, parent block, sibling
+// position, child level).  Averaging is one thread per parent cell written
+// (k fastest, so the 2x2(x2) child reads of neighbouring threads are
+// contiguous).  Prediction and details stage the parent's stencil window --
+// the predicted parent cells plus two on each side, at most 16 per axis
+// (32 KiB of doubles in 3D) -- in LDS once per field and let each thread
+// predict the 2^DIM children of one parent cell from it: the 3D stencil reads
+// 125 parent values per parent cell, which LDS serves instead of L1/L2.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "fv_device.h"
+
+namespace alpaca {
+namespace mr {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxLevels = 32;
+
+// std::min / std::max of the reference: the first extreme, NaN never picked
+// by a comparison.
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+
+// ConsistencyManagedSum, three and four values (mathematical_functions.h:79-91,
+// 115-124), with std::min/std::max over the initializer list.
+__device__ __forceinline__ double cms3(double a, double b, double c) {
+  double const s1 = (a + b) + c;
+  double const s2 = (a + c) + b;
+  double const s3 = (b + c) + a;
+  double const mn = smin(smin(s1, s2), s3);
+  double const mx = smax(smax(s1, s2), s3);
+  return 0.5 * (mn + mx);
+}
+__device__ __forceinline__ double cms4(double a, double b, double c, double d) {
+  double const s1 = (a + b) + (c + d);
+  double const s2 = (a + c) + (b + d);
+  double const s3 = (a + d) + (b + c);
+  double const mn = smin(smin(s1, s2), s3);
+  double const mx = smax(smax(s1, s2), s3);
+  return 0.5 * (mn + mx);
+}
+// DimensionAwareConsistencyManagedSum (mathematical_functions.h:98-108).
+template <int DIM> __device__ __forceinline__ double dams(double a, double b, double c) {
+  if constexpr (DIM == 2) {
+    (void)c;
+    return a + b;
+  } else {
+    return cms3(a, b, c);
+  }
+}
+
+template <int DIM> __device__ __forceinline__ bool link_ok(const alpaca_mr_link &l, int nblocks) {
+  return l.child >= 0 && l.child < nblocks && l.parent >= 0 && l.parent < nblocks &&
+         l.child != l.parent && l.position >= 0 && l.position < (1 << DIM);
+}
+
+// ---------------------------------------------------------------- averaging
+
+// Parent internal cells covered by one child: IC/2 per axis, starting at HS
+// (low child) or TC/2 (high child) (xyz_look_up_table_, multiresolution.h:
+// 35-45); child cells from HS in steps of two.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void average_kernel(double *__restrict__ blocks, int nblocks,
+                                                           const alpaca_mr_link *__restrict__ links,
+                                                           int nlinks) {
+  using D = Dims<DIM>;
+  constexpr int H = kIC / 2;
+  constexpr int HZ = DIM == 3 ? H : 1;
+  constexpr int CELLS = H * H * HZ;
+  size_t const t = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= (size_t)nlinks * D::NE * CELLS) return;
+  int const r = int(t % CELLS);
+  int const e = int((t / CELLS) % D::NE);
+  alpaca_mr_link const l = links[t / ((size_t)CELLS * D::NE)];
+  if (!link_ok<DIM>(l, nblocks)) return;
+  int const i = r / (H * HZ), j = (r / HZ) % H, k = r % HZ;
+  const double *c = blocks + (size_t)l.child * D::BLOCK + (size_t)e * D::FIELD;
+  double *p = blocks + (size_t)l.parent * D::BLOCK + (size_t)e * D::FIELD;
+  int const x0 = (l.position & 1) ? kTC / 2 : kHS;
+  int const y0 = (l.position & 2) ? kTC / 2 : kHS;
+  int const ci = kHS + 2 * i, cj = kHS + 2 * j;
+  constexpr int SX = D::SX, SY = D::SY;
+  if constexpr (DIM == 2) {
+    const double *q = c + ci * SX + cj * SY;
+    p[(x0 + i) * SX + (y0 + j) * SY] = ((q[0] + q[SX + SY]) + (q[SX] + q[SY])) * 0.25;
+  } else {
+    int const z0 = (l.position & 4) ? kTC / 2 : kHS;
+    int const ck = kHS + 2 * k;
+    const double *q = c + ci * SX + cj * SY + ck;
+    p[(x0 + i) * SX + (y0 + j) * SY + z0 + k] =
+        0.125 * cms4(q[0] + q[SX + SY + 1], q[1] + q[SX + SY], q[SY] + q[SX + 1], q[SY + 1] + q[SX]);
+  }
+}
+
+// --------------------------------------------------------------- prediction
+
+// Window of parent cells a prediction reads along one axis: the predicted
+// parent cells [p0, p0 + n) (n = count / 2, p0 = start / 2 + 2 for the low
+// child, start / 2 + TC / 2 - 2 for the high one; PIOLCH/PIOHCH,
+// compile_time_constants.h:598-608) and two more on each side.
+struct Range {
+  int start[3];
+  int count[3];  // predicted parent cells per axis (child count / 2)
+};
+
+// The 2^DIM predicted child values of the parent cell at LDS index `o` of a
+// window with strides (sx, sy, 1): out[dx | dy << 1 | dz << 2].  In 2D the
+// z terms are the reference's zero-initialised Qz, Qxz, Qyz, Qxyz, and enter
+// the sums as such (their signs matter for signed zeros).
+template <int DIM>
+__device__ __forceinline__ void predict_cell(const double *__restrict__ s, int o, int sx, int sy,
+                                             double out[1 << DIM]) {
+  constexpr double c0 = -22.0 / 128.0, c1 = 3.0 / 128.0;
+  constexpr double c00 = c0 * c0, c01 = c0 * c1, c10 = c01, c11 = c1 * c1;
+  constexpr double c000 = c00 * c0, c001 = c00 * c1, c010 = c001, c011 = c0 * c11;
+  constexpr double c100 = c001, c101 = c011, c110 = c011, c111 = c1 * c11;
+#define P(a, b, c) s[o + (a) * sx + (b) * sy + (c)]
+  double Qz = 0.0, Qxz = 0.0, Qyz = 0.0, Qxyz = 0.0;
+  double const Qx = c0 * (P(1, 0, 0) - P(-1, 0, 0)) + c1 * (P(2, 0, 0) - P(-2, 0, 0));
+  double const Qy = c0 * (P(0, 1, 0) - P(0, -1, 0)) + c1 * (P(0, 2, 0) - P(0, -2, 0));
+  double const Qxy =
+      (c00 * ((P(1, 1, 0) + P(-1, -1, 0)) - (P(-1, 1, 0) + P(1, -1, 0))) +
+       c11 * ((P(2, 2, 0) + P(-2, -2, 0)) - (P(-2, 2, 0) + P(2, -2, 0)))) +
+      (c01 * ((P(1, 2, 0) + P(-1, -2, 0)) - (P(-1, 2, 0) + P(1, -2, 0))) +
+       c10 * ((P(2, 1, 0) + P(-2, -1, 0)) - (P(-2, 1, 0) + P(2, -1, 0))));
+  if constexpr (DIM == 3) {
+    Qz = c0 * (P(0, 0, 1) - P(0, 0, -1)) + c1 * (P(0, 0, 2) - P(0, 0, -2));
+    Qxz = (c00 * ((P(1, 0, 1) + P(-1, 0, -1)) - (P(-1, 0, 1) + P(1, 0, -1))) +
+           c11 * ((P(2, 0, 2) + P(-2, 0, -2)) - (P(-2, 0, 2) + P(2, 0, -2)))) +
+          (c01 * ((P(1, 0, 2) + P(-1, 0, -2)) - (P(-1, 0, 2) + P(1, 0, -2))) +
+           c10 * ((P(2, 0, 1) + P(-2, 0, -1)) - (P(-2, 0, 1) + P(2, 0, -1))));
+    Qyz = (c00 * ((P(0, 1, 1) + P(0, -1, -1)) - (P(0, -1, 1) + P(0, 1, -1))) +
+           c11 * ((P(0, 2, 2) + P(0, -2, -2)) - (P(0, -2, 2) + P(0, 2, -2)))) +
+          (c01 * ((P(0, 1, 2) + P(0, -1, -2)) - (P(0, -1, 2) + P(0, 1, -2))) +
+           c10 * ((P(0, 2, 1) + P(0, -2, -1)) - (P(0, -2, 1) + P(0, 2, -1))));
+    // The four antisymmetric differences over the corners (+-a, +-b, +-c).
+#define G(a, b, c)                                     \
+  cms4(P(a, b, c) - P(-(a), -(b), -(c)), P(-(a), -(b), c) - P(a, b, -(c)), \
+       P(a, -(b), -(c)) - P(-(a), b, c), P(-(a), b, -(c)) - P(a, -(b), c))
+    Qxyz = cms4(c000 * G(1, 1, 1) + c111 * G(2, 2, 2), c001 * G(1, 1, 2) + c110 * G(2, 2, 1),
+                c011 * G(1, 2, 2) + c100 * G(2, 1, 1), c010 * G(1, 2, 1) + c101 * G(2, 1, 2));
+#undef G
+  }
+  double const p = P(0, 0, 0);
+#undef P
+  out[0] = p + dams<DIM>(Qx, Qy, Qz) + dams<DIM>(Qxy, Qxz, Qyz) + Qxyz;
+  out[1] = p + dams<DIM>(-Qx, Qy, Qz) + dams<DIM>(-Qxy, -Qxz, Qyz) - Qxyz;
+  out[2] = p + dams<DIM>(Qx, -Qy, Qz) + dams<DIM>(-Qxy, Qxz, -Qyz) - Qxyz;
+  out[3] = p + dams<DIM>(-Qx, -Qy, Qz) + dams<DIM>(Qxy, -Qxz, -Qyz) + Qxyz;
+  if constexpr (DIM == 3) {
+    out[4] = p + cms3(Qx, Qy, -Qz) + cms3(Qxy, -Qxz, -Qyz) - Qxyz;
+    out[5] = p + cms3(-Qx, Qy, -Qz) + cms3(-Qxy, Qxz, -Qyz) + Qxyz;
+    out[6] = p + cms3(Qx, -Qy, -Qz) + cms3(-Qxy, -Qxz, Qyz) + Qxyz;
+    out[7] = p + cms3(-Qx, -Qy, -Qz) + cms3(Qxy, Qxz, Qyz) - Qxyz;
+  }
+}
+
+// Window geometry of one link: first parent cell of the window per axis and
+// the window's strides.
+template <int DIM> struct Window {
+  int p0[3];   // first window cell (predicted cell - 2) per axis
+  int w[3];    // window extent per axis
+  int sx, sy;  // LDS strides of i and j
+  __device__ Window(const Range &r, int position) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      int const off = ((position >> a) & 1) ? kTC / 2 - kHS / 2 : kHS / 2;
+      bool const live = a < DIM;
+      p0[a] = live ? r.start[a] / 2 + off - 2 : 0;
+      w[a] = live ? r.count[a] + 4 : 1;
+    }
+    sy = w[2];
+    sx = w[1] * w[2];
+  }
+  __device__ int size() const { return w[0] * w[1] * w[2]; }
+  // LDS index of predicted parent cell (a, b, c) (counted from the range's
+  // first predicted cell).
+  __device__ int centre(int a, int b, int c) const {
+    return (a + 2) * sx + (b + 2) * sy + (DIM == 3 ? c + 2 : 0);
+  }
+};
+
+template <int DIM>
+__device__ __forceinline__ void stage_window(double *s, const double *__restrict__ field,
+                                             const Window<DIM> &win) {
+  using D = Dims<DIM>;
+  int const n = win.size();
+  for (int t = threadIdx.x; t < n; t += kThreads) {
+    int const a = t / win.sx, b = (t / win.sy) % win.w[1], c = t % win.w[2];
+    s[t] = field[(win.p0[0] + a) * D::SX + (win.p0[1] + b) * D::SY + (win.p0[2] + c)];
+  }
+}
+
+// One workgroup per (link, equation): the child cells of the range,
+// predicted from the parent's field.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void prediction_kernel(double *__restrict__ blocks, int nblocks,
+                                                              const alpaca_mr_link *__restrict__ links,
+                                                              Range r) {
+  using D = Dims<DIM>;
+  __shared__ double s[16 * 16 * (DIM == 3 ? 16 : 1)];
+  int const e = blockIdx.x % D::NE;
+  alpaca_mr_link const l = links[blockIdx.x / D::NE];
+  if (!link_ok<DIM>(l, nblocks)) return;
+  Window<DIM> const win(r, l.position);
+  stage_window<DIM>(s, blocks + (size_t)l.parent * D::BLOCK + (size_t)e * D::FIELD, win);
+  __syncthreads();
+  double *child = blocks + (size_t)l.child * D::BLOCK + (size_t)e * D::FIELD;
+  int const nz = DIM == 3 ? r.count[2] : 1;
+  int const tasks = r.count[0] * r.count[1] * nz;
+  for (int t = threadIdx.x; t < tasks; t += kThreads) {
+    int const a = t / (r.count[1] * nz), b = (t / nz) % r.count[1], c = t % nz;
+    double v[1 << DIM];
+    predict_cell<DIM>(s, win.centre(a, b, c), win.sx, win.sy, v);
+    int const ci = r.start[0] + 2 * a, cj = r.start[1] + 2 * b;
+    int const ck = DIM == 3 ? r.start[2] + 2 * c : 0;
+    double *q = child + ci * D::SX + cj * D::SY + ck;
+    q[0] = v[0];
+    q[D::SX] = v[1];
+    q[D::SY] = v[2];
+    q[D::SX + D::SY] = v[3];
+    if constexpr (DIM == 3) {
+      q[1] = v[4];
+      q[D::SX + 1] = v[5];
+      q[D::SY + 1] = v[6];
+      q[D::SX + D::SY + 1] = v[7];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ details
+
+struct Thresholds {
+  double eps[kMaxLevels];  // coarsening threshold per level
+  int nlevels;
+};
+
+// One workgroup per link: over Mass and Energy (the wavelet equations,
+// field_setup.h:243-247) and every child cell, halo included, the largest
+// |exact - predicted| / |exact|, the prediction taken over the whole block.
+// std::max(max_detail, x) drops NaN quotients, so every partial maximum is a
+// non-NaN value >= 0 and the order of the reduction does not matter.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void details_kernel(const double *__restrict__ blocks, int nblocks,
+                                                           const alpaca_mr_link *__restrict__ links,
+                                                           Thresholds th, double *__restrict__ details,
+                                                           int *__restrict__ decisions) {
+  using D = Dims<DIM>;
+  constexpr int HZ = DIM == 3 ? kTC / 2 : 1;
+  constexpr int TASKS = (kTC / 2) * (kTC / 2) * HZ;
+  __shared__ double s[16 * 16 * (DIM == 3 ? 16 : 1)];
+  __shared__ double part[kThreads / 64];
+  alpaca_mr_link const l = links[blockIdx.x];
+  if (!link_ok<DIM>(l, nblocks)) {
+    if (threadIdx.x == 0) {
+      details[blockIdx.x] = __builtin_nan("");
+      if (decisions) decisions[blockIdx.x] = -1;
+    }
+    return;
+  }
+  Range const full{{0, 0, 0}, {kTC / 2, kTC / 2, HZ}};
+  Window<DIM> const win(full, l.position);
+  double m = 0.0;
+  for (int e = kMass; e <= kEnergy; ++e) {
+    if (e != kMass) __syncthreads();  // the previous field's window is consumed
+    stage_window<DIM>(s, blocks + (size_t)l.parent * D::BLOCK + (size_t)e * D::FIELD, win);
+    __syncthreads();
+    const double *child = blocks + (size_t)l.child * D::BLOCK + (size_t)e * D::FIELD;
+    for (int t = threadIdx.x; t < TASKS; t += kThreads) {
+      int const a = t / ((kTC / 2) * HZ), b = (t / HZ) % (kTC / 2), c = t % HZ;
+      double v[1 << DIM];
+      predict_cell<DIM>(s, win.centre(a, b, c), win.sx, win.sy, v);
+      const double *q = child + 2 * a * D::SX + 2 * b * D::SY + (DIM == 3 ? 2 * c : 0);
+      double x[1 << DIM];
+      x[0] = q[0];
+      x[1] = q[D::SX];
+      x[2] = q[D::SY];
+      x[3] = q[D::SX + D::SY];
+      if constexpr (DIM == 3) {
+        x[4] = q[1];
+        x[5] = q[D::SX + 1];
+        x[6] = q[D::SY + 1];
+        x[7] = q[D::SX + D::SY + 1];
+      }
+#pragma unroll
+      for (int n = 0; n < (1 << DIM); ++n)
+        m = smax(m, __builtin_fabs(x[n] - v[n]) / __builtin_fabs(x[n]));
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = smax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double d = part[0];
+#pragma unroll
+    for (int w = 1; w < kThreads / 64; ++w) d = smax(d, part[w]);
+    details[blockIdx.x] = d;
+    if (decisions) {
+      int dec = -1;
+      if (l.level >= 0 && l.level < th.nlevels) {
+        // RemeshingDecision (multiresolution.cpp:408-428): Coarse 2,
+        // Refine 1, Neutral 0 (remesh_identifier.h:29).
+        double const eps_coarsen = th.eps[l.level];
+        double const eps_refine = eps_coarsen * 32.0;
+        dec = d <= eps_coarsen ? 2 : (d >= eps_refine ? 1 : 0);
+      }
+      decisions[blockIdx.x] = dec;
+    }
+  }
+}
+
+int hip_status(hipError_t e) { return e == hipSuccess ? ALPACA_OK : ALPACA_ERR_HIP; }
+
+template <int DIM> int average_impl(double *blocks, int nblocks, const alpaca_mr_link *links,
+                                    int nlinks, hipStream_t s) {
+  constexpr int H = kIC / 2;
+  size_t const total = (size_t)nlinks * Dims<DIM>::NE * H * H * (DIM == 3 ? H : 1);
+  dim3 grid((unsigned)((total + kThreads - 1) / kThreads));
+  hipLaunchKernelGGL((average_kernel<DIM>), grid, dim3(kThreads), 0, s, blocks, nblocks, links, nlinks);
+  return hip_status(hipGetLastError());
+}
+
+template <int DIM> int prediction_impl(double *blocks, int nblocks, const alpaca_mr_link *links,
+                                       int nlinks, const Range &r, hipStream_t s) {
+  dim3 grid((unsigned)nlinks * Dims<DIM>::NE);
+  hipLaunchKernelGGL((prediction_kernel<DIM>), grid, dim3(kThreads), 0, s, blocks, nblocks, links, r);
+  return hip_status(hipGetLastError());
+}
+
+template <int DIM> int details_impl(const double *blocks, int nblocks, const alpaca_mr_link *links,
+                                    int nlinks, const Thresholds &th, double *details, int *decisions,
+                                    hipStream_t s) {
+  hipLaunchKernelGGL((details_kernel<DIM>), dim3((unsigned)nlinks), dim3(kThreads), 0, s, blocks,
+                     nblocks, links, th, details, decisions);
+  return hip_status(hipGetLastError());
+}
+
+}  // namespace
+}  // namespace mr
+}  // namespace alpaca
+
+using namespace alpaca;
+
+extern "C" {
+
+int alpaca_mr_average(int dim, double *d_blocks, int nblocks, const alpaca_mr_link *d_links,
+                      int nlinks, void *stream) {
+  if ((dim != 2 && dim != 3) || nblocks < 1 || nlinks < 0 || !d_blocks || (nlinks > 0 && !d_links))
+    return ALPACA_ERR_INVALID;
+  if (nlinks == 0) return ALPACA_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dim == 2) return mr::average_impl<2>(d_blocks, nblocks, d_links, nlinks, s);
+  return mr::average_impl<3>(d_blocks, nblocks, d_links, nlinks, s);
+}
+
+int alpaca_mr_prediction(int dim, double *d_blocks, int nblocks, const alpaca_mr_link *d_links,
+                         int nlinks, const int *range, void *stream) {
+  if ((dim != 2 && dim != 3) || nblocks < 1 || nlinks < 0 || !d_blocks || !range ||
+      (nlinks > 0 && !d_links))
+    return ALPACA_ERR_INVALID;
+  mr::Range r{{0, 0, 0}, {1, 1, 1}};
+  for (int a = 0; a < dim; ++a) {
+    int const start = range[2 * a], count = range[2 * a + 1];
+    // Even start (the reference's own precondition, multiresolution.cpp:154-
+    // 159) and inside the block: every parent cell read, start / 2 + 2 - 2 up
+    // to start / 2 + count / 2 + TC / 2 - 2 + 1, then lies in [0, TC).
+    if (start < 0 || count < 0 || (start & 1) || start + count > kTC) return ALPACA_ERR_INVALID;
+    r.start[a] = start;
+    r.count[a] = count / 2;
+  }
+  if (nlinks == 0 || r.count[0] == 0 || r.count[1] == 0 || r.count[2] == 0)
+    return ALPACA_OK;  // nothing to predict
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dim == 2) return mr::prediction_impl<2>(d_blocks, nblocks, d_links, nlinks, r, s);
+  return mr::prediction_impl<3>(d_blocks, nblocks, d_links, nlinks, r, s);
+}
+
+int alpaca_mr_child_needs_remeshing(int dim, const double *d_blocks, int nblocks,
+                                    const alpaca_mr_link *d_links, int nlinks,
+                                    const double *eps_coarsen, int nlevels, double *d_details,
+                                    int *d_decisions, void *stream) {
+  if ((dim != 2 && dim != 3) || nblocks < 1 || nlinks < 0 || !d_blocks ||
+      (nlinks > 0 && (!d_links || !d_details)) || nlevels < 0 || nlevels > mr::kMaxLevels ||
+      (nlevels > 0 && !eps_coarsen) || (d_decisions && nlevels == 0))
+    return ALPACA_ERR_INVALID;
+  if (nlinks == 0) return ALPACA_OK;
+  mr::Thresholds th;
+  std::memset(&th, 0, sizeof(th));
+  th.nlevels = nlevels;
+  for (int i = 0; i < nlevels; ++i) th.eps[i] = eps_coarsen[i];
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dim == 2)
+    return mr::details_impl<2>(d_blocks, nblocks, d_links, nlinks, th, d_details, d_decisions, s);
+  return mr::details_impl<3>(d_blocks, nblocks, d_links, nlinks, th, d_details, d_decisions, s);
+}
+
+}  // extern "C"

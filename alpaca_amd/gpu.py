@@ -90,6 +90,16 @@ _SIGNATURES = {
     "alpaca_advance": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_void_p,
                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
                        ctypes.c_int),
+    "alpaca_mr_average": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                           ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_mr_prediction": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                              ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p],
+                             ctypes.c_int),
+    "alpaca_mr_child_needs_remeshing": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+                                        ctypes.c_int),
 }
 
 

@@ -1,0 +1,145 @@
+"""Multiresolution operators on a device pool of haloed blocks.
+
+Host side of the C ABI's alpaca_mr_* entry points, named after the
+reference's classes: ``ThresholdComputer`` (threshold_computer.h:35-97) and
+``Multiresolution`` (multiresolution.h:59-188) with ``average``
+(Multiresolution::Average), ``prediction`` (Multiresolution::Prediction) and
+``child_needs_remeshing`` (ChildNeedsRemeshing<Norm::Linfinity> with
+RemeshingDecision).  Where the reference works on one parent/child pair per
+call, these take a batch of links -- (child block, parent block, sibling
+position, child level) rows of an int32 device tensor -- and run one launch
+over all of them.  Blocks are the pool's [nblocks][NE][24][24][24|1] float64
+device tensor.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import gpu
+
+# RemeshIdentifier (remesh_identifier.h:29).
+NEUTRAL, REFINE, COARSE = 0, 1, 2
+MAX_LEVELS = 32
+
+# Compile-time constants of the threshold: STDO, the limiting convergence
+# order of space and time (compile_time_constants.h:617), and the refinement
+# factor 2^(p+1) of the fifth-order prediction (multiresolution.cpp:412-418).
+STDO = 3
+REFINE_FACTOR = 32.0
+
+
+class ThresholdComputer:
+    """Level-dependent coarsening threshold (threshold_computer.h:35-97):
+    eps_ref = eps_user * 2^(-(STDO + 1)(maximum_level - reference_level)),
+    ThresholdOnLevel(l) = eps_ref * 2^(-dim (maximum_level - l))."""
+
+    def __init__(self, dim: int, maximum_level: int, user_reference_level: int,
+                 user_reference_epsilon: float):
+        self.dim = int(dim)
+        self.maximum_level = int(maximum_level)
+        self.reference_epsilon = float(user_reference_epsilon) * 2.0 ** (
+            -1.0 * (STDO + 1.0) * (float(maximum_level) - float(user_reference_level)))
+
+    def threshold_on_level(self, level: int) -> float:
+        return self.reference_epsilon * 2.0 ** (
+            -1.0 * float(self.dim) * (float(self.maximum_level) - float(level)))
+
+    def table(self, nlevels: int | None = None) -> list:
+        n = self.maximum_level + 1 if nlevels is None else nlevels
+        return [self.threshold_on_level(level) for level in range(n)]
+
+
+def remeshing_decision(detail: float, eps_coarsen: float) -> int:
+    """RemeshingDecision (multiresolution.cpp:408-428) on the host."""
+    if detail <= eps_coarsen:
+        return COARSE
+    if detail >= eps_coarsen * REFINE_FACTOR:
+        return REFINE
+    return NEUTRAL
+
+
+def make_links(rows) -> np.ndarray:
+    """int32 [n][4] links from (child, parent, position, level) rows."""
+    a = np.asarray(list(rows), dtype=np.int32).reshape(-1, 4)
+    return np.ascontiguousarray(a)
+
+
+def _check_pool(blocks, dim):
+    if blocks.dtype != gpu._torch().float64 or not blocks.is_contiguous():
+        raise gpu.AlpacaError("blocks must be a contiguous float64 device tensor")
+    n = blocks.numel() // gpu.block_doubles(dim)
+    if n * gpu.block_doubles(dim) != blocks.numel() or n < 1:
+        raise gpu.AlpacaError("blocks is not a whole number of haloed blocks")
+    return n
+
+
+def _check_links(links):
+    torch = gpu._torch()
+    if links.dtype != torch.int32 or not links.is_contiguous() or links.dim() != 2 \
+            or links.shape[1] != 4:
+        raise gpu.AlpacaError("links must be a contiguous int32 [n][4] device tensor")
+    return links.shape[0]
+
+
+def _stream(stream):
+    return gpu._stream_ptr() if stream is None else stream
+
+
+class Multiresolution:
+    """Batched Multiresolution operators; ``thresholder`` (a ThresholdComputer)
+    is needed by child_needs_remeshing only."""
+
+    def __init__(self, thresholder: ThresholdComputer | None = None):
+        self.thresholder = thresholder
+
+    @staticmethod
+    def average(blocks, dim, links, stream=None):
+        """Averages every link's child into its parent's octant (quadrant)."""
+        lib = gpu.load_library()
+        nb, nl = _check_pool(blocks, dim), _check_links(links)
+        gpu.check(lib.alpaca_mr_average(dim, ctypes.c_void_p(blocks.data_ptr()), nb,
+                                        ctypes.c_void_p(links.data_ptr()), nl, _stream(stream)),
+                  "mr_average")
+
+    @staticmethod
+    def prediction(blocks, dim, links, rng=None, stream=None):
+        """Predicts every link's child cells in rng = (x_start, x_count,
+        y_start, y_count[, z_start, z_count]) from its parent (default: the
+        whole block, as ChildNeedsRemeshing uses it)."""
+        lib = gpu.load_library()
+        nb, nl = _check_pool(blocks, dim), _check_links(links)
+        if rng is None:
+            rng = (0, gpu.TC) * dim
+        r = (ctypes.c_int * 6)(*(list(rng) + [0, 1] * (3 - len(rng) // 2)))
+        gpu.check(lib.alpaca_mr_prediction(dim, ctypes.c_void_p(blocks.data_ptr()), nb,
+                                           ctypes.c_void_p(links.data_ptr()), nl, r,
+                                           _stream(stream)), "mr_prediction")
+
+    def child_needs_remeshing(self, blocks, dim, links, stream=None, out=None):
+        """(details float64[n], decisions int32[n]) device tensors; decisions
+        are RemeshIdentifier values, -1 for a skipped link.  `out` may hand
+        in the two tensors to fill."""
+        torch = gpu._torch()
+        if self.thresholder is None:
+            raise gpu.AlpacaError("child_needs_remeshing needs a ThresholdComputer")
+        lib = gpu.load_library()
+        nb, nl = _check_pool(blocks, dim), _check_links(links)
+        table = self.thresholder.table()
+        if len(table) > MAX_LEVELS:
+            raise gpu.AlpacaError(f"at most {MAX_LEVELS} levels")
+        eps = (ctypes.c_double * len(table))(*table)
+        if out is None:
+            details = torch.empty(nl, dtype=torch.float64, device=blocks.device)
+            decisions = torch.empty(nl, dtype=torch.int32, device=blocks.device)
+        else:
+            details, decisions = out
+            if details.dtype != torch.float64 or decisions.dtype != torch.int32 \
+                    or details.numel() < nl or decisions.numel() < nl:
+                raise gpu.AlpacaError("out: float64 and int32 tensors of at least n entries")
+        gpu.check(lib.alpaca_mr_child_needs_remeshing(
+            dim, ctypes.c_void_p(blocks.data_ptr()), nb, ctypes.c_void_p(links.data_ptr()), nl,
+            eps, len(table), ctypes.c_void_p(details.data_ptr()),
+            ctypes.c_void_p(decisions.data_ptr()), _stream(stream)), "mr_child_needs_remeshing")
+        return details, decisions

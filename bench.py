@@ -14,7 +14,9 @@ Prints one JSON line.  `roofline` covers the dominant kernel (the fused
 stage kernel) timed with HIP events on its launch stream during the timed
 region; `modes` repeats the workload in the FP64-with-FMA and mixed forms
 (with the mixed density error against FP64); `roe_fp64` runs it with Roe
-flux splitting in FP64; `cpu_baseline` times the reference's own numerics
+flux splitting in FP64; `mr_operators` times the multiresolution kernels
+(averaging, prediction, wavelet details) on 512 parents and their 4096
+children; `cpu_baseline` times the reference's own numerics
 (oracle/_ref, threads over blocks) on a bounded sample of the same workload
 on this host's cores.
 """
@@ -306,6 +308,69 @@ def run_accuracy(args):
                       "rows": rows}), flush=True)
 
 
+# Algorithmic HBM bytes per link of the multiresolution kernels (3D, five
+# equations, doubles): averaging reads the child's 16^3 internal cells and
+# writes the parent's 8^3 octant; prediction reads the parent's stencil window
+# (predicted cells + 2 per side: 16^3 for the whole block, 12^3 for the
+# internal cells) and writes the child cells (24^3 or 16^3); the details read
+# the parent window and all 24^3 child cells of Mass and Energy.
+MR_BYTES_PER_LINK = {
+    "average": (16 ** 3 + 8 ** 3) * 5 * 8,
+    "prediction_block": (16 ** 3 + 24 ** 3) * 5 * 8,
+    "prediction_internal": (12 ** 3 + 16 ** 3) * 5 * 8,
+    "details": (16 ** 3 + 24 ** 3) * 2 * 8,
+}
+
+
+def run_mr(args, nparents=512):
+    """Multiresolution kernels on 512 parents and their 4096 children (3D;
+    the 256^3 cells of level 1): each operator over all links per launch,
+    timed with HIP events on the launch stream."""
+    import torch
+
+    from alpaca_amd import gpu
+    from alpaca_amd import multiresolution as mr
+
+    dim = 3
+    n = nparents * 9
+    bd = gpu.block_doubles(dim)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    blocks = (torch.rand(n * bd, dtype=torch.float64, device="cuda", generator=g) + 0.5)
+    rows = [(p * 9 + 1 + pos, p * 9, pos, 1) for p in range(nparents) for pos in range(8)]
+    links = torch.as_tensor(mr.make_links(rows), device="cuda")
+    nl = len(rows)
+    m = mr.Multiresolution(mr.ThresholdComputer(dim, 1, 1, 1e-3))
+    out = (torch.empty(nl, dtype=torch.float64, device="cuda"),
+           torch.empty(nl, dtype=torch.int32, device="cuda"))
+    ops = {
+        "average": lambda: m.average(blocks, dim, links),
+        "prediction_block": lambda: m.prediction(blocks, dim, links),
+        "prediction_internal": lambda: m.prediction(blocks, dim, links, (4, 16, 4, 16, 4, 16)),
+        "details": lambda: m.child_needs_remeshing(blocks, dim, links, out=out),
+    }
+    stream = torch.cuda.current_stream()
+    reps = max(args.steps, 1)
+    res = {}
+    for name, op in ops.items():
+        for _ in range(max(args.warmup, 1)):
+            op()
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(reps):
+            op()
+        t1.record(stream)
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / reps
+        bytes_launch = MR_BYTES_PER_LINK[name] * nl
+        gbs = bytes_launch / (ms / 1e3) / 1e9
+        res[name] = {"avg_launch_ms": ms, "links": nl, "bytes_per_link": MR_BYTES_PER_LINK[name],
+                     "hbm_GBs": gbs, "frac": gbs / (PEAK_HBM / 1e9)}
+    return {"workload": f"3D MR operators, {nparents} parents x 8 children (256^3 on level 1), "
+                        "FP64", "kernels": res}
+
+
 def run_copy_probe():
     """A plain device copy of known size, for scaling the HBM counters."""
     import torch
@@ -332,7 +397,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-modes", action="store_true",
                     help="time only --precision (skip the other arithmetic modes)")
-    ap.add_argument("--workload", choices=["uniform", "batched", "serving", "accuracy"],
+    ap.add_argument("--workload", choices=["uniform", "batched", "serving", "accuracy", "mr"],
                     default="uniform",
                     help="uniform: the 256^3 case (default); batched: 512 uneven 2D cases; "
                          "serving: Poisson request stream")
@@ -363,6 +428,9 @@ def main():
         return
     if args.workload == "batched":
         run_batched(args)
+        return
+    if args.workload == "mr":
+        print(json.dumps(run_mr(args)), flush=True)
         return
     if args.workload == "serving":
         from alpaca_amd import serving
@@ -466,6 +534,9 @@ def main():
                                 "stage_avg_ms": roe["stage_avg_ms"],
                                 "roofline_frac": (roe["cells_per_launch"] * FLOP_PER_CELL_STAGE[(3, 1)]
                                                   / (roe["stage_avg_ms"] / 1e3) / PEAK_FP64)}
+        # The multiresolution operators (averaging, prediction, details) of
+        # the multi-level mode at the same cell count on level 1.
+        line["mr_operators"] = run_mr(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(flux, cpu_threads())
     print(json.dumps(line), flush=True)

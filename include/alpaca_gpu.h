@@ -262,6 +262,59 @@ int alpaca_ensemble_copy_case(alpaca_ensemble *e, int case_id, double *d_dst);
 int alpaca_ensemble_capacity(alpaca_ensemble *e, int *free_blocks, int *largest_range,
                              int *free_slots);
 
+/* ------------------------------------------------------------------------
+ * Multiresolution operators between parent and child blocks of one pool of
+ * haloed blocks (the building blocks of ALPACA's multi-level mode).  A link
+ * names a child block, its parent block (pool indices), the child's position
+ * among its siblings (id & 7, id_information.h:158-160: bit 0 x, bit 1 y,
+ * bit 2 z; 0..3 in 2D) and the child's level.  Links are in device memory;
+ * a link with an index outside [0, nblocks), child == parent or a position
+ * outside [0, 2^dim) is skipped.  Within one call no block may be both a
+ * parent and a child (the reference walks the levels one at a time). */
+typedef struct {
+  int child;
+  int parent;
+  int position;
+  int level;
+} alpaca_mr_link;
+
+/* Multiresolution::Average (multiresolution.h:94-167; used by
+ * Averager::AverageMaterial, averager.cpp:62-140): for every link and every
+ * equation, the parent internal cells the child covers -- IC/2 per axis from
+ * HS or TC/2 -- become the conservative mean of the 2^dim child cells under
+ * each.  Children of one parent may share a call. */
+int alpaca_mr_average(int dim, double *d_blocks, int nblocks,
+                      const alpaca_mr_link *d_links, int nlinks, void *stream);
+
+/* Multiresolution::Prediction (multiresolution.cpp:146-292) of every
+ * equation field, for refinement (RefineNode, modular_algorithm_assembler.cpp:
+ * 2547-2553: range = HS, IC per axis) and for halos across a level jump
+ * (InternalHaloManager::UpdateMaterialJumpNoMpi, internal_halo_manager.cpp:
+ * 231-284: the halo region).  range (host, int[2*dim]: x_start, x_count,
+ * y_start, y_count, z_start, z_count) selects the child cells written:
+ * [start, start + 2*(count/2)) per axis; starts must be even and
+ * start + count <= 24 (ALPACA_ERR_INVALID otherwise). */
+int alpaca_mr_prediction(int dim, double *d_blocks, int nblocks,
+                         const alpaca_mr_link *d_links, int nlinks,
+                         const int *range, void *stream);
+
+/* ChildNeedsRemeshing<Norm::Linfinity> (multiresolution.cpp:299-323, called
+ * by DetermineRemeshingNodes, modular_algorithm_assembler.cpp:2425-2480) on
+ * the blocks given (the reference passes the right-hand-side buffers):
+ * d_details[n] (device) = the largest |exact - predicted| / |exact| over Mass
+ * and Energy (field_setup.h:243-247) and every child cell, halo included
+ * (NaN quotients dropped; NaN for a skipped link).  With d_decisions (device,
+ * may be NULL) also RemeshingDecision (multiresolution.cpp:408-428):
+ * 2 Coarse (detail <= eps), 1 Refine (detail >= 32 eps), 0 Neutral, -1 for a
+ * skipped link or a level outside the table, where eps = eps_coarsen[level]
+ * (host, nlevels <= 32 entries, ThresholdComputer::ThresholdOnLevel,
+ * threshold_computer.h:91-94). */
+int alpaca_mr_child_needs_remeshing(int dim, const double *d_blocks, int nblocks,
+                                    const alpaca_mr_link *d_links, int nlinks,
+                                    const double *eps_coarsen, int nlevels,
+                                    double *d_details, int *d_decisions,
+                                    void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -961,3 +961,181 @@ void ora_domain_sums(int dim, const int *ncell, const double *state,
     out[e] = s;
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Multiresolution operators between a parent block and one of its children.  */
+/* Fields are [TC][TC][TCZ] (TCZ = 1 in 2D); blocks are [NE] such fields.      */
+/* A child's position among its siblings is id & 7: bit 0 x, bit 1 y, bit 2 z  */
+/* (id_information.h:158-160).                                                 */
+
+/* ConsistencyManagedSum of four values, full symmetry
+ * (mathematical_functions.h:115-124): the mean of the smallest and largest of
+ * the three pairings, with std::min/std::max picking the first extreme. */
+static double cms4(double a, double b, double c, double d) {
+  double const s1 = (a + b) + (c + d);
+  double const s2 = (a + c) + (b + d);
+  double const s3 = (a + d) + (b + c);
+  double mn = s1, mx = s1;
+  mn = smin(mn, s2);
+  mn = smin(mn, s3);
+  mx = smax(mx, s2);
+  mx = smax(mx, s3);
+  return 0.5 * (mn + mx);
+}
+
+static inline size_t fidx(int dim, int i, int j, int k) {
+  return ((size_t)i * TC + (size_t)j) * (size_t)(dim == 3 ? TC : 1) + (size_t)k;
+}
+
+/* Multiresolution::Average (multiresolution.h:94-167): the conservative mean
+ * of the 2^dim child cells under each parent cell, written into the quarter
+ * (octant) of the parent's internal cells that the child covers: parent start
+ * HS or TC/2 per axis (xyz_look_up_table_, multiresolution.h:35-45), IC/2
+ * cells per axis (PSOCIC, compile_time_constants.h:578-590). */
+void ora_mr_average(int dim, const double *child, double *parent, int position) {
+  int const ne = dim + 2, tz = dim == 3 ? TC : 1;
+  int const x0 = (position & 1) ? TC / 2 : HS;
+  int const y0 = (position & 2) ? TC / 2 : HS;
+  int const z0 = dim == 3 ? ((position & 4) ? TC / 2 : HS) : 0;
+  int const nz = dim == 3 ? IC / 2 : 1;
+  size_t const field = (size_t)TC * TC * tz;
+  for (int e = 0; e < ne; ++e) {
+    const double *c = child + (size_t)e * field;
+    double *p = parent + (size_t)e * field;
+    for (int i = 0; i < IC / 2; ++i)
+      for (int j = 0; j < IC / 2; ++j)
+        for (int k = 0; k < nz; ++k) {
+          int const ci = HS + 2 * i, cj = HS + 2 * j, ck = dim == 3 ? HS + 2 * k : 0;
+          double v;
+          if (dim == 2) {
+            v = ((c[fidx(dim, ci, cj, 0)] + c[fidx(dim, ci + 1, cj + 1, 0)]) +
+                 (c[fidx(dim, ci + 1, cj, 0)] + c[fidx(dim, ci, cj + 1, 0)])) *
+                0.25;
+          } else {
+            v = 0.125 *
+                cms4(c[fidx(dim, ci, cj, ck)] + c[fidx(dim, ci + 1, cj + 1, ck + 1)],
+                     c[fidx(dim, ci, cj, ck + 1)] + c[fidx(dim, ci + 1, cj + 1, ck)],
+                     c[fidx(dim, ci, cj + 1, ck)] + c[fidx(dim, ci + 1, cj, ck + 1)],
+                     c[fidx(dim, ci, cj + 1, ck + 1)] + c[fidx(dim, ci + 1, cj, ck)]);
+          }
+          p[fidx(dim, x0 + i, y0 + j, z0 + k)] = v;
+        }
+  }
+}
+
+/* Multiresolution::Prediction (multiresolution.cpp:146-292): Harten's
+ * fifth-order interpolation of the parent field into the child cells
+ * [start, start + 2*(count/2)) per axis (range = x_start, x_count, y_start,
+ * y_count, z_start, z_count; starts even; the z pair is ignored in 2D).  The
+ * parent cell over child cell 2m+start is m + start/2 + 2 (low child) or
+ * m + start/2 + TC/2 - 2 (high child) (PIOLCH/PIOHCH,
+ * compile_time_constants.h:598-608).  In 2D the z terms are 0.0 and enter the
+ * sums as the reference's zero-initialised variables do. */
+void ora_mr_prediction(int dim, const double *parent, double *child, int position,
+                       const int *range) {
+  static const double c0 = -22.0 / 128.0, c1 = 3.0 / 128.0;
+  double const c00 = c0 * c0, c01 = c0 * c1, c10 = c01, c11 = c1 * c1;
+  double const c000 = c00 * c0, c001 = c00 * c1, c010 = c001, c011 = c0 * c11;
+  double const c100 = c001, c101 = c011, c110 = c011, c111 = c1 * c11;
+  int const ox = (position & 1) ? TC / 2 - HS / 2 : HS / 2;
+  int const oy = (position & 2) ? TC / 2 - HS / 2 : HS / 2;
+  int const oz = (position & 4) ? TC / 2 - HS / 2 : HS / 2;
+  int const px0 = range[0] / 2 + ox, px1 = px0 + range[1] / 2;
+  int const py0 = range[2] / 2 + oy, py1 = py0 + range[3] / 2;
+  int const pz0 = dim == 3 ? range[4] / 2 + oz : 0;
+  int const pz1 = dim == 3 ? pz0 + range[5] / 2 : 1;
+#define P(a, b, c) parent[fidx(dim, (a), (b), (c))]
+  int ci = range[0];
+  for (int i = px0; i < px1; ++i, ci += 2) {
+    int cj = range[2];
+    for (int j = py0; j < py1; ++j, cj += 2) {
+      int ck = dim == 3 ? range[4] : 0;
+      for (int k = pz0; k < pz1; ++k, ck += 2) {
+        double Qz = 0.0, Qxz = 0.0, Qyz = 0.0, Qxyz = 0.0;
+        double const Qx = c0 * (P(i + 1, j, k) - P(i - 1, j, k)) + c1 * (P(i + 2, j, k) - P(i - 2, j, k));
+        double const Qy = c0 * (P(i, j + 1, k) - P(i, j - 1, k)) + c1 * (P(i, j + 2, k) - P(i, j - 2, k));
+        double const Qxy =
+            (c00 * ((P(i + 1, j + 1, k) + P(i - 1, j - 1, k)) - (P(i - 1, j + 1, k) + P(i + 1, j - 1, k))) +
+             c11 * ((P(i + 2, j + 2, k) + P(i - 2, j - 2, k)) - (P(i - 2, j + 2, k) + P(i + 2, j - 2, k)))) +
+            (c01 * ((P(i + 1, j + 2, k) + P(i - 1, j - 2, k)) - (P(i - 1, j + 2, k) + P(i + 1, j - 2, k))) +
+             c10 * ((P(i + 2, j + 1, k) + P(i - 2, j - 1, k)) - (P(i - 2, j + 1, k) + P(i + 2, j - 1, k))));
+        if (dim == 3) {
+          Qz = c0 * (P(i, j, k + 1) - P(i, j, k - 1)) + c1 * (P(i, j, k + 2) - P(i, j, k - 2));
+          Qxz = (c00 * ((P(i + 1, j, k + 1) + P(i - 1, j, k - 1)) - (P(i - 1, j, k + 1) + P(i + 1, j, k - 1))) +
+                 c11 * ((P(i + 2, j, k + 2) + P(i - 2, j, k - 2)) - (P(i - 2, j, k + 2) + P(i + 2, j, k - 2)))) +
+                (c01 * ((P(i + 1, j, k + 2) + P(i - 1, j, k - 2)) - (P(i - 1, j, k + 2) + P(i + 1, j, k - 2))) +
+                 c10 * ((P(i + 2, j, k + 1) + P(i - 2, j, k - 1)) - (P(i - 2, j, k + 1) + P(i + 2, j, k - 1))));
+          Qyz = (c00 * ((P(i, j + 1, k + 1) + P(i, j - 1, k - 1)) - (P(i, j - 1, k + 1) + P(i, j + 1, k - 1))) +
+                 c11 * ((P(i, j + 2, k + 2) + P(i, j - 2, k - 2)) - (P(i, j - 2, k + 2) + P(i, j + 2, k - 2)))) +
+                (c01 * ((P(i, j + 1, k + 2) + P(i, j - 1, k - 2)) - (P(i, j - 1, k + 2) + P(i, j + 1, k - 2))) +
+                 c10 * ((P(i, j + 2, k + 1) + P(i, j - 2, k - 1)) - (P(i, j - 2, k + 1) + P(i, j + 2, k - 1))));
+          /* One corner group: the four antisymmetric differences of the
+           * stencil points (i+-a, j+-b, k+-c). */
+#define G(a, b, c)                                                                  \
+  cms4(P(i + a, j + b, k + c) - P(i - a, j - b, k - c),                             \
+       P(i - a, j - b, k + c) - P(i + a, j + b, k - c),                             \
+       P(i + a, j - b, k - c) - P(i - a, j + b, k + c),                             \
+       P(i - a, j + b, k - c) - P(i + a, j - b, k + c))
+          Qxyz = cms4(c000 * G(1, 1, 1) + c111 * G(2, 2, 2), c001 * G(1, 1, 2) + c110 * G(2, 2, 1),
+                      c011 * G(1, 2, 2) + c100 * G(2, 1, 1), c010 * G(1, 2, 1) + c101 * G(2, 1, 2));
+#undef G
+        }
+        double const p = P(i, j, k);
+        child[fidx(dim, ci, cj, ck)] = p + dams(dim, Qx, Qy, Qz) + dams(dim, Qxy, Qxz, Qyz) + Qxyz;
+        child[fidx(dim, ci + 1, cj, ck)] = p + dams(dim, -Qx, Qy, Qz) + dams(dim, -Qxy, -Qxz, Qyz) - Qxyz;
+        child[fidx(dim, ci, cj + 1, ck)] = p + dams(dim, Qx, -Qy, Qz) + dams(dim, -Qxy, Qxz, -Qyz) - Qxyz;
+        child[fidx(dim, ci + 1, cj + 1, ck)] = p + dams(dim, -Qx, -Qy, Qz) + dams(dim, Qxy, -Qxz, -Qyz) + Qxyz;
+        if (dim == 3) {
+          child[fidx(dim, ci, cj, ck + 1)] = p + cms3(Qx, Qy, -Qz) + cms3(Qxy, -Qxz, -Qyz) - Qxyz;
+          child[fidx(dim, ci + 1, cj, ck + 1)] = p + cms3(-Qx, Qy, -Qz) + cms3(-Qxy, Qxz, -Qyz) + Qxyz;
+          child[fidx(dim, ci, cj + 1, ck + 1)] = p + cms3(Qx, -Qy, -Qz) + cms3(-Qxy, -Qxz, Qyz) + Qxyz;
+          child[fidx(dim, ci + 1, cj + 1, ck + 1)] = p + cms3(-Qx, -Qy, -Qz) + cms3(Qxy, Qxz, Qyz) - Qxyz;
+        }
+      }
+    }
+  }
+#undef P
+}
+
+/* ChildNeedsRemeshing<Norm::Linfinity> (multiresolution.cpp:299-323) without
+ * the decision: the largest |exact - predicted| / |exact| over the wavelet
+ * equations -- Mass and Energy (MF::EWA(), field_setup.h:243-247) -- and
+ * every cell of the child block (halo included), the prediction taken over
+ * the full block from the parent.  std::max(max_detail, x) keeps max_detail
+ * when x is NaN, so NaN quotients (0/0) drop out. */
+double ora_mr_detail_linf(int dim, const double *parent, const double *child, int position) {
+  int const tz = dim == 3 ? TC : 1;
+  size_t const field = (size_t)TC * TC * tz;
+  int const full[6] = {0, TC, 0, TC, 0, tz};
+  double *pred = (double *)malloc(field * sizeof(double));
+  double max_detail = 0.0;
+  for (int e = MASS; e <= ENERGY; ++e) {
+    ora_mr_prediction(dim, parent + (size_t)e * field, pred, position, full);
+    const double *x = child + (size_t)e * field;
+    for (size_t c = 0; c < field; ++c)
+      max_detail = smax(max_detail, fabs(x[c] - pred[c]) / fabs(x[c]));
+  }
+  free(pred);
+  return max_detail;
+}
+
+/* ThresholdComputer<DIM, STDO = 3> (threshold_computer.h:53-59, 91-94): the
+ * coarsening threshold on a level. */
+double ora_mr_threshold(int dim, int maximum_level, int reference_level,
+                        double reference_epsilon, int level) {
+  double const alpha = 3.0, d = (double)dim;
+  double const eps_ref =
+      reference_epsilon * pow(2.0, -1.0 * (alpha + 1.0) * ((double)maximum_level - (double)reference_level));
+  return eps_ref * pow(2.0, -1.0 * d * ((double)maximum_level - (double)level));
+}
+
+/* Multiresolution::RemeshingDecision (multiresolution.cpp:408-428):
+ * RemeshIdentifier Neutral 0, Refine 1, Coarse 2 (remesh_identifier.h:29). */
+int ora_mr_decision(double detail, double eps_coarsen) {
+  double const eps_refine = eps_coarsen * 32.0;
+  if (detail <= eps_coarsen)
+    return 2;
+  if (detail >= eps_refine)
+    return 1;
+  return 0;
+}

@@ -70,6 +70,19 @@ int ora_run(const ora_grid *g, double cfl, double t_end, int max_steps,
 void ora_domain_sums(int dim, const int *ncell, const double *state,
                      double *out);
 
+/* Multiresolution operators between a parent block and the child at sibling
+ * position `position` (bit 0 x, bit 1 y, bit 2 z).  Blocks are haloed blocks
+ * as above; ora_mr_prediction works on one field [TC][TC][TCZ]. */
+void ora_mr_average(int dim, const double *child, double *parent, int position);
+/* range = x_start, x_count, y_start, y_count, z_start, z_count (child cells). */
+void ora_mr_prediction(int dim, const double *parent, double *child, int position,
+                       const int *range);
+double ora_mr_detail_linf(int dim, const double *parent, const double *child, int position);
+double ora_mr_threshold(int dim, int maximum_level, int reference_level,
+                        double reference_epsilon, int level);
+/* 0 Neutral, 1 Refine, 2 Coarse. */
+int ora_mr_decision(double detail, double eps_coarsen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -174,3 +174,91 @@ def run(dim, nb, bc, flux, cfl, t_end, max_steps, state, gamma=1.4, b=0.0,
                              ctypes.c_double(t_end), max_steps, threads, _dp(st), _dp(dts),
                              ctypes.byref(tf))
     return st, dts[:n], tf.value
+
+
+# Multiresolution operators (child at sibling position `position` of a parent).
+MR_NEUTRAL, MR_REFINE, MR_COARSE = 0, 1, 2
+
+
+def _block(dim, a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    assert a.shape == (dim + 2, 24, 24, 24 if dim == 3 else 1), a.shape
+    return a
+
+
+def mr_average(dim, child, parent, position, which="oracle"):
+    """Parent block with the child's octant (quadrant) replaced by the
+    conservative average of the child's internal cells."""
+    child = _block(dim, child)
+    out = _block(dim, parent).copy()
+    if which == "oracle":
+        oracle().ora_mr_average(dim, _dp(child), _dp(out), position)
+    else:
+        ref(dim).ref_mr_average(_dp(child), _dp(out), position)
+    return out
+
+
+def mr_prediction(dim, parent_field, child_field, position, rng=None, which="oracle"):
+    """Child field [24][24][24|1] with the cells of rng = (x_start, x_count,
+    y_start, y_count, z_start, z_count) predicted from the parent field."""
+    tz = 24 if dim == 3 else 1
+    if rng is None:
+        rng = (0, 24, 0, 24, 0, tz)
+    pf = np.ascontiguousarray(parent_field, dtype=np.float64)
+    out = np.ascontiguousarray(child_field, dtype=np.float64).copy()
+    assert pf.shape == (24, 24, tz) and out.shape == (24, 24, tz)
+    r = (ctypes.c_int * 6)(*rng)
+    if which == "oracle":
+        oracle().ora_mr_prediction(dim, _dp(pf), _dp(out), position, r)
+    else:
+        ref(dim).ref_mr_prediction(_dp(pf), _dp(out), position, r)
+    return out
+
+
+def mr_detail_linf(dim, parent, child, position):
+    f = oracle().ora_mr_detail_linf
+    f.restype = ctypes.c_double
+    return f(dim, _dp(_block(dim, parent)), _dp(_block(dim, child)), position)
+
+
+def mr_threshold(dim, maximum_level, reference_level, reference_epsilon, level, which="oracle"):
+    if which == "oracle":
+        f = oracle().ora_mr_threshold
+        f.restype = ctypes.c_double
+        return f(dim, maximum_level, reference_level, ctypes.c_double(reference_epsilon), level)
+    f = ref(dim).ref_mr_threshold
+    f.restype = ctypes.c_double
+    return f(maximum_level, reference_level, ctypes.c_double(reference_epsilon), level)
+
+
+def mr_decision(detail, eps_coarsen):
+    return oracle().ora_mr_decision(ctypes.c_double(detail), ctypes.c_double(eps_coarsen))
+
+
+def ref_mr_child_needs_remeshing(dim, parent, child, position, epsilon):
+    """The reference's ChildNeedsRemeshing (L-infinity) with the coarsening
+    threshold on the child's level equal to `epsilon`."""
+    return ref(dim).ref_mr_child_needs_remeshing(_dp(_block(dim, parent)), _dp(_block(dim, child)),
+                                                 position, ctypes.c_double(epsilon))
+
+
+def ref_mr_detail(dim, parent, child, position):
+    """The reference's L-infinity detail, read off its remeshing decision: the
+    decision is Coarse exactly when detail <= epsilon, so the smallest
+    non-negative double epsilon with a Coarse decision is the detail itself
+    (bisection over the ordered bit patterns of non-negative doubles)."""
+    lo, hi = 0, 0x7FF0000000000000  # 0.0 .. +inf as int64 bit patterns
+
+    def coarse(bits):
+        eps = np.array([bits], dtype=np.int64).view(np.float64)[0]
+        return ref_mr_child_needs_remeshing(dim, parent, child, position, eps) == MR_COARSE
+
+    if coarse(lo):
+        return 0.0
+    while hi - lo > 1:          # invariant: not coarse at lo, coarse at hi
+        mid = (lo + hi) // 2
+        if coarse(mid):
+            hi = mid
+        else:
+            lo = mid
+    return float(np.array([hi], dtype=np.int64).view(np.float64)[0])

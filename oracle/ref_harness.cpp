@@ -11,8 +11,9 @@
 // (FiniteVolumeScheme + HllcRiemannSolver/Einfeldt, FluxSplittingScheme/Roe,
 // EigenDecomposition, CharacteristicStateReconstruction, WENO5), the
 // stiffened-gas EOS, ConservativesToPrimeStates, the RungeKutta3TVD time
-// integrator on reference Nodes/Blocks, Zero-gradient/Symmetry boundary conditions and
-// ConsistencyManagedSum.  What this file restates around them (all cited):
+// integrator on reference Nodes/Blocks, Zero-gradient/Symmetry boundary conditions,
+// ConsistencyManagedSum, and the multiresolution operators (Multiresolution::
+// Average/Prediction/ChildNeedsRemeshing, ThresholdComputer).  What this file restates around them (all cited):
 // the SpaceSolver divergence loop (space_solver.cpp:50-153, source terms off),
 // the uniform-level Advance choreography (modular_algorithm_assembler.cpp:
 // 543-910), ComputeTimestepSize (:1777-1997, inviscid terms only) and the
@@ -24,6 +25,7 @@
 #include "integrator/runge_kutta_3_TVD.h"
 #include "materials/equations_of_state/stiffened_gas.h"
 #include "materials/material_manager.h"
+#include "multiresolution/multiresolution.h"
 #include "prime_states/prime_state_handler.h"
 #include "solvers/convective_term_contributions/finite_volume_scheme.h"
 #include "solvers/convective_term_contributions/flux_splitting_scheme.h"
@@ -539,6 +541,82 @@ void ref_level0_ids(int const *nb, unsigned long long *out) {
     id = TopNeighborOfNodeWithId(list[(nb[1] * nb[0]) * i]);
   }
   for (size_t n = 0; n < list.size(); ++n) out[n] = list[n];
+}
+
+} // extern "C"
+
+// Multiresolution operators on the RHS buffers of two reference Blocks, the
+// child at sibling position `position` of a level-0 parent (child id =
+// parent id << 3 | position, id_information.h:158-160).  Blocks are
+// [NE][TX][TY][TZ] arrays.
+namespace {
+void LoadRhs(Block &block, double const *in) {
+  Conservatives &a = block.GetRightHandSideBuffer();
+  for (unsigned int e = 0; e < NE; ++e)
+    std::memcpy(&a[e][0][0][0], in + std::size_t(e) * TX * TY * TZ,
+                sizeof(double) * TX * TY * TZ);
+}
+void StoreRhs(Block const &block, double *out) {
+  Conservatives const &a = block.GetRightHandSideBuffer();
+  for (unsigned int e = 0; e < NE; ++e)
+    std::memcpy(out + std::size_t(e) * TX * TY * TZ, &a[e][0][0][0],
+                sizeof(double) * TX * TY * TZ);
+}
+nid_t ChildId(int position) { return (IdSeed() << 3) | nid_t(position); }
+} // namespace
+
+extern "C" {
+
+// Multiresolution::Average over the whole Conservatives buffer; parent is
+// read and written (only the child's octant changes).
+void ref_mr_average(double const *child, double *parent, int position) {
+  auto c = std::make_unique<Block>();
+  auto p = std::make_unique<Block>();
+  LoadRhs(*c, child);
+  LoadRhs(*p, parent);
+  Multiresolution::Average(c->GetRightHandSideBuffer(),
+                           p->GetRightHandSideBuffer(), ChildId(position));
+  StoreRhs(*p, parent);
+}
+
+// Multiresolution::Prediction of one field [TX][TY][TZ]; child is read and
+// written (cells outside the range keep their values).
+void ref_mr_prediction(double const *parent, double *child, int position,
+                       int const *range) {
+  auto c = std::make_unique<Block>();
+  auto p = std::make_unique<Block>();
+  std::memcpy(&p->GetRightHandSideBuffer(Equation::Mass)[0][0][0], parent,
+              sizeof(double) * TX * TY * TZ);
+  std::memcpy(&c->GetRightHandSideBuffer(Equation::Mass)[0][0][0], child,
+              sizeof(double) * TX * TY * TZ);
+  Multiresolution::Prediction(p->GetRightHandSideBuffer(Equation::Mass),
+                              c->GetRightHandSideBuffer(Equation::Mass),
+                              ChildId(position), range[0], range[1], range[2],
+                              range[3], range[4], range[5]);
+  std::memcpy(child, &c->GetRightHandSideBuffer(Equation::Mass)[0][0][0],
+              sizeof(double) * TX * TY * TZ);
+}
+
+// ChildNeedsRemeshing<CC::NFWA()> of a level-1 child under a Thresholder
+// built from (maximum level 1, reference level 1, epsilon): the coarsening
+// threshold on the child's level is then exactly `epsilon`.  Returns the
+// RemeshIdentifier (0 Neutral, 1 Refine, 2 Coarse).
+int ref_mr_child_needs_remeshing(double const *parent, double const *child,
+                                 int position, double epsilon) {
+  auto c = std::make_unique<Block>();
+  auto p = std::make_unique<Block>();
+  LoadRhs(*c, child);
+  LoadRhs(*p, parent);
+  Multiresolution mr(Thresholder(1, 1, epsilon));
+  return int(mr.ChildNeedsRemeshing<CC::NFWA()>(*p, *c, ChildId(position)));
+}
+
+// Reference ThresholdComputer::ThresholdOnLevel.
+double ref_mr_threshold(int maximum_level, int reference_level,
+                        double reference_epsilon, int level) {
+  return Thresholder(unsigned(maximum_level), unsigned(reference_level),
+                     reference_epsilon)
+      .ThresholdOnLevel(unsigned(level));
 }
 
 } // extern "C"

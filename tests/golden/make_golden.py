@@ -29,7 +29,9 @@ half an hour per scheme on 8 cores) and writes
 With ``--ids`` it writes level0_ids.json only (level-0 node ids, used to pin
 the restart files' node-id encoding); with ``--edge`` edge_blocks.json only
 (hashes of the reference's per-block fluxes and right-hand sides on the
-special-branch states of tests/edge_states.py).
+special-branch states of tests/edge_states.py); with ``--mr`` mr_golden.json
+only (the reference's multiresolution averaging, prediction, details and
+thresholds on the block pairs of tests/mr_states.py).
 """
 from __future__ import annotations
 
@@ -282,6 +284,46 @@ def main_ids():
         json.dump(meta, fh, indent=1)
 
 
+def main_mr():
+    """mr_golden.json: per dimension, pair kind (tests/mr_states.py) and
+    sibling position, the reference's Multiresolution::Average output (the
+    parent block after averaging), Prediction output per range of
+    mr_states.ranges (every field, child cells outside the range untouched)
+    as SHA-256 with NaNs canonical, and the L-infinity detail of
+    ChildNeedsRemeshing (read off its decisions, oracle_py.ref_mr_detail) as
+    a hex float; plus ThresholdOnLevel for a few settings."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    import edge_states
+    import mr_states
+
+    meta = {"pairs": {}, "thresholds": []}
+    with np.errstate(all="ignore"):
+        for dim in (2, 3):
+            for kind in mr_states.KINDS:
+                for pos in range(2 ** dim):
+                    parent, child = mr_states.pair(dim, kind, pos)
+                    avg = oracle_py.mr_average(dim, child, parent, pos, which="ref")
+                    preds = []
+                    for rng in mr_states.ranges(dim):
+                        pred = np.stack([oracle_py.mr_prediction(dim, parent[e], child[e], pos, rng,
+                                                                 which="ref")
+                                         for e in range(dim + 2)])
+                        preds.append(hashlib.sha256(edge_states.canonical(pred)).hexdigest())
+                    detail = oracle_py.ref_mr_detail(dim, parent, child, pos)
+                    meta["pairs"][f"{dim}d_{kind}_{pos}"] = {
+                        "average_sha256": hashlib.sha256(edge_states.canonical(avg)).hexdigest(),
+                        "prediction_sha256": preds,
+                        "detail": float(detail).hex()}
+        for dim in (2, 3):
+            for ml, rl, eps in ((1, 1, 1e-3), (4, 2, 1e-3), (6, 6, 0.01), (3, 0, 0.5)):
+                for level in range(ml + 1):
+                    meta["thresholds"].append(
+                        [dim, ml, rl, eps, level,
+                         float(oracle_py.mr_threshold(dim, ml, rl, eps, level, which="ref")).hex()])
+    with open(os.path.join(HERE, "mr_golden.json"), "w") as fh:
+        json.dump(meta, fh, indent=1)
+
+
 if __name__ == "__main__":
     import argparse
 
@@ -291,8 +333,11 @@ if __name__ == "__main__":
     ap.add_argument("--flux", choices=["roe", "hllc", "both"], default="both")
     ap.add_argument("--ids", action="store_true", help="write level0_ids.json only")
     ap.add_argument("--edge", action="store_true", help="write edge_blocks.json only")
+    ap.add_argument("--mr", action="store_true", help="write mr_golden.json only")
     a = ap.parse_args()
-    if a.ids:
+    if a.mr:
+        main_mr()
+    elif a.ids:
         main_ids()
     elif a.edge:
         main_edge()

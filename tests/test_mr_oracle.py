@@ -1,0 +1,189 @@
+"""The oracle's multiresolution operators (oracle/alpaca_oracle.c: averaging,
+prediction, L-infinity detail, threshold, remeshing decision) pinned three
+ways:
+
+* against the committed fixture tests/golden/mr_golden.json, written by
+  tests/golden/make_golden.py --mr from the reference's own Multiresolution
+  and ThresholdComputer code (multiresolution.cpp, multiresolution.h,
+  threshold_computer.h compiled from the reference tree): bit for bit, NaN
+  positions included;
+* against that harness directly on further seeds, where it is built;
+* against what the mathematics fixes: the fifth-order prediction reproduces
+  the child cell averages of tensor products of quartics (a wrong
+  coefficient, sign or corner term breaks this), the 2^dim predicted
+  children of a parent cell average back to the parent cell, a child that
+  is its own prediction has detail 0, one perturbed cell gives the detail
+  of that perturbation, and the decision's boundaries are inclusive.
+
+The host module alpaca_amd.multiresolution (thresholds and decisions) is
+checked against the same fixture.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+from numpy.polynomial import polynomial as Pn
+
+from alpaca_amd import multiresolution as mr
+from oracle import oracle_py as O
+
+import edge_states
+import mr_states
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+IC, HS, TC = 16, 4, 24
+
+with open(os.path.join(HERE, "golden", "mr_golden.json")) as fh:
+    GOLDEN = json.load(fh)
+
+
+def _sha(a):
+    return hashlib.sha256(edge_states.canonical(a)).hexdigest()
+
+
+def _pairs():
+    for dim in (2, 3):
+        for kind in mr_states.KINDS:
+            for pos in range(2 ** dim):
+                yield dim, kind, pos
+
+
+@pytest.mark.parametrize("dim,kind,pos", list(_pairs()))
+def test_oracle_matches_reference_fixture(dim, kind, pos):
+    g = GOLDEN["pairs"][f"{dim}d_{kind}_{pos}"]
+    parent, child = mr_states.pair(dim, kind, pos)
+    with np.errstate(all="ignore"):
+        assert _sha(O.mr_average(dim, child, parent, pos)) == g["average_sha256"]
+        for rng, want in zip(mr_states.ranges(dim), g["prediction_sha256"]):
+            pred = np.stack([O.mr_prediction(dim, parent[e], child[e], pos, rng)
+                             for e in range(dim + 2)])
+            assert _sha(pred) == want, rng
+        assert O.mr_detail_linf(dim, parent, child, pos) == float.fromhex(g["detail"])
+
+
+def test_thresholds_match_reference_fixture():
+    for dim, ml, rl, eps, level, want in GOLDEN["thresholds"]:
+        want = float.fromhex(want)
+        assert O.mr_threshold(dim, ml, rl, eps, level) == want
+        assert mr.ThresholdComputer(dim, ml, rl, eps).threshold_on_level(level) == want
+
+
+@pytest.mark.skipif(not (O.ref_available(2) and O.ref_available(3)),
+                    reason="reference harness not built (oracle/_ref)")
+@pytest.mark.parametrize("dim", [2, 3])
+def test_oracle_matches_reference_harness_more_seeds(dim):
+    with np.errstate(all="ignore"):
+        for seed in (1, 2):
+            for kind in ("smooth", "random"):
+                for pos in range(2 ** dim):
+                    parent, child = mr_states.pair(dim, kind, pos, seed=seed)
+                    a = O.mr_average(dim, child, parent, pos)
+                    b = O.mr_average(dim, child, parent, pos, which="ref")
+                    assert np.array_equal(a, b)
+                    e = seed % (dim + 2)
+                    for rng in mr_states.ranges(dim):
+                        f = O.mr_prediction(dim, parent[e], child[e], pos, rng)
+                        g = O.mr_prediction(dim, parent[e], child[e], pos, rng, which="ref")
+                        assert np.array_equal(f, g)
+                    assert O.mr_detail_linf(dim, parent, child, pos) == \
+                        O.ref_mr_detail(dim, parent, child, pos)
+
+
+def _cell_averages(coef, h, origin):
+    """Cell averages of the polynomial `coef` over the TC cells of width h of
+    a block whose first internal cell starts at `origin`."""
+    Q = Pn.polyint(coef)
+    a = origin + (np.arange(TC) - HS) * h
+    return (Pn.polyval(a + h, Q) - Pn.polyval(a, Q)) / h
+
+
+def _tensor(factors, dim):
+    if dim == 3:
+        return np.einsum("i,j,k->ijk", *factors)
+    return np.einsum("i,j->ij", *factors)[..., None]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_prediction_exact_for_quartic_tensor_products(dim):
+    rng = np.random.default_rng(7)
+    h = 1.0 / IC
+    for pos in range(2 ** dim):
+        coefs = [rng.uniform(-1.0, 1.0, 5) for _ in range(dim)]
+        parent = _tensor([_cell_averages(coefs[a], h, 0.0) for a in range(dim)], dim)
+        child = _tensor([_cell_averages(coefs[a], h / 2, 0.5 * ((pos >> a) & 1))
+                         for a in range(dim)], dim)
+        pred = O.mr_prediction(dim, parent, np.zeros_like(parent), pos)
+        assert np.max(np.abs(pred - child)) <= 1e-13 * np.max(np.abs(child)), pos
+    # a quintic is not reproduced: the check has teeth
+    coefs = [rng.uniform(-1.0, 1.0, 6) for _ in range(dim)]
+    parent = _tensor([_cell_averages(coefs[a], h, 0.0) for a in range(dim)], dim)
+    child = _tensor([_cell_averages(coefs[a], h / 2, 0.0) for a in range(dim)], dim)
+    pred = O.mr_prediction(dim, parent, np.zeros_like(parent), 0)
+    assert np.max(np.abs(pred - child)) > 1e-9 * np.max(np.abs(child))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_predicted_children_average_back_to_parent(dim):
+    tz = TC if dim == 3 else 1
+    for pos in range(2 ** dim):
+        parent, child = mr_states.pair(dim, "random", pos)
+        pred = np.stack([O.mr_prediction(dim, parent[e], child[e], pos) for e in range(dim + 2)])
+        back = O.mr_average(dim, pred, np.zeros_like(parent), pos)
+        x0 = [TC // 2 if (pos >> a) & 1 else HS for a in range(3)]
+        sl = (slice(None), slice(x0[0], x0[0] + IC // 2), slice(x0[1], x0[1] + IC // 2),
+              slice(x0[2], x0[2] + IC // 2) if dim == 3 else slice(0, tz))
+        # parent cells under the child's internal cells: window start 2 or 10
+        assert np.allclose(back[sl], parent[(slice(None),) + tuple(
+            slice(2 + 8 * ((pos >> a) & 1) + 2, 2 + 8 * ((pos >> a) & 1) + 2 + IC // 2)
+            if a < dim else slice(0, 1) for a in range(3))], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_average_is_the_mean_of_the_children(dim):
+    rng = np.random.default_rng(3)
+    tz = TC if dim == 3 else 1
+    for pos in range(2 ** dim):
+        child = rng.uniform(0.5, 2.0, (dim + 2, TC, TC, tz))
+        parent = rng.uniform(0.5, 2.0, (dim + 2, TC, TC, tz))
+        out = O.mr_average(dim, child, parent, pos)
+        zs = slice(HS, HS + IC) if dim == 3 else slice(0, 1)
+        inner = child[:, HS:HS + IC, HS:HS + IC, zs]
+        shape = (dim + 2, IC // 2, 2, IC // 2, 2) + ((IC // 2, 2) if dim == 3 else (1, 1))
+        mean = inner.reshape(shape).mean(axis=(2, 4, 6))
+        x0 = [TC // 2 if (pos >> a) & 1 else HS for a in range(3)]
+        sl = (slice(None), slice(x0[0], x0[0] + IC // 2), slice(x0[1], x0[1] + IC // 2),
+              slice(x0[2], x0[2] + IC // 2) if dim == 3 else slice(0, 1))
+        assert np.allclose(out[sl], mean, rtol=1e-15, atol=0)
+        untouched = np.ones(out.shape, dtype=bool)
+        untouched[sl] = False
+        assert np.array_equal(out[untouched], parent[untouched])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_detail_of_exact_prediction_and_of_one_perturbed_cell(dim):
+    parent, _ = mr_states.pair(dim, "smooth", 1)
+    pos = 1
+    pred = np.stack([O.mr_prediction(dim, parent[e], np.zeros(parent.shape[1:]), pos)
+                     for e in range(dim + 2)])
+    assert O.mr_detail_linf(dim, parent, pred, pos) == 0.0
+    pert = pred.copy()
+    pert[1, 5, 7, 3 if dim == 3 else 0] *= 1.0 + 1e-3
+    want = 1e-3 / (1.0 + 1e-3)
+    assert abs(O.mr_detail_linf(dim, parent, pert, pos) - want) < 1e-12
+    # momentum equations are not wavelet equations
+    pert = pred.copy()
+    pert[2, 5, 7, 3 if dim == 3 else 0] *= 2.0
+    assert O.mr_detail_linf(dim, parent, pert, pos) == 0.0
+
+
+def test_decision_boundaries_are_inclusive():
+    eps = 1e-3
+    for f in (O.mr_decision, mr.remeshing_decision):
+        assert f(eps, eps) == mr.COARSE
+        assert f(np.nextafter(eps, 1.0), eps) == mr.NEUTRAL
+        assert f(32.0 * eps, eps) == mr.REFINE
+        assert f(np.nextafter(32.0 * eps, 0.0), eps) == mr.NEUTRAL
+        assert f(0.0, 0.0) == mr.COARSE
+        assert f(float("inf"), eps) == mr.REFINE
