@@ -12,9 +12,9 @@
 // out so that non-finite inputs propagate as they do there.
 //
 // Work decomposition.  A link names (child block, parent block, sibling
-// position, child level).  Averaging is one thread per parent cell written
-// (k fastest, so the 2x2(x2) child reads of neighbouring threads are
-// contiguous).  Prediction and details stage the parent's stencil window --
+// position, child level).  Averaging is one thread per two parent cells
+// written (adjacent along the fastest axis, so the child reads of neighbouring
+// threads are contiguous 16-byte accesses).  Prediction and details stage the parent's stencil window --
 // the predicted parent cells plus two on each side, at most 16 per axis
 // (32 KiB of doubles in 3D) -- in LDS once per field and let each thread
 // predict the 2^DIM children of one parent cell from it: the 3D stencil reads
@@ -29,31 +29,39 @@ namespace alpaca {
 namespace mr {
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;         // averaging
+constexpr int kStencilThreads = 192;  // prediction and details: 1728 = 9 x 192 parent cells
 constexpr int kMaxLevels = 32;
 
-// std::min / std::max of the reference: the first extreme, NaN never picked
-// by a comparison.
-__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+// LDS layout of a parent stencil window: [16][16][16] doubles in 3D, [16][16]
+// in 2D, whatever part of it a range fills, so that every stencil offset is a
+// compile-time constant.
+template <int DIM> struct Lds {
+  static constexpr int SX = DIM == 3 ? 256 : 16;
+  static constexpr int SY = DIM == 3 ? 16 : 1;
+  static constexpr int SIZE = 16 * SX;
+};
 
 // ConsistencyManagedSum, three and four values (mathematical_functions.h:79-91,
-// 115-124), with std::min/std::max over the initializer list.
+// 115-124): 0.5 (min + max) of the three pairings s1, s2, s3, where std::min
+// and std::max over the initializer list keep the first extreme and never
+// pick a NaN by comparison.  The hardware minimum/maximum (one instruction
+// each) pick the same value whenever s1 is not NaN:
+//  * a NaN among s2, s3 is skipped by both (std: the comparison is false);
+//  * ties: a rounded sum is -0 only when both addends are -0, so the s_i are
+//    zeros of one sign or of none -- equal values are the same bits.
+// When s1 is NaN, std keeps it for both extremes and the result is NaN.
+__device__ __forceinline__ double cms_pick(double s1, double s2, double s3) {
+  double const mn = __builtin_fmin(__builtin_fmin(s1, s2), s3);
+  double const mx = __builtin_fmax(__builtin_fmax(s1, s2), s3);
+  double const r = 0.5 * (mn + mx);
+  return s1 != s1 ? s1 : r;
+}
 __device__ __forceinline__ double cms3(double a, double b, double c) {
-  double const s1 = (a + b) + c;
-  double const s2 = (a + c) + b;
-  double const s3 = (b + c) + a;
-  double const mn = smin(smin(s1, s2), s3);
-  double const mx = smax(smax(s1, s2), s3);
-  return 0.5 * (mn + mx);
+  return cms_pick((a + b) + c, (a + c) + b, (b + c) + a);
 }
 __device__ __forceinline__ double cms4(double a, double b, double c, double d) {
-  double const s1 = (a + b) + (c + d);
-  double const s2 = (a + c) + (b + d);
-  double const s3 = (a + d) + (b + c);
-  double const mn = smin(smin(s1, s2), s3);
-  double const mx = smax(smax(s1, s2), s3);
-  return 0.5 * (mn + mx);
+  return cms_pick((a + b) + (c + d), (a + c) + (b + d), (a + d) + (b + c));
 }
 // DimensionAwareConsistencyManagedSum (mathematical_functions.h:98-108).
 template <int DIM> __device__ __forceinline__ double dams(double a, double b, double c) {
@@ -74,37 +82,61 @@ template <int DIM> __device__ __forceinline__ bool link_ok(const alpaca_mr_link 
 
 // Parent internal cells covered by one child: IC/2 per axis, starting at HS
 // (low child) or TC/2 (high child) (xyz_look_up_table_, multiresolution.h:
-// 35-45); child cells from HS in steps of two.
+// 35-45); child cells from HS in steps of two.  Each thread writes two parent
+// cells adjacent along the fastest axis (k in 3D, j in 2D) and reads the four
+// contiguous child cells under them per child row as two 16-byte loads; every
+// such pair starts at an even element offset of a block (16-byte aligned).
+template <int DIM> struct AvgShape {
+  static constexpr int H = kIC / 2;                  // parent cells per axis
+  static constexpr int PAIRS = DIM == 3 ? H * H * (H / 2) : H * (H / 2);  // threads per field
+};
+
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void average_kernel(double *__restrict__ blocks, int nblocks,
                                                            const alpaca_mr_link *__restrict__ links,
                                                            int nlinks) {
   using D = Dims<DIM>;
-  constexpr int H = kIC / 2;
-  constexpr int HZ = DIM == 3 ? H : 1;
-  constexpr int CELLS = H * H * HZ;
+  constexpr int H = AvgShape<DIM>::H;
+  constexpr int PAIRS = AvgShape<DIM>::PAIRS;
+  constexpr int SX = D::SX, SY = D::SY;
   size_t const t = (size_t)blockIdx.x * kThreads + threadIdx.x;
-  if (t >= (size_t)nlinks * D::NE * CELLS) return;
-  int const r = int(t % CELLS);
-  int const e = int((t / CELLS) % D::NE);
-  alpaca_mr_link const l = links[t / ((size_t)CELLS * D::NE)];
+  if (t >= (size_t)nlinks * D::NE * PAIRS) return;
+  int const r = int(t % PAIRS);
+  int const e = int((t / PAIRS) % D::NE);
+  alpaca_mr_link const l = links[t / ((size_t)PAIRS * D::NE)];
   if (!link_ok<DIM>(l, nblocks)) return;
-  int const i = r / (H * HZ), j = (r / HZ) % H, k = r % HZ;
   const double *c = blocks + (size_t)l.child * D::BLOCK + (size_t)e * D::FIELD;
   double *p = blocks + (size_t)l.parent * D::BLOCK + (size_t)e * D::FIELD;
   int const x0 = (l.position & 1) ? kTC / 2 : kHS;
   int const y0 = (l.position & 2) ? kTC / 2 : kHS;
-  int const ci = kHS + 2 * i, cj = kHS + 2 * j;
-  constexpr int SX = D::SX, SY = D::SY;
   if constexpr (DIM == 2) {
-    const double *q = c + ci * SX + cj * SY;
-    p[(x0 + i) * SX + (y0 + j) * SY] = ((q[0] + q[SX + SY]) + (q[SX] + q[SY])) * 0.25;
+    int const i = r / (H / 2), j = 2 * (r % (H / 2));
+    const double *q = c + (kHS + 2 * i) * SX + (kHS + 2 * j) * SY;
+    double2 const a0 = *reinterpret_cast<const double2 *>(q);           // (ci,   cj..cj+1)
+    double2 const a1 = *reinterpret_cast<const double2 *>(q + 2);       // (ci,   cj+2..cj+3)
+    double2 const b0 = *reinterpret_cast<const double2 *>(q + SX);      // (ci+1, cj..cj+1)
+    double2 const b1 = *reinterpret_cast<const double2 *>(q + SX + 2);  // (ci+1, cj+2..cj+3)
+    double2 out;
+    out.x = ((a0.x + b0.y) + (b0.x + a0.y)) * 0.25;
+    out.y = ((a1.x + b1.y) + (b1.x + a1.y)) * 0.25;
+    *reinterpret_cast<double2 *>(p + (x0 + i) * SX + (y0 + j) * SY) = out;
   } else {
+    int const i = r / (H * H / 2), j = (r / (H / 2)) % H, k = 2 * (r % (H / 2));
     int const z0 = (l.position & 4) ? kTC / 2 : kHS;
-    int const ck = kHS + 2 * k;
-    const double *q = c + ci * SX + cj * SY + ck;
-    p[(x0 + i) * SX + (y0 + j) * SY + z0 + k] =
-        0.125 * cms4(q[0] + q[SX + SY + 1], q[1] + q[SX + SY], q[SY] + q[SX + 1], q[SY + 1] + q[SX]);
+    const double *q = c + (kHS + 2 * i) * SX + (kHS + 2 * j) * SY + kHS + 2 * k;
+    // Rows (dx, dy) of four child cells along k.
+    double2 const r00a = *reinterpret_cast<const double2 *>(q);
+    double2 const r00b = *reinterpret_cast<const double2 *>(q + 2);
+    double2 const r01a = *reinterpret_cast<const double2 *>(q + SY);
+    double2 const r01b = *reinterpret_cast<const double2 *>(q + SY + 2);
+    double2 const r10a = *reinterpret_cast<const double2 *>(q + SX);
+    double2 const r10b = *reinterpret_cast<const double2 *>(q + SX + 2);
+    double2 const r11a = *reinterpret_cast<const double2 *>(q + SX + SY);
+    double2 const r11b = *reinterpret_cast<const double2 *>(q + SX + SY + 2);
+    double2 out;
+    out.x = 0.125 * cms4(r00a.x + r11a.y, r00a.y + r11a.x, r01a.x + r10a.y, r01a.y + r10a.x);
+    out.y = 0.125 * cms4(r00b.x + r11b.y, r00b.y + r11b.x, r01b.x + r10b.y, r01b.y + r10b.x);
+    *reinterpret_cast<double2 *>(p + (x0 + i) * SX + (y0 + j) * SY + z0 + k) = out;
   }
 }
 
@@ -120,12 +152,13 @@ struct Range {
 };
 
 // The 2^DIM predicted child values of the parent cell at LDS index `o` of a
-// window with strides (sx, sy, 1): out[dx | dy << 1 | dz << 2].  In 2D the
+// window (Lds layout): out[dx | dy << 1 | dz << 2].  In 2D the
 // z terms are the reference's zero-initialised Qz, Qxz, Qyz, Qxyz, and enter
 // the sums as such (their signs matter for signed zeros).
 template <int DIM>
-__device__ __forceinline__ void predict_cell(const double *__restrict__ s, int o, int sx, int sy,
+__device__ __forceinline__ void predict_cell(const double *__restrict__ s, int o,
                                              double out[1 << DIM]) {
+  constexpr int sx = Lds<DIM>::SX, sy = Lds<DIM>::SY;
   constexpr double c0 = -22.0 / 128.0, c1 = 3.0 / 128.0;
   constexpr double c00 = c0 * c0, c01 = c0 * c1, c10 = c01, c11 = c1 * c1;
   constexpr double c000 = c00 * c0, c001 = c00 * c1, c010 = c001, c011 = c0 * c11;
@@ -172,11 +205,10 @@ __device__ __forceinline__ void predict_cell(const double *__restrict__ s, int o
 }
 
 // Window geometry of one link: first parent cell of the window per axis and
-// the window's strides.
+// its extent (at most 16).
 template <int DIM> struct Window {
-  int p0[3];   // first window cell (predicted cell - 2) per axis
-  int w[3];    // window extent per axis
-  int sx, sy;  // LDS strides of i and j
+  int p0[3];  // first window cell (predicted cell - 2) per axis
+  int w[3];   // window extent per axis
   __device__ Window(const Range &r, int position) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -185,14 +217,11 @@ template <int DIM> struct Window {
       p0[a] = live ? r.start[a] / 2 + off - 2 : 0;
       w[a] = live ? r.count[a] + 4 : 1;
     }
-    sy = w[2];
-    sx = w[1] * w[2];
   }
-  __device__ int size() const { return w[0] * w[1] * w[2]; }
   // LDS index of predicted parent cell (a, b, c) (counted from the range's
   // first predicted cell).
-  __device__ int centre(int a, int b, int c) const {
-    return (a + 2) * sx + (b + 2) * sy + (DIM == 3 ? c + 2 : 0);
+  __device__ static int centre(int a, int b, int c) {
+    return (a + 2) * Lds<DIM>::SX + (b + 2) * Lds<DIM>::SY + (DIM == 3 ? c + 2 : 0);
   }
 };
 
@@ -200,21 +229,59 @@ template <int DIM>
 __device__ __forceinline__ void stage_window(double *s, const double *__restrict__ field,
                                              const Window<DIM> &win) {
   using D = Dims<DIM>;
-  int const n = win.size();
-  for (int t = threadIdx.x; t < n; t += kThreads) {
-    int const a = t / win.sx, b = (t / win.sy) % win.w[1], c = t % win.w[2];
-    s[t] = field[(win.p0[0] + a) * D::SX + (win.p0[1] + b) * D::SY + (win.p0[2] + c)];
+  int const n = win.w[0] * win.w[1] * win.w[2];
+  for (int t = threadIdx.x; t < n; t += kStencilThreads) {
+    int const a = t / (win.w[1] * win.w[2]), b = (t / win.w[2]) % win.w[1], c = t % win.w[2];
+    s[a * Lds<DIM>::SX + b * Lds<DIM>::SY + c] =
+        field[(win.p0[0] + a) * D::SX + (win.p0[1] + b) * D::SY + (win.p0[2] + c)];
+  }
+}
+
+// Child cells (2a + dx, 2b + dy, 2c + dz) from `q`, the child's cell
+// (2a, 2b, 2c): the two cells adjacent along the fastest axis (dz in 3D, dy in
+// 2D) are one 16-byte access (the offset is even: aligned).
+template <int DIM> __device__ __forceinline__ void store_children(double *q, const double v[1 << DIM]) {
+  using D = Dims<DIM>;
+  if constexpr (DIM == 3) {
+    *reinterpret_cast<double2 *>(q) = make_double2(v[0], v[4]);
+    *reinterpret_cast<double2 *>(q + D::SX) = make_double2(v[1], v[5]);
+    *reinterpret_cast<double2 *>(q + D::SY) = make_double2(v[2], v[6]);
+    *reinterpret_cast<double2 *>(q + D::SX + D::SY) = make_double2(v[3], v[7]);
+  } else {
+    *reinterpret_cast<double2 *>(q) = make_double2(v[0], v[2]);
+    *reinterpret_cast<double2 *>(q + D::SX) = make_double2(v[1], v[3]);
+  }
+}
+template <int DIM> __device__ __forceinline__ void load_children(const double *q, double x[1 << DIM]) {
+  using D = Dims<DIM>;
+  if constexpr (DIM == 3) {
+    double2 const a = *reinterpret_cast<const double2 *>(q);
+    double2 const b = *reinterpret_cast<const double2 *>(q + D::SX);
+    double2 const c = *reinterpret_cast<const double2 *>(q + D::SY);
+    double2 const d = *reinterpret_cast<const double2 *>(q + D::SX + D::SY);
+    x[0] = a.x, x[4] = a.y, x[1] = b.x, x[5] = b.y, x[2] = c.x, x[6] = c.y, x[3] = d.x, x[7] = d.y;
+  } else {
+    double2 const a = *reinterpret_cast<const double2 *>(q);
+    double2 const b = *reinterpret_cast<const double2 *>(q + D::SX);
+    x[0] = a.x, x[2] = a.y, x[1] = b.x, x[3] = b.y;
   }
 }
 
 // One workgroup per (link, equation): the child cells of the range,
-// predicted from the parent's field.
-template <int DIM>
-__global__ __launch_bounds__(kThreads) void prediction_kernel(double *__restrict__ blocks, int nblocks,
-                                                              const alpaca_mr_link *__restrict__ links,
-                                                              Range r) {
+// predicted from the parent's field.  FULL: the range is the whole block
+// (12 predicted parent cells per axis, known at compile time).
+template <int DIM, bool FULL>
+__global__ __launch_bounds__(kStencilThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void prediction_kernel(
+    double *__restrict__ blocks, int nblocks, const alpaca_mr_link *__restrict__ links, Range r) {
   using D = Dims<DIM>;
-  __shared__ double s[16 * 16 * (DIM == 3 ? 16 : 1)];
+  __shared__ double s[Lds<DIM>::SIZE];
+  if constexpr (FULL) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      r.start[a] = 0;
+      r.count[a] = a < DIM ? kTC / 2 : 1;
+    }
+  }
   int const e = blockIdx.x % D::NE;
   alpaca_mr_link const l = links[blockIdx.x / D::NE];
   if (!link_ok<DIM>(l, nblocks)) return;
@@ -224,23 +291,13 @@ __global__ __launch_bounds__(kThreads) void prediction_kernel(double *__restrict
   double *child = blocks + (size_t)l.child * D::BLOCK + (size_t)e * D::FIELD;
   int const nz = DIM == 3 ? r.count[2] : 1;
   int const tasks = r.count[0] * r.count[1] * nz;
-  for (int t = threadIdx.x; t < tasks; t += kThreads) {
+  for (int t = threadIdx.x; t < tasks; t += kStencilThreads) {
     int const a = t / (r.count[1] * nz), b = (t / nz) % r.count[1], c = t % nz;
     double v[1 << DIM];
-    predict_cell<DIM>(s, win.centre(a, b, c), win.sx, win.sy, v);
+    predict_cell<DIM>(s, Window<DIM>::centre(a, b, c), v);
     int const ci = r.start[0] + 2 * a, cj = r.start[1] + 2 * b;
     int const ck = DIM == 3 ? r.start[2] + 2 * c : 0;
-    double *q = child + ci * D::SX + cj * D::SY + ck;
-    q[0] = v[0];
-    q[D::SX] = v[1];
-    q[D::SY] = v[2];
-    q[D::SX + D::SY] = v[3];
-    if constexpr (DIM == 3) {
-      q[1] = v[4];
-      q[D::SX + 1] = v[5];
-      q[D::SY + 1] = v[6];
-      q[D::SX + D::SY + 1] = v[7];
-    }
+    store_children<DIM>(child + ci * D::SX + cj * D::SY + ck, v);
   }
 }
 
@@ -255,17 +312,17 @@ struct Thresholds {
 // field_setup.h:243-247) and every child cell, halo included, the largest
 // |exact - predicted| / |exact|, the prediction taken over the whole block.
 // std::max(max_detail, x) drops NaN quotients, so every partial maximum is a
-// non-NaN value >= 0 and the order of the reduction does not matter.
+// non-NaN value >= +0 and the order of the reduction does not matter; the
+// hardware maximum (which skips a NaN operand) picks the same value: the
+// quotients of absolute values are never -0.
 template <int DIM>
-__global__ __launch_bounds__(kThreads) void details_kernel(const double *__restrict__ blocks, int nblocks,
-                                                           const alpaca_mr_link *__restrict__ links,
-                                                           Thresholds th, double *__restrict__ details,
-                                                           int *__restrict__ decisions) {
+__global__ __launch_bounds__(kStencilThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void details_kernel(
+    const double *__restrict__ blocks, int nblocks, const alpaca_mr_link *__restrict__ links,
+    Thresholds th, double *__restrict__ details, int *__restrict__ decisions) {
   using D = Dims<DIM>;
   constexpr int HZ = DIM == 3 ? kTC / 2 : 1;
   constexpr int TASKS = (kTC / 2) * (kTC / 2) * HZ;
-  __shared__ double s[16 * 16 * (DIM == 3 ? 16 : 1)];
-  __shared__ double part[kThreads / 64];
+  __shared__ double s[Lds<DIM>::SIZE];
   alpaca_mr_link const l = links[blockIdx.x];
   if (!link_ok<DIM>(l, nblocks)) {
     if (threadIdx.x == 0) {
@@ -282,35 +339,29 @@ __global__ __launch_bounds__(kThreads) void details_kernel(const double *__restr
     stage_window<DIM>(s, blocks + (size_t)l.parent * D::BLOCK + (size_t)e * D::FIELD, win);
     __syncthreads();
     const double *child = blocks + (size_t)l.child * D::BLOCK + (size_t)e * D::FIELD;
-    for (int t = threadIdx.x; t < TASKS; t += kThreads) {
+    for (int t = threadIdx.x; t < TASKS; t += kStencilThreads) {
       int const a = t / ((kTC / 2) * HZ), b = (t / HZ) % (kTC / 2), c = t % HZ;
       double v[1 << DIM];
-      predict_cell<DIM>(s, win.centre(a, b, c), win.sx, win.sy, v);
-      const double *q = child + 2 * a * D::SX + 2 * b * D::SY + (DIM == 3 ? 2 * c : 0);
+      predict_cell<DIM>(s, Window<DIM>::centre(a, b, c), v);
       double x[1 << DIM];
-      x[0] = q[0];
-      x[1] = q[D::SX];
-      x[2] = q[D::SY];
-      x[3] = q[D::SX + D::SY];
-      if constexpr (DIM == 3) {
-        x[4] = q[1];
-        x[5] = q[D::SX + 1];
-        x[6] = q[D::SY + 1];
-        x[7] = q[D::SX + D::SY + 1];
-      }
+      load_children<DIM>(child + 2 * a * D::SX + 2 * b * D::SY + (DIM == 3 ? 2 * c : 0), x);
 #pragma unroll
       for (int n = 0; n < (1 << DIM); ++n)
-        m = smax(m, __builtin_fabs(x[n] - v[n]) / __builtin_fabs(x[n]));
+        m = __builtin_fmax(m, __builtin_fabs(x[n] - v[n]) / __builtin_fabs(x[n]));
     }
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) m = smax(m, __shfl_xor(m, off, 64));
+  for (int off = 32; off > 0; off >>= 1) m = __builtin_fmax(m, __shfl_xor(m, off, 64));
+  // The window is consumed: its first entries take the per-wave maxima (one
+  // LDS allocation of 32 KiB keeps five workgroups per CU).
+  double *part = s;
+  __syncthreads();
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     double d = part[0];
 #pragma unroll
-    for (int w = 1; w < kThreads / 64; ++w) d = smax(d, part[w]);
+    for (int w = 1; w < kStencilThreads / 64; ++w) d = __builtin_fmax(d, part[w]);
     details[blockIdx.x] = d;
     if (decisions) {
       int dec = -1;
@@ -330,8 +381,7 @@ int hip_status(hipError_t e) { return e == hipSuccess ? ALPACA_OK : ALPACA_ERR_H
 
 template <int DIM> int average_impl(double *blocks, int nblocks, const alpaca_mr_link *links,
                                     int nlinks, hipStream_t s) {
-  constexpr int H = kIC / 2;
-  size_t const total = (size_t)nlinks * Dims<DIM>::NE * H * H * (DIM == 3 ? H : 1);
+  size_t const total = (size_t)nlinks * Dims<DIM>::NE * AvgShape<DIM>::PAIRS;
   dim3 grid((unsigned)((total + kThreads - 1) / kThreads));
   hipLaunchKernelGGL((average_kernel<DIM>), grid, dim3(kThreads), 0, s, blocks, nblocks, links, nlinks);
   return hip_status(hipGetLastError());
@@ -340,14 +390,22 @@ template <int DIM> int average_impl(double *blocks, int nblocks, const alpaca_mr
 template <int DIM> int prediction_impl(double *blocks, int nblocks, const alpaca_mr_link *links,
                                        int nlinks, const Range &r, hipStream_t s) {
   dim3 grid((unsigned)nlinks * Dims<DIM>::NE);
-  hipLaunchKernelGGL((prediction_kernel<DIM>), grid, dim3(kThreads), 0, s, blocks, nblocks, links, r);
+  bool const full = r.start[0] == 0 && r.start[1] == 0 && r.start[2] == 0 &&
+                    r.count[0] == kTC / 2 && r.count[1] == kTC / 2 &&
+                    r.count[2] == (DIM == 3 ? kTC / 2 : 1);
+  if (full)
+    hipLaunchKernelGGL((prediction_kernel<DIM, true>), grid, dim3(kStencilThreads), 0, s, blocks,
+                       nblocks, links, r);
+  else
+    hipLaunchKernelGGL((prediction_kernel<DIM, false>), grid, dim3(kStencilThreads), 0, s, blocks,
+                       nblocks, links, r);
   return hip_status(hipGetLastError());
 }
 
 template <int DIM> int details_impl(const double *blocks, int nblocks, const alpaca_mr_link *links,
                                     int nlinks, const Thresholds &th, double *details, int *decisions,
                                     hipStream_t s) {
-  hipLaunchKernelGGL((details_kernel<DIM>), dim3((unsigned)nlinks), dim3(kThreads), 0, s, blocks,
+  hipLaunchKernelGGL((details_kernel<DIM>), dim3((unsigned)nlinks), dim3(kStencilThreads), 0, s, blocks,
                      nblocks, links, th, details, decisions);
   return hip_status(hipGetLastError());
 }
@@ -364,6 +422,7 @@ int alpaca_mr_average(int dim, double *d_blocks, int nblocks, const alpaca_mr_li
                       int nlinks, void *stream) {
   if ((dim != 2 && dim != 3) || nblocks < 1 || nlinks < 0 || !d_blocks || (nlinks > 0 && !d_links))
     return ALPACA_ERR_INVALID;
+  if (reinterpret_cast<uintptr_t>(d_blocks) % 16) return ALPACA_ERR_INVALID;  // 16-byte accesses
   if (nlinks == 0) return ALPACA_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (dim == 2) return mr::average_impl<2>(d_blocks, nblocks, d_links, nlinks, s);
@@ -375,6 +434,7 @@ int alpaca_mr_prediction(int dim, double *d_blocks, int nblocks, const alpaca_mr
   if ((dim != 2 && dim != 3) || nblocks < 1 || nlinks < 0 || !d_blocks || !range ||
       (nlinks > 0 && !d_links))
     return ALPACA_ERR_INVALID;
+  if (reinterpret_cast<uintptr_t>(d_blocks) % 16) return ALPACA_ERR_INVALID;  // 16-byte accesses
   mr::Range r{{0, 0, 0}, {1, 1, 1}};
   for (int a = 0; a < dim; ++a) {
     int const start = range[2 * a], count = range[2 * a + 1];
@@ -400,6 +460,7 @@ int alpaca_mr_child_needs_remeshing(int dim, const double *d_blocks, int nblocks
       (nlinks > 0 && (!d_links || !d_details)) || nlevels < 0 || nlevels > mr::kMaxLevels ||
       (nlevels > 0 && !eps_coarsen) || (d_decisions && nlevels == 0))
     return ALPACA_ERR_INVALID;
+  if (reinterpret_cast<uintptr_t>(d_blocks) % 16) return ALPACA_ERR_INVALID;  // 16-byte accesses
   if (nlinks == 0) return ALPACA_OK;
   mr::Thresholds th;
   std::memset(&th, 0, sizeof(th));

@@ -270,7 +270,9 @@ int alpaca_ensemble_capacity(alpaca_ensemble *e, int *free_blocks, int *largest_
  * bit 2 z; 0..3 in 2D) and the child's level.  Links are in device memory;
  * a link with an index outside [0, nblocks), child == parent or a position
  * outside [0, 2^dim) is skipped.  Within one call no block may be both a
- * parent and a child (the reference walks the levels one at a time). */
+ * parent and a child (the reference walks the levels one at a time).
+ * d_blocks must be 16-byte aligned (ALPACA_ERR_INVALID otherwise): the
+ * kernels move pairs of adjacent cells as one 16-byte access. */
 typedef struct {
   int child;
   int parent;
