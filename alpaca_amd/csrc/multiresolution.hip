@@ -309,20 +309,34 @@ struct Thresholds {
 };
 
 // One workgroup per link: over Mass and Energy (the wavelet equations,
-// field_setup.h:243-247) and every child cell, halo included, the largest
-// |exact - predicted| / |exact|, the prediction taken over the whole block.
-// std::max(max_detail, x) drops NaN quotients, so every partial maximum is a
-// non-NaN value >= +0 and the order of the reduction does not matter; the
-// hardware maximum (which skips a NaN operand) picks the same value: the
-// quotients of absolute values are never -0.
-template <int DIM>
+// field_setup.h:243-247) and every child cell, halo included, the relative
+// detail between the child and the parent's prediction over the whole block,
+// ChildNeedsRemeshing<N> (multiresolution.cpp:299-398):
+//   NORM 0 (Linfinity): the largest |exact - predicted| / |exact|;
+//   NORM 1 (Lone): per equation the sum of |exact - predicted| / |exact|
+//                  times 1/(TC^2 TCZ), the largest over the equations;
+//   NORM 2 (Ltwo): per equation sqrt(sum of ((exact - predicted)/exact)^2)
+//                  times 1/(TC^2 TCZ), the largest over the equations.
+// std::max(max_detail, x) drops a NaN x, so every running maximum is a
+// non-NaN value >= +0 and the hardware maximum (which skips a NaN operand)
+// picks the same value; for L-infinity the order of the reduction does not
+// matter (the quotients of absolute values are never -0).  The L-one and
+// L-two sums are reduced as a tree (lanes, then waves) instead of the
+// reference's cell order: they agree to rounding, and a NaN or infinite term
+// gives a NaN or infinite sum either way.
+template <int DIM, int NORM>
 __global__ __launch_bounds__(kStencilThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void details_kernel(
     const double *__restrict__ blocks, int nblocks, const alpaca_mr_link *__restrict__ links,
     Thresholds th, double *__restrict__ details, int *__restrict__ decisions) {
   using D = Dims<DIM>;
   constexpr int HZ = DIM == 3 ? kTC / 2 : 1;
   constexpr int TASKS = (kTC / 2) * (kTC / 2) * HZ;
+  constexpr int kWaves = kStencilThreads / 64;
+  constexpr double kOneNumberOfCells = 1.0 / double(kTC * kTC * D::TZ);
   __shared__ double s[Lds<DIM>::SIZE];
+  // The window's first entries take the per-wave partial results once it is
+  // consumed (one LDS allocation of 32 KiB keeps five workgroups per CU).
+  double *part = s;
   alpaca_mr_link const l = links[blockIdx.x];
   if (!link_ok<DIM>(l, nblocks)) {
     if (threadIdx.x == 0) {
@@ -339,6 +353,7 @@ __global__ __launch_bounds__(kStencilThreads) __attribute__((amdgpu_waves_per_eu
     stage_window<DIM>(s, blocks + (size_t)l.parent * D::BLOCK + (size_t)e * D::FIELD, win);
     __syncthreads();
     const double *child = blocks + (size_t)l.child * D::BLOCK + (size_t)e * D::FIELD;
+    double acc = 0.0;
     for (int t = threadIdx.x; t < TASKS; t += kStencilThreads) {
       int const a = t / ((kTC / 2) * HZ), b = (t / HZ) % (kTC / 2), c = t % HZ;
       double v[1 << DIM];
@@ -346,23 +361,47 @@ __global__ __launch_bounds__(kStencilThreads) __attribute__((amdgpu_waves_per_eu
       double x[1 << DIM];
       load_children<DIM>(child + 2 * a * D::SX + 2 * b * D::SY + (DIM == 3 ? 2 * c : 0), x);
 #pragma unroll
-      for (int n = 0; n < (1 << DIM); ++n)
-        m = __builtin_fmax(m, __builtin_fabs(x[n] - v[n]) / __builtin_fabs(x[n]));
+      for (int n = 0; n < (1 << DIM); ++n) {
+        if constexpr (NORM == 0) {
+          m = __builtin_fmax(m, __builtin_fabs(x[n] - v[n]) / __builtin_fabs(x[n]));
+        } else if constexpr (NORM == 1) {
+          acc += __builtin_fabs(x[n] - v[n]) / __builtin_fabs(x[n]);
+        } else {
+          double const q = (x[n] - v[n]) / x[n];
+          acc = acc + q * q;
+        }
+      }
+    }
+    if constexpr (NORM != 0) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+      __syncthreads();  // every thread is done with the window
+      if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double sum = part[0];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) sum += part[w];
+        double const val =
+            NORM == 1 ? sum * kOneNumberOfCells : __builtin_sqrt(sum) * kOneNumberOfCells;
+        m = __builtin_fmax(m, val);
+      }
     }
   }
+  if constexpr (NORM == 0) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) m = __builtin_fmax(m, __shfl_xor(m, off, 64));
-  // The window is consumed: its first entries take the per-wave maxima (one
-  // LDS allocation of 32 KiB keeps five workgroups per CU).
-  double *part = s;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
-  __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) m = __builtin_fmax(m, __shfl_xor(m, off, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int w = 1; w < kWaves; ++w) m = __builtin_fmax(m, part[w]);
+      m = __builtin_fmax(part[0], m);
+    }
+  }
   if (threadIdx.x == 0) {
-    double d = part[0];
-#pragma unroll
-    for (int w = 1; w < kStencilThreads / 64; ++w) d = __builtin_fmax(d, part[w]);
-    details[blockIdx.x] = d;
+    details[blockIdx.x] = m;
     if (decisions) {
       int dec = -1;
       if (l.level >= 0 && l.level < th.nlevels) {
@@ -370,7 +409,7 @@ __global__ __launch_bounds__(kStencilThreads) __attribute__((amdgpu_waves_per_eu
         // Refine 1, Neutral 0 (remesh_identifier.h:29).
         double const eps_coarsen = th.eps[l.level];
         double const eps_refine = eps_coarsen * 32.0;
-        dec = d <= eps_coarsen ? 2 : (d >= eps_refine ? 1 : 0);
+        dec = m <= eps_coarsen ? 2 : (m >= eps_refine ? 1 : 0);
       }
       decisions[blockIdx.x] = dec;
     }
@@ -403,10 +442,18 @@ template <int DIM> int prediction_impl(double *blocks, int nblocks, const alpaca
 }
 
 template <int DIM> int details_impl(const double *blocks, int nblocks, const alpaca_mr_link *links,
-                                    int nlinks, const Thresholds &th, double *details, int *decisions,
-                                    hipStream_t s) {
-  hipLaunchKernelGGL((details_kernel<DIM>), dim3((unsigned)nlinks), dim3(kStencilThreads), 0, s, blocks,
-                     nblocks, links, th, details, decisions);
+                                    int nlinks, int norm, const Thresholds &th, double *details,
+                                    int *decisions, hipStream_t s) {
+  dim3 const grid((unsigned)nlinks), block(kStencilThreads);
+  if (norm == ALPACA_MR_NORM_L1)
+    hipLaunchKernelGGL((details_kernel<DIM, 1>), grid, block, 0, s, blocks, nblocks, links, th,
+                       details, decisions);
+  else if (norm == ALPACA_MR_NORM_L2)
+    hipLaunchKernelGGL((details_kernel<DIM, 2>), grid, block, 0, s, blocks, nblocks, links, th,
+                       details, decisions);
+  else
+    hipLaunchKernelGGL((details_kernel<DIM, 0>), grid, block, 0, s, blocks, nblocks, links, th,
+                       details, decisions);
   return hip_status(hipGetLastError());
 }
 
@@ -453,10 +500,11 @@ int alpaca_mr_prediction(int dim, double *d_blocks, int nblocks, const alpaca_mr
 }
 
 int alpaca_mr_child_needs_remeshing(int dim, const double *d_blocks, int nblocks,
-                                    const alpaca_mr_link *d_links, int nlinks,
+                                    const alpaca_mr_link *d_links, int nlinks, int norm,
                                     const double *eps_coarsen, int nlevels, double *d_details,
                                     int *d_decisions, void *stream) {
   if ((dim != 2 && dim != 3) || nblocks < 1 || nlinks < 0 || !d_blocks ||
+      (norm != ALPACA_MR_NORM_LINF && norm != ALPACA_MR_NORM_L1 && norm != ALPACA_MR_NORM_L2) ||
       (nlinks > 0 && (!d_links || !d_details)) || nlevels < 0 || nlevels > mr::kMaxLevels ||
       (nlevels > 0 && !eps_coarsen) || (d_decisions && nlevels == 0))
     return ALPACA_ERR_INVALID;
@@ -468,8 +516,10 @@ int alpaca_mr_child_needs_remeshing(int dim, const double *d_blocks, int nblocks
   for (int i = 0; i < nlevels; ++i) th.eps[i] = eps_coarsen[i];
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (dim == 2)
-    return mr::details_impl<2>(d_blocks, nblocks, d_links, nlinks, th, d_details, d_decisions, s);
-  return mr::details_impl<3>(d_blocks, nblocks, d_links, nlinks, th, d_details, d_decisions, s);
+    return mr::details_impl<2>(d_blocks, nblocks, d_links, nlinks, norm, th, d_details,
+                               d_decisions, s);
+  return mr::details_impl<3>(d_blocks, nblocks, d_links, nlinks, norm, th, d_details,
+                             d_decisions, s);
 }
 
 }  // extern "C"

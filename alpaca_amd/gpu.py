@@ -96,7 +96,7 @@ _SIGNATURES = {
                               ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p],
                              ctypes.c_int),
     "alpaca_mr_child_needs_remeshing": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
-                                         ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
                                         ctypes.c_int),

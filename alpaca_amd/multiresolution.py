@@ -21,6 +21,9 @@ from . import gpu
 
 # RemeshIdentifier (remesh_identifier.h:29).
 NEUTRAL, REFINE, COARSE = 0, 1, 2
+# Norm of the wavelet details (norms.h); the reference's default
+# CC::NFWA() is L-infinity (compile_time_constants.h:133).
+LINF, L1, L2 = 0, 1, 2
 MAX_LEVELS = 32
 
 # Compile-time constants of the threshold: STDO, the limiting convergence
@@ -89,10 +92,14 @@ def _stream(stream):
 
 class Multiresolution:
     """Batched Multiresolution operators; ``thresholder`` (a ThresholdComputer)
-    is needed by child_needs_remeshing only."""
+    and ``norm`` (LINF, L1, L2: the template argument of ChildNeedsRemeshing)
+    are used by child_needs_remeshing only."""
 
-    def __init__(self, thresholder: ThresholdComputer | None = None):
+    def __init__(self, thresholder: ThresholdComputer | None = None, norm: int = LINF):
+        if norm not in (LINF, L1, L2):
+            raise gpu.AlpacaError(f"unknown norm {norm}")
         self.thresholder = thresholder
+        self.norm = norm
 
     @staticmethod
     def average(blocks, dim, links, stream=None):
@@ -140,6 +147,6 @@ class Multiresolution:
                 raise gpu.AlpacaError("out: float64 and int32 tensors of at least n entries")
         gpu.check(lib.alpaca_mr_child_needs_remeshing(
             dim, ctypes.c_void_p(blocks.data_ptr()), nb, ctypes.c_void_p(links.data_ptr()), nl,
-            eps, len(table), ctypes.c_void_p(details.data_ptr()),
+            self.norm, eps, len(table), ctypes.c_void_p(details.data_ptr()),
             ctypes.c_void_p(decisions.data_ptr()), _stream(stream)), "mr_child_needs_remeshing")
         return details, decisions

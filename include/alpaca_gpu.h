@@ -300,19 +300,27 @@ int alpaca_mr_prediction(int dim, double *d_blocks, int nblocks,
                          const alpaca_mr_link *d_links, int nlinks,
                          const int *range, void *stream);
 
-/* ChildNeedsRemeshing<Norm::Linfinity> (multiresolution.cpp:299-323, called
- * by DetermineRemeshingNodes, modular_algorithm_assembler.cpp:2425-2480) on
- * the blocks given (the reference passes the right-hand-side buffers):
- * d_details[n] (device) = the largest |exact - predicted| / |exact| over Mass
- * and Energy (field_setup.h:243-247) and every child cell, halo included
- * (NaN quotients dropped; NaN for a skipped link).  With d_decisions (device,
- * may be NULL) also RemeshingDecision (multiresolution.cpp:408-428):
+/* Norm of the wavelet details (the reference's Norm, norms.h; its default
+ * CC::NFWA() is L-infinity, compile_time_constants.h:133). */
+enum { ALPACA_MR_NORM_LINF = 0, ALPACA_MR_NORM_L1 = 1, ALPACA_MR_NORM_L2 = 2 };
+
+/* ChildNeedsRemeshing<N> (multiresolution.cpp:299-398, called by
+ * DetermineRemeshingNodes, modular_algorithm_assembler.cpp:2425-2480) on the
+ * blocks given (the reference passes the right-hand-side buffers):
+ * d_details[n] (device) over Mass and Energy (field_setup.h:243-247) and
+ * every child cell, halo included, with q = (exact - predicted) / exact:
+ * L-infinity the largest |q|; L-one per equation sum |q| / (TC^2 TCZ);
+ * L-two per equation sqrt(sum q^2) / (TC^2 TCZ); the largest over the
+ * equations, NaN values dropped (std::max), NaN for a skipped link.  The
+ * L-one/L-two sums are tree reductions: equal to the reference's sequential
+ * sums to rounding.  With d_decisions (device, may be NULL) also
+ * RemeshingDecision (multiresolution.cpp:408-428):
  * 2 Coarse (detail <= eps), 1 Refine (detail >= 32 eps), 0 Neutral, -1 for a
  * skipped link or a level outside the table, where eps = eps_coarsen[level]
  * (host, nlevels <= 32 entries, ThresholdComputer::ThresholdOnLevel,
  * threshold_computer.h:91-94). */
 int alpaca_mr_child_needs_remeshing(int dim, const double *d_blocks, int nblocks,
-                                    const alpaca_mr_link *d_links, int nlinks,
+                                    const alpaca_mr_link *d_links, int nlinks, int norm,
                                     const double *eps_coarsen, int nlevels,
                                     double *d_details, int *d_decisions,
                                     void *stream);

@@ -1097,26 +1097,51 @@ void ora_mr_prediction(int dim, const double *parent, double *child, int positio
 #undef P
 }
 
-/* ChildNeedsRemeshing<Norm::Linfinity> (multiresolution.cpp:299-323) without
- * the decision: the largest |exact - predicted| / |exact| over the wavelet
- * equations -- Mass and Energy (MF::EWA(), field_setup.h:243-247) -- and
- * every cell of the child block (halo included), the prediction taken over
- * the full block from the parent.  std::max(max_detail, x) keeps max_detail
- * when x is NaN, so NaN quotients (0/0) drop out. */
-double ora_mr_detail_linf(int dim, const double *parent, const double *child, int position) {
+/* ChildNeedsRemeshing<N> (multiresolution.cpp:299-398) without the decision:
+ * over the wavelet equations -- Mass and Energy (MF::EWA(),
+ * field_setup.h:243-247) -- and every cell of the child block (halo
+ * included), the relative detail between the child's values and the
+ * parent's prediction over the full block:
+ *   norm 0, L-infinity (:299-323): max of |exact - predicted| / |exact|;
+ *   norm 1, L-one (:329-361): per equation the sum over cells (i, j, k order)
+ *           of |exact - predicted| / |exact|, times 1/(TC^2 TCZ);
+ *   norm 2, L-two (:367-398): per equation sqrt(sum of ((exact - predicted) /
+ *           exact)^2) times 1/(TC^2 TCZ);
+ * the result is the largest over the equations, by std::max(max_detail, x),
+ * which keeps max_detail when x is NaN. */
+double ora_mr_detail(int dim, const double *parent, const double *child, int position, int norm) {
   int const tz = dim == 3 ? TC : 1;
   size_t const field = (size_t)TC * TC * tz;
   int const full[6] = {0, TC, 0, TC, 0, tz};
+  double const one_number_of_cells = 1.0 / (double)(TC * TC * tz);
   double *pred = (double *)malloc(field * sizeof(double));
   double max_detail = 0.0;
   for (int e = MASS; e <= ENERGY; ++e) {
     ora_mr_prediction(dim, parent + (size_t)e * field, pred, position, full);
     const double *x = child + (size_t)e * field;
-    for (size_t c = 0; c < field; ++c)
-      max_detail = smax(max_detail, fabs(x[c] - pred[c]) / fabs(x[c]));
+    if (norm == 0) {
+      for (size_t c = 0; c < field; ++c)
+        max_detail = smax(max_detail, fabs(x[c] - pred[c]) / fabs(x[c]));
+    } else if (norm == 1) {
+      double error_norm = 0.0;
+      for (size_t c = 0; c < field; ++c)
+        error_norm += fabs(x[c] - pred[c]) / fabs(x[c]);
+      max_detail = smax(max_detail, error_norm * one_number_of_cells);
+    } else {
+      double error_norm = 0.0;
+      for (size_t c = 0; c < field; ++c) {
+        double const temp = (x[c] - pred[c]) / x[c];
+        error_norm = error_norm + temp * temp;
+      }
+      max_detail = smax(max_detail, sqrt(error_norm) * one_number_of_cells);
+    }
   }
   free(pred);
   return max_detail;
+}
+
+double ora_mr_detail_linf(int dim, const double *parent, const double *child, int position) {
+  return ora_mr_detail(dim, parent, child, position, 0);
 }
 
 /* ThresholdComputer<DIM, STDO = 3> (threshold_computer.h:53-59, 91-94): the

@@ -77,6 +77,8 @@ void ora_mr_average(int dim, const double *child, double *parent, int position);
 /* range = x_start, x_count, y_start, y_count, z_start, z_count (child cells). */
 void ora_mr_prediction(int dim, const double *parent, double *child, int position,
                        const int *range);
+/* norm: 0 L-infinity, 1 L-one, 2 L-two (the reference's Norm enum order). */
+double ora_mr_detail(int dim, const double *parent, const double *child, int position, int norm);
 double ora_mr_detail_linf(int dim, const double *parent, const double *child, int position);
 double ora_mr_threshold(int dim, int maximum_level, int reference_level,
                         double reference_epsilon, int level);

@@ -215,10 +215,17 @@ def mr_prediction(dim, parent_field, child_field, position, rng=None, which="ora
     return out
 
 
-def mr_detail_linf(dim, parent, child, position):
-    f = oracle().ora_mr_detail_linf
+MR_LINF, MR_L1, MR_L2 = 0, 1, 2
+
+
+def mr_detail(dim, parent, child, position, norm=MR_LINF):
+    f = oracle().ora_mr_detail
     f.restype = ctypes.c_double
-    return f(dim, _dp(_block(dim, parent)), _dp(_block(dim, child)), position)
+    return f(dim, _dp(_block(dim, parent)), _dp(_block(dim, child)), position, norm)
+
+
+def mr_detail_linf(dim, parent, child, position):
+    return mr_detail(dim, parent, child, position, MR_LINF)
 
 
 def mr_threshold(dim, maximum_level, reference_level, reference_epsilon, level, which="oracle"):
@@ -235,14 +242,14 @@ def mr_decision(detail, eps_coarsen):
     return oracle().ora_mr_decision(ctypes.c_double(detail), ctypes.c_double(eps_coarsen))
 
 
-def ref_mr_child_needs_remeshing(dim, parent, child, position, epsilon):
-    """The reference's ChildNeedsRemeshing (L-infinity) with the coarsening
-    threshold on the child's level equal to `epsilon`."""
+def ref_mr_child_needs_remeshing(dim, parent, child, position, epsilon, norm=MR_LINF):
+    """The reference's ChildNeedsRemeshing<norm> with the coarsening threshold
+    on the child's level equal to `epsilon`."""
     return ref(dim).ref_mr_child_needs_remeshing(_dp(_block(dim, parent)), _dp(_block(dim, child)),
-                                                 position, ctypes.c_double(epsilon))
+                                                 position, ctypes.c_double(epsilon), norm)
 
 
-def ref_mr_detail(dim, parent, child, position):
+def ref_mr_detail(dim, parent, child, position, norm=MR_LINF):
     """The reference's L-infinity detail, read off its remeshing decision: the
     decision is Coarse exactly when detail <= epsilon, so the smallest
     non-negative double epsilon with a Coarse decision is the detail itself
@@ -251,7 +258,7 @@ def ref_mr_detail(dim, parent, child, position):
 
     def coarse(bits):
         eps = np.array([bits], dtype=np.int64).view(np.float64)[0]
-        return ref_mr_child_needs_remeshing(dim, parent, child, position, eps) == MR_COARSE
+        return ref_mr_child_needs_remeshing(dim, parent, child, position, eps, norm) == MR_COARSE
 
     if coarse(lo):
         return 0.0

@@ -597,18 +597,23 @@ void ref_mr_prediction(double const *parent, double *child, int position,
               sizeof(double) * TX * TY * TZ);
 }
 
-// ChildNeedsRemeshing<CC::NFWA()> of a level-1 child under a Thresholder
-// built from (maximum level 1, reference level 1, epsilon): the coarsening
-// threshold on the child's level is then exactly `epsilon`.  Returns the
-// RemeshIdentifier (0 Neutral, 1 Refine, 2 Coarse).
+// ChildNeedsRemeshing<N> (norm 0 Linfinity, 1 Lone, 2 Ltwo) of a level-1
+// child under a Thresholder built from (maximum level 1, reference level 1,
+// epsilon): the coarsening threshold on the child's level is then exactly
+// `epsilon`.  Returns the RemeshIdentifier (0 Neutral, 1 Refine, 2 Coarse).
 int ref_mr_child_needs_remeshing(double const *parent, double const *child,
-                                 int position, double epsilon) {
+                                 int position, double epsilon, int norm) {
   auto c = std::make_unique<Block>();
   auto p = std::make_unique<Block>();
   LoadRhs(*c, child);
   LoadRhs(*p, parent);
   Multiresolution mr(Thresholder(1, 1, epsilon));
-  return int(mr.ChildNeedsRemeshing<CC::NFWA()>(*p, *c, ChildId(position)));
+  nid_t const id = ChildId(position);
+  if (norm == 1)
+    return int(mr.ChildNeedsRemeshing<Norm::Lone>(*p, *c, id));
+  if (norm == 2)
+    return int(mr.ChildNeedsRemeshing<Norm::Ltwo>(*p, *c, id));
+  return int(mr.ChildNeedsRemeshing<Norm::Linfinity>(*p, *c, id));
 }
 
 // Reference ThresholdComputer::ThresholdOnLevel.

@@ -289,9 +289,10 @@ def main_mr():
     sibling position, the reference's Multiresolution::Average output (the
     parent block after averaging), Prediction output per range of
     mr_states.ranges (every field, child cells outside the range untouched)
-    as SHA-256 with NaNs canonical, and the L-infinity detail of
-    ChildNeedsRemeshing (read off its decisions, oracle_py.ref_mr_detail) as
-    a hex float; plus ThresholdOnLevel for a few settings."""
+    as SHA-256 with NaNs canonical, and the L-infinity, L-one and L-two
+    details of ChildNeedsRemeshing (read off its decisions,
+    oracle_py.ref_mr_detail) as hex floats; plus ThresholdOnLevel for a few
+    settings."""
     sys.path.insert(0, os.path.dirname(HERE))
     import edge_states
     import mr_states
@@ -309,11 +310,14 @@ def main_mr():
                                                                  which="ref")
                                          for e in range(dim + 2)])
                         preds.append(hashlib.sha256(edge_states.canonical(pred)).hexdigest())
-                    detail = oracle_py.ref_mr_detail(dim, parent, child, pos)
+                    details = [float(oracle_py.ref_mr_detail(dim, parent, child, pos, norm)).hex()
+                               for norm in (oracle_py.MR_LINF, oracle_py.MR_L1, oracle_py.MR_L2)]
                     meta["pairs"][f"{dim}d_{kind}_{pos}"] = {
                         "average_sha256": hashlib.sha256(edge_states.canonical(avg)).hexdigest(),
                         "prediction_sha256": preds,
-                        "detail": float(detail).hex()}
+                        "detail": details[0],
+                        "detail_l1": details[1],
+                        "detail_l2": details[2]}
         for dim in (2, 3):
             for ml, rl, eps in ((1, 1, 1e-3), (4, 2, 1e-3), (6, 6, 0.01), (3, 0, 0.5)):
                 for level in range(ml + 1):

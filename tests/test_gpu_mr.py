@@ -115,6 +115,31 @@ def test_details_and_decisions(torch, dim):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("norm", [mr.L1, mr.L2])
+def test_details_l1_l2(torch, dim, norm):
+    """The L-one and L-two details: tree-reduced sums, equal to the oracle's
+    sequential sums to 1e-13 relative; non-finite sums classified alike."""
+    print(f"[mr details norm {norm}] dim={dim}", flush=True)
+    host, dev, links, pairs = _pool(torch, dim, mr_states.KINDS)
+    with np.errstate(all="ignore"):
+        want = np.array([O.mr_detail(dim, p, c, pos, norm) for _, pos, p, c in pairs])
+    th = mr.ThresholdComputer(dim, 2, 2, 4.0e-3)
+    details, decisions = mr.Multiresolution(th, norm).child_needs_remeshing(dev, dim, links)
+    torch.cuda.synchronize()
+    got = details.cpu().numpy()
+    dec = decisions.cpu().numpy()
+    for i, (kind, pos, _, _) in enumerate(pairs):
+        w, g = want[i], got[i]
+        if np.isfinite(w):
+            assert abs(g - w) <= 1e-13 * abs(w), (kind, pos, g, w)
+            eps = th.threshold_on_level(i % 3)
+            if abs(w - eps) > 1e-10 * eps and abs(w - 32 * eps) > 1e-10 * eps:
+                assert dec[i] == O.mr_decision(w, eps), (kind, pos, dec[i], w)
+        else:
+            assert g == w, (kind, pos, g, w)  # +inf (NaN sums are dropped)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
 def test_all_children_of_one_parent(torch, dim):
     """The octree use: one parent, all 2^dim children in one call."""
     print(f"[mr siblings] dim={dim}", flush=True)
@@ -173,15 +198,18 @@ def test_skipped_links_and_bad_arguments(torch, dim):
     eps = (ctypes.c_double * 33)(*([1e-3] * 33))
     out = torch.empty(1, dtype=torch.float64, device="cuda")
     dec = torch.empty(1, dtype=torch.int32, device="cuda")
-    assert lib.alpaca_mr_child_needs_remeshing(*args, eps, 33, ctypes.c_void_p(out.data_ptr()),
+    assert lib.alpaca_mr_child_needs_remeshing(*args, 0, eps, 33, ctypes.c_void_p(out.data_ptr()),
                                                ctypes.c_void_p(dec.data_ptr()), st) \
         == 1
-    assert lib.alpaca_mr_child_needs_remeshing(*args, None, 0, ctypes.c_void_p(out.data_ptr()),
+    assert lib.alpaca_mr_child_needs_remeshing(*args, 0, None, 0, ctypes.c_void_p(out.data_ptr()),
+                                               ctypes.c_void_p(dec.data_ptr()), st) \
+        == 1
+    assert lib.alpaca_mr_child_needs_remeshing(*args, 3, eps, 2, ctypes.c_void_p(out.data_ptr()),
                                                ctypes.c_void_p(dec.data_ptr()), st) \
         == 1
     assert lib.alpaca_mr_average(4, *args[1:], st) == 1
     # details alone (no decisions) need no threshold table
-    assert lib.alpaca_mr_child_needs_remeshing(*args, None, 0, ctypes.c_void_p(out.data_ptr()),
+    assert lib.alpaca_mr_child_needs_remeshing(*args, 0, None, 0, ctypes.c_void_p(out.data_ptr()),
                                                None, st) == 0
     torch.cuda.synchronize()
     want = O.mr_detail_linf(dim, host[0], host[1], 0)

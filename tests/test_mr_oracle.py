@@ -61,6 +61,8 @@ def test_oracle_matches_reference_fixture(dim, kind, pos):
                              for e in range(dim + 2)])
             assert _sha(pred) == want, rng
         assert O.mr_detail_linf(dim, parent, child, pos) == float.fromhex(g["detail"])
+        assert O.mr_detail(dim, parent, child, pos, O.MR_L1) == float.fromhex(g["detail_l1"])
+        assert O.mr_detail(dim, parent, child, pos, O.MR_L2) == float.fromhex(g["detail_l2"])
 
 
 def test_thresholds_match_reference_fixture():
@@ -87,8 +89,9 @@ def test_oracle_matches_reference_harness_more_seeds(dim):
                         f = O.mr_prediction(dim, parent[e], child[e], pos, rng)
                         g = O.mr_prediction(dim, parent[e], child[e], pos, rng, which="ref")
                         assert np.array_equal(f, g)
-                    assert O.mr_detail_linf(dim, parent, child, pos) == \
-                        O.ref_mr_detail(dim, parent, child, pos)
+                    for norm in (O.MR_LINF, O.MR_L1, O.MR_L2):
+                        assert O.mr_detail(dim, parent, child, pos, norm) == \
+                            O.ref_mr_detail(dim, parent, child, pos, norm)
 
 
 def _cell_averages(coef, h, origin):
@@ -159,6 +162,26 @@ def test_average_is_the_mean_of_the_children(dim):
         untouched = np.ones(out.shape, dtype=bool)
         untouched[sl] = False
         assert np.array_equal(out[untouched], parent[untouched])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_detail_norms_closed_form(dim):
+    """One Energy cell off its prediction by a relative 1e-3 (and one Mass
+    cell by 4e-3): L1 = 4e-3/(1+4e-3)/N, L2 the same (one term), L-infinity
+    the larger quotient; N = TC^2 TCZ cells per field."""
+    parent, _ = mr_states.pair(dim, "smooth", 2)
+    pos = 2
+    pred = np.stack([O.mr_prediction(dim, parent[e], np.zeros(parent.shape[1:]), pos)
+                     for e in range(dim + 2)])
+    z = 3 if dim == 3 else 0
+    pert = pred.copy()
+    pert[1, 5, 7, z] *= 1.0 + 1e-3
+    pert[0, 9, 2, z] *= 1.0 + 4e-3
+    n = TC * TC * (TC if dim == 3 else 1)
+    q = 4e-3 / (1.0 + 4e-3)
+    assert abs(O.mr_detail(dim, parent, pert, pos, O.MR_L1) - q / n) < 1e-12 / n
+    assert abs(O.mr_detail(dim, parent, pert, pos, O.MR_L2) - q / n) < 1e-12 / n
+    assert abs(O.mr_detail(dim, parent, pert, pos, O.MR_LINF) - q) < 1e-12
 
 
 @pytest.mark.parametrize("dim", [2, 3])
