@@ -1,6 +1,7 @@
 // Multiresolution operators of ALPACA between parent and child blocks held in
 // one pool of haloed blocks on gfx950: conservative averaging of a child into
-// its parent (Multiresolution::Average, multiresolution.h:94-167), Harten's
+// its parent (Multiresolution::Average, multiresolution.h:94-167; of its jump
+// surfaces, AverageJumpBuffer, multiresolution.cpp:47-131), Harten's
 // fifth-order prediction of a parent into a child (Multiresolution::
 // Prediction, multiresolution.cpp:146-292) and the L-infinity wavelet detail
 // with the remeshing decision (ChildNeedsRemeshing<Norm::Linfinity>,
@@ -138,6 +139,49 @@ __global__ __launch_bounds__(kThreads) void average_kernel(double *__restrict__ 
     out.y = 0.125 * cms4(r00b.x + r11b.y, r00b.y + r11b.x, r01b.x + r10b.y, r01b.y + r10b.x);
     *reinterpret_cast<double2 *>(p + (x0 + i) * SX + (y0 + j) * SY + z0 + k) = out;
   }
+}
+
+// ------------------------------------------------------- jump surfaces
+
+// Multiresolution::AverageJumpBuffer (multiresolution.cpp:47-131) on jump
+// surfaces [6][NE][IC][ICZ] per block (SurfaceBuffer, block.h:34-41: East,
+// West, North, South, Top, Bottom).  One thread per parent face entry of
+// every face a child may touch; a child touches the high face of an axis when
+// its bit of that axis is set, the low face otherwise, and covers the quarter
+// (half in 2D) of it starting at IC/2 times its bits of the face's transverse
+// axes.  The average is ADDED to the parent's value.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void average_jump_kernel(double *__restrict__ surfaces,
+                                                                int nblocks,
+                                                                const alpaca_mr_link *__restrict__ links,
+                                                                int nlinks) {
+  constexpr int NE = Dims<DIM>::NE;
+  constexpr int ICZ = DIM == 3 ? kIC : 1;
+  constexpr int HJ = DIM == 3 ? kIC / 2 : 1;      // parent entries along the second index
+  constexpr int ENTRIES = NE * (kIC / 2) * HJ;    // per face
+  constexpr int FACE = NE * kIC * ICZ;            // doubles per face
+  constexpr int BLOCK = 6 * FACE;                 // doubles per block's surfaces
+  size_t const t = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= (size_t)nlinks * DIM * ENTRIES) return;
+  int const r = int(t % ENTRIES);
+  int const axis = int((t / ENTRIES) % DIM);
+  alpaca_mr_link const l = links[t / ((size_t)ENTRIES * DIM)];
+  if (!link_ok<DIM>(l, nblocks)) return;
+  // The face of this axis the child touches: East/North/Top (2 axis) when its
+  // bit is set, West/South/Bottom (2 axis + 1) otherwise.
+  int const loc = 2 * axis + (((l.position >> axis) & 1) ? 0 : 1);
+  int const t1 = axis == 0 ? 1 : 0, t2 = axis == 2 ? 1 : 2;
+  int const s1 = ((l.position >> t1) & 1) * (kIC / 2);
+  int const s2 = DIM == 3 ? ((l.position >> t2) & 1) * (kIC / 2) : 0;
+  int const e = r / ((kIC / 2) * HJ), i = (r / HJ) % (kIC / 2), j = r % HJ;
+  const double *q = surfaces + (size_t)l.child * BLOCK + loc * FACE + (e * kIC + 2 * i) * ICZ +
+                    (DIM == 3 ? 2 * j : 0);
+  double v;
+  if constexpr (DIM == 2)
+    v = (q[0] + q[ICZ]) * 0.5;
+  else
+    v = ((q[0] + q[ICZ + 1]) + (q[ICZ] + q[1])) * 0.25;
+  surfaces[(size_t)l.parent * BLOCK + loc * FACE + (e * kIC + s1 + i) * ICZ + s2 + j] += v;
 }
 
 // --------------------------------------------------------------- prediction
@@ -426,6 +470,15 @@ template <int DIM> int average_impl(double *blocks, int nblocks, const alpaca_mr
   return hip_status(hipGetLastError());
 }
 
+template <int DIM> int average_jump_impl(double *surfaces, int nblocks, const alpaca_mr_link *links,
+                                         int nlinks, hipStream_t s) {
+  size_t const total = (size_t)nlinks * DIM * Dims<DIM>::NE * (kIC / 2) * (DIM == 3 ? kIC / 2 : 1);
+  dim3 grid((unsigned)((total + kThreads - 1) / kThreads));
+  hipLaunchKernelGGL((average_jump_kernel<DIM>), grid, dim3(kThreads), 0, s, surfaces, nblocks, links,
+                     nlinks);
+  return hip_status(hipGetLastError());
+}
+
 template <int DIM> int prediction_impl(double *blocks, int nblocks, const alpaca_mr_link *links,
                                        int nlinks, const Range &r, hipStream_t s) {
   dim3 grid((unsigned)nlinks * Dims<DIM>::NE);
@@ -474,6 +527,17 @@ int alpaca_mr_average(int dim, double *d_blocks, int nblocks, const alpaca_mr_li
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (dim == 2) return mr::average_impl<2>(d_blocks, nblocks, d_links, nlinks, s);
   return mr::average_impl<3>(d_blocks, nblocks, d_links, nlinks, s);
+}
+
+int alpaca_mr_average_jump_buffers(int dim, double *d_surfaces, int nblocks,
+                                   const alpaca_mr_link *d_links, int nlinks, void *stream) {
+  if ((dim != 2 && dim != 3) || nblocks < 1 || nlinks < 0 || !d_surfaces ||
+      (nlinks > 0 && !d_links))
+    return ALPACA_ERR_INVALID;
+  if (nlinks == 0) return ALPACA_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dim == 2) return mr::average_jump_impl<2>(d_surfaces, nblocks, d_links, nlinks, s);
+  return mr::average_jump_impl<3>(d_surfaces, nblocks, d_links, nlinks, s);
 }
 
 int alpaca_mr_prediction(int dim, double *d_blocks, int nblocks, const alpaca_mr_link *d_links,

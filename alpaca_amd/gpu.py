@@ -92,6 +92,9 @@ _SIGNATURES = {
                        ctypes.c_int),
     "alpaca_mr_average": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                            ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_mr_average_jump_buffers": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
+                                       ctypes.c_int),
     "alpaca_mr_prediction": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                               ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p],
                              ctypes.c_int),

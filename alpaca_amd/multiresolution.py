@@ -3,7 +3,8 @@
 Host side of the C ABI's alpaca_mr_* entry points, named after the
 reference's classes: ``ThresholdComputer`` (threshold_computer.h:35-97) and
 ``Multiresolution`` (multiresolution.h:59-188) with ``average``
-(Multiresolution::Average), ``prediction`` (Multiresolution::Prediction) and
+(Multiresolution::Average), ``average_jump_buffer`` (AverageJumpBuffer),
+``prediction`` (Multiresolution::Prediction) and
 ``child_needs_remeshing`` (ChildNeedsRemeshing<Norm::Linfinity> with
 RemeshingDecision).  Where the reference works on one parent/child pair per
 call, these take a batch of links -- (child block, parent block, sibling
@@ -109,6 +110,24 @@ class Multiresolution:
         gpu.check(lib.alpaca_mr_average(dim, ctypes.c_void_p(blocks.data_ptr()), nb,
                                         ctypes.c_void_p(links.data_ptr()), nl, _stream(stream)),
                   "mr_average")
+
+    @staticmethod
+    def average_jump_buffer(surfaces, dim, links, stream=None):
+        """AverageJumpBuffer for every link: surfaces is the pool's float64
+        device tensor of [nblocks][6][NE][16][16|1] jump surfaces; the
+        children's averaged face values are added into their parents'."""
+        torch = gpu._torch()
+        lib = gpu.load_library()
+        per = 6 * (dim + 2) * gpu.IC * (gpu.IC if dim == 3 else 1)
+        if surfaces.dtype != torch.float64 or not surfaces.is_contiguous() \
+                or surfaces.numel() % per or surfaces.numel() == 0:
+            raise gpu.AlpacaError("surfaces must be a contiguous float64 tensor of whole "
+                                  "jump-surface buffers")
+        nl = _check_links(links)
+        gpu.check(lib.alpaca_mr_average_jump_buffers(dim, ctypes.c_void_p(surfaces.data_ptr()),
+                                                     surfaces.numel() // per,
+                                                     ctypes.c_void_p(links.data_ptr()), nl,
+                                                     _stream(stream)), "mr_average_jump_buffers")
 
     @staticmethod
     def prediction(blocks, dim, links, rng=None, stream=None):

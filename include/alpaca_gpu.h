@@ -288,6 +288,19 @@ typedef struct {
 int alpaca_mr_average(int dim, double *d_blocks, int nblocks,
                       const alpaca_mr_link *d_links, int nlinks, void *stream);
 
+/* Multiresolution::AverageJumpBuffer (multiresolution.cpp:47-131; used by
+ * JumpFluxAdjustment, modular_algorithm_assembler.cpp:1347-1408): d_surfaces
+ * holds nblocks jump-surface buffers laid out as the reference's
+ * SurfaceBuffer (block.h:34-41), [6][NE][16][16] doubles in 3D and
+ * [6][NE][16][1] in 2D (East, West, North, South, Top, Bottom; 2D uses the
+ * first four).  For every link, on each face of the parent the child touches,
+ * the child's values averaged over 2x2 (2D: 2) entries are ADDED (+=) to the
+ * quarter (half) of the parent's face the child covers.  Children of one
+ * parent may share a call (they cover disjoint entries). */
+int alpaca_mr_average_jump_buffers(int dim, double *d_surfaces, int nblocks,
+                                   const alpaca_mr_link *d_links, int nlinks,
+                                   void *stream);
+
 /* Multiresolution::Prediction (multiresolution.cpp:146-292) of every
  * equation field, for refinement (RefineNode, modular_algorithm_assembler.cpp:
  * 2547-2553: range = HS, IC per axis) and for halos across a level jump

@@ -1164,3 +1164,38 @@ int ora_mr_decision(double detail, double eps_coarsen) {
     return 1;
   return 0;
 }
+
+/* Multiresolution::AverageJumpBuffer (multiresolution.cpp:47-131): a block's
+ * jump surfaces are [6][NE][IC][ICZ] doubles (SurfaceBuffer, block.h:34-41:
+ * East, West, North, South, Top, Bottom; ICZ = 1 in 2D, where only the first
+ * four are used).  For every face of the parent that the child touches (East
+ * when the child's x bit is set, West when it is clear, and so on), the
+ * child's face values averaged over 2x2 (3D) or 2 (2D) cells are ADDED to the
+ * quarter (half) of the parent's face the child covers, which starts at
+ * IC/2 times the child's bits of the face's two transverse axes in
+ * increasing axis order (first_/second_index_look_up_table). */
+void ora_mr_average_jump(int dim, const double *child, double *parent, int position) {
+  int const ne = dim + 2, icz = dim == 3 ? IC : 1;
+  size_t const face = (size_t)ne * IC * icz;
+  for (int loc = 0; loc < 2 * dim; ++loc) {
+    int const axis = loc / 2, high = (loc % 2) == 0; /* East, North, Top are high */
+    if (((position >> axis) & 1) != high)
+      continue;
+    int const t1 = axis == 0 ? 1 : 0, t2 = axis == 2 ? 1 : 2;
+    int const s1 = ((position >> t1) & 1) * (IC / 2);
+    int const s2 = dim == 3 ? ((position >> t2) & 1) * (IC / 2) : 0;
+    const double *c = child + (size_t)loc * face;
+    double *p = parent + (size_t)loc * face;
+    for (int e = 0; e < ne; ++e)
+      for (int i = 0; i < IC / 2; ++i)
+        for (int j = 0; j < (dim == 3 ? IC / 2 : 1); ++j) {
+          const double *q = c + ((size_t)e * IC + 2 * i) * icz + (dim == 3 ? 2 * j : 0);
+          double v;
+          if (dim == 2)
+            v = (q[0] + q[icz]) * 0.5;
+          else
+            v = ((q[0] + q[icz + 1]) + (q[icz] + q[1])) * 0.25;
+          p[((size_t)e * IC + s1 + i) * icz + s2 + j] += v;
+        }
+  }
+}

@@ -84,6 +84,8 @@ double ora_mr_threshold(int dim, int maximum_level, int reference_level,
                         double reference_epsilon, int level);
 /* 0 Neutral, 1 Refine, 2 Coarse. */
 int ora_mr_decision(double detail, double eps_coarsen);
+/* Jump surfaces [6][NE][16][16|1] of a child added into its parent's. */
+void ora_mr_average_jump(int dim, const double *child, double *parent, int position);
 
 #ifdef __cplusplus
 }

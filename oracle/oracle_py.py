@@ -269,3 +269,17 @@ def ref_mr_detail(dim, parent, child, position, norm=MR_LINF):
         else:
             lo = mid
     return float(np.array([hi], dtype=np.int64).view(np.float64)[0])
+
+
+def mr_average_jump(dim, child, parent, position, which="oracle"):
+    """Parent jump surfaces [6][NE][16][16|1] plus the child's averaged ones
+    on the faces the child touches."""
+    icz = 16 if dim == 3 else 1
+    child = np.ascontiguousarray(child, dtype=np.float64)
+    out = np.ascontiguousarray(parent, dtype=np.float64).copy()
+    assert child.shape == (6, dim + 2, 16, icz) and out.shape == child.shape
+    if which == "oracle":
+        oracle().ora_mr_average_jump(dim, _dp(child), _dp(out), position)
+    else:
+        ref(dim).ref_mr_average_jump(_dp(child), _dp(out), position)
+    return out

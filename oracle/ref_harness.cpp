@@ -616,6 +616,17 @@ int ref_mr_child_needs_remeshing(double const *parent, double const *child,
   return int(mr.ChildNeedsRemeshing<Norm::Linfinity>(*p, *c, id));
 }
 
+// Multiresolution::AverageJumpBuffer on SurfaceBuffers given as
+// [6][NE][IY][IZ] arrays; parent is read and written.
+void ref_mr_average_jump(double const *child, double *parent, int position) {
+  auto c = std::make_unique<SurfaceBuffer>();
+  auto p = std::make_unique<SurfaceBuffer>();
+  std::memcpy(c.get(), child, sizeof(SurfaceBuffer));
+  std::memcpy(p.get(), parent, sizeof(SurfaceBuffer));
+  Multiresolution::AverageJumpBuffer(*c, *p, ChildId(position));
+  std::memcpy(parent, p.get(), sizeof(SurfaceBuffer));
+}
+
 // Reference ThresholdComputer::ThresholdOnLevel.
 double ref_mr_threshold(int maximum_level, int reference_level,
                         double reference_epsilon, int level) {

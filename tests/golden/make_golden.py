@@ -291,8 +291,9 @@ def main_mr():
     mr_states.ranges (every field, child cells outside the range untouched)
     as SHA-256 with NaNs canonical, and the L-infinity, L-one and L-two
     details of ChildNeedsRemeshing (read off its decisions,
-    oracle_py.ref_mr_detail) as hex floats; plus ThresholdOnLevel for a few
-    settings."""
+    oracle_py.ref_mr_detail) as hex floats; the reference's AverageJumpBuffer
+    on seeded jump surfaces (seed 40 + dim, child then parent) for every
+    sibling position; plus ThresholdOnLevel for a few settings."""
     sys.path.insert(0, os.path.dirname(HERE))
     import edge_states
     import mr_states
@@ -318,6 +319,15 @@ def main_mr():
                         "detail": details[0],
                         "detail_l1": details[1],
                         "detail_l2": details[2]}
+        meta["jump"] = {}
+        for dim in (2, 3):
+            icz = 16 if dim == 3 else 1
+            rng = np.random.default_rng(40 + dim)
+            for pos in range(2 ** dim):
+                child = rng.standard_normal((6, dim + 2, 16, icz))
+                parent = rng.standard_normal((6, dim + 2, 16, icz))
+                out = oracle_py.mr_average_jump(dim, child, parent, pos, which="ref")
+                meta["jump"][f"{dim}d_{pos}"] = hashlib.sha256(out.tobytes()).hexdigest()
         for dim in (2, 3):
             for ml, rl, eps in ((1, 1, 1e-3), (4, 2, 1e-3), (6, 6, 0.01), (3, 0, 0.5)):
                 for level in range(ml + 1):

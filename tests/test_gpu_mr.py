@@ -1,5 +1,6 @@
 """Multiresolution operators on the GPU (alpaca_mr_average,
-alpaca_mr_prediction, alpaca_mr_child_needs_remeshing) against the oracle.
+alpaca_mr_average_jump_buffers, alpaca_mr_prediction,
+alpaca_mr_child_needs_remeshing) against the oracle.
 
 Averaging and prediction use only additions, subtractions and
 multiplications in the reference's order, so they are compared bit for bit
@@ -163,6 +164,31 @@ def test_all_children_of_one_parent(torch, dim):
     got_mean = dev.cpu().numpy()[0][inner].mean(axis=(1, 2, 3))
     child_mean = np.mean([c[inner].mean(axis=(1, 2, 3)) for c in children], axis=0)
     assert np.allclose(got_mean, child_mean, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_jump_buffer_average_bitexact(torch, dim):
+    """AverageJumpBuffer: three parents with all their children (shuffled
+    link order, one call): every parent's surfaces equal the oracle's
+    children-by-children accumulation bit for bit; children unchanged."""
+    print(f"[mr jump average] dim={dim}", flush=True)
+    n = 2 ** dim
+    icz = 16 if dim == 3 else 1
+    rng = np.random.default_rng(21 + dim)
+    nparents = 3
+    nb = nparents * (n + 1)
+    host = rng.standard_normal((nb, 6, dim + 2, 16, icz))
+    rows = [(p * (n + 1) + 1 + pos, p * (n + 1), pos, 1) for p in range(nparents) for pos in range(n)]
+    order = rng.permutation(len(rows))
+    links = torch.as_tensor(mr.make_links([rows[k] for k in order]), device="cuda")
+    dev = torch.as_tensor(host, device="cuda").contiguous()
+    mr.Multiresolution.average_jump_buffer(dev, dim, links)
+    torch.cuda.synchronize()
+    got = dev.cpu().numpy()
+    want = host.copy()
+    for child, parent, pos, _ in rows:
+        want[parent] = O.mr_average_jump(dim, host[child], want[parent], pos)
+    assert np.array_equal(got, want)
 
 
 @pytest.mark.parametrize("dim", [2, 3])

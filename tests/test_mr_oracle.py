@@ -65,6 +65,45 @@ def test_oracle_matches_reference_fixture(dim, kind, pos):
         assert O.mr_detail(dim, parent, child, pos, O.MR_L2) == float.fromhex(g["detail_l2"])
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_jump_average_matches_reference_fixture(dim):
+    icz = 16 if dim == 3 else 1
+    rng = np.random.default_rng(40 + dim)
+    for pos in range(2 ** dim):
+        child = rng.standard_normal((6, dim + 2, 16, icz))
+        parent = rng.standard_normal((6, dim + 2, 16, icz))
+        out = O.mr_average_jump(dim, child, parent, pos)
+        assert hashlib.sha256(out.tobytes()).hexdigest() == GOLDEN["jump"][f"{dim}d_{pos}"], pos
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_jump_average_of_all_children_is_the_face_mean(dim):
+    """Adding the averaged faces of all 2^dim children into a zero parent:
+    on every face, each parent entry is the mean of the child entries under
+    it (the children of that face cover it exactly once), and a constant
+    surface stays constant."""
+    icz = 16 if dim == 3 else 1
+    rng = np.random.default_rng(9)
+    parent = np.zeros((6, dim + 2, 16, icz))
+    for pos in range(2 ** dim):
+        child = np.full((6, dim + 2, 16, icz), 2.5)
+        parent = O.mr_average_jump(dim, child, parent, pos)
+    assert np.all(parent[:2 * dim] == 2.5)
+    assert np.all(parent[2 * dim:] == 0.0)
+    child = rng.standard_normal((6, dim + 2, 16, icz))
+    out = O.mr_average_jump(dim, child, np.zeros_like(child), 0)
+    # child 0 touches West (1), South (3) and Bottom (5): their low quarter
+    for loc in (1, 3) + ((5,) if dim == 3 else ()):
+        c = child[loc]
+        want = c[:, 0::2] + c[:, 1::2]
+        if dim == 3:
+            want = want[:, :, 0::2] + want[:, :, 1::2]
+        want = want / (2 ** (dim - 1))
+        got = out[loc, :, :8, :8] if dim == 3 else out[loc, :, :8, :]
+        assert np.allclose(got, want, rtol=1e-15, atol=1e-15), loc
+    assert np.all(out[[0, 2, 4]] == 0.0)
+
+
 def test_thresholds_match_reference_fixture():
     for dim, ml, rl, eps, level, want in GOLDEN["thresholds"]:
         want = float.fromhex(want)
