@@ -10,6 +10,11 @@
 //   GpuUniformSolver<DIM>         ModularAlgorithmAssembler::ComputeLoop /
 //                                 Advance on a uniform level 0
 //                                 (modular_algorithm_assembler.cpp:97-195, 543-910)
+//   GpuThresholdComputer<DIM>     ThresholdComputer<DIM, STDO>
+//                                 (threshold_computer.h:35-97)
+//   GpuMultiresolution<DIM>       Multiresolution (multiresolution.h:59-188):
+//                                 Average, AverageJumpBuffer, Prediction,
+//                                 ChildNeedsRemeshing, per pair and batched
 //
 // Errors: the C ABI returns status codes; these adapters turn a non-OK status
 // into AlpacaGpuError (the reference path has no error returns and traps FP
@@ -18,7 +23,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstddef>
+#include <cstring>
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -332,6 +340,171 @@ template <int DIM> class GpuUniformSolver {
   GpuRungeKutta3TVD<DIM> integrator_;
   double cfl_, end_time_;
   int steps_ = 0;
+};
+
+// ThresholdComputer<DIM, STDO = 3> (threshold_computer.h:35-97).
+template <int DIM> class GpuThresholdComputer {
+ public:
+  GpuThresholdComputer(unsigned int maximum_level, unsigned int user_reference_level,
+                       double user_reference_epsilon)
+      : maximum_level_(maximum_level),
+        reference_epsilon_(user_reference_epsilon *
+                           std::pow(2.0, -1.0 * (kAlpha + 1.0) *
+                                             (double(maximum_level) - double(user_reference_level)))) {}
+  double ThresholdOnLevel(unsigned int level) const {
+    return reference_epsilon_ * std::pow(2.0, -1.0 * double(DIM) * (double(maximum_level_) - double(level)));
+  }
+  unsigned int MaximumLevel() const { return maximum_level_; }
+
+ private:
+  static constexpr double kAlpha = 3.0;  // CC::STDO() (compile_time_constants.h:617)
+  unsigned int maximum_level_;
+  double reference_epsilon_;
+};
+
+// RemeshIdentifier (remesh_identifier.h:29).
+enum class RemeshIdentifier { Neutral, Refine, Coarse };
+
+// LevelOfNode / PositionOfNodeAmongSiblings (id_information.h:59-69, 158-160).
+inline unsigned int LevelOfNodeId(std::uint64_t id) {
+  std::uint64_t w = id >> 21;
+  unsigned int level = 0;
+  while (w > 10) {
+    w >>= 3;
+    ++level;
+  }
+  return level;
+}
+inline int PositionOfNodeIdAmongSiblings(std::uint64_t id) { return int(id & 0x7); }
+
+// Multiresolution (multiresolution.h:59-188) on the device.  The batched
+// members take a pool of haloed blocks and a device array of links
+// (child, parent, position, level) and run one launch over all of them; the
+// reference-shaped members take one parent/child pair in host memory (one
+// upload and download per call, for per-pair use and parity checks).
+template <int DIM> class GpuMultiresolution {
+  using G = Geometry<DIM>;
+
+ public:
+  using Field = double[G::TC][G::TC][G::TCZ];
+  static constexpr std::size_t kSurfaceDoubles = std::size_t(6) * G::NE * G::IC * G::ICZ;
+
+  explicit GpuMultiresolution(GpuThresholdComputer<DIM> thresholder, int norm = ALPACA_MR_NORM_LINF,
+                              hipStream_t stream = nullptr)
+      : thresholder_(thresholder), norm_(norm), stream_(stream) {}
+
+  // ---- batched, device-resident
+  void AverageLinks(double *d_pool, int nblocks, alpaca_mr_link const *d_links, int n) const {
+    Check(alpaca_mr_average(DIM, d_pool, nblocks, d_links, n, stream_), "alpaca_mr_average");
+  }
+  void AverageJumpBufferLinks(double *d_surfaces, int nblocks, alpaca_mr_link const *d_links,
+                              int n) const {
+    Check(alpaca_mr_average_jump_buffers(DIM, d_surfaces, nblocks, d_links, n, stream_),
+          "alpaca_mr_average_jump_buffers");
+  }
+  void PredictionLinks(double *d_pool, int nblocks, alpaca_mr_link const *d_links, int n,
+                       int const range[6]) const {
+    Check(alpaca_mr_prediction(DIM, d_pool, nblocks, d_links, n, range, stream_),
+          "alpaca_mr_prediction");
+  }
+  // Details (device, n doubles) and RemeshIdentifier values (device, n ints)
+  // on the thresholds of levels 0..MaximumLevel.
+  void ChildNeedsRemeshingLinks(double const *d_pool, int nblocks, alpaca_mr_link const *d_links,
+                                int n, double *d_details, int *d_decisions) const {
+    std::vector<double> eps(thresholder_.MaximumLevel() + 1);
+    for (unsigned int l = 0; l < eps.size(); ++l) eps[l] = thresholder_.ThresholdOnLevel(l);
+    Check(alpaca_mr_child_needs_remeshing(DIM, d_pool, nblocks, d_links, n, norm_, eps.data(),
+                                          int(eps.size()), d_details, d_decisions, stream_),
+          "alpaca_mr_child_needs_remeshing");
+  }
+
+  // ---- reference-shaped, one pair in host memory
+  // Average(child_buffer, parent_buffer, child_id): overrides the child's
+  // octant of the parent's internal cells.
+  void Average(typename G::Conservatives const &child, typename G::Conservatives &parent,
+               std::uint64_t child_id) const {
+    PairCall(&parent, &child, child_id, [&](double *d, alpaca_mr_link const *l) { AverageLinks(d, 2, l, 1); },
+             true, false);
+  }
+  // Prediction(parent_field, child_field, child_id, x_start, x_count, ...).
+  void Prediction(Field const &parent, Field &child, std::uint64_t child_id, unsigned int x_start = 0,
+                  unsigned int x_count = G::TC, unsigned int y_start = 0, unsigned int y_count = G::TC,
+                  unsigned int z_start = 0, unsigned int z_count = G::TCZ) const {
+    int const range[6] = {int(x_start), int(x_count), int(y_start), int(y_count), int(z_start), int(z_count)};
+    std::vector<double> p(G::kBlockDoubles, 0.0), c(G::kBlockDoubles, 0.0);
+    std::memcpy(p.data(), &parent[0][0][0], sizeof(Field));
+    std::memcpy(c.data(), &child[0][0][0], sizeof(Field));
+    PairCall(p.data(), c.data(), child_id,
+             [&](double *d, alpaca_mr_link const *l) { PredictionLinks(d, 2, l, 1, range); }, false, true);
+    std::memcpy(&child[0][0][0], c.data(), sizeof(Field));
+  }
+  // ChildNeedsRemeshing<norm>(parent, child, child_id) on the blocks given
+  // (the reference passes the right-hand-side buffers).
+  RemeshIdentifier ChildNeedsRemeshing(typename G::Conservatives const &parent,
+                                       typename G::Conservatives const &child,
+                                       std::uint64_t child_id, double *detail = nullptr) const {
+    DeviceBuffer pool(2 * sizeof(double) * G::kBlockDoubles), link(sizeof(alpaca_mr_link)),
+        out(sizeof(double) + sizeof(int));
+    Upload(pool.get<double>(), &parent, G::kBlockDoubles);
+    Upload(pool.get<double>() + G::kBlockDoubles, &child, G::kBlockDoubles);
+    alpaca_mr_link const l{1, 0, PositionOfNodeIdAmongSiblings(child_id), int(LevelOfNodeId(child_id))};
+    CheckHip(hipMemcpyAsync(link.get(), &l, sizeof(l), hipMemcpyHostToDevice, stream_), "upload link");
+    ChildNeedsRemeshingLinks(pool.get<double>(), 2, link.get<alpaca_mr_link>(), 1, out.get<double>(),
+                             reinterpret_cast<int *>(out.get<char>() + sizeof(double)));
+    struct {
+      double d;
+      int r;
+    } h{};
+    CheckHip(hipMemcpyAsync(&h.d, out.get(), sizeof(double), hipMemcpyDeviceToHost, stream_), "download");
+    CheckHip(hipMemcpyAsync(&h.r, out.get<char>() + sizeof(double), sizeof(int), hipMemcpyDeviceToHost,
+                            stream_), "download");
+    CheckHip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    if (detail) *detail = h.d;
+    if (h.r < 0) throw AlpacaGpuError(ALPACA_ERR_INVALID, "ChildNeedsRemeshing: level beyond the thresholds");
+    return RemeshIdentifier(h.r);
+  }
+  // AverageJumpBuffer(child_surfaces, parent_surfaces, child_id): surfaces of
+  // kSurfaceDoubles doubles in SurfaceBuffer layout; adds into the parent.
+  void AverageJumpBuffer(double const *child, double *parent, std::uint64_t child_id) const {
+    DeviceBuffer pool(2 * sizeof(double) * kSurfaceDoubles), link(sizeof(alpaca_mr_link));
+    Upload(pool.get<double>(), parent, kSurfaceDoubles);
+    Upload(pool.get<double>() + kSurfaceDoubles, child, kSurfaceDoubles);
+    alpaca_mr_link const l{1, 0, PositionOfNodeIdAmongSiblings(child_id), int(LevelOfNodeId(child_id))};
+    CheckHip(hipMemcpyAsync(link.get(), &l, sizeof(l), hipMemcpyHostToDevice, stream_), "upload link");
+    AverageJumpBufferLinks(pool.get<double>(), 2, link.get<alpaca_mr_link>(), 1);
+    CheckHip(hipMemcpyAsync(parent, pool.get(), sizeof(double) * kSurfaceDoubles, hipMemcpyDeviceToHost,
+                            stream_), "download");
+    CheckHip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  }
+
+ private:
+  void Upload(double *d, void const *h, std::size_t n) const {
+    CheckHip(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, stream_), "upload");
+  }
+  // Pool [parent, child] on the device, one link; downloads the parent or
+  // the child afterwards.
+  template <typename F>
+  void PairCall(void *parent, void const *child, std::uint64_t child_id, F const &f, bool back_parent,
+                bool back_child) const {
+    DeviceBuffer pool(2 * sizeof(double) * G::kBlockDoubles), link(sizeof(alpaca_mr_link));
+    Upload(pool.get<double>(), parent, G::kBlockDoubles);
+    Upload(pool.get<double>() + G::kBlockDoubles, child, G::kBlockDoubles);
+    alpaca_mr_link const l{1, 0, PositionOfNodeIdAmongSiblings(child_id), int(LevelOfNodeId(child_id))};
+    CheckHip(hipMemcpyAsync(link.get(), &l, sizeof(l), hipMemcpyHostToDevice, stream_), "upload link");
+    f(pool.get<double>(), link.get<alpaca_mr_link>());
+    if (back_parent)
+      CheckHip(hipMemcpyAsync(parent, pool.get(), sizeof(double) * G::kBlockDoubles, hipMemcpyDeviceToHost,
+                              stream_), "download");
+    if (back_child)
+      CheckHip(hipMemcpyAsync(const_cast<void *>(child), pool.get<double>() + G::kBlockDoubles,
+                              sizeof(double) * G::kBlockDoubles, hipMemcpyDeviceToHost, stream_),
+               "download");
+    CheckHip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  }
+
+  GpuThresholdComputer<DIM> thresholder_;
+  int norm_;
+  hipStream_t stream_;
 };
 
 }  // namespace alpaca_host

@@ -6,6 +6,14 @@
 //   alpaca_selftest fluxes <in.bin> <out.bin>
 //       GpuConvectiveTermSolver::UpdateConvectiveFluxes on one haloed block;
 //       out = the three accumulated face-flux arrays.
+//   alpaca_selftest mr <in.bin> <out.bin>
+//       GpuMultiresolution on one level-0 parent and its level-1 child:
+//       in = int32 dim, position, norm, pad; float64 epsilon; parent block,
+//       child block, parent jump surfaces, child jump surfaces.  out = the
+//       averaged parent, the child predicted over the whole block and over
+//       its internal cells (every field), the detail and the decision (as a
+//       float64) under GpuThresholdComputer(1, 1, epsilon), and the parent's
+//       jump surfaces after AverageJumpBuffer.
 //
 // Binary header (little endian): int32 dim, nb[3], bc[3], flux, precision,
 // max_steps; float64 gamma, background_pressure, cell_size, cfl, end_time.
@@ -69,14 +77,83 @@ template <int DIM> int Fluxes(Header const &h, std::vector<double> const &block,
   return 0;
 }
 
+struct MrHeader {
+  int32_t dim, position, norm, pad;
+  double epsilon;
+};
+
+template <int DIM> int Mr(MrHeader const &h, std::vector<double> const &data, char const *out_path) {
+  using G = alpaca_host::Geometry<DIM>;
+  using M = alpaca_host::GpuMultiresolution<DIM>;
+  std::size_t const nb = G::kBlockDoubles, ns = M::kSurfaceDoubles;
+  if (data.size() != 2 * nb + 2 * ns) {
+    std::cerr << "mr input size mismatch\n";
+    return 2;
+  }
+  M const mr(alpaca_host::GpuThresholdComputer<DIM>(1, 1, h.epsilon), h.norm);
+  std::uint64_t const child_id = (std::uint64_t(0x1400000) << 3) | std::uint64_t(h.position);
+  auto *parent = new typename G::Conservatives[1];
+  auto *child = new typename G::Conservatives[4];  // [1] whole block, [2] internal cells
+  std::memcpy(parent, data.data(), nb * sizeof(double));
+  for (int c = 0; c < 3; ++c) std::memcpy(child + c, data.data() + nb, nb * sizeof(double));
+  std::vector<double> out;
+  auto append = [&](void const *p, std::size_t n) {
+    double const *d = static_cast<double const *>(p);
+    out.insert(out.end(), d, d + n);
+  };
+  // ChildNeedsRemeshing first: the blocks as given.
+  double detail = 0.0;
+  alpaca_host::RemeshIdentifier const r = mr.ChildNeedsRemeshing(parent[0], child[0], child_id, &detail);
+  typename G::Conservatives *avg = new typename G::Conservatives[1];
+  std::memcpy(avg, parent, nb * sizeof(double));
+  mr.Average(child[0], avg[0], child_id);
+  append(avg, nb);
+  for (int e = 0; e < G::NE; ++e) {
+    mr.Prediction(parent[0][e], child[1][e], child_id);
+    mr.Prediction(parent[0][e], child[2][e], child_id, G::HS, G::IC, G::HS, G::IC,
+                  DIM == 3 ? G::HS : 0, DIM == 3 ? G::IC : 1);
+  }
+  append(child + 1, nb);
+  append(child + 2, nb);
+  append(&detail, 1);
+  double const rd = double(int(r));
+  append(&rd, 1);
+  std::vector<double> surf(data.begin() + 2 * nb, data.begin() + 2 * nb + ns);
+  mr.AverageJumpBuffer(data.data() + 2 * nb + ns, surf.data(), child_id);
+  append(surf.data(), ns);
+  std::ofstream o(out_path, std::ios::binary);
+  o.write(reinterpret_cast<char const *>(out.data()), out.size() * sizeof(double));
+  delete[] parent;
+  delete[] child;
+  delete[] avg;
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
   if (argc != 4) {
-    std::cerr << "usage: alpaca_selftest run|fluxes <in.bin> <out.bin>\n";
+    std::cerr << "usage: alpaca_selftest run|fluxes|mr <in.bin> <out.bin>\n";
     return 2;
   }
   std::ifstream in(argv[2], std::ios::binary);
+  if (std::strcmp(argv[1], "mr") == 0) {
+    MrHeader m{};
+    if (!in.read(reinterpret_cast<char *>(&m), sizeof(m))) {
+      std::cerr << "cannot read header\n";
+      return 2;
+    }
+    std::vector<double> const data = ReadRest(in);
+    try {
+      if (m.dim == 2) return Mr<2>(m, data, argv[3]);
+      if (m.dim == 3) return Mr<3>(m, data, argv[3]);
+      std::cerr << "bad dim\n";
+      return 2;
+    } catch (alpaca_host::AlpacaGpuError const &e) {
+      std::cerr << "alpaca error: " << e.what() << "\n";
+      return 3;
+    }
+  }
   Header h{};
   if (!in.read(reinterpret_cast<char *>(&h), sizeof(h))) {
     std::cerr << "cannot read header\n";

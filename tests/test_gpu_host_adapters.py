@@ -1,7 +1,8 @@
 """The C++ host adapters (alpaca_amd/csrc/host/alpaca_host.h), exercised
 through their command-line driver, against the CPU oracle: the
-ConvectiveTermSolver-shaped per-block call and the uniform-level
-ComputeLoop/Advance built on the RungeKutta3TVD-shaped stage calls."""
+ConvectiveTermSolver-shaped per-block call, the uniform-level
+ComputeLoop/Advance built on the RungeKutta3TVD-shaped stage calls, and the
+Multiresolution-shaped per-pair calls."""
 import os
 import struct
 import subprocess
@@ -74,3 +75,40 @@ def test_uniform_solver_adapter_3d_periodic(tmp_path):
     want, _, _ = oracle_py.run(3, [2, 2, 2], [1, 1, 1], 0, 0.5, 1e300, 10, st0)
     assert steps == 10
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("norm", [0, 1, 2])
+def test_multiresolution_adapter(tmp_path, dim, norm):
+    """GpuMultiresolution's reference-shaped calls (Average, Prediction over
+    the block and over the internal cells, ChildNeedsRemeshing,
+    AverageJumpBuffer) on one parent/child pair against the oracle."""
+    import mr_states
+
+    pos = (3 * norm + dim) % (2 ** dim)
+    print(f"[host mr] dim={dim} norm={norm} position={pos}", flush=True)
+    parent, child = mr_states.pair(dim, "smooth", pos, seed=norm)
+    icz = 16 if dim == 3 else 1
+    rng = np.random.default_rng(5 + norm)
+    ps = rng.standard_normal((6, dim + 2, 16, icz))
+    cs = rng.standard_normal((6, dim + 2, 16, icz))
+    eps = 1.0e-3
+    raw = _run("mr", struct.pack("<4id", dim, pos, norm, 0, eps) + parent.tobytes() +
+               child.tobytes() + ps.tobytes() + cs.tobytes(), tmp_path)
+    got = np.frombuffer(raw, dtype=np.float64)
+    nb = parent.size
+    avg, whole, inner = got[:nb], got[nb:2 * nb], got[2 * nb:3 * nb]
+    detail, decision = got[3 * nb], int(got[3 * nb + 1])
+    surf = got[3 * nb + 2:]
+    assert np.array_equal(avg, oracle_py.mr_average(dim, child, parent, pos).ravel())
+    rng_inner = (4, 16, 4, 16) + ((4, 16) if dim == 3 else (0, 1))
+    want_whole = np.stack([oracle_py.mr_prediction(dim, parent[e], child[e], pos)
+                           for e in range(dim + 2)])
+    want_inner = np.stack([oracle_py.mr_prediction(dim, parent[e], child[e], pos, rng_inner)
+                           for e in range(dim + 2)])
+    assert np.array_equal(whole, want_whole.ravel())
+    assert np.array_equal(inner, want_inner.ravel())
+    want = oracle_py.mr_detail(dim, parent, child, pos, norm)
+    assert abs(detail - want) <= 1e-13 * want
+    assert decision == oracle_py.mr_decision(want, eps)
+    assert np.array_equal(surf, oracle_py.mr_average_jump(dim, cs, ps, pos).ravel())
