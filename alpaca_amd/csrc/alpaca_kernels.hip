@@ -475,6 +475,8 @@ int validate(const alpaca_case_desc *d) {
                          d->bc[a] != ALPACA_BC_SYMMETRY))
       return ALPACA_ERR_INVALID;
   if (d->dim == 2 && d->nb[2] != 1) return ALPACA_ERR_INVALID;
+  if ((long long)d->nb[0] * d->nb[1] * (d->dim == 3 ? d->nb[2] : 1) > kMaxBlocks)
+    return ALPACA_ERR_INVALID;
   if (d->flux != ALPACA_FLUX_HLLC && d->flux != ALPACA_FLUX_ROE) return ALPACA_ERR_INVALID;
   if (d->precision != ALPACA_FP64 && d->precision != ALPACA_MIXED &&
       d->precision != ALPACA_FP64_FMA)
@@ -717,7 +719,7 @@ int alpaca_convective_fluxes(const alpaca_case_desc *desc, int nblocks, const do
                              double *d_fluxes, void *stream) {
   int st = validate(desc);
   if (st != ALPACA_OK) return st;
-  if (nblocks < 1 || !d_avg || !d_fluxes) return ALPACA_ERR_INVALID;
+  if (nblocks < 1 || nblocks > kMaxBlocks || !d_avg || !d_fluxes) return ALPACA_ERR_INVALID;
   hipStream_t s = static_cast<hipStream_t>(stream);
   double *fl = nullptr;
   if (desc->dim == 2) {
@@ -740,7 +742,8 @@ int alpaca_right_hand_side(const alpaca_case_desc *desc, int nblocks, const doub
                            double *d_rhs, void *stream) {
   int st = validate(desc);
   if (st != ALPACA_OK) return st;
-  if (nblocks < 1 || !d_avg || !d_rhs || d_avg == d_rhs) return ALPACA_ERR_INVALID;
+  if (nblocks < 1 || nblocks > kMaxBlocks || !d_avg || !d_rhs || d_avg == d_rhs)
+    return ALPACA_ERR_INVALID;
   hipStream_t s = static_cast<hipStream_t>(stream);
   size_t const bytes = alpaca_block_doubles(desc->dim) * sizeof(double) * (size_t)nblocks;
   if (hipMemsetAsync(d_rhs, 0, bytes, s) != hipSuccess) return ALPACA_ERR_HIP;
@@ -775,7 +778,8 @@ int alpaca_fill_halos(const alpaca_case_desc *desc, double *d_blocks, void *stre
 
 int alpaca_fill_halos_table(int dim, double *d_blocks, int nblocks, const int *d_neighbors,
                             const int *side_bc, void *stream) {
-  if ((dim != 2 && dim != 3) || !d_blocks || !d_neighbors || !side_bc || nblocks < 1)
+  if ((dim != 2 && dim != 3) || !d_blocks || !d_neighbors || !side_bc || nblocks < 1 ||
+      nblocks > kMaxBlocks)
     return ALPACA_ERR_INVALID;
   SideBc bc;
   for (int i = 0; i < 6; ++i) {

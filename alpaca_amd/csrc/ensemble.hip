@@ -274,7 +274,9 @@ int capture_step(alpaca_ensemble *e) {
 extern "C" {
 
 size_t alpaca_ensemble_bytes(int dim, int max_blocks, int max_cases) {
-  if ((dim != 2 && dim != 3) || max_blocks < 1 || max_cases < 1) return 0;
+  if ((dim != 2 && dim != 3) || max_blocks < 1 || max_cases < 1 || max_blocks > kMaxBlocks ||
+      max_cases > kMaxBlocks)
+    return 0;
   return 3 * block_bytes(dim) * (size_t)max_blocks +
          (size_t)max_cases * (sizeof(CaseSlot) + sizeof(alpaca_case_state)) +
          (size_t)kClasses * max_blocks * sizeof(WorkItem) + kClasses * sizeof(int);
@@ -283,7 +285,7 @@ size_t alpaca_ensemble_bytes(int dim, int max_blocks, int max_cases) {
 int alpaca_ensemble_create(int dim, int max_blocks, int max_cases, unsigned class_mask,
                            void *stream, alpaca_ensemble **out) {
   if (!out || (dim != 2 && dim != 3) || max_blocks < 1 || max_cases < 1 || class_mask == 0 ||
-      (class_mask >> kClasses) != 0)
+      (class_mask >> kClasses) != 0 || max_blocks > kMaxBlocks || max_cases > kMaxBlocks)
     return ALPACA_ERR_INVALID;
   *out = nullptr;
   auto *e = new (std::nothrow) alpaca_ensemble();

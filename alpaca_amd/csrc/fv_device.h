@@ -22,6 +22,11 @@
 
 namespace alpaca {
 
+// Largest block count (and link count) an entry point accepts: every grid
+// size computed from it stays inside 32-bit launch dimensions (3D: 2^24
+// blocks are 9 TB of haloed state, far beyond one GPU's HBM).
+constexpr long long kMaxBlocks = 1LL << 24;
+
 constexpr int kIC = 16;  // internal cells per block side (compile_time_constants.h:39)
 constexpr int kHS = 4;   // halo width (compile_time_constants.h:41)
 constexpr int kTC = 24;  // cells per side with halo

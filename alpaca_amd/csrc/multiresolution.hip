@@ -462,6 +462,11 @@ __global__ __launch_bounds__(kStencilThreads) __attribute__((amdgpu_waves_per_eu
 
 int hip_status(hipError_t e) { return e == hipSuccess ? ALPACA_OK : ALPACA_ERR_HIP; }
 
+bool bad_sizes(int dim, int nblocks, int nlinks) {
+  return (dim != 2 && dim != 3) || nblocks < 1 || nblocks > kMaxBlocks || nlinks < 0 ||
+         nlinks > kMaxBlocks;
+}
+
 template <int DIM> int average_impl(double *blocks, int nblocks, const alpaca_mr_link *links,
                                     int nlinks, hipStream_t s) {
   size_t const total = (size_t)nlinks * Dims<DIM>::NE * AvgShape<DIM>::PAIRS;
@@ -520,7 +525,7 @@ extern "C" {
 
 int alpaca_mr_average(int dim, double *d_blocks, int nblocks, const alpaca_mr_link *d_links,
                       int nlinks, void *stream) {
-  if ((dim != 2 && dim != 3) || nblocks < 1 || nlinks < 0 || !d_blocks || (nlinks > 0 && !d_links))
+  if (mr::bad_sizes(dim, nblocks, nlinks) || !d_blocks || (nlinks > 0 && !d_links))
     return ALPACA_ERR_INVALID;
   if (reinterpret_cast<uintptr_t>(d_blocks) % 16) return ALPACA_ERR_INVALID;  // 16-byte accesses
   if (nlinks == 0) return ALPACA_OK;
@@ -531,7 +536,7 @@ int alpaca_mr_average(int dim, double *d_blocks, int nblocks, const alpaca_mr_li
 
 int alpaca_mr_average_jump_buffers(int dim, double *d_surfaces, int nblocks,
                                    const alpaca_mr_link *d_links, int nlinks, void *stream) {
-  if ((dim != 2 && dim != 3) || nblocks < 1 || nlinks < 0 || !d_surfaces ||
+  if (mr::bad_sizes(dim, nblocks, nlinks) || !d_surfaces ||
       (nlinks > 0 && !d_links))
     return ALPACA_ERR_INVALID;
   if (nlinks == 0) return ALPACA_OK;
@@ -542,7 +547,7 @@ int alpaca_mr_average_jump_buffers(int dim, double *d_surfaces, int nblocks,
 
 int alpaca_mr_prediction(int dim, double *d_blocks, int nblocks, const alpaca_mr_link *d_links,
                          int nlinks, const int *range, void *stream) {
-  if ((dim != 2 && dim != 3) || nblocks < 1 || nlinks < 0 || !d_blocks || !range ||
+  if (mr::bad_sizes(dim, nblocks, nlinks) || !d_blocks || !range ||
       (nlinks > 0 && !d_links))
     return ALPACA_ERR_INVALID;
   if (reinterpret_cast<uintptr_t>(d_blocks) % 16) return ALPACA_ERR_INVALID;  // 16-byte accesses
@@ -567,7 +572,7 @@ int alpaca_mr_child_needs_remeshing(int dim, const double *d_blocks, int nblocks
                                     const alpaca_mr_link *d_links, int nlinks, int norm,
                                     const double *eps_coarsen, int nlevels, double *d_details,
                                     int *d_decisions, void *stream) {
-  if ((dim != 2 && dim != 3) || nblocks < 1 || nlinks < 0 || !d_blocks ||
+  if (mr::bad_sizes(dim, nblocks, nlinks) || !d_blocks ||
       (norm != ALPACA_MR_NORM_LINF && norm != ALPACA_MR_NORM_L1 && norm != ALPACA_MR_NORM_L2) ||
       (nlinks > 0 && (!d_links || !d_details)) || nlevels < 0 || nlevels > mr::kMaxLevels ||
       (nlevels > 0 && !eps_coarsen) || (d_decisions && nlevels == 0))

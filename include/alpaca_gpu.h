@@ -15,6 +15,9 @@
  * equations in the order Mass, Energy, MomX, MomY, (MomZ)
  * (field_setup.h:101-131).  A case is nb[0]*nb[1]*nb[2] consecutive blocks,
  * block (bx,by,bz) at index (bx*nb[1] + by)*nb[2] + bz.
+ *
+ * Counts of blocks, links and cases above 2^24 are refused with
+ * ALPACA_ERR_INVALID (every launch grid derived from them then fits 32 bits).
  */
 #ifndef ALPACA_GPU_H
 #define ALPACA_GPU_H

@@ -62,6 +62,34 @@ def test_invalid_descriptors_are_rejected_before_any_launch():
                               ctypes.c_void_p(2), ctypes.c_void_p(3), 1, None) == 1
 
 
+def test_oversized_counts_are_rejected_before_any_launch():
+    """Block and link counts beyond 2^24 (every launch grid derived from them
+    fits 32 bits below that) are refused with ALPACA_ERR_INVALID."""
+    lib = gpu.load_library()
+    huge = gpu.make_desc(3, [256, 256, 257], [1, 1, 1])
+    assert lib.alpaca_fill_halos(ctypes.byref(huge), ctypes.c_void_p(16), None) == 1
+    big = 1 << 25
+    side = (ctypes.c_int * 6)(0, 0, 0, 0, 0, 0)
+    assert lib.alpaca_fill_halos_table(3, ctypes.c_void_p(16), big, ctypes.c_void_p(16), side,
+                                       None) == 1
+    p = ctypes.c_void_p(256)
+    assert lib.alpaca_mr_average(3, p, big, p, 1, None) == 1
+    assert lib.alpaca_mr_average(3, p, 10, p, big, None) == 1
+    assert lib.alpaca_mr_average_jump_buffers(2, p, 10, p, -1, None) == 1
+    rng = (ctypes.c_int * 6)(0, 24, 0, 24, 0, 24)
+    assert lib.alpaca_mr_prediction(3, p, 10, p, big, rng, None) == 1
+    eps = (ctypes.c_double * 1)(1e-3)
+    assert lib.alpaca_mr_child_needs_remeshing(3, p, big, p, 1, 0, eps, 1, p, None, None) == 1
+    # misaligned pools and unknown norms are refused as well
+    assert lib.alpaca_mr_average(3, ctypes.c_void_p(264), 10, p, 1, None) == 1
+    assert lib.alpaca_mr_child_needs_remeshing(3, p, 10, p, 1, 5, eps, 1, p, None, None) == 1
+    assert lib.alpaca_ensemble_bytes(2, big, 4) == 0
+    out = ctypes.c_void_p()
+    assert lib.alpaca_ensemble_create(2, big, 4, 1, None, ctypes.byref(out)) == 1
+    # no links: nothing to do, and no device call is made
+    assert lib.alpaca_mr_average(3, p, 10, None, 0, None) == 0
+
+
 def test_grid_neighbour_table():
     """alpaca_fill_halos_table's table for a grid: offsets reciprocate
     (B at +o of A means A at -o of B), periodic directions wrap, other
