@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-HIP_SOURCES = ["alpaca_kernels.hip", "ensemble.hip", "multiresolution.hip"]
+HIP_SOURCES = ["alpaca_kernels.hip", "ensemble.hip", "multiresolution.hip", "lts.hip"]
 
 def _run(cmd, cwd=None, verbose=False):
     if verbose:

@@ -1,0 +1,191 @@
+// Level-wise (local time stepping) integration of ALPACA's
+// TimeIntegrator<RungeKutta3TVD> on gfx950: the per-node operations a
+// multi-level step applies to the leaves and jump halos of one level, over a
+// list of pool blocks per launch.
+//
+//   IntegrateConservatives / IntegrateHalo (time_integrator.h:98-153):
+//       u_new = u_old + timestep * u_new on the internal cells or a halo range;
+//   IntegrateJumpConservatives (time_integrator.h:64-84):
+//       jump conservatives += timestep * jump fluxes, jump fluxes = 0;
+//   PrepareBufferForIntegration (time_integrator.h:434-457):
+//       avg = m0 * avg + m1 * init for stages 1 and 2.
+//
+// The level's time step -- the last n micro time steps summed from the newest,
+// times the stage multiplier (time_integrator.h:281-301, 312-340) -- is host
+// arithmetic (alpaca_lts_timestep).  Every expression keeps the reference's
+// operation order; the translation unit is built without contraction.
+#include <hip/hip_runtime.h>
+
+#include "fv_device.h"
+
+namespace alpaca {
+namespace lts {
+namespace {
+
+constexpr int kThreads = 256;
+
+struct Box {
+  int start[3];
+  int count[3];
+};
+
+__device__ __forceinline__ bool block_ok(int b, int nblocks) { return b >= 0 && b < nblocks; }
+
+// u_old may be u_new (no restrict): each element is read, then written.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void integrate_kernel(const double *u_old,
+                                                             double *u_new, int nblocks,
+                                                             const int *__restrict__ list, int n,
+                                                             Box box, double timestep) {
+  using D = Dims<DIM>;
+  int const cells = box.count[0] * box.count[1] * box.count[2];
+  size_t const total = (size_t)n * D::NE * cells;
+  for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * kThreads) {
+    int const b = list[t / ((size_t)D::NE * cells)];
+    if (!block_ok(b, nblocks)) continue;
+    int const r = int(t % cells);
+    int const e = int((t / cells) % D::NE);
+    int const i = box.start[0] + r / (box.count[1] * box.count[2]);
+    int const j = box.start[1] + (r / box.count[2]) % box.count[1];
+    int const k = box.start[2] + r % box.count[2];
+    size_t const c = (size_t)b * D::BLOCK + (size_t)e * D::FIELD + i * D::SX + j * D::SY + k;
+    u_new[c] = u_old[c] + timestep * u_new[c];
+  }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void jump_kernel(double *__restrict__ cons,
+                                                        double *__restrict__ flux, int nblocks,
+                                                        const int *__restrict__ list, int n,
+                                                        double timestep) {
+  constexpr int ICZ = DIM == 3 ? kIC : 1;
+  constexpr int FACE = Dims<DIM>::NE * kIC * ICZ;
+  constexpr int LIVE = 2 * DIM * FACE;  // the faces CC::ANBS() names
+  constexpr int BLOCK = 6 * FACE;       // SurfaceBuffer: six faces
+  size_t const total = (size_t)n * LIVE;
+  for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * kThreads) {
+    int const b = list[t / LIVE];
+    if (!block_ok(b, nblocks)) continue;
+    size_t const c = (size_t)b * BLOCK + t % LIVE;
+    cons[c] += timestep * flux[c];
+    flux[c] = 0.0;
+  }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void prepare_kernel(double *__restrict__ avg,
+                                                           const double *__restrict__ init,
+                                                           int nblocks, const int *__restrict__ list,
+                                                           int n, double m0, double m1) {
+  using D = Dims<DIM>;
+  size_t const total = (size_t)n * D::BLOCK;
+  for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * kThreads) {
+    int const b = list[t / D::BLOCK];
+    if (!block_ok(b, nblocks)) continue;
+    size_t const c = (size_t)b * D::BLOCK + t % D::BLOCK;
+    avg[c] = m0 * avg[c] + m1 * init[c];
+  }
+}
+
+int hip_status(hipError_t e) { return e == hipSuccess ? ALPACA_OK : ALPACA_ERR_HIP; }
+
+bool bad_sizes(int dim, int nblocks, int n) {
+  return (dim != 2 && dim != 3) || nblocks < 1 || nblocks > kMaxBlocks || n < 0 || n > kMaxBlocks;
+}
+
+// Grid of a grid-stride launch: one thread per element up to 2^22 groups.
+dim3 grid_for(size_t total) {
+  size_t const groups = (total + kThreads - 1) / kThreads;
+  return dim3((unsigned)(groups < (size_t(1) << 22) ? groups : (size_t(1) << 22)));
+}
+
+}  // namespace
+}  // namespace lts
+}  // namespace alpaca
+
+using namespace alpaca;
+
+extern "C" {
+
+int alpaca_lts_timestep(const double *micro, int count, int n, int stage, int jump, double *out) {
+  // RungeKutta3TVD multipliers (runge_kutta_3_TVD.h:42-53).
+  static const double mc[3] = {1.0, 0.25, 2.0 / 3.0};
+  static const double mj[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
+  if (!out || count < 0 || n < 0 || n > count || (count > 0 && !micro) || stage < 0 || stage > 2)
+    return ALPACA_ERR_INVALID;
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s = s + micro[count - 1 - i];
+  *out = s * (jump ? mj[stage] : mc[stage]);
+  return ALPACA_OK;
+}
+
+int alpaca_lts_integrate(int dim, const double *d_old, double *d_new, int nblocks, const int *d_list,
+                         int n, const int *range, double timestep, void *stream) {
+  if (lts::bad_sizes(dim, nblocks, n) || !d_old || !d_new || (n > 0 && !d_list))
+    return ALPACA_ERR_INVALID;
+  lts::Box box{{kHS, kHS, dim == 3 ? kHS : 0}, {kIC, kIC, dim == 3 ? kIC : 1}};
+  if (range) {
+    int const tz = dim == 3 ? kTC : 1;
+    for (int a = 0; a < 3; ++a) {
+      int const start = a < dim ? range[2 * a] : 0, count = a < dim ? range[2 * a + 1] : 1;
+      int const extent = a < 2 ? kTC : tz;
+      if (start < 0 || count < 0 || start + count > extent) return ALPACA_ERR_INVALID;
+      box.start[a] = start;
+      box.count[a] = count;
+    }
+  }
+  size_t const cells = (size_t)box.count[0] * box.count[1] * box.count[2];
+  if (n == 0 || cells == 0) return ALPACA_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  size_t const total = (size_t)n * (dim + 2) * cells;
+  if (dim == 2)
+    hipLaunchKernelGGL((lts::integrate_kernel<2>), lts::grid_for(total), dim3(lts::kThreads), 0, s,
+                       d_old, d_new, nblocks, d_list, n, box, timestep);
+  else
+    hipLaunchKernelGGL((lts::integrate_kernel<3>), lts::grid_for(total), dim3(lts::kThreads), 0, s,
+                       d_old, d_new, nblocks, d_list, n, box, timestep);
+  return lts::hip_status(hipGetLastError());
+}
+
+int alpaca_lts_integrate_jump_conservatives(int dim, double *d_cons, double *d_flux, int nblocks,
+                                            const int *d_list, int n, double timestep,
+                                            void *stream) {
+  if (lts::bad_sizes(dim, nblocks, n) || !d_cons || !d_flux || d_cons == d_flux ||
+      (n > 0 && !d_list))
+    return ALPACA_ERR_INVALID;
+  if (n == 0) return ALPACA_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  size_t const live = (size_t)2 * dim * (dim + 2) * kIC * (dim == 3 ? kIC : 1);
+  if (dim == 2)
+    hipLaunchKernelGGL((lts::jump_kernel<2>), lts::grid_for(n * live), dim3(lts::kThreads), 0, s,
+                       d_cons, d_flux, nblocks, d_list, n, timestep);
+  else
+    hipLaunchKernelGGL((lts::jump_kernel<3>), lts::grid_for(n * live), dim3(lts::kThreads), 0, s,
+                       d_cons, d_flux, nblocks, d_list, n, timestep);
+  return lts::hip_status(hipGetLastError());
+}
+
+int alpaca_lts_prepare_buffer(int dim, double *d_avg, const double *d_init, int nblocks,
+                              const int *d_list, int n, int stage, void *stream) {
+  if (lts::bad_sizes(dim, nblocks, n) || !d_avg || !d_init || stage < 0 || stage > 2 ||
+      (n > 0 && !d_list))
+    return ALPACA_ERR_INVALID;
+  if (n == 0 || stage == 0) return ALPACA_OK;  // only later stages combine the buffers
+  // GetBufferMultiplier (runge_kutta_3_TVD.h:55-59).
+  double const m0 = stage == 1 ? 0.25 : 2.0 / 3.0;
+  double const m1 = stage == 1 ? 0.75 : 1.0 / 3.0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  size_t const per = (size_t)(dim + 2) * kTC * kTC * (dim == 3 ? kTC : 1);
+  if (dim == 2)
+    hipLaunchKernelGGL((lts::prepare_kernel<2>), lts::grid_for(n * per), dim3(lts::kThreads), 0, s,
+                       d_avg, d_init, nblocks, d_list, n, m0, m1);
+  else
+    hipLaunchKernelGGL((lts::prepare_kernel<3>), lts::grid_for(n * per), dim3(lts::kThreads), 0, s,
+                       d_avg, d_init, nblocks, d_list, n, m0, m1);
+  return lts::hip_status(hipGetLastError());
+}
+
+}  // extern "C"

@@ -95,6 +95,18 @@ _SIGNATURES = {
     "alpaca_mr_average_jump_buffers": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
                                        ctypes.c_int),
+    "alpaca_lts_timestep": ([ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int,
+                             ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+                            ctypes.c_int),
+    "alpaca_lts_integrate": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                              ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                              ctypes.c_double, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_lts_integrate_jump_conservatives": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.c_double, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_lts_prepare_buffer": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                   ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
+                                  ctypes.c_int),
     "alpaca_mr_prediction": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                               ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p],
                              ctypes.c_int),

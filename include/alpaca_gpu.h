@@ -341,6 +341,44 @@ int alpaca_mr_child_needs_remeshing(int dim, const double *d_blocks, int nblocks
                                     double *d_details, int *d_decisions,
                                     void *stream);
 
+/* ------------------------------------------------------------------------
+ * Level-wise (local time stepping) integration: TimeIntegrator<RungeKutta3TVD>
+ * operations on the leaves and jump halos of one level (Integrate,
+ * modular_algorithm_assembler.cpp:1068-1098).  d_list (device, n int32) names
+ * the pool blocks to treat; entries outside [0, nblocks) are skipped. */
+
+/* The time step of a level's stage (host arithmetic): the last n of `count`
+ * micro time steps summed from the newest, starting at 0.0 (std::accumulate
+ * over crbegin..crbegin + n), times the stage multiplier of the
+ * conservatives (1, 1/4, 2/3) or, with jump != 0, of the jump conservatives
+ * (1/6, 1/6, 2/3) (time_integrator.h:281-301, 312-340; runge_kutta_3_TVD.h:
+ * 42-53). */
+int alpaca_lts_timestep(const double *micro, int count, int n, int stage, int jump,
+                        double *out);
+
+/* IntegrateConservatives / IntegrateHalo (time_integrator.h:98-153):
+ * d_new = d_old + timestep * d_new on every equation of the listed blocks,
+ * over range (host, int[6]: x_start, x_count, y_start, y_count, z_start,
+ * z_count; the z pair is ignored in 2D) or, with range NULL, the internal
+ * cells.  d_old may equal d_new. */
+int alpaca_lts_integrate(int dim, const double *d_old, double *d_new, int nblocks,
+                         const int *d_list, int n, const int *range, double timestep,
+                         void *stream);
+
+/* IntegrateJumpConservatives (time_integrator.h:64-84) on jump surfaces in
+ * SurfaceBuffer layout ([nblocks][6][NE][16][16|1]): d_cons += timestep *
+ * d_flux, then d_flux = 0, on the faces the dimension has (4 in 2D, 6 in
+ * 3D). */
+int alpaca_lts_integrate_jump_conservatives(int dim, double *d_cons, double *d_flux,
+                                            int nblocks, const int *d_list, int n,
+                                            double timestep, void *stream);
+
+/* PrepareBufferForIntegration (time_integrator.h:434-457): for stage 1 or 2,
+ * d_avg = m0 * d_avg + m1 * d_init on every cell of the listed blocks,
+ * (m0, m1) = (1/4, 3/4), (2/3, 1/3); stage 0 does nothing. */
+int alpaca_lts_prepare_buffer(int dim, double *d_avg, const double *d_init, int nblocks,
+                              const int *d_list, int n, int stage, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1199,3 +1199,63 @@ void ora_mr_average_jump(int dim, const double *child, double *parent, int posit
         }
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Level-wise (local time stepping) integration, TimeIntegrator with          */
+/* RungeKutta3TVD (time_integrator.h, runge_kutta_3_TVD.h:39-59).             */
+
+/* The time step of a level's stage: the last n micro time steps summed from
+ * the newest, starting at 0.0 (std::accumulate over crbegin..crbegin+n),
+ * times the stage multiplier -- of the conservatives (1, 1/4, 2/3) or, with
+ * jump != 0, of the jump conservatives (1/6, 1/6, 2/3)
+ * (time_integrator.h:281-301, 312-340). */
+double ora_lts_timestep(const double *micro, int count, int n, int stage, int jump) {
+  static const double mc[3] = {1.0, 0.25, 2.0 / 3.0};
+  static const double mj[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
+  double s = 0.0;
+  for (int i = 0; i < n; ++i)
+    s = s + micro[count - 1 - i];
+  return s * (jump ? mj[stage] : mc[stage]);
+}
+
+/* IntegrateConservatives / IntegrateHalo (time_integrator.h:98-153):
+ * u_new = u_old + timestep * u_new over the cells of range (x_start,
+ * x_count, y_start, y_count, z_start, z_count) of every equation; range NULL
+ * means the internal cells. */
+void ora_lts_integrate(int dim, const double *u_old, double *u_new, const int *range,
+                       double timestep) {
+  int const ne = dim + 2, tz = dim == 3 ? TC : 1;
+  int const internal[6] = {HS, IC, HS, IC, dim == 3 ? HS : 0, dim == 3 ? IC : 1};
+  const int *r = range ? range : internal;
+  size_t const field = (size_t)TC * TC * tz;
+  for (int e = 0; e < ne; ++e)
+    for (int i = r[0]; i < r[0] + r[1]; ++i)
+      for (int j = r[2]; j < r[2] + r[3]; ++j)
+        for (int k = r[4]; k < r[4] + r[5]; ++k) {
+          size_t const c = (size_t)e * field + fidx(dim, i, j, k);
+          u_new[c] = u_old[c] + timestep * u_new[c];
+        }
+}
+
+/* IntegrateJumpConservatives (time_integrator.h:64-84) on jump surfaces
+ * [6][NE][IC][ICZ]: conservatives += timestep * fluxes, fluxes = 0, on the
+ * 2*dim faces (CC::ANBS()). */
+void ora_lts_integrate_jump(int dim, double *cons, double *flux, double timestep) {
+  size_t const n = (size_t)2 * dim * (dim + 2) * IC * (dim == 3 ? IC : 1);
+  for (size_t c = 0; c < n; ++c) {
+    cons[c] += timestep * flux[c];
+    flux[c] = 0.0;
+  }
+}
+
+/* PrepareBufferForIntegration (time_integrator.h:434-457) for stage 1 or 2:
+ * avg = m0 * avg + m1 * init over every cell, (m0, m1) = (1/4, 3/4) and
+ * (2/3, 1/3) (runge_kutta_3_TVD.h:55-59); stage 0 leaves avg unchanged. */
+void ora_lts_prepare_buffer(int dim, double *avg, const double *init, int stage) {
+  static const double m[2][2] = {{0.25, 0.75}, {2.0 / 3.0, 1.0 / 3.0}};
+  if (stage == 0)
+    return;
+  size_t const n = (size_t)(dim + 2) * TC * TC * (dim == 3 ? TC : 1);
+  for (size_t c = 0; c < n; ++c)
+    avg[c] = m[stage - 1][0] * avg[c] + m[stage - 1][1] * init[c];
+}

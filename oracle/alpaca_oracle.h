@@ -87,6 +87,13 @@ int ora_mr_decision(double detail, double eps_coarsen);
 /* Jump surfaces [6][NE][16][16|1] of a child added into its parent's. */
 void ora_mr_average_jump(int dim, const double *child, double *parent, int position);
 
+/* Level-wise integration (RungeKutta3TVD on a level with n micro steps). */
+double ora_lts_timestep(const double *micro, int count, int n, int stage, int jump);
+void ora_lts_integrate(int dim, const double *u_old, double *u_new, const int *range,
+                       double timestep);
+void ora_lts_integrate_jump(int dim, double *cons, double *flux, double timestep);
+void ora_lts_prepare_buffer(int dim, double *avg, const double *init, int stage);
+
 #ifdef __cplusplus
 }
 #endif

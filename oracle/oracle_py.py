@@ -283,3 +283,45 @@ def mr_average_jump(dim, child, parent, position, which="oracle"):
     else:
         ref(dim).ref_mr_average_jump(_dp(child), _dp(out), position)
     return out
+
+
+# Level-wise integration (RungeKutta3TVD with micro time steps).
+def lts_timestep(micro, n, stage, jump=False):
+    m = np.ascontiguousarray(micro, dtype=np.float64)
+    f = oracle().ora_lts_timestep
+    f.restype = ctypes.c_double
+    return f(_dp(m), len(m), n, stage, int(bool(jump)))
+
+
+def lts_integrate(dim, u_old, u_new, rng, timestep):
+    """u_new with the cells of rng (None: internal cells) integrated."""
+    out = _block(dim, u_new).copy()
+    r = None if rng is None else (ctypes.c_int * 6)(*rng)
+    oracle().ora_lts_integrate(dim, _dp(_block(dim, u_old)), _dp(out), r, ctypes.c_double(timestep))
+    return out
+
+
+def lts_integrate_jump(dim, cons, flux, timestep):
+    c = np.ascontiguousarray(cons, dtype=np.float64).copy()
+    f = np.ascontiguousarray(flux, dtype=np.float64).copy()
+    oracle().ora_lts_integrate_jump(dim, _dp(c), _dp(f), ctypes.c_double(timestep))
+    return c, f
+
+
+def lts_prepare_buffer(dim, avg, init, stage):
+    out = _block(dim, avg).copy()
+    oracle().ora_lts_prepare_buffer(dim, _dp(out), _dp(_block(dim, init)), stage)
+    return out
+
+
+def ref_lts_integrate(dim, op, stage, n, micro, start, size, avg, rhs, init, jcons, jflux):
+    """The reference's RungeKutta3TVD on one Node: op 0 PrepareBufferForIntegration
+    + IntegrateNode, op 1 IntegrateJumpHalos; returns the five buffers."""
+    m = np.ascontiguousarray(micro, dtype=np.float64)
+    bufs = [_block(dim, avg).copy(), _block(dim, rhs).copy(), _block(dim, init).copy(),
+            np.ascontiguousarray(jcons, dtype=np.float64).copy(),
+            np.ascontiguousarray(jflux, dtype=np.float64).copy()]
+    st = (ctypes.c_int * 3)(*start)
+    sz = (ctypes.c_int * 3)(*size)
+    ref(dim).ref_lts_integrate(op, stage, n, _dp(m), len(m), st, sz, *[_dp(b) for b in bufs])
+    return bufs

@@ -627,6 +627,44 @@ void ref_mr_average_jump(double const *child, double *parent, int position) {
   std::memcpy(parent, p.get(), sizeof(SurfaceBuffer));
 }
 
+// TimeIntegrator<RungeKutta3TVD> on one Node holding Average, RHS, Initial
+// and the two jump surfaces (each given as the reference's arrays, read and
+// written): the `count` micro time steps are appended, then
+//   op 0: PrepareBufferForIntegration(stage) + IntegrateNode(stage, n)
+//   op 1: IntegrateJumpHalos(stage, n, start, size)
+void ref_lts_integrate(int op, int stage, int n, double const *micro, int count,
+                       int const *start, int const *size, double *avg, double *rhs,
+                       double *init, double *jump_cons, double *jump_flux) {
+  RungeKutta3TVD ti(0.0);
+  for (int i = 0; i < count; ++i)
+    ti.AppendMicroTimestep(micro[i]);
+  auto node = std::make_unique<Node>(IdSeed(), 1.0,
+                                     std::vector<MaterialName>{MaterialName::MaterialOne});
+  Block &b = node->GetSinglePhase();
+  std::size_t const field = sizeof(double) * TX * TY * TZ;
+  for (unsigned int e = 0; e < NE; ++e) {
+    std::memcpy(&b.GetAverageBuffer()[e][0][0][0], avg + e * TX * TY * TZ, field);
+    std::memcpy(&b.GetRightHandSideBuffer()[e][0][0][0], rhs + e * TX * TY * TZ, field);
+    std::memcpy(&b.GetInitialBuffer()[e][0][0][0], init + e * TX * TY * TZ, field);
+  }
+  std::memcpy(&b.GetBoundaryJumpConservatives(), jump_cons, sizeof(SurfaceBuffer));
+  std::memcpy(&b.GetBoundaryJumpFluxes(), jump_flux, sizeof(SurfaceBuffer));
+  if (op == 0) {
+    ti.PrepareBufferForIntegration(*node, unsigned(stage));
+    ti.IntegrateNode(*node, unsigned(stage), unsigned(n));
+  } else {
+    ti.IntegrateJumpHalos(*node, unsigned(stage), unsigned(n), {start[0], start[1], start[2]},
+                          {size[0], size[1], size[2]});
+  }
+  for (unsigned int e = 0; e < NE; ++e) {
+    std::memcpy(avg + e * TX * TY * TZ, &b.GetAverageBuffer()[e][0][0][0], field);
+    std::memcpy(rhs + e * TX * TY * TZ, &b.GetRightHandSideBuffer()[e][0][0][0], field);
+    std::memcpy(init + e * TX * TY * TZ, &b.GetInitialBuffer()[e][0][0][0], field);
+  }
+  std::memcpy(jump_cons, &b.GetBoundaryJumpConservatives(), sizeof(SurfaceBuffer));
+  std::memcpy(jump_flux, &b.GetBoundaryJumpFluxes(), sizeof(SurfaceBuffer));
+}
+
 // Reference ThresholdComputer::ThresholdOnLevel.
 double ref_mr_threshold(int maximum_level, int reference_level,
                         double reference_epsilon, int level) {

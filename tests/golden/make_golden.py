@@ -284,6 +284,11 @@ def main_ids():
         json.dump(meta, fh, indent=1)
 
 
+# Halo range (start, size) integrated by the IntegrateJumpHalos vectors: the
+# +y halo slab over the internal x (and z) cells.
+LTS_HALO = {2: ((4, 20, 0), (16, 4, 1)), 3: ((4, 20, 4), (16, 4, 16))}
+
+
 def main_mr():
     """mr_golden.json: per dimension, pair kind (tests/mr_states.py) and
     sibling position, the reference's Multiresolution::Average output (the
@@ -293,7 +298,10 @@ def main_mr():
     details of ChildNeedsRemeshing (read off its decisions,
     oracle_py.ref_mr_detail) as hex floats; the reference's AverageJumpBuffer
     on seeded jump surfaces (seed 40 + dim, child then parent) for every
-    sibling position; plus ThresholdOnLevel for a few settings."""
+    sibling position; the reference's RungeKutta3TVD on one Node
+    (PrepareBufferForIntegration + IntegrateNode, IntegrateJumpHalos) with
+    four of eight seeded micro time steps per stage; plus ThresholdOnLevel
+    for a few settings."""
     sys.path.insert(0, os.path.dirname(HERE))
     import edge_states
     import mr_states
@@ -328,6 +336,19 @@ def main_mr():
                 parent = rng.standard_normal((6, dim + 2, 16, icz))
                 out = oracle_py.mr_average_jump(dim, child, parent, pos, which="ref")
                 meta["jump"][f"{dim}d_{pos}"] = hashlib.sha256(out.tobytes()).hexdigest()
+        meta["lts"] = {}
+        for dim in (2, 3):
+            tz, icz = (24, 16) if dim == 3 else (1, 1)
+            for op in (0, 1):
+                for stage in range(3):
+                    rng = np.random.default_rng(1000 * dim + 10 * op + stage)
+                    micro = rng.uniform(1e-4, 1e-3, 8)
+                    bufs = [rng.standard_normal((dim + 2, 24, 24, tz)) for _ in range(3)]
+                    bufs += [rng.standard_normal((6, dim + 2, 16, icz)) for _ in range(2)]
+                    start, size = LTS_HALO[dim]
+                    out = oracle_py.ref_lts_integrate(dim, op, stage, 4, micro, start, size, *bufs)
+                    meta["lts"][f"{dim}d_op{op}_stage{stage}"] = hashlib.sha256(
+                        b"".join(o.tobytes() for o in out)).hexdigest()
         for dim in (2, 3):
             for ml, rl, eps in ((1, 1, 1e-3), (4, 2, 1e-3), (6, 6, 0.01), (3, 0, 0.5)):
                 for level in range(ml + 1):

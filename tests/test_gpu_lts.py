@@ -1,0 +1,89 @@
+"""Level-wise integration on the GPU (alpaca_lts_prepare_buffer,
+alpaca_lts_integrate for internal cells and halo ranges,
+alpaca_lts_integrate_jump_conservatives) through alpaca_amd.lts against the
+oracle, bit for bit: a pool of blocks of which a shuffled list (with an
+out-of-range entry, skipped) is treated, every stage and several micro-step
+counts, blocks off the list untouched."""
+import numpy as np
+import pytest
+
+from alpaca_amd import gpu, lts
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available()
+    gpu.load_library()
+    return _t
+
+
+def _pools(torch, dim, nb, seed):
+    tz, icz = (24, 16) if dim == 3 else (1, 1)
+    rng = np.random.default_rng(seed)
+    host = {k: rng.standard_normal((nb, dim + 2, 24, 24, tz)) for k in ("avg", "rhs", "init")}
+    host.update({k: rng.standard_normal((nb, 6, dim + 2, 16, icz)) for k in ("jc", "jf")})
+    dev = {k: torch.as_tensor(v, device="cuda").contiguous() for k, v in host.items()}
+    return host, dev
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_level_stage_operations_bitexact(torch, dim):
+    nb = 7
+    rng = np.random.default_rng(dim)
+    chosen = [5, 1, 3, 6]
+    lst = torch.as_tensor(np.array(chosen + [nb + 3], dtype=np.int32), device="cuda")
+    micro = rng.uniform(1e-4, 1e-3, 8)
+    for stage in range(3):
+        for n in (1, 4):
+            print(f"[lts] dim={dim} stage={stage} n={n}", flush=True)
+            host, dev = _pools(torch, dim, nb, 10 * stage + n + dim)
+            ti = lts.LevelTimeIntegrator(dim)
+            for dt in micro:
+                ti.append_micro_timestep(dt)
+            ti.prepare_buffer_for_integration(dev["avg"], dev["init"], lst, stage)
+            ti.integrate_node(dev["avg"], dev["rhs"], dev["jc"], dev["jf"], lst, stage, n)
+            start, size = (20, 4, 4 if dim == 3 else 0), (4, 16, 16 if dim == 3 else 1)
+            ti.integrate_jump_halos(dev["avg"], dev["init"], lst, stage, n, start, size)
+            torch.cuda.synchronize()
+            got = {k: v.cpu().numpy() for k, v in dev.items()}
+            k = 2 * dim
+            for b in range(nb):
+                want = {key: host[key][b].copy() for key in host}
+                if b in chosen:
+                    want["avg"] = O.lts_prepare_buffer(dim, want["avg"], want["init"], stage)
+                    c, f = O.lts_integrate_jump(dim, want["jc"][:k], want["jf"][:k],
+                                                O.lts_timestep(micro, n, stage, True))
+                    want["jc"][:k], want["jf"][:k] = c, f
+                    want["rhs"] = O.lts_integrate(dim, want["avg"], want["rhs"], None,
+                                                  O.lts_timestep(micro, n, stage))
+                    rng6 = (start[0], size[0], start[1], size[1], start[2], size[2])
+                    want["init"] = O.lts_integrate(dim, want["avg"], want["init"], rng6,
+                                                   O.lts_timestep(micro, n, stage))
+                for key in host:
+                    assert np.array_equal(got[key][b], want[key]), (b, key)
+
+
+def test_fill_initial_buffer_and_bad_arguments(torch):
+    dim, nb = 3, 3
+    host, dev = _pools(torch, dim, nb, 4)
+    lst = torch.as_tensor(np.array([2, 0], dtype=np.int32), device="cuda")
+    ti = lts.LevelTimeIntegrator(dim)
+    ti.fill_initial_buffer(dev["avg"], dev["init"], lst, 0)
+    torch.cuda.synchronize()
+    init = dev["init"].cpu().numpy()
+    assert np.array_equal(init[0], host["avg"][0]) and np.array_equal(init[2], host["avg"][2])
+    assert np.array_equal(init[1], host["init"][1])
+    lib = gpu.load_library()
+    p = gpu.ctypes.c_void_p(dev["avg"].data_ptr())
+    q = gpu.ctypes.c_void_p(dev["rhs"].data_ptr())
+    l = gpu.ctypes.c_void_p(lst.data_ptr())
+    st = gpu._stream_ptr()
+    bad = (gpu.ctypes.c_int * 6)(20, 8, 0, 24, 0, 24)
+    assert lib.alpaca_lts_integrate(3, p, q, nb, l, 2, bad, gpu.ctypes.c_double(1.0), st) == 1
+    assert lib.alpaca_lts_prepare_buffer(3, p, q, nb, l, 2, 3, st) == 1
+    assert lib.alpaca_lts_integrate_jump_conservatives(3, p, p, nb, l, 2,
+                                                       gpu.ctypes.c_double(1.0), st) == 1
