@@ -90,6 +90,44 @@ __global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__res
   for (int e = 0; e < NE; ++e) o[e * NF] = fl[e];
 }
 
+// ------------------------------------------------------- jump fluxes ---
+// The boundary-flux save of SpaceSolver::UpdateFluxes (space_solver.cpp:
+// 160-182): the fluxes of a block's six boundary faces ADDED into its jump
+// surfaces [6][NE][IC][ICZ] (SurfaceBuffer: East, West, North, South, Top,
+// Bottom).  One thread per surface entry; in 2D the z faces do not exist and
+// Bottom/Top receive += 0.0, as from the reference's zeroed z face array.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void jump_flux_kernel(const double *__restrict__ flux,
+                                                             double *__restrict__ jump,
+                                                             int nblocks) {
+  using D = Dims<DIM>;
+  constexpr int NE = D::NE;
+  constexpr int NF = faces_per_dir<DIM>();
+  constexpr int ICZ = D::ICZ;
+  constexpr int FACE = NE * kIC * ICZ;
+  size_t const t = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= (size_t)nblocks * 6 * FACE) return;
+  int const blk = int(t / (6 * FACE));
+  int const loc = int((t / FACE) % 6);
+  int const r = int(t % FACE);
+  int const e = r / (kIC * ICZ), i = (r / ICZ) % kIC, j = r % ICZ;
+  const double *fx = flux + ((size_t)blk * DIM + 0) * NE * NF + (size_t)e * NF;
+  const double *fy = flux + ((size_t)blk * DIM + 1) * NE * NF + (size_t)e * NF;
+  double v = 0.0;
+  switch (loc) {
+    case 0: v = fx[(kIC * kIC + i) * ICZ + j]; break;        // East: x face IC
+    case 1: v = fx[i * ICZ + j]; break;                        // West: x face 0
+    case 2: v = fy[(i * (kIC + 1) + kIC) * ICZ + j]; break;  // North: y face IC
+    case 3: v = fy[(i * (kIC + 1)) * ICZ + j]; break;          // South: y face 0
+    default:
+      if constexpr (DIM == 3) {
+        const double *fz = flux + ((size_t)blk * DIM + 2) * NE * NF + (size_t)e * NF;
+        v = fz[(i * kIC + j) * (kIC + 1) + (loc == 4 ? kIC : 0)];  // Top / Bottom
+      }
+  }
+  jump[t] += v;
+}
+
 // ------------------------------------------------------------- combine ---
 struct StageCoef {
   double m0, m1, c;
@@ -529,7 +567,8 @@ void launch_faces(const alpaca_case_desc *d, int nblocks, const double *in, doub
 
 template <int DIM>
 int stage_impl(const alpaca_case_desc *d, int nblocks, int stage, int mode, const double *avg,
-               double *init, double *out, alpaca_case_state *state, hipStream_t s) {
+               double *init, double *out, alpaca_case_state *state, hipStream_t s,
+               double *jump = nullptr) {
   double *fl = nullptr;
   int st = workspace(flux_doubles<DIM>(nblocks) * sizeof(double), &fl);
   if (st != ALPACA_OK) return st;
@@ -547,6 +586,11 @@ int stage_impl(const alpaca_case_desc *d, int nblocks, int stage, int mode, cons
   } else {
     hipLaunchKernelGGL((combine_kernel<DIM, 1>), grid, dim3(kThreads), 0, s, fl, avg, nullptr,
                        out, nblocks, 0, inv_dx, state);
+  }
+  if (jump) {
+    size_t const total = (size_t)nblocks * 6 * Dims<DIM>::NE * kIC * Dims<DIM>::ICZ;
+    hipLaunchKernelGGL((jump_flux_kernel<DIM>), dim3((unsigned)((total + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, s, fl, jump, nblocks);
   }
   return hip_status(hipGetLastError());
 }
@@ -749,6 +793,21 @@ int alpaca_right_hand_side(const alpaca_case_desc *desc, int nblocks, const doub
   if (hipMemsetAsync(d_rhs, 0, bytes, s) != hipSuccess) return ALPACA_ERR_HIP;
   if (desc->dim == 2) return stage_impl<2>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s);
   return stage_impl<3>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s);
+}
+
+int alpaca_right_hand_side_jump_fluxes(const alpaca_case_desc *desc, int nblocks,
+                                       const double *d_avg, double *d_rhs, double *d_jump_fluxes,
+                                       void *stream) {
+  int st = validate(desc);
+  if (st != ALPACA_OK) return st;
+  if (nblocks < 1 || nblocks > kMaxBlocks || !d_avg || !d_rhs || !d_jump_fluxes || d_avg == d_rhs)
+    return ALPACA_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  size_t const bytes = alpaca_block_doubles(desc->dim) * sizeof(double) * (size_t)nblocks;
+  if (hipMemsetAsync(d_rhs, 0, bytes, s) != hipSuccess) return ALPACA_ERR_HIP;
+  if (desc->dim == 2)
+    return stage_impl<2>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, d_jump_fluxes);
+  return stage_impl<3>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, d_jump_fluxes);
 }
 
 int alpaca_rk3_stage(const alpaca_case_desc *desc, int stage, const double *d_avg, double *d_init,

@@ -51,6 +51,9 @@ _SIGNATURES = {
                                   ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "alpaca_right_hand_side": ([ctypes.POINTER(CaseDesc), ctypes.c_int, ctypes.c_void_p,
                                 ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_right_hand_side_jump_fluxes": ([ctypes.POINTER(CaseDesc), ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+                                           ctypes.c_int),
     "alpaca_rk3_stage": ([ctypes.POINTER(CaseDesc), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "alpaca_fill_halos_table": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,

@@ -133,6 +133,15 @@ int alpaca_convective_fluxes(const alpaca_case_desc *desc, int nblocks,
 int alpaca_right_hand_side(const alpaca_case_desc *desc, int nblocks,
                            const double *d_avg, double *d_rhs, void *stream);
 
+/* (ii) with the rest of SpaceSolver::UpdateFluxes: also ADDS (+=) the fluxes
+ * of each block's six boundary faces into its jump-flux surfaces
+ * (space_solver.cpp:160-182), d_jump_fluxes = nblocks SurfaceBuffers
+ * ([6][NE][16][16] doubles in 3D, [6][NE][16][1] in 2D: East, West, North,
+ * South, Top, Bottom; in 2D Top/Bottom receive += 0.0). */
+int alpaca_right_hand_side_jump_fluxes(const alpaca_case_desc *desc, int nblocks,
+                                       const double *d_avg, double *d_rhs,
+                                       double *d_jump_fluxes, void *stream);
+
 /* (iii) One SSP-RK3 stage on every block of a case: FillInitialBuffer,
  * UpdateFluxes, PrepareBufferForIntegration and IntegrateNode
  * (time_integrator.h:98-113, 312-457; runge_kutta_3_TVD.h:39-59):

@@ -1259,3 +1259,35 @@ void ora_lts_prepare_buffer(int dim, double *avg, const double *init, int stage)
   for (size_t c = 0; c < n; ++c)
     avg[c] = m[stage - 1][0] * avg[c] + m[stage - 1][1] * init[c];
 }
+
+/* The boundary-flux save of SpaceSolver::UpdateFluxes (space_solver.cpp:
+ * 160-182): the face fluxes of the block's six boundary faces ADDED into its
+ * jump-flux surfaces [6][NE][IC][ICZ] (East, West, North, South, Top,
+ * Bottom): West/East from the x faces 0 and IC, South/North from the y faces,
+ * Bottom/Top from the z faces, at transverse face index (i+1, j+1) of the
+ * reference's face arrays [NE][IC+1][IC+1][ICZ+1].  In 2D the z face arrays
+ * are zero, so Bottom and Top receive += 0.0. */
+void ora_block_jump_fluxes(int dim, int flux, double gamma, double b, const double *avg,
+                           double *jump) {
+  int const ne = dim + 2, icz = dim == 3 ? IC : 1;
+  size_t const n1 = IC + 1, nz = (size_t)icz + 1, per = n1 * n1 * nz;
+  double *f = (double *)calloc(3 * (size_t)ne * per, sizeof(double));
+  block_face_fluxes(dim, flux, gamma, b, avg, f, f + ne * per, f + 2 * ne * per);
+  const double *fx = f, *fy = f + ne * per, *fz = f + 2 * ne * per;
+  size_t const face = (size_t)ne * IC * icz;
+#define FA(a, e, i, j, k) a[(((size_t)(e) * n1 + (i)) * n1 + (j)) * nz + (k)]
+#define JB(loc, e, i, j) jump[(loc) * face + ((size_t)(e) * IC + (i)) * icz + (j)]
+  for (int e = 0; e < ne; ++e)
+    for (int i = 0; i < IC; ++i)
+      for (int j = 0; j < icz; ++j) {
+        JB(1, e, i, j) += FA(fx, e, 0, i + 1, j + 1);  /* West */
+        JB(0, e, i, j) += FA(fx, e, IC, i + 1, j + 1); /* East */
+        JB(3, e, i, j) += FA(fy, e, i + 1, 0, j + 1);  /* South */
+        JB(2, e, i, j) += FA(fy, e, i + 1, IC, j + 1); /* North */
+        JB(5, e, i, j) += FA(fz, e, i + 1, j + 1, 0);  /* Bottom */
+        JB(4, e, i, j) += FA(fz, e, i + 1, j + 1, icz); /* Top */
+      }
+#undef FA
+#undef JB
+  free(f);
+}

@@ -87,6 +87,11 @@ int ora_mr_decision(double detail, double eps_coarsen);
 /* Jump surfaces [6][NE][16][16|1] of a child added into its parent's. */
 void ora_mr_average_jump(int dim, const double *child, double *parent, int position);
 
+/* Boundary face fluxes of one haloed block added into its jump-flux surfaces
+ * [6][NE][16][16|1] (SpaceSolver::UpdateFluxes). */
+void ora_block_jump_fluxes(int dim, int flux, double gamma, double b, const double *avg,
+                           double *jump);
+
 /* Level-wise integration (RungeKutta3TVD on a level with n micro steps). */
 double ora_lts_timestep(const double *micro, int count, int n, int stage, int jump);
 void ora_lts_integrate(int dim, const double *u_old, double *u_new, const int *range,

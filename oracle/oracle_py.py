@@ -325,3 +325,12 @@ def ref_lts_integrate(dim, op, stage, n, micro, start, size, avg, rhs, init, jco
     sz = (ctypes.c_int * 3)(*size)
     ref(dim).ref_lts_integrate(op, stage, n, _dp(m), len(m), st, sz, *[_dp(b) for b in bufs])
     return bufs
+
+
+def block_jump_fluxes(dim, flux, gamma, b, blk, jump):
+    """Jump-flux surfaces [6][NE][16][16|1] plus the block's boundary face
+    fluxes (SpaceSolver::UpdateFluxes' boundary-flux save)."""
+    out = np.ascontiguousarray(jump, dtype=np.float64).copy()
+    oracle().ora_block_jump_fluxes(dim, flux, ctypes.c_double(gamma), ctypes.c_double(b),
+                                   _dp(_block(dim, blk)), _dp(out))
+    return out

@@ -87,3 +87,39 @@ def test_fill_initial_buffer_and_bad_arguments(torch):
     assert lib.alpaca_lts_prepare_buffer(3, p, q, nb, l, 2, 3, st) == 1
     assert lib.alpaca_lts_integrate_jump_conservatives(3, p, p, nb, l, 2,
                                                        gpu.ctypes.c_double(1.0), st) == 1
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("flux", [0, 1])
+def test_right_hand_side_with_jump_fluxes(torch, dim, flux):
+    """alpaca_right_hand_side_jump_fluxes: the per-block right-hand side as
+    alpaca_right_hand_side gives it, and the boundary face fluxes added into
+    the jump-flux surfaces, both bit for bit against the oracle."""
+    from alpaca_amd import cases
+
+    print(f"[rhs+jump] dim={dim} flux={flux}", flush=True)
+    icz = 16 if dim == 3 else 1
+    nb = 3
+    blocks = np.stack([cases.random_block(dim, 40 + b + 3 * flux) for b in range(nb)])
+    rng = np.random.default_rng(flux + dim)
+    jump0 = rng.standard_normal((nb, 6, dim + 2, 16, icz))
+    dev = torch.as_tensor(blocks, device="cuda").contiguous()
+    rhs = torch.empty_like(dev)
+    jump = torch.as_tensor(jump0, device="cuda").contiguous()
+    desc = gpu.make_desc(dim, [1, 1, 1][:dim], [0] * dim, flux=flux, cell_size=1.0 / 64)
+    lib = gpu.load_library()
+    gpu.check(lib.alpaca_right_hand_side_jump_fluxes(gpu.ctypes.byref(desc), nb,
+                                                     gpu.ctypes.c_void_p(dev.data_ptr()),
+                                                     gpu.ctypes.c_void_p(rhs.data_ptr()),
+                                                     gpu.ctypes.c_void_p(jump.data_ptr()),
+                                                     gpu._stream_ptr()), "rhs+jump")
+    torch.cuda.synchronize()
+    got_rhs = rhs.cpu().numpy()
+    got_jump = jump.cpu().numpy()
+    hs, ic = 4, 16
+    zs = slice(hs, hs + ic) if dim == 3 else slice(0, 1)
+    for b in range(nb):
+        want = O.block_rhs(dim, flux, 1.4, 0.0, blocks[b], 1.0 / 64)
+        assert np.array_equal(got_rhs[b][:, hs:hs + ic, hs:hs + ic, zs], want), b
+        assert np.array_equal(got_jump[b], O.block_jump_fluxes(dim, flux, 1.4, 0.0, blocks[b],
+                                                               jump0[b])), b

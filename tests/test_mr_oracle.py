@@ -249,3 +249,36 @@ def test_decision_boundaries_are_inclusive():
         assert f(np.nextafter(32.0 * eps, 0.0), eps) == mr.NEUTRAL
         assert f(0.0, 0.0) == mr.COARSE
         assert f(float("inf"), eps) == mr.REFINE
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("flux", [0, 1])
+def test_boundary_jump_fluxes_telescope(dim, flux):
+    """The saved boundary fluxes are the faces the divergence telescopes to:
+    per equation, the sum of the block's right-hand side times dx equals
+    (West - East) + (South - North) + (Bottom - Top) of the saved fluxes (to
+    rounding), and they are exactly the face arrays' boundary entries."""
+    from alpaca_amd import cases
+
+    icz = 16 if dim == 3 else 1
+    blk = cases.random_block(dim, 31 + flux, smooth=True)
+    dx = 1.0 / 64
+    jump = np.zeros((6, dim + 2, 16, icz))
+    jump[4:] = -0.0
+    got = O.block_jump_fluxes(dim, flux, 1.4, 0.0, blk, jump)
+    rhs = O.block_rhs(dim, flux, 1.4, 0.0, blk, dx)
+    net = (got[1] - got[0]) + (got[3] - got[2]) + (got[5] - got[4])
+    lhs = rhs.reshape(dim + 2, -1).sum(axis=1) * dx
+    assert np.allclose(lhs, net.reshape(dim + 2, -1).sum(axis=1), rtol=1e-11, atol=1e-13)
+    faces = O.block_fluxes(dim, flux, 1.4, 0.0, blk, dx)
+    fx, fy, fz = faces
+    assert np.array_equal(got[1], fx[:, 0, 1:, 1:icz + 1])
+    assert np.array_equal(got[0], fx[:, 16, 1:, 1:icz + 1])
+    assert np.array_equal(got[3], fy[:, 1:, 0, 1:icz + 1])
+    assert np.array_equal(got[2], fy[:, 1:, 16, 1:icz + 1])
+    if dim == 3:
+        assert np.array_equal(got[5], fz[:, 1:, 1:, 0])
+        assert np.array_equal(got[4], fz[:, 1:, 1:, 16])
+    else:
+        # += 0.0 from the zero z faces: -0.0 becomes +0.0
+        assert np.all(got[4:] == 0.0) and not np.any(np.signbit(got[4:]))
