@@ -90,6 +90,100 @@ __global__ __launch_bounds__(kThreads) void prepare_kernel(double *__restrict__ 
   }
 }
 
+// ConsistencyManagedSum of three values (mathematical_functions.h:79-91) with
+// std::min/std::max picks (NaN never chosen by a comparison).
+__device__ __forceinline__ double cms3_std(double a, double b, double c) {
+  double const s1 = (a + b) + c, s2 = (a + c) + b, s3 = (b + c) + a;
+  double mn = s1, mx = s1;
+  mn = (s2 < mn) ? s2 : mn;
+  mn = (s3 < mn) ? s3 : mn;
+  mx = (mx < s2) ? s2 : mx;
+  mx = (mx < s3) ? s3 : mx;
+  return 0.5 * (mn + mx);
+}
+
+// JumpFluxAdjustment's exchange (modular_algorithm_assembler.cpp:1410-1620):
+// one thread per (leaf, equation, internal cell).  entries[l] = {leaf block,
+// neighbour block per face location East, West, North, South, Top, Bottom or
+// -1}: a neighbour that exists and is not a leaf.  Boundary-layer cells take
+// c_L = J_leaf[L] * s * dir_L and f_L = J_nb[opposite L] * s * dir_L (dir -1 on
+// East/North/Top, +1 on West/South/Bottom), and every internal cell becomes
+// (u - CMS(c_E + c_W, c_N + c_S, c_T + c_B)) + CMS(f_E + f_W, f_N + f_S, f_T + f_B).
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void adjust_kernel(double *__restrict__ cells,
+                                                          const double *__restrict__ jump,
+                                                          int nblocks, const int *__restrict__ entries,
+                                                          int n, double one_cell_size) {
+  using D = Dims<DIM>;
+  constexpr int ICZ = D::ICZ;
+  constexpr int FACE = D::NE * kIC * ICZ;
+  constexpr int SURF = 6 * FACE;
+  size_t const total = (size_t)n * D::NE * D::CELLS;
+  for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * kThreads) {
+    const int *en = entries + (t / ((size_t)D::NE * D::CELLS)) * 7;
+    int const leaf = en[0];
+    if (!block_ok(leaf, nblocks)) continue;
+    int const r = int(t % D::CELLS);
+    int const e = int((t / D::CELLS) % D::NE);
+    int const x[3] = {r / (kIC * ICZ), (r / ICZ) % kIC, r % ICZ};
+    double c[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int loc = 0; loc < 2 * DIM; ++loc) {
+      int const nb = en[1 + loc];
+      int const axis = loc / 2;
+      bool const high = (loc % 2) == 0;
+      if (!block_ok(nb, nblocks) || x[axis] != (high ? kIC - 1 : 0)) continue;
+      int const t1 = axis == 0 ? 1 : 0, t2 = axis == 2 ? 1 : 2;
+      int const idx = (e * kIC + x[t1]) * ICZ + (DIM == 3 ? x[t2] : 0);
+      double const dir = high ? -1.0 : 1.0;
+      int const opp = high ? loc + 1 : loc - 1;
+      c[loc] = jump[(size_t)leaf * SURF + loc * FACE + idx] * one_cell_size * dir;
+      f[loc] = jump[(size_t)nb * SURF + opp * FACE + idx] * one_cell_size * dir;
+    }
+    double *u = cells + (size_t)leaf * D::BLOCK + (size_t)e * D::FIELD + (x[0] + kHS) * D::SX +
+                (x[1] + kHS) * D::SY + (DIM == 3 ? x[2] + kHS : 0);
+    double v = *u;
+    v -= cms3_std(c[0] + c[1], c[2] + c[3], c[4] + c[5]);
+    v += cms3_std(f[0] + f[1], f[2] + f[3], f[4] + f[5]);
+    *u = v;
+  }
+}
+
+// The leaf's jump surface at each exchanged face takes the neighbour's
+// opposite one (after adjust_kernel has read it).
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void adjust_copy_kernel(double *__restrict__ jump, int nblocks,
+                                                               const int *__restrict__ entries, int n) {
+  constexpr int FACE = Dims<DIM>::NE * kIC * (DIM == 3 ? kIC : 1);
+  constexpr int SURF = 6 * FACE;
+  size_t const total = (size_t)n * 2 * DIM * FACE;
+  for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * kThreads) {
+    const int *en = entries + (t / ((size_t)2 * DIM * FACE)) * 7;
+    int const loc = int((t / FACE) % (2 * DIM));
+    int const leaf = en[0], nb = en[1 + loc];
+    if (!block_ok(leaf, nblocks) || !block_ok(nb, nblocks)) continue;
+    int const opp = (loc % 2) == 0 ? loc + 1 : loc - 1;
+    jump[(size_t)leaf * SURF + loc * FACE + t % FACE] = jump[(size_t)nb * SURF + opp * FACE + t % FACE];
+  }
+}
+
+// ResetJumpConservatives on the faces CC::ANBS() names.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void reset_jump_kernel(double *__restrict__ jump, int nblocks,
+                                                              const int *__restrict__ list, int n) {
+  constexpr int FACE = Dims<DIM>::NE * kIC * (DIM == 3 ? kIC : 1);
+  constexpr int LIVE = 2 * DIM * FACE;
+  size_t const total = (size_t)n * LIVE;
+  for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * kThreads) {
+    int const b = list[t / LIVE];
+    if (!block_ok(b, nblocks)) continue;
+    jump[(size_t)b * 6 * FACE + t % LIVE] = 0.0;
+  }
+}
+
 int hip_status(hipError_t e) { return e == hipSuccess ? ALPACA_OK : ALPACA_ERR_HIP; }
 
 bool bad_sizes(int dim, int nblocks, int n) {
@@ -185,6 +279,45 @@ int alpaca_lts_prepare_buffer(int dim, double *d_avg, const double *d_init, int 
   else
     hipLaunchKernelGGL((lts::prepare_kernel<3>), lts::grid_for(n * per), dim3(lts::kThreads), 0, s,
                        d_avg, d_init, nblocks, d_list, n, m0, m1);
+  return lts::hip_status(hipGetLastError());
+}
+
+int alpaca_lts_jump_flux_adjustment(int dim, double *d_cells, double *d_jump_cons, int nblocks,
+                                    const int *d_entries, int n, double one_cell_size,
+                                    void *stream) {
+  if (lts::bad_sizes(dim, nblocks, n) || !d_cells || !d_jump_cons || (n > 0 && !d_entries))
+    return ALPACA_ERR_INVALID;
+  if (n == 0) return ALPACA_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  size_t const cells = (size_t)n * (dim + 2) * kIC * kIC * (dim == 3 ? kIC : 1);
+  size_t const faces = (size_t)n * 2 * dim * (dim + 2) * kIC * (dim == 3 ? kIC : 1);
+  if (dim == 2) {
+    hipLaunchKernelGGL((lts::adjust_kernel<2>), lts::grid_for(cells), dim3(lts::kThreads), 0, s,
+                       d_cells, d_jump_cons, nblocks, d_entries, n, one_cell_size);
+    hipLaunchKernelGGL((lts::adjust_copy_kernel<2>), lts::grid_for(faces), dim3(lts::kThreads), 0,
+                       s, d_jump_cons, nblocks, d_entries, n);
+  } else {
+    hipLaunchKernelGGL((lts::adjust_kernel<3>), lts::grid_for(cells), dim3(lts::kThreads), 0, s,
+                       d_cells, d_jump_cons, nblocks, d_entries, n, one_cell_size);
+    hipLaunchKernelGGL((lts::adjust_copy_kernel<3>), lts::grid_for(faces), dim3(lts::kThreads), 0,
+                       s, d_jump_cons, nblocks, d_entries, n);
+  }
+  return lts::hip_status(hipGetLastError());
+}
+
+int alpaca_lts_reset_jump_conservatives(int dim, double *d_jump_cons, int nblocks,
+                                        const int *d_list, int n, void *stream) {
+  if (lts::bad_sizes(dim, nblocks, n) || !d_jump_cons || (n > 0 && !d_list))
+    return ALPACA_ERR_INVALID;
+  if (n == 0) return ALPACA_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  size_t const live = (size_t)n * 2 * dim * (dim + 2) * kIC * (dim == 3 ? kIC : 1);
+  if (dim == 2)
+    hipLaunchKernelGGL((lts::reset_jump_kernel<2>), lts::grid_for(live), dim3(lts::kThreads), 0, s,
+                       d_jump_cons, nblocks, d_list, n);
+  else
+    hipLaunchKernelGGL((lts::reset_jump_kernel<3>), lts::grid_for(live), dim3(lts::kThreads), 0, s,
+                       d_jump_cons, nblocks, d_list, n);
   return lts::hip_status(hipGetLastError());
 }
 

@@ -110,6 +110,12 @@ _SIGNATURES = {
     "alpaca_lts_prepare_buffer": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                    ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
                                   ctypes.c_int),
+    "alpaca_lts_jump_flux_adjustment": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_void_p], ctypes.c_int),
+    "alpaca_lts_reset_jump_conservatives": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
+                                            ctypes.c_int),
     "alpaca_mr_prediction": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                               ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p],
                              ctypes.c_int),

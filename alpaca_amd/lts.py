@@ -132,6 +132,34 @@ class LevelTimeIntegrator:
                   "lts_integrate (halo)")
 
 
+def jump_flux_adjustment(dim, cells, jump_cons, entries, one_cell_size, stream=None):
+    """JumpFluxAdjustment's exchange on one level's leaves; entries is an
+    int32 [n][7] device tensor (leaf, then the non-leaf neighbour per face
+    location East, West, North, South, Top, Bottom or -1)."""
+    torch = gpu._torch()
+    lib = gpu.load_library()
+    if entries.dtype != torch.int32 or not entries.is_contiguous() or entries.dim() != 2 \
+            or entries.shape[1] != 7:
+        raise gpu.AlpacaError("entries must be a contiguous int32 [n][7] device tensor")
+    nb = _pool(cells, dim)
+    gpu.check(lib.alpaca_lts_jump_flux_adjustment(dim, ctypes.c_void_p(cells.data_ptr()),
+                                                  ctypes.c_void_p(jump_cons.data_ptr()), nb,
+                                                  ctypes.c_void_p(entries.data_ptr()),
+                                                  entries.shape[0], float(one_cell_size),
+                                                  _stream(stream)), "lts_jump_flux_adjustment")
+
+
+def reset_jump_conservatives(dim, jump_cons, block_list, stream=None):
+    """ResetJumpConservativeBuffers on the listed blocks."""
+    lib = gpu.load_library()
+    per = 6 * (dim + 2) * gpu.IC * (gpu.IC if dim == 3 else 1)
+    gpu.check(lib.alpaca_lts_reset_jump_conservatives(dim, ctypes.c_void_p(jump_cons.data_ptr()),
+                                                      jump_cons.numel() // per,
+                                                      ctypes.c_void_p(block_list.data_ptr()),
+                                                      _list(block_list), _stream(stream)),
+              "lts_reset_jump_conservatives")
+
+
 def levels_to_update(timestep: int, maximum_level: int) -> list:
     """GetLevels (modular_algorithm_assembler.cpp:2563-2572): the levels that
     advance in micro step `timestep`, in descending order from the finest:

@@ -388,6 +388,30 @@ int alpaca_lts_integrate_jump_conservatives(int dim, double *d_cons, double *d_f
 int alpaca_lts_prepare_buffer(int dim, double *d_avg, const double *d_init, int nblocks,
                               const int *d_list, int n, int stage, void *stream);
 
+/* The exchange of JumpFluxAdjustment (modular_algorithm_assembler.cpp:
+ * 1410-1620) on the leaves of one level (one_cell_size = 1 / their cell
+ * size).  d_entries (device, int32[n][7]): the leaf's pool block, then per
+ * face location East, West, North, South, Top, Bottom the pool block of the
+ * neighbour there when it exists and is not a leaf (its jump conservatives
+ * hold its children's averages), else -1 (2D: the last two are ignored).
+ * Every internal cell of each leaf becomes
+ *   (u - CMS(c_E + c_W, c_N + c_S, c_T + c_B)) + CMS(f_E + f_W, f_N + f_S, f_T + f_B)
+ * with c_L = J_leaf[L] * one_cell_size * dir_L, f_L = J_neighbour[opposite L]
+ * * one_cell_size * dir_L on the boundary layer of a listed face (dir = -1 on
+ * East/North/Top, +1 on West/South/Bottom) and 0 elsewhere; then the leaf's
+ * jump conservatives at each listed face take the neighbour's.  d_cells is
+ * the pool the reference corrects (its right-hand-side buffers); d_jump_cons
+ * the jump-conservative surfaces ([nblocks][6][NE][16][16|1]).  Each leaf at
+ * most once per call; no listed neighbour may be a listed leaf. */
+int alpaca_lts_jump_flux_adjustment(int dim, double *d_cells, double *d_jump_cons, int nblocks,
+                                    const int *d_entries, int n, double one_cell_size,
+                                    void *stream);
+
+/* ResetJumpConservativeBuffers (modular_algorithm_assembler.cpp:2021-2033):
+ * the jump conservatives of the listed blocks (faces of CC::ANBS()) set to 0. */
+int alpaca_lts_reset_jump_conservatives(int dim, double *d_jump_cons, int nblocks,
+                                        const int *d_list, int n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1291,3 +1291,52 @@ void ora_block_jump_fluxes(int dim, int flux, double gamma, double b, const doub
 #undef JB
   free(f);
 }
+
+/* The exchange part of JumpFluxAdjustment (modular_algorithm_assembler.cpp:
+ * 1410-1620) for one leaf: for every face location L (East, West, North,
+ * South, Top, Bottom; 2*dim of them) whose neighbour exists and is not a leaf
+ * (nb[L] non-NULL: that neighbour's jump surfaces, holding its children's
+ * averaged jump conservatives), the boundary layer of internal cells at L
+ * takes the coarse contribution c_L = J_leaf[L] * one_cell_size * dir_L and
+ * the fine one f_L = J_nb[opposite L] * one_cell_size * dir_L (dir = -1 on
+ * East/North/Top, +1 on West/South/Bottom; transverse indices in increasing
+ * axis order), and every internal cell becomes
+ *   (u - CMS(c_E + c_W, c_N + c_S, c_T + c_B)) + CMS(f_E + f_W, f_N + f_S, f_T + f_B)
+ * with zero contributions elsewhere (so every internal cell of the leaf is
+ * rewritten, -0 becoming +0).  The leaf's jump surface at each such L is then
+ * overwritten with the neighbour's (the copy the reference makes before
+ * reading the fine fluxes). */
+void ora_lts_jump_flux_adjustment(int dim, double *cells, double *jump_leaf,
+                                  const double *const *jump_nb, double one_cell_size) {
+  int const ne = dim + 2, icz = dim == 3 ? IC : 1, tz = dim == 3 ? TC : 1;
+  size_t const face = (size_t)ne * IC * icz, field = (size_t)TC * TC * tz;
+  for (int e = 0; e < ne; ++e)
+    for (int i = 0; i < IC; ++i)
+      for (int j = 0; j < IC; ++j)
+        for (int k = 0; k < icz; ++k) {
+          double c[6] = {0, 0, 0, 0, 0, 0}, f[6] = {0, 0, 0, 0, 0, 0};
+          int const x[3] = {i, j, k};
+          for (int loc = 0; loc < 2 * dim; ++loc) {
+            if (!jump_nb[loc])
+              continue;
+            int const axis = loc / 2, high = (loc % 2) == 0;
+            if (x[axis] != (high ? IC - 1 : 0))
+              continue;
+            int const t1 = axis == 0 ? 1 : 0, t2 = axis == 2 ? 1 : 2;
+            int const a = x[t1], b = dim == 3 ? x[t2] : 0;
+            double const dir = high ? -1.0 : 1.0;
+            int const opp = high ? loc + 1 : loc - 1;
+            c[loc] = jump_leaf[loc * face + ((size_t)e * IC + a) * icz + b] * one_cell_size * dir;
+            f[loc] = jump_nb[loc][opp * face + ((size_t)e * IC + a) * icz + b] * one_cell_size * dir;
+          }
+          double *u = cells + (size_t)e * field + fidx(dim, i + HS, j + HS, dim == 3 ? k + HS : 0);
+          *u -= cms3(c[0] + c[1], c[2] + c[3], c[4] + c[5]);
+          *u += cms3(f[0] + f[1], f[2] + f[3], f[4] + f[5]);
+        }
+  for (int loc = 0; loc < 2 * dim; ++loc) {
+    if (!jump_nb[loc])
+      continue;
+    int const opp = (loc % 2) == 0 ? loc + 1 : loc - 1;
+    memcpy(jump_leaf + loc * face, jump_nb[loc] + opp * face, face * sizeof(double));
+  }
+}

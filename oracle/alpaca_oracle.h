@@ -98,6 +98,10 @@ void ora_lts_integrate(int dim, const double *u_old, double *u_new, const int *r
                        double timestep);
 void ora_lts_integrate_jump(int dim, double *cons, double *flux, double timestep);
 void ora_lts_prepare_buffer(int dim, double *avg, const double *init, int stage);
+/* JumpFluxAdjustment's exchange for one leaf block; jump_nb[6]: per face
+ * location the jump surfaces of the non-leaf neighbour there, or NULL. */
+void ora_lts_jump_flux_adjustment(int dim, double *cells, double *jump_leaf,
+                                  const double *const *jump_nb, double one_cell_size);
 
 #ifdef __cplusplus
 }

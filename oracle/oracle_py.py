@@ -334,3 +334,14 @@ def block_jump_fluxes(dim, flux, gamma, b, blk, jump):
     oracle().ora_block_jump_fluxes(dim, flux, ctypes.c_double(gamma), ctypes.c_double(b),
                                    _dp(_block(dim, blk)), _dp(out))
     return out
+
+
+def lts_jump_flux_adjustment(dim, cells, jump_leaf, jump_nb, one_cell_size):
+    """(cells, jump_leaf) after JumpFluxAdjustment's exchange for one leaf;
+    jump_nb: six neighbour surfaces or None per face location."""
+    c = _block(dim, cells).copy()
+    j = np.ascontiguousarray(jump_leaf, dtype=np.float64).copy()
+    keep = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in jump_nb]
+    ptrs = (_P * 6)(*[(_dp(x) if x is not None else _P()) for x in keep])
+    oracle().ora_lts_jump_flux_adjustment(dim, _dp(c), _dp(j), ptrs, ctypes.c_double(one_cell_size))
+    return c, j

@@ -123,3 +123,40 @@ def test_right_hand_side_with_jump_fluxes(torch, dim, flux):
         assert np.array_equal(got_rhs[b][:, hs:hs + ic, hs:hs + ic, zs], want), b
         assert np.array_equal(got_jump[b], O.block_jump_fluxes(dim, flux, 1.4, 0.0, blocks[b],
                                                                jump0[b])), b
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_jump_flux_adjustment_and_reset_bitexact(torch, dim):
+    """JumpFluxAdjustment's exchange on three leaves with different sets of
+    jump faces (a parent neighbour per face, others -1) against the oracle,
+    bit for bit, cells and jump surfaces; then the reset zeroes the listed
+    blocks' faces."""
+    print(f"[lts exchange] dim={dim}", flush=True)
+    tz, icz = (24, 16) if dim == 3 else (1, 1)
+    rng = np.random.default_rng(60 + dim)
+    nb = 6  # blocks 0..2 leaves, 3..5 non-leaf neighbours
+    cells = rng.standard_normal((nb, dim + 2, 24, 24, tz))
+    jump = rng.standard_normal((nb, 6, dim + 2, 16, icz))
+    faces = [[3, -1, 4, -1, 5, -1], [-1, 5, -1, -1, -1, 3], [4, 3, 5, 4, 3, 5]]
+    entries = np.array([[leaf] + faces[leaf] for leaf in range(3)], dtype=np.int32)
+    dc = torch.as_tensor(cells, device="cuda").contiguous()
+    dj = torch.as_tensor(jump, device="cuda").contiguous()
+    s = 48.0
+    lts.jump_flux_adjustment(dim, dc, dj, torch.as_tensor(entries, device="cuda"), s)
+    torch.cuda.synchronize()
+    got_c, got_j = dc.cpu().numpy(), dj.cpu().numpy()
+    for leaf in range(3):
+        nbs = [jump[x] if (x >= 0 and loc < 2 * dim) else None for loc, x in enumerate(faces[leaf])]
+        wc, wj = O.lts_jump_flux_adjustment(dim, cells[leaf], jump[leaf], nbs, s)
+        assert np.array_equal(got_c[leaf], wc), leaf
+        assert np.array_equal(got_j[leaf], wj), leaf
+    for b in range(3, nb):
+        assert np.array_equal(got_c[b], cells[b]) and np.array_equal(got_j[b], jump[b])
+    lts.reset_jump_conservatives(dim, dj, torch.as_tensor(np.array([0, 4], dtype=np.int32),
+                                                          device="cuda"))
+    torch.cuda.synchronize()
+    after = dj.cpu().numpy()
+    for b in (0, 4):
+        assert np.all(after[b, :2 * dim] == 0.0)
+        assert np.array_equal(after[b, 2 * dim:], got_j[b, 2 * dim:])
+    assert np.array_equal(after[1], got_j[1])

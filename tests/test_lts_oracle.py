@@ -121,3 +121,58 @@ def test_level_schedule_over_a_macro_step(maximum_level):
             counts[level] += 1
     assert counts == [2 ** level for level in range(L + 1)]
     assert lts.levels_to_update(2 ** L - 1, L) == list(range(L, -1, -1))
+
+
+def _surf(dim, rng):
+    icz = 16 if dim == 3 else 1
+    return rng.standard_normal((6, dim + 2, 16, icz))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_jump_exchange_closed_forms(dim):
+    """One jump face (West): its boundary layer becomes exactly
+    (u - J_leaf*s) + J_nb[East]*s (CMS of one non-zero term is that term), all
+    other internal cells u - 0 + 0, the halos untouched, and the leaf's West
+    surface takes the neighbour's East one."""
+    rng = np.random.default_rng(5 + dim)
+    tz = 24 if dim == 3 else 1
+    cells = rng.standard_normal((dim + 2, 24, 24, tz))
+    cells[0, 10, 10, 0 if dim == 2 else 10] = -0.0
+    jl, jn = _surf(dim, rng), _surf(dim, rng)
+    s = 64.0
+    nb = [None, jn, None, None, None, None]
+    out, jout = O.lts_jump_flux_adjustment(dim, cells, jl, nb, s)
+    zs = slice(4, 20) if dim == 3 else slice(0, 1)
+    layer = cells[:, 4, 4:20, zs]
+    want = (layer - jl[1] * s * 1.0) + jn[0] * s * 1.0
+    assert np.array_equal(out[:, 4, 4:20, zs], want)
+    inner = (slice(None), slice(5, 20), slice(4, 20), zs)
+    assert np.array_equal(out[inner], (cells[inner] - 0.0) + 0.0)
+    assert not np.signbit(out[0, 10, 10, 0 if dim == 2 else 10])  # -0 became +0
+    halo = np.ones(cells.shape, dtype=bool)
+    halo[:, 4:20, 4:20, zs] = False
+    assert np.array_equal(out[halo], cells[halo])
+    assert np.array_equal(jout[1], jn[0])
+    assert np.array_equal(np.delete(jout, 1, axis=0), np.delete(jl, 1, axis=0))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_jump_exchange_conserves(dim):
+    """Every face a jump: the leaf's internal total changes by the net
+    exchanged flux, sum over faces of (J_nb[opp] - J_leaf) * s * dir (to
+    rounding); a sign or face mix-up breaks it."""
+    rng = np.random.default_rng(11 + dim)
+    tz = 24 if dim == 3 else 1
+    cells = rng.uniform(0.5, 1.5, (dim + 2, 24, 24, tz))
+    jl = _surf(dim, rng)
+    nbs = [_surf(dim, rng) for _ in range(2 * dim)] + [None] * (6 - 2 * dim)
+    s = 32.0
+    out, _ = O.lts_jump_flux_adjustment(dim, cells, jl, nbs, s)
+    zs = slice(4, 20) if dim == 3 else slice(0, 1)
+    delta = (out - cells)[:, 4:20, 4:20, zs].reshape(dim + 2, -1).sum(axis=1)
+    want = np.zeros(dim + 2)
+    for loc in range(2 * dim):
+        d = -1.0 if loc % 2 == 0 else 1.0
+        opp = loc + 1 if loc % 2 == 0 else loc - 1
+        want += ((nbs[loc][opp] - jl[loc]) * s * d).reshape(dim + 2, -1).sum(axis=1)
+    assert np.allclose(delta, want, rtol=1e-12, atol=1e-9)
