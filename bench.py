@@ -319,6 +319,12 @@ MR_BYTES_PER_LINK = {
     "prediction_block": (16 ** 3 + 24 ** 3) * 5 * 8,
     "prediction_internal": (12 ** 3 + 16 ** 3) * 5 * 8,
     "details": (16 ** 3 + 24 ** 3) * 2 * 8,
+    # level-wise integration, per listed block: buffer preparation reads two
+    # and writes one 24^3 x 5 block; IntegrateNode reads and writes the jump
+    # surfaces (cons, flux: read 2, write 2 of 6 x 5 x 16^2) and the internal
+    # cells (read 2, write 1 of 16^3 x 5).
+    "lts_prepare_buffer": 3 * 24 ** 3 * 5 * 8,
+    "lts_integrate_node": (4 * 6 * 5 * 16 ** 2 + 3 * 16 ** 3 * 5) * 8,
 }
 
 
@@ -348,6 +354,21 @@ def run_mr(args, nparents=512):
         "prediction_internal": lambda: m.prediction(blocks, dim, links, (4, 16, 4, 16, 4, 16)),
         "details": lambda: m.child_needs_remeshing(blocks, dim, links, out=out),
     }
+    # Level-wise integration on the 4096 children as one level (LTS stage
+    # operations): jump surfaces for the exchange-free operations.
+    from alpaca_amd import lts as lts_mod
+
+    per_surf = 6 * (dim + 2) * 16 * 16
+    jc = torch.rand(n * per_surf, dtype=torch.float64, device="cuda", generator=g)
+    jf = torch.rand(n * per_surf, dtype=torch.float64, device="cuda", generator=g)
+    init = torch.rand(n * bd, dtype=torch.float64, device="cuda", generator=g)
+    rhs = torch.rand(n * bd, dtype=torch.float64, device="cuda", generator=g)
+    level = torch.as_tensor([r[0] for r in rows], dtype=torch.int32, device="cuda")
+    ti = lts_mod.LevelTimeIntegrator(dim)
+    for dt in (1e-4, 1.1e-4):
+        ti.append_micro_timestep(dt)
+    ops["lts_prepare_buffer"] = lambda: ti.prepare_buffer_for_integration(blocks, init, level, 1)
+    ops["lts_integrate_node"] = lambda: ti.integrate_node(blocks, rhs, jc, jf, level, 1, 2)
     stream = torch.cuda.current_stream()
     reps = max(args.steps, 1)
     res = {}
@@ -368,7 +389,7 @@ def run_mr(args, nparents=512):
         res[name] = {"avg_launch_ms": ms, "links": nl, "bytes_per_link": MR_BYTES_PER_LINK[name],
                      "hbm_GBs": gbs, "frac": gbs / (PEAK_HBM / 1e9)}
     return {"workload": f"3D MR operators, {nparents} parents x 8 children (256^3 on level 1), "
-                        "FP64", "kernels": res}
+                        "FP64; lts_* over the 4096 children as one level", "kernels": res}
 
 
 def run_copy_probe():
