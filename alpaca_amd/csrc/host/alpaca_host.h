@@ -15,6 +15,9 @@
 //   GpuMultiresolution<DIM>       Multiresolution (multiresolution.h:59-188):
 //                                 Average, AverageJumpBuffer, Prediction,
 //                                 ChildNeedsRemeshing, per pair and batched
+//   GpuLevelTimeIntegrator<DIM>   TimeIntegrator<RungeKutta3TVD> with micro
+//                                 time steps on a level's block list
+//                                 (time_integrator.h:240-526)
 //
 // Errors: the C ABI returns status codes; these adapters turn a non-OK status
 // into AlpacaGpuError (the reference path has no error returns and traps FP
@@ -505,6 +508,68 @@ template <int DIM> class GpuMultiresolution {
   GpuThresholdComputer<DIM> thresholder_;
   int norm_;
   hipStream_t stream_;
+};
+
+// TimeIntegrator<RungeKutta3TVD> for the multi-level mode
+// (time_integrator.h:240-526): micro time steps appended on the finest
+// level; a level advancing with n of them integrates with their sum (newest
+// first, from 0.0) times the stage multiplier.  The per-node calls take a
+// device list of the level's pool blocks and run one launch each.
+template <int DIM> class GpuLevelTimeIntegrator {
+ public:
+  explicit GpuLevelTimeIntegrator(double start_time = 0.0, hipStream_t stream = nullptr)
+      : current_run_time_(start_time), stream_(stream) {}
+  static constexpr unsigned int NumberOfStages() { return 3; }
+  static constexpr bool IsLastStage(unsigned int stage) { return stage == 2; }
+  void AppendMicroTimestep(double dt) { micro_.push_back(dt); }
+  std::vector<double> const &MicroTimestepSizes() const { return micro_; }
+  double CurrentRunTime() const { return current_run_time_; }
+  void FinishMacroTimestep() {
+    double s = 0.0;
+    for (double dt : micro_) s += dt;
+    current_run_time_ += s;
+    micro_.clear();
+  }
+  double Timestep(unsigned int stage, unsigned int n, bool jump = false) const {
+    double out = 0.0;
+    Check(alpaca_lts_timestep(micro_.data(), int(micro_.size()), int(n), int(stage), jump ? 1 : 0, &out),
+          "alpaca_lts_timestep");
+    return out;
+  }
+  // PrepareBufferForIntegration on the listed blocks.
+  void PrepareBufferForIntegration(double *d_avg, double const *d_init, int nblocks, int const *d_list,
+                                   int n, unsigned int stage) const {
+    Check(alpaca_lts_prepare_buffer(DIM, d_avg, d_init, nblocks, d_list, n, int(stage), stream_),
+          "alpaca_lts_prepare_buffer");
+  }
+  // IntegrateNode: jump conservatives (jump multiplier), then the internal
+  // cells rhs = avg + timestep * rhs.
+  void IntegrateNode(double const *d_avg, double *d_rhs, double *d_jump_cons, double *d_jump_flux,
+                     int nblocks, int const *d_list, int n, unsigned int stage,
+                     unsigned int number_of_timesteps) const {
+    if (d_jump_cons)
+      Check(alpaca_lts_integrate_jump_conservatives(DIM, d_jump_cons, d_jump_flux, nblocks, d_list, n,
+                                                    Timestep(stage, number_of_timesteps, true), stream_),
+            "alpaca_lts_integrate_jump_conservatives");
+    Check(alpaca_lts_integrate(DIM, d_avg, d_rhs, nblocks, d_list, n, nullptr,
+                               Timestep(stage, number_of_timesteps), stream_),
+          "alpaca_lts_integrate");
+  }
+  // IntegrateJumpHalos: the halo range (start, size per axis) of the listed
+  // blocks, with the conservatives' time step.
+  void IntegrateJumpHalos(double const *d_avg, double *d_rhs, int nblocks, int const *d_list, int n,
+                          unsigned int stage, unsigned int number_of_timesteps,
+                          int const start[3], int const size[3]) const {
+    int const range[6] = {start[0], size[0], start[1], size[1], start[2], size[2]};
+    Check(alpaca_lts_integrate(DIM, d_avg, d_rhs, nblocks, d_list, n, range,
+                               Timestep(stage, number_of_timesteps), stream_),
+          "alpaca_lts_integrate (halo)");
+  }
+
+ private:
+  double current_run_time_;
+  hipStream_t stream_;
+  std::vector<double> micro_;
 };
 
 }  // namespace alpaca_host

@@ -14,6 +14,12 @@
 //       its internal cells (every field), the detail and the decision (as a
 //       float64) under GpuThresholdComputer(1, 1, epsilon), and the parent's
 //       jump surfaces after AverageJumpBuffer.
+//   alpaca_selftest lts <in.bin> <out.bin>
+//       GpuLevelTimeIntegrator on one block: in = int32 dim, stage, n,
+//       count; float64 micro[count]; avg, rhs, init blocks, jump
+//       conservatives and fluxes.  out = avg, rhs, jump conservatives, jump
+//       fluxes after PrepareBufferForIntegration + IntegrateNode, then the
+//       rhs after IntegrateJumpHalos over the -x halo slab.
 //
 // Binary header (little endian): int32 dim, nb[3], bc[3], flux, precision,
 // max_steps; float64 gamma, background_pressure, cell_size, cfl, end_time.
@@ -129,14 +135,70 @@ template <int DIM> int Mr(MrHeader const &h, std::vector<double> const &data, ch
   return 0;
 }
 
+struct LtsHeader {
+  int32_t dim, stage, n, count;
+};
+
+template <int DIM> int Lts(LtsHeader const &h, std::vector<double> const &data, char const *out_path) {
+  using G = alpaca_host::Geometry<DIM>;
+  std::size_t const nb = G::kBlockDoubles;
+  std::size_t const ns = std::size_t(6) * G::NE * G::IC * G::ICZ;
+  if (h.count < 0 || data.size() != std::size_t(h.count) + 3 * nb + 2 * ns) {
+    std::cerr << "lts input size mismatch\n";
+    return 2;
+  }
+  alpaca_host::GpuLevelTimeIntegrator<DIM> ti;
+  for (int i = 0; i < h.count; ++i) ti.AppendMicroTimestep(data[i]);
+  double const *p = data.data() + h.count;
+  std::size_t const bytes = sizeof(double) * (3 * nb + 2 * ns);
+  alpaca_host::DeviceBuffer d(bytes), list(sizeof(int));
+  alpaca_host::CheckHip(hipMemcpy(d.get(), p, bytes, hipMemcpyHostToDevice), "upload");
+  int const zero = 0;
+  alpaca_host::CheckHip(hipMemcpy(list.get(), &zero, sizeof(int), hipMemcpyHostToDevice), "upload");
+  double *avg = d.get<double>(), *rhs = avg + nb, *init = rhs + nb, *jc = init + nb, *jf = jc + ns;
+  int const *l = list.get<int>();
+  ti.PrepareBufferForIntegration(avg, init, 1, l, 1, unsigned(h.stage));
+  ti.IntegrateNode(avg, rhs, jc, jf, 1, l, 1, unsigned(h.stage), unsigned(h.n));
+  std::vector<double> out(3 * nb + 2 * ns);
+  alpaca_host::CheckHip(hipMemcpy(out.data(), d.get(), bytes, hipMemcpyDeviceToHost), "download");
+  int const start[3] = {0, G::HS, DIM == 3 ? G::HS : 0}, size[3] = {G::HS, G::IC, G::ICZ};
+  ti.IntegrateJumpHalos(avg, rhs, 1, l, 1, unsigned(h.stage), unsigned(h.n), start, size);
+  std::vector<double> rhs2(nb);
+  alpaca_host::CheckHip(hipMemcpy(rhs2.data(), rhs, nb * sizeof(double), hipMemcpyDeviceToHost),
+                        "download");
+  std::ofstream o(out_path, std::ios::binary);
+  o.write(reinterpret_cast<char const *>(out.data()), nb * sizeof(double));           // avg
+  o.write(reinterpret_cast<char const *>(out.data() + nb), nb * sizeof(double));      // rhs
+  o.write(reinterpret_cast<char const *>(out.data() + 3 * nb), 2 * ns * sizeof(double));  // jc, jf
+  o.write(reinterpret_cast<char const *>(rhs2.data()), nb * sizeof(double));
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
   if (argc != 4) {
-    std::cerr << "usage: alpaca_selftest run|fluxes|mr <in.bin> <out.bin>\n";
+    std::cerr << "usage: alpaca_selftest run|fluxes|mr|lts <in.bin> <out.bin>\n";
     return 2;
   }
   std::ifstream in(argv[2], std::ios::binary);
+  if (std::strcmp(argv[1], "lts") == 0) {
+    LtsHeader m{};
+    if (!in.read(reinterpret_cast<char *>(&m), sizeof(m))) {
+      std::cerr << "cannot read header\n";
+      return 2;
+    }
+    std::vector<double> const data = ReadRest(in);
+    try {
+      if (m.dim == 2) return Lts<2>(m, data, argv[3]);
+      if (m.dim == 3) return Lts<3>(m, data, argv[3]);
+      std::cerr << "bad dim\n";
+      return 2;
+    } catch (alpaca_host::AlpacaGpuError const &e) {
+      std::cerr << "alpaca error: " << e.what() << "\n";
+      return 3;
+    }
+  }
   if (std::strcmp(argv[1], "mr") == 0) {
     MrHeader m{};
     if (!in.read(reinterpret_cast<char *>(&m), sizeof(m))) {

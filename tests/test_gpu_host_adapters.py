@@ -112,3 +112,38 @@ def test_multiresolution_adapter(tmp_path, dim, norm):
     assert abs(detail - want) <= 1e-13 * want
     assert decision == oracle_py.mr_decision(want, eps)
     assert np.array_equal(surf, oracle_py.mr_average_jump(dim, cs, ps, pos).ravel())
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("stage", [0, 1, 2])
+def test_level_time_integrator_adapter(tmp_path, dim, stage):
+    """GpuLevelTimeIntegrator (C++): PrepareBufferForIntegration,
+    IntegrateNode and IntegrateJumpHalos on one block with summed micro time
+    steps, against the oracle's restatement of the reference's
+    RungeKutta3TVD."""
+    print(f"[host lts] dim={dim} stage={stage}", flush=True)
+    tz, icz = (24, 16) if dim == 3 else (1, 1)
+    rng = np.random.default_rng(17 * dim + stage)
+    micro = rng.uniform(1e-4, 1e-3, 4)
+    n = 2
+    avg, rhs, init = (rng.standard_normal((dim + 2, 24, 24, tz)) for _ in range(3))
+    jc, jf = (rng.standard_normal((6, dim + 2, 16, icz)) for _ in range(2))
+    raw = _run("lts", struct.pack("<4i", dim, stage, n, len(micro)) + micro.tobytes() +
+               avg.tobytes() + rhs.tobytes() + init.tobytes() + jc.tobytes() + jf.tobytes(),
+               tmp_path)
+    got = np.frombuffer(raw, dtype=np.float64)
+    nb, ns = avg.size, jc.size
+    g_avg, g_rhs = got[:nb], got[nb:2 * nb]
+    g_jc, g_jf, g_rhs2 = got[2 * nb:2 * nb + ns], got[2 * nb + ns:2 * nb + 2 * ns], got[2 * nb + 2 * ns:]
+    w_avg = oracle_py.lts_prepare_buffer(dim, avg, init, stage)
+    w_rhs = oracle_py.lts_integrate(dim, w_avg, rhs, None, oracle_py.lts_timestep(micro, n, stage))
+    k = 2 * dim
+    w_jc, w_jf = jc.copy(), jf.copy()
+    c, f = oracle_py.lts_integrate_jump(dim, jc[:k], jf[:k], oracle_py.lts_timestep(micro, n, stage, True))
+    w_jc[:k], w_jf[:k] = c, f
+    halo = (0, 4, 4, 16, 4 if dim == 3 else 0, 16 if dim == 3 else 1)
+    w_rhs2 = oracle_py.lts_integrate(dim, w_avg, w_rhs, halo, oracle_py.lts_timestep(micro, n, stage))
+    assert np.array_equal(g_avg, w_avg.ravel())
+    assert np.array_equal(g_rhs, w_rhs.ravel())
+    assert np.array_equal(g_jc, w_jc.ravel()) and np.array_equal(g_jf, w_jf.ravel())
+    assert np.array_equal(g_rhs2, w_rhs2.ravel())
