@@ -32,12 +32,20 @@ struct Box {
 __device__ __forceinline__ bool block_ok(int b, int nblocks) { return b >= 0 && b < nblocks; }
 
 // u_old may be u_new (no restrict): each element is read, then written.
-template <int DIM>
+// INTERNAL: the box is the internal cells (its extents known at compile time).
+template <int DIM, bool INTERNAL>
 __global__ __launch_bounds__(kThreads) void integrate_kernel(const double *u_old,
                                                              double *u_new, int nblocks,
                                                              const int *__restrict__ list, int n,
                                                              Box box, double timestep) {
   using D = Dims<DIM>;
+  if constexpr (INTERNAL) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      box.start[a] = a < DIM ? kHS : 0;
+      box.count[a] = a < DIM ? kIC : 1;
+    }
+  }
   int const cells = box.count[0] * box.count[1] * box.count[2];
   size_t const total = (size_t)n * D::NE * cells;
   for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < total;
@@ -235,12 +243,22 @@ int alpaca_lts_integrate(int dim, const double *d_old, double *d_new, int nblock
   if (n == 0 || cells == 0) return ALPACA_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   size_t const total = (size_t)n * (dim + 2) * cells;
-  if (dim == 2)
-    hipLaunchKernelGGL((lts::integrate_kernel<2>), lts::grid_for(total), dim3(lts::kThreads), 0, s,
-                       d_old, d_new, nblocks, d_list, n, box, timestep);
-  else
-    hipLaunchKernelGGL((lts::integrate_kernel<3>), lts::grid_for(total), dim3(lts::kThreads), 0, s,
-                       d_old, d_new, nblocks, d_list, n, box, timestep);
+  dim3 const grid = lts::grid_for(total), block(lts::kThreads);
+  if (dim == 2) {
+    if (range)
+      hipLaunchKernelGGL((lts::integrate_kernel<2, false>), grid, block, 0, s, d_old, d_new, nblocks,
+                         d_list, n, box, timestep);
+    else
+      hipLaunchKernelGGL((lts::integrate_kernel<2, true>), grid, block, 0, s, d_old, d_new, nblocks,
+                         d_list, n, box, timestep);
+  } else {
+    if (range)
+      hipLaunchKernelGGL((lts::integrate_kernel<3, false>), grid, block, 0, s, d_old, d_new, nblocks,
+                         d_list, n, box, timestep);
+    else
+      hipLaunchKernelGGL((lts::integrate_kernel<3, true>), grid, block, 0, s, d_old, d_new, nblocks,
+                         d_list, n, box, timestep);
+  }
   return lts::hip_status(hipGetLastError());
 }
 
