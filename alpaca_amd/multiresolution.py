@@ -40,11 +40,12 @@ class ThresholdComputer:
     ThresholdOnLevel(l) = eps_ref * 2^(-dim (maximum_level - l))."""
 
     def __init__(self, dim: int, maximum_level: int, user_reference_level: int,
-                 user_reference_epsilon: float):
+                 user_reference_epsilon: float, convergence_order: int = STDO):
         self.dim = int(dim)
         self.maximum_level = int(maximum_level)
         self.reference_epsilon = float(user_reference_epsilon) * 2.0 ** (
-            -1.0 * (STDO + 1.0) * (float(maximum_level) - float(user_reference_level)))
+            -1.0 * (float(convergence_order) + 1.0) *
+            (float(maximum_level) - float(user_reference_level)))
 
     def threshold_on_level(self, level: int) -> float:
         return self.reference_epsilon * 2.0 ** (

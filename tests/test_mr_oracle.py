@@ -282,3 +282,24 @@ def test_boundary_jump_fluxes_telescope(dim, flux):
     else:
         # += 0.0 from the zero z faces: -0.0 becomes +0.0
         assert np.all(got[4:] == 0.0) and not np.any(np.signbit(got[4:]))
+
+
+# The reference's own unit test of ThresholdComputer (test/multiresolution/
+# test_threshold_computer.cpp:75-181): (dimension, convergence order, maximum
+# level, reference level, user epsilon, internal reference epsilon), and
+# ThresholdOnLevel(l) = eps_ref * 2^(dim * (l - lmax)) for l = 0..9.
+THRESHOLD_KNOWN_ANSWERS = [
+    (1, 0, 6, 6, 1.0, 1.0),
+    (1, 0, 6, 4, 0.5, 0.125),
+    (1, 0, 6, 8, 1.0, 4.0),
+    (2, 2, 7, 5, 1.0, 0.015625),
+    (3, 5, 7, 5, 1.0, 0.000244140625),
+]
+
+
+@pytest.mark.parametrize("dim,order,lmax,lref,eps,eps_ref", THRESHOLD_KNOWN_ANSWERS)
+def test_threshold_known_answers_of_the_reference_tests(dim, order, lmax, lref, eps, eps_ref):
+    th = mr.ThresholdComputer(dim, lmax, lref, eps, convergence_order=order)
+    assert th.reference_epsilon == eps_ref
+    for level in range(10):
+        assert th.threshold_on_level(level) == 2.0 ** (dim * (level - lmax)) * eps_ref
