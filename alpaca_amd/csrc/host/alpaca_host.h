@@ -410,6 +410,13 @@ template <int DIM> class GpuMultiresolution {
     Check(alpaca_mr_prediction(DIM, d_pool, nblocks, d_links, n, range, stream_),
           "alpaca_mr_prediction");
   }
+  // InternalHaloManager::UpdateMaterialJumpNoMpi for a batch of (child,
+  // BoundaryLocation) pairs: d_locations (device, n ints) in the enum's order.
+  void JumpHaloUpdateLinks(double *d_pool, int nblocks, alpaca_mr_link const *d_links,
+                           int const *d_locations, int n) const {
+    Check(alpaca_mr_jump_halo_update(DIM, d_pool, nblocks, d_links, d_locations, n, stream_),
+          "alpaca_mr_jump_halo_update");
+  }
   // Details (device, n doubles) and RemeshIdentifier values (device, n ints)
   // on the thresholds of levels 0..MaximumLevel.
   void ChildNeedsRemeshingLinks(double const *d_pool, int nblocks, alpaca_mr_link const *d_links,

@@ -127,6 +127,32 @@ template <int DIM> int Mr(MrHeader const &h, std::vector<double> const &data, ch
   std::vector<double> surf(data.begin() + 2 * nb, data.begin() + 2 * nb + ns);
   mr.AverageJumpBuffer(data.data() + 2 * nb + ns, surf.data(), child_id);
   append(surf.data(), ns);
+  // Jump halos: the child's halo slabs at every location of the dimension,
+  // predicted from the parent in one batched call.
+  {
+    std::vector<alpaca_mr_link> links;
+    std::vector<int> locs;
+    for (int loc = 0; loc < 26; ++loc) {
+      bool const z = (loc >= 4 && loc <= 13) || loc >= 18;  // a Top/Bottom part
+      if (DIM == 2 && z) continue;
+      links.push_back(alpaca_mr_link{1, 0, int(h.position), 1});
+      locs.push_back(loc);
+    }
+    alpaca_host::DeviceBuffer pool(2 * nb * sizeof(double)),
+        dl(links.size() * sizeof(alpaca_mr_link)), dloc(locs.size() * sizeof(int));
+    alpaca_host::CheckHip(hipMemcpy(pool.get(), parent, nb * sizeof(double), hipMemcpyHostToDevice),
+                          "upload");
+    alpaca_host::CheckHip(hipMemcpy(pool.get<double>() + nb, data.data() + nb, nb * sizeof(double),
+                                    hipMemcpyHostToDevice), "upload");
+    alpaca_host::CheckHip(hipMemcpy(dl.get(), links.data(), dl.bytes(), hipMemcpyHostToDevice), "upload");
+    alpaca_host::CheckHip(hipMemcpy(dloc.get(), locs.data(), dloc.bytes(), hipMemcpyHostToDevice),
+                          "upload");
+    mr.JumpHaloUpdateLinks(pool.get<double>(), 2, dl.get<alpaca_mr_link>(), dloc.get<int>(),
+                           int(locs.size()));
+    alpaca_host::CheckHip(hipMemcpy(child + 3, pool.get<double>() + nb, nb * sizeof(double),
+                                    hipMemcpyDeviceToHost), "download");
+    append(child + 3, nb);
+  }
   std::ofstream o(out_path, std::ios::binary);
   o.write(reinterpret_cast<char const *>(out.data()), out.size() * sizeof(double));
   delete[] parent;

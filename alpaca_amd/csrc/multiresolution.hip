@@ -314,6 +314,30 @@ template <int DIM> __device__ __forceinline__ void load_children(const double *q
 // One workgroup per (link, equation): the child cells of the range,
 // predicted from the parent's field.  FULL: the range is the whole block
 // (12 predicted parent cells per axis, known at compile time).
+// Predicts range r of child l.child's equation e from its parent: stages the
+// parent window in s, then two children per axis per task.  The caller's
+// early exits are uniform over the workgroup, so every thread reaches the
+// barrier.
+template <int DIM>
+__device__ __forceinline__ void predict_range(double *s, double *__restrict__ blocks,
+                                              const alpaca_mr_link &l, int e, const Range &r) {
+  using D = Dims<DIM>;
+  Window<DIM> const win(r, l.position);
+  stage_window<DIM>(s, blocks + (size_t)l.parent * D::BLOCK + (size_t)e * D::FIELD, win);
+  __syncthreads();
+  double *child = blocks + (size_t)l.child * D::BLOCK + (size_t)e * D::FIELD;
+  int const nz = DIM == 3 ? r.count[2] : 1;
+  int const tasks = r.count[0] * r.count[1] * nz;
+  for (int t = threadIdx.x; t < tasks; t += kStencilThreads) {
+    int const a = t / (r.count[1] * nz), b = (t / nz) % r.count[1], c = t % nz;
+    double v[1 << DIM];
+    predict_cell<DIM>(s, Window<DIM>::centre(a, b, c), v);
+    int const ci = r.start[0] + 2 * a, cj = r.start[1] + 2 * b;
+    int const ck = DIM == 3 ? r.start[2] + 2 * c : 0;
+    store_children<DIM>(child + ci * D::SX + cj * D::SY + ck, v);
+  }
+}
+
 template <int DIM, bool FULL>
 __global__ __launch_bounds__(kStencilThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void prediction_kernel(
     double *__restrict__ blocks, int nblocks, const alpaca_mr_link *__restrict__ links, Range r) {
@@ -329,20 +353,55 @@ __global__ __launch_bounds__(kStencilThreads) __attribute__((amdgpu_waves_per_eu
   int const e = blockIdx.x % D::NE;
   alpaca_mr_link const l = links[blockIdx.x / D::NE];
   if (!link_ok<DIM>(l, nblocks)) return;
-  Window<DIM> const win(r, l.position);
-  stage_window<DIM>(s, blocks + (size_t)l.parent * D::BLOCK + (size_t)e * D::FIELD, win);
-  __syncthreads();
-  double *child = blocks + (size_t)l.child * D::BLOCK + (size_t)e * D::FIELD;
-  int const nz = DIM == 3 ? r.count[2] : 1;
-  int const tasks = r.count[0] * r.count[1] * nz;
-  for (int t = threadIdx.x; t < tasks; t += kStencilThreads) {
-    int const a = t / (r.count[1] * nz), b = (t / nz) % r.count[1], c = t % nz;
-    double v[1 << DIM];
-    predict_cell<DIM>(s, Window<DIM>::centre(a, b, c), v);
-    int const ci = r.start[0] + 2 * a, cj = r.start[1] + 2 * b;
-    int const ck = DIM == 3 ? r.start[2] + 2 * c : 0;
-    store_children<DIM>(child + ci * D::SX + cj * D::SY + ck, v);
+  predict_range<DIM>(s, blocks, l, e, r);
+}
+
+// Per-axis direction of each BoundaryLocation, in the enum's order
+// (boundary_specifications.h:56-86): +1 East/North/Top, -1 West/South/Bottom,
+// 0 along an axis the location does not name.
+__constant__ signed char kLocationDir[26][3] = {
+    {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},    {0, 0, -1},  {0, 1, -1},
+    {0, -1, -1}, {0, 1, 1},   {0, -1, 1},  {1, 0, -1},  {-1, 0, -1},  {1, 0, 1},   {-1, 0, 1},
+    {1, 1, 0},   {-1, 1, 0},  {1, -1, 0},  {-1, -1, 0}, {1, 1, 1},    {1, 1, -1},  {1, -1, 1},
+    {1, -1, -1}, {-1, 1, 1},  {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+
+// The halo slab of a location (CommunicationTypes::GetStartIndicesHaloRecv and
+// GetHaloSize, communication_types.h:76-132,190-197): per axis, direction +1
+// gives the cells [IC+HS, TC), -1 gives [0, HS), 0 the internal cells
+// [HS, HS+IC) (in 2D the single z cell).  False for a location outside the
+// enum or, in 2D, one with a Top/Bottom part.
+template <int DIM> __device__ __forceinline__ bool location_range(int loc, Range &r) {
+  if (loc < 0 || loc >= 26) return false;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int const d = kLocationDir[loc][a];
+    if (DIM == 2 && a == 2) {
+      if (d != 0) return false;
+      r.start[2] = 0;
+      r.count[2] = 1;
+    } else {
+      r.start[a] = d > 0 ? kIC + kHS : (d < 0 ? 0 : kHS);
+      r.count[a] = (d != 0 ? kHS : kIC) / 2;
+    }
   }
+  return true;
+}
+
+// UpdateMaterialJumpNoMpi (internal_halo_manager.cpp:231-262) for a batch:
+// link i's child gets the halo slab of location locations[i] predicted from
+// its parent, one workgroup per (link, equation).
+template <int DIM>
+__global__ __launch_bounds__(kStencilThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void jump_halo_kernel(
+    double *__restrict__ blocks, int nblocks, const alpaca_mr_link *__restrict__ links,
+    const int *__restrict__ locations) {
+  using D = Dims<DIM>;
+  __shared__ double s[Lds<DIM>::SIZE];
+  int const e = blockIdx.x % D::NE;
+  int const i = blockIdx.x / D::NE;
+  alpaca_mr_link const l = links[i];
+  Range r;
+  if (!link_ok<DIM>(l, nblocks) || !location_range<DIM>(locations[i], r)) return;
+  predict_range<DIM>(s, blocks, l, e, r);
 }
 
 // ------------------------------------------------------------------ details
@@ -499,6 +558,14 @@ template <int DIM> int prediction_impl(double *blocks, int nblocks, const alpaca
   return hip_status(hipGetLastError());
 }
 
+template <int DIM> int jump_halo_impl(double *blocks, int nblocks, const alpaca_mr_link *links,
+                                      const int *locations, int nlinks, hipStream_t s) {
+  dim3 grid((unsigned)nlinks * Dims<DIM>::NE);
+  hipLaunchKernelGGL((jump_halo_kernel<DIM>), grid, dim3(kStencilThreads), 0, s, blocks, nblocks,
+                     links, locations);
+  return hip_status(hipGetLastError());
+}
+
 template <int DIM> int details_impl(const double *blocks, int nblocks, const alpaca_mr_link *links,
                                     int nlinks, int norm, const Thresholds &th, double *details,
                                     int *decisions, hipStream_t s) {
@@ -566,6 +633,19 @@ int alpaca_mr_prediction(int dim, double *d_blocks, int nblocks, const alpaca_mr
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (dim == 2) return mr::prediction_impl<2>(d_blocks, nblocks, d_links, nlinks, r, s);
   return mr::prediction_impl<3>(d_blocks, nblocks, d_links, nlinks, r, s);
+}
+
+int alpaca_mr_jump_halo_update(int dim, double *d_blocks, int nblocks,
+                               const alpaca_mr_link *d_links, const int *d_locations, int nlinks,
+                               void *stream) {
+  if (mr::bad_sizes(dim, nblocks, nlinks) || !d_blocks ||
+      (nlinks > 0 && (!d_links || !d_locations)))
+    return ALPACA_ERR_INVALID;
+  if (reinterpret_cast<uintptr_t>(d_blocks) % 16) return ALPACA_ERR_INVALID;  // 16-byte accesses
+  if (nlinks == 0) return ALPACA_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dim == 2) return mr::jump_halo_impl<2>(d_blocks, nblocks, d_links, d_locations, nlinks, s);
+  return mr::jump_halo_impl<3>(d_blocks, nblocks, d_links, d_locations, nlinks, s);
 }
 
 int alpaca_mr_child_needs_remeshing(int dim, const double *d_blocks, int nblocks,

@@ -119,6 +119,8 @@ _SIGNATURES = {
     "alpaca_mr_prediction": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                               ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p],
                              ctypes.c_int),
+    "alpaca_mr_jump_halo_update": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "alpaca_mr_child_needs_remeshing": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_double), ctypes.c_int,

@@ -65,6 +65,41 @@ def remeshing_decision(detail: float, eps_coarsen: float) -> int:
     return NEUTRAL
 
 
+# BoundaryLocation (boundary_specifications.h:56-86) in enum order, with the
+# per-axis direction each name gives: East/North/Top +1, West/South/Bottom -1.
+LOCATIONS = ("East", "West", "North", "South", "Top", "Bottom",
+             "BottomNorth", "BottomSouth", "TopNorth", "TopSouth",
+             "BottomEast", "BottomWest", "TopEast", "TopWest",
+             "NorthEast", "NorthWest", "SouthEast", "SouthWest",
+             "EastNorthTop", "EastNorthBottom", "EastSouthTop", "EastSouthBottom",
+             "WestNorthTop", "WestNorthBottom", "WestSouthTop", "WestSouthBottom")
+_AXIS_WORDS = (("East", "West"), ("North", "South"), ("Top", "Bottom"))
+
+
+def location_direction(location: int) -> tuple:
+    """Per-axis direction (+1, -1 or 0) of a BoundaryLocation, from its name."""
+    name = LOCATIONS[location]
+    return tuple(1 if hi in name else (-1 if lo in name else 0) for hi, lo in _AXIS_WORDS)
+
+
+def halo_recv_range(dim: int, location: int) -> tuple:
+    """(x_start, x_count, y_start, y_count[, z_start, z_count]) of the halo slab
+    at a location: GetStartIndicesHaloRecv and GetHaloSize
+    (communication_types.h:76-132,190-197)."""
+    d = location_direction(location)
+    if dim == 2 and d[2] != 0:
+        raise gpu.AlpacaError(f"location {LOCATIONS[location]} has no halo in 2D")
+    out = []
+    for a in range(dim):
+        if d[a] > 0:
+            out += [gpu.IC + gpu.HS, gpu.HS]
+        elif d[a] < 0:
+            out += [0, gpu.HS]
+        else:
+            out += [gpu.HS, gpu.IC]
+    return tuple(out)
+
+
 def make_links(rows) -> np.ndarray:
     """int32 [n][4] links from (child, parent, position, level) rows."""
     a = np.asarray(list(rows), dtype=np.int32).reshape(-1, 4)
@@ -143,6 +178,23 @@ class Multiresolution:
         gpu.check(lib.alpaca_mr_prediction(dim, ctypes.c_void_p(blocks.data_ptr()), nb,
                                            ctypes.c_void_p(links.data_ptr()), nl, r,
                                            _stream(stream)), "mr_prediction")
+
+    @staticmethod
+    def jump_halo_update(blocks, dim, links, locations, stream=None):
+        """Halo update across a level jump (UpdateMaterialJumpNoMpi,
+        internal_halo_manager.cpp:231-262): every link's child gets its halo
+        slab at locations[i] (int32 device tensor, BoundaryLocation order)
+        predicted from its parent."""
+        torch = gpu._torch()
+        lib = gpu.load_library()
+        nb, nl = _check_pool(blocks, dim), _check_links(links)
+        if locations.dtype != torch.int32 or not locations.is_contiguous() \
+                or locations.numel() != nl:
+            raise gpu.AlpacaError("locations must be a contiguous int32 device tensor, one per link")
+        gpu.check(lib.alpaca_mr_jump_halo_update(dim, ctypes.c_void_p(blocks.data_ptr()), nb,
+                                                 ctypes.c_void_p(links.data_ptr()),
+                                                 ctypes.c_void_p(locations.data_ptr()), nl,
+                                                 _stream(stream)), "mr_jump_halo_update")
 
     def child_needs_remeshing(self, blocks, dim, links, stream=None, out=None):
         """(details float64[n], decisions int32[n]) device tensors; decisions

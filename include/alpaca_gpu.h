@@ -325,6 +325,23 @@ int alpaca_mr_prediction(int dim, double *d_blocks, int nblocks,
                          const alpaca_mr_link *d_links, int nlinks,
                          const int *range, void *stream);
 
+/* UpdateMaterialJumpNoMpi (internal_halo_manager.cpp:231-262), the halo
+ * update across a level jump, batched: for every link i the child's halo
+ * slab at boundary location d_locations[i] (device int, the reference's
+ * BoundaryLocation order, boundary_specifications.h:56-86: 0 East, 1 West,
+ * 2 North, 3 South, 4 Top, 5 Bottom, 6-17 the sticks BottomNorth ...
+ * SouthWest, 18-25 the cubes EastNorthTop ... WestSouthBottom) is predicted
+ * from the parent on every equation, over GetStartIndicesHaloRecv(loc) and
+ * GetHaloSize(loc) (communication_types.h:76-132,190-197): per axis the
+ * cells [20, 24) for East/North/Top, [0, 4) for West/South/Bottom and the
+ * internal [4, 20) otherwise.  The reference calls it on the right-hand-side
+ * buffers for conservatives; pass the pool that holds them.  Links with an
+ * invalid block index or location (in 2D: one with a Top/Bottom part) are
+ * skipped. */
+int alpaca_mr_jump_halo_update(int dim, double *d_blocks, int nblocks,
+                               const alpaca_mr_link *d_links, const int *d_locations, int nlinks,
+                               void *stream);
+
 /* Norm of the wavelet details (the reference's Norm, norms.h; its default
  * CC::NFWA() is L-infinity, compile_time_constants.h:133). */
 enum { ALPACA_MR_NORM_LINF = 0, ALPACA_MR_NORM_L1 = 1, ALPACA_MR_NORM_L2 = 2 };

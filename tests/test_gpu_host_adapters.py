@@ -99,7 +99,9 @@ def test_multiresolution_adapter(tmp_path, dim, norm):
     nb = parent.size
     avg, whole, inner = got[:nb], got[nb:2 * nb], got[2 * nb:3 * nb]
     detail, decision = got[3 * nb], int(got[3 * nb + 1])
-    surf = got[3 * nb + 2:]
+    ns = 6 * (dim + 2) * 16 * icz
+    surf = got[3 * nb + 2:3 * nb + 2 + ns]
+    halos = got[3 * nb + 2 + ns:]
     assert np.array_equal(avg, oracle_py.mr_average(dim, child, parent, pos).ravel())
     rng_inner = (4, 16, 4, 16) + ((4, 16) if dim == 3 else (0, 1))
     want_whole = np.stack([oracle_py.mr_prediction(dim, parent[e], child[e], pos)
@@ -112,6 +114,16 @@ def test_multiresolution_adapter(tmp_path, dim, norm):
     assert abs(detail - want) <= 1e-13 * want
     assert decision == oracle_py.mr_decision(want, eps)
     assert np.array_equal(surf, oracle_py.mr_average_jump(dim, cs, ps, pos).ravel())
+    # JumpHaloUpdateLinks over every location of the dimension
+    from alpaca_amd import multiresolution as mr
+    want_halo = child.copy()
+    for loc in range(26):
+        if dim == 2 and mr.location_direction(loc)[2] != 0:
+            continue
+        r = mr.halo_recv_range(dim, loc)
+        for e in range(dim + 2):
+            want_halo[e] = oracle_py.mr_prediction(dim, parent[e], want_halo[e], pos, r)
+    assert np.array_equal(halos, want_halo.ravel())
 
 
 @pytest.mark.parametrize("dim", [2, 3])

@@ -288,3 +288,72 @@ def test_full_size_level_one_properties(torch):
         want = np.stack([O.mr_prediction(dim, p[e], np.zeros((TC, TC, TC)), rows[k][2])
                          for e in range(dim + 2)])
         assert np.array_equal(blocks[rows[k][0]].cpu().numpy(), want), k
+
+
+def _valid_locations(dim):
+    return [loc for loc in range(26) if dim == 3 or mr.location_direction(loc)[2] == 0]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_jump_halo_update_bitexact(torch, dim):
+    """alpaca_mr_jump_halo_update: each link's halo slab at its own location
+    equals the oracle's prediction over that location's receive range; the
+    rest of the child and the parent are untouched; an invalid location (and
+    in 2D one with a Top/Bottom part) leaves the child as it was."""
+    print(f"[mr jump halo] dim={dim}", flush=True)
+    host, dev, links, pairs = _pool(torch, dim, mr_states.KINDS)
+    valid = _valid_locations(dim)
+    bad = [26, -1] + ([4, 8, 25] if dim == 2 else [])
+    locs = [valid[i % len(valid)] if i % 7 else bad[(i // 7) % len(bad)]
+            for i in range(len(pairs))]
+    mr.Multiresolution.jump_halo_update(dev, dim, links,
+                                        torch.tensor(locs, dtype=torch.int32, device="cuda"))
+    torch.cuda.synchronize()
+    got = dev.cpu().numpy()
+    with np.errstate(all="ignore"):
+        for i, (kind, pos, p, c) in enumerate(pairs):
+            if locs[i] in valid:
+                rng = mr.halo_recv_range(dim, locs[i])
+                want = np.stack([O.mr_prediction(dim, p[e], c[e], pos, rng)
+                                 for e in range(dim + 2)])
+            else:
+                want = c
+            assert _same(got[2 * i + 1], want), (kind, pos, locs[i])
+            assert _same(got[2 * i], p), (kind, pos)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_jump_halo_update_every_location_of_one_child(torch, dim):
+    """One launch over every location of the same child (disjoint slabs of
+    one block written by concurrent workgroups) equals the oracle applying
+    the ranges one after the other: the whole halo is predicted, the internal
+    cells keep their values."""
+    print(f"[mr jump halo, all locations] dim={dim}", flush=True)
+    valid = _valid_locations(dim)
+    for pos in range(2 ** dim):
+        p, c = mr_states.pair(dim, "smooth", pos, seed=5)
+        dev = torch.as_tensor(np.ascontiguousarray(np.stack([p, c])), device="cuda").contiguous()
+        links = torch.as_tensor(mr.make_links([(1, 0, pos, 1)] * len(valid)), device="cuda")
+        mr.Multiresolution.jump_halo_update(dev, dim, links,
+                                            torch.tensor(valid, dtype=torch.int32, device="cuda"))
+        torch.cuda.synchronize()
+        want = c.copy()
+        for loc in valid:
+            rng = mr.halo_recv_range(dim, loc)
+            for e in range(dim + 2):
+                want[e] = O.mr_prediction(dim, p[e], want[e], pos, rng)
+        assert _same(dev.cpu().numpy()[1], want), pos
+
+
+def test_jump_halo_update_bad_arguments(torch):
+    lib = gpu.load_library()
+    dev = torch.zeros(2 * gpu.block_doubles(3), dtype=torch.float64, device="cuda")
+    links = torch.as_tensor(mr.make_links([(1, 0, 0, 1)]), device="cuda")
+    st = ctypes.c_void_p(0)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+    assert lib.alpaca_mr_jump_halo_update(3, ptr(dev), 2, ptr(links), None, 1, st) == 1
+    assert lib.alpaca_mr_jump_halo_update(4, ptr(dev), 2, ptr(links), ptr(links), 1, st) == 1
+    assert lib.alpaca_mr_jump_halo_update(3, ptr(dev), 2, None, None, 0, st) == 0
+    with pytest.raises(gpu.AlpacaError):
+        mr.Multiresolution.jump_halo_update(dev.view(2, -1), 3, links,
+                                            torch.zeros(2, dtype=torch.int32, device="cuda"))

@@ -26,6 +26,7 @@ import numpy as np
 import pytest
 from numpy.polynomial import polynomial as Pn
 
+from alpaca_amd import gpu
 from alpaca_amd import multiresolution as mr
 from oracle import oracle_py as O
 
@@ -303,3 +304,64 @@ def test_threshold_known_answers_of_the_reference_tests(dim, order, lmax, lref, 
     assert th.reference_epsilon == eps_ref
     for level in range(10):
         assert th.threshold_on_level(level) == 2.0 ** (dim * (level - lmax)) * eps_ref
+
+
+# Halo slabs of the 26 boundary locations, as given by the reference's
+# CommunicationTypes tables (communication_types.h:76-132): a few entries read
+# off those tables, and the invariants a wrong entry breaks -- each slab holds
+# exactly the halo cells on its side(s), and the slabs of all locations of a
+# dimension cover every halo cell once.
+HALO_TABLE_ENTRIES = {
+    "East": (20, 4, 4, 16, 4, 16), "West": (0, 4, 4, 16, 4, 16),
+    "BottomNorth": (4, 16, 20, 4, 0, 4), "TopWest": (0, 4, 4, 16, 20, 4),
+    "SouthEast": (20, 4, 0, 4, 4, 16), "WestSouthBottom": (0, 4, 0, 4, 0, 4),
+    "EastNorthTop": (20, 4, 20, 4, 20, 4),
+}
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_halo_recv_ranges_partition_the_halo(dim):
+    for name, want in HALO_TABLE_ENTRIES.items():
+        loc = mr.LOCATIONS.index(name)
+        if dim == 3:
+            assert mr.halo_recv_range(3, loc) == want
+        elif mr.location_direction(loc)[2] == 0:
+            assert mr.halo_recv_range(2, loc) == want[:4]
+    nz = TC if dim == 3 else 1
+    hits = np.zeros((TC, TC, nz), dtype=int)
+    side = lambda i, n: 0 if n == 1 else (-1 if i < HS else (1 if i >= HS + IC else 0))
+    used = 0
+    for loc in range(26):
+        d = mr.location_direction(loc)
+        if dim == 2 and d[2] != 0:
+            with pytest.raises(gpu.AlpacaError):
+                mr.halo_recv_range(2, loc)
+            continue
+        used += 1
+        r = list(mr.halo_recv_range(dim, loc)) + ([0, 1] if dim == 2 else [])
+        sl = tuple(slice(r[2 * a], r[2 * a] + r[2 * a + 1]) for a in range(3))
+        hits[sl] += 1
+        ii, jj, kk = np.meshgrid(*[np.arange(s.start, s.stop) for s in sl], indexing="ij")
+        for i, j, k in zip(ii.ravel(), jj.ravel(), kk.ravel()):
+            assert (side(i, TC), side(j, TC), side(k, nz)) == d, (loc, i, j, k)
+    assert used == (26 if dim == 3 else 8)
+    internal = np.zeros_like(hits)
+    internal[HS:HS + IC, HS:HS + IC, (HS if dim == 3 else 0):(HS + IC if dim == 3 else 1)] = 1
+    assert np.array_equal(hits + internal, np.ones_like(hits))
+
+
+@pytest.mark.skipif(not (O.ref_available(2) and O.ref_available(3)),
+                    reason="reference harness not built (oracle/_ref)")
+@pytest.mark.parametrize("dim", [2, 3])
+def test_jump_halo_prediction_matches_reference_harness(dim):
+    with np.errstate(all="ignore"):
+        for pos in range(2 ** dim):
+            parent, child = mr_states.pair(dim, "random", pos, seed=3)
+            for loc in range(26):
+                if dim == 2 and mr.location_direction(loc)[2] != 0:
+                    continue
+                rng = mr.halo_recv_range(dim, loc)
+                e = loc % (dim + 2)
+                f = O.mr_prediction(dim, parent[e], child[e], pos, rng)
+                g = O.mr_prediction(dim, parent[e], child[e], pos, rng, which="ref")
+                assert np.array_equal(f, g), (pos, loc)
