@@ -325,6 +325,10 @@ MR_BYTES_PER_LINK = {
     # cells (read 2, write 1 of 16^3 x 5).
     "lts_prepare_buffer": 3 * 24 ** 3 * 5 * 8,
     "lts_integrate_node": (4 * 6 * 5 * 16 ** 2 + 3 * 16 ** 3 * 5) * 8,
+    # halo update across a level jump, per (child, face location): writes the
+    # 4 x 16 x 16 slab, reads the parent window of its 2 x 8 x 8 parent cells
+    # + 2 per side (6 x 12 x 12).
+    "jump_halos_faces": (4 * 16 * 16 + 6 * 12 * 12) * 5 * 8,
 }
 
 
@@ -369,6 +373,13 @@ def run_mr(args, nparents=512):
         ti.append_micro_timestep(dt)
     ops["lts_prepare_buffer"] = lambda: ti.prepare_buffer_for_integration(blocks, init, level, 1)
     ops["lts_integrate_node"] = lambda: ti.integrate_node(blocks, rhs, jc, jf, level, 1, 2)
+    # Jump halos: every child's six face slabs (East ... Bottom) in one launch.
+    jrows = [r for r in rows for _ in range(6)]
+    jlinks = torch.as_tensor(mr.make_links(jrows), device="cuda")
+    jlocs = torch.as_tensor([loc for _ in rows for loc in range(6)], dtype=torch.int32,
+                            device="cuda")
+    ops["jump_halos_faces"] = lambda: m.jump_halo_update(blocks, dim, jlinks, jlocs)
+    counts = {"jump_halos_faces": len(jrows)}
     stream = torch.cuda.current_stream()
     reps = max(args.steps, 1)
     res = {}
@@ -384,12 +395,14 @@ def run_mr(args, nparents=512):
         t1.record(stream)
         torch.cuda.synchronize()
         ms = t0.elapsed_time(t1) / reps
-        bytes_launch = MR_BYTES_PER_LINK[name] * nl
+        cnt = counts.get(name, nl)
+        bytes_launch = MR_BYTES_PER_LINK[name] * cnt
         gbs = bytes_launch / (ms / 1e3) / 1e9
-        res[name] = {"avg_launch_ms": ms, "links": nl, "bytes_per_link": MR_BYTES_PER_LINK[name],
+        res[name] = {"avg_launch_ms": ms, "links": cnt, "bytes_per_link": MR_BYTES_PER_LINK[name],
                      "hbm_GBs": gbs, "frac": gbs / (PEAK_HBM / 1e9)}
     return {"workload": f"3D MR operators, {nparents} parents x 8 children (256^3 on level 1), "
-                        "FP64; lts_* over the 4096 children as one level", "kernels": res}
+                        "FP64; lts_* over the 4096 children as one level; jump_halos_faces: the six "
+                        "face slabs of every child", "kernels": res}
 
 
 def run_copy_probe():
