@@ -78,6 +78,12 @@ def test_oversized_counts_are_rejected_before_any_launch():
     assert lib.alpaca_mr_average_jump_buffers(2, p, 10, p, -1, None) == 1
     rng = (ctypes.c_int * 6)(0, 24, 0, 24, 0, 24)
     assert lib.alpaca_mr_prediction(3, p, 10, p, big, rng, None) == 1
+    assert lib.alpaca_mr_jump_halo_update(3, p, 10, p, p, big, None) == 1
+    assert lib.alpaca_mr_jump_halo_update(3, p, big, p, p, 1, None) == 1
+    assert lib.alpaca_mr_jump_halo_update(3, p, 10, p, None, 1, None) == 1  # no locations
+    assert lib.alpaca_mr_jump_halo_update(3, ctypes.c_void_p(264), 10, p, p, 1, None) == 1
+    assert lib.alpaca_mr_jump_halo_update(1, p, 10, p, p, 1, None) == 1
+    assert lib.alpaca_mr_jump_halo_update(2, p, 10, None, None, 0, None) == 0
     eps = (ctypes.c_double * 1)(1e-3)
     assert lib.alpaca_mr_child_needs_remeshing(3, p, big, p, 1, 0, eps, 1, p, None, None) == 1
     # misaligned pools and unknown norms are refused as well
