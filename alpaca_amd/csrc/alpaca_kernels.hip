@@ -15,11 +15,25 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
 #include "../../include/alpaca_gpu.h"
 #include "fv_device.h"
+
+// Opaque context of the per-block entries (include/alpaca_gpu.h): the HIP
+// stream its calls are queued on and the face-flux scratch they use, so
+// contexts on different streams never share scratch.
+struct alpaca_context {
+  int device;
+  hipStream_t stream;
+  std::mutex mutex;
+  double *ptr = nullptr;
+  size_t bytes = 0;
+  std::vector<double *> retired;  // outgrown buffers, freed at destroy
+  alpaca_context(int dev, hipStream_t s) : device(dev), stream(s) {}
+};
 
 namespace alpaca {
 namespace {
@@ -334,6 +348,88 @@ __global__ __launch_bounds__(kThreads) void halo_table_kernel(double *blocks, in
   for (int e = 0; e < NE; ++e) blocks[di + (size_t)e * D::FIELD] = sign[e] * blocks[si + (size_t)e * D::FIELD];
 }
 
+// Halo update of listed blocks of a multi-level pool: the same-level part
+// of MaterialHaloUpdateOnLevel (internal_halo_manager.cpp:53-97) and the
+// external conditions after it (halo_manager.cpp:89-123).  Row i of the
+// table belongs to block list[i]; an entry is the neighbour's pool block,
+// -1 where the offset leaves the domain, or -2 where the neighbour does not
+// exist on this level (a level jump: the region is predicted from the parent
+// by the jump update).  Mode 0: internal copies and external conditions,
+// leaving every cell whose source lies in a jump region.  Mode 1, after the
+// jump update: only those cells -- the external condition's source is then
+// the block's own halo cell in the jump region (the reference applies the
+// external conditions last, over the full transverse extent, so they read
+// the predicted values).  Writes and reads of one launch never meet: mode 0
+// reads internal cells, mode 1 reads jump-region halo cells and writes cells
+// of external slabs.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void halo_list_kernel(double *blocks, int nblocks,
+                                                             const int *__restrict__ list, int n,
+                                                             const int *__restrict__ nbr,
+                                                             SideBc bc, int mode) {
+  using D = Dims<DIM>;
+  constexpr int NE = D::NE;
+  constexpr int SLOTS = DIM == 3 ? 27 : 9;
+  constexpr int CELLS = kTC * kTC * D::TZ;
+  size_t const t = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= (size_t)n * CELLS) return;
+  int const row_i = int(t / CELLS);
+  int const blk = list[row_i];
+  if (blk < 0 || blk >= nblocks) return;
+  int const r = int(t % CELLS);
+  int c[3] = {r / (kTC * D::TZ), (r / D::TZ) % kTC, DIM == 3 ? r % kTC : 0};
+  int const dst[3] = {c[0], c[1], c[2]};
+  int o[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) o[a] = c[a] < kHS ? -1 : (c[a] >= kHS + kIC ? 1 : 0);
+  if (o[0] == 0 && o[1] == 0 && o[2] == 0) return;  // internal cell
+  const int *tb = nbr + (size_t)row_i * SLOTS;
+  int src_blk = tb[offset_slot<DIM>(o)];
+  if (src_blk == -2) return;  // jump region
+  bool external = false;
+  double sign[NE];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) sign[e] = 1.0;
+  if (src_blk < 0) {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      if (o[a] == 0) continue;
+      int f[3] = {0, 0, 0};
+      f[a] = o[a];
+      if (tb[offset_slot<DIM>(f)] != -1) continue;  // a neighbour or a jump across this axis
+      int const kind = bc.kind[2 * a + (o[a] > 0)];
+      if (kind == ALPACA_BC_ZERO_GRADIENT) {
+        c[a] = o[a] < 0 ? kHS : kHS + kIC - 1;
+      } else if (kind == ALPACA_BC_SYMMETRY) {
+        c[a] = (o[a] < 0 ? 2 * kHS - 1 : 2 * (kHS + kIC) - 1) - c[a];
+        sign[kMomX + a] = -1.0 * sign[kMomX + a];
+      } else {
+        return;  // a periodic side must appear in the table
+      }
+      o[a] = 0;
+      external = true;
+    }
+    src_blk = (o[0] == 0 && o[1] == 0 && o[2] == 0) ? blk : tb[offset_slot<DIM>(o)];
+    if (src_blk == -2) {
+      if (mode != 1 || !external) return;
+      // The external condition reads the block's own predicted halo cell.
+      size_t const di = (size_t)blk * D::BLOCK + dst[0] * D::SX + dst[1] * D::SY + dst[2];
+      size_t const si = (size_t)blk * D::BLOCK + c[0] * D::SX + c[1] * D::SY + c[2];
+#pragma unroll
+      for (int e = 0; e < NE; ++e)
+        blocks[di + (size_t)e * D::FIELD] = sign[e] * blocks[si + (size_t)e * D::FIELD];
+      return;
+    }
+    if (src_blk < 0) return;  // no covering neighbour across the remaining axes
+  }
+  if (mode != 0 || src_blk >= nblocks) return;
+  size_t const di = (size_t)blk * D::BLOCK + dst[0] * D::SX + dst[1] * D::SY + dst[2];
+  size_t const si = (size_t)src_blk * D::BLOCK + (c[0] - o[0] * kIC) * D::SX +
+                    (c[1] - o[1] * kIC) * D::SY + (c[2] - o[2] * kIC);
+#pragma unroll
+  for (int e = 0; e < NE; ++e) blocks[di + (size_t)e * D::FIELD] = sign[e] * blocks[si + (size_t)e * D::FIELD];
+}
+
 // ----------------------------------------------------------- time step ---
 
 // Per block: max over internal cells of sum_d(|u_d| + c)
@@ -391,6 +487,13 @@ __global__ void dt_kernel(alpaca_case_state *state, double cell_size) {
   state->dt = dt;
   state->smax = 0;
   if (dt < 2.220446049250313080847263336181640625e-16 && dt > 0.0) state->flags |= 2;
+}
+
+// Zero fill of n doubles (grid-stride): used in place of hipMemsetAsync so
+// that a caller may capture the per-block entries in a HIP graph.
+__global__ __launch_bounds__(kThreads) void zero_kernel(double *p, size_t n) {
+  size_t const stride = (size_t)gridDim.x * kThreads;
+  for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < n; t += stride) p[t] = 0.0;
 }
 
 // FinishMacroTimestep (time_integrator.h:244-248).
@@ -476,32 +579,42 @@ struct ProfileScope {
   }
 };
 
-struct Workspace {
-  double *ptr = nullptr;
-  size_t bytes = 0;
-};
-std::mutex g_ws_mutex;
-std::vector<Workspace> g_ws;  // per device
-
 int hip_status(hipError_t e) { return e == hipSuccess ? ALPACA_OK : ALPACA_ERR_HIP; }
 
-int workspace(size_t bytes, double **out) {
+// Implicit contexts of the stream-argument entries: one per (device, stream),
+// created on first use and kept until the process ends.
+std::mutex g_ctx_mutex;
+std::map<std::pair<int, hipStream_t>, alpaca_context *> g_stream_ctx;
+
+int stream_context(hipStream_t s, alpaca_context **out) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return ALPACA_ERR_HIP;
-  std::lock_guard<std::mutex> lock(g_ws_mutex);
-  if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
-  Workspace &w = g_ws[dev];
-  if (w.bytes < bytes) {
-    if (w.ptr) {
-      if (hipDeviceSynchronize() != hipSuccess) return ALPACA_ERR_HIP;
-      if (hipFree(w.ptr) != hipSuccess) return ALPACA_ERR_HIP;
-      w.ptr = nullptr;
-      w.bytes = 0;
-    }
-    if (hipMalloc(&w.ptr, bytes) != hipSuccess) return ALPACA_ERR_HIP;
-    w.bytes = bytes;
+  std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  alpaca_context *&c = g_stream_ctx[{dev, s}];
+  if (!c) c = new alpaca_context(dev, s);
+  *out = c;
+  return ALPACA_OK;
+}
+
+// Face-flux scratch of at least `bytes` from a context on the current
+// device.  A buffer that is too small is retired, not freed: work already
+// queued on the context's stream may still read it, and freeing it here would
+// need a device synchronisation.  Growth is geometric, so the retired
+// buffers together stay below the live one.
+int context_scratch(alpaca_context *c, size_t bytes, double **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return ALPACA_ERR_HIP;
+  if (dev != c->device) return ALPACA_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(c->mutex);
+  if (c->bytes < bytes) {
+    size_t const grown = std::max(bytes, c->bytes + c->bytes / 2);
+    double *p = nullptr;
+    if (hipMalloc(&p, grown) != hipSuccess) return ALPACA_ERR_HIP;
+    if (c->ptr) c->retired.push_back(c->ptr);
+    c->ptr = p;
+    c->bytes = grown;
   }
-  *out = w.ptr;
+  *out = c->ptr;
   return ALPACA_OK;
 }
 
@@ -565,13 +678,12 @@ void launch_faces(const alpaca_case_desc *d, int nblocks, const double *in, doub
     launch_faces_flux<DIM, ALPACA_FLUX_ROE>(d, nblocks, in, fl, s);
 }
 
+// Faces into the scratch fl (flux_doubles<DIM>(nblocks) doubles), then the
+// divergence (mode 1) or divergence + RK combination (mode 0) into out.
 template <int DIM>
 int stage_impl(const alpaca_case_desc *d, int nblocks, int stage, int mode, const double *avg,
-               double *init, double *out, alpaca_case_state *state, hipStream_t s,
+               double *init, double *out, alpaca_case_state *state, hipStream_t s, double *fl,
                double *jump = nullptr) {
-  double *fl = nullptr;
-  int st = workspace(flux_doubles<DIM>(nblocks) * sizeof(double), &fl);
-  if (st != ALPACA_OK) return st;
   launch_faces<DIM>(d, nblocks, avg, fl, s);
   constexpr int cells = Dims<DIM>::CELLS;
   constexpr int gpb = (cells + kThreads - 1) / kThreads;
@@ -759,21 +871,50 @@ size_t alpaca_block_doubles(int dim) {
   return 0;
 }
 
-int alpaca_convective_fluxes(const alpaca_case_desc *desc, int nblocks, const double *d_avg,
-                             double *d_fluxes, void *stream) {
+int alpaca_context_create(void *stream, alpaca_context **out) {
+  if (!out) return ALPACA_ERR_INVALID;
+  *out = nullptr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return ALPACA_ERR_HIP;
+  *out = new alpaca_context(dev, static_cast<hipStream_t>(stream));
+  return ALPACA_OK;
+}
+
+int alpaca_context_destroy(alpaca_context *ctx) {
+  if (!ctx) return ALPACA_ERR_INVALID;
+  int st = hip_status(hipStreamSynchronize(ctx->stream));
+  for (double *p : ctx->retired)
+    if (hipFree(p) != hipSuccess) st = ALPACA_ERR_HIP;
+  if (ctx->ptr && hipFree(ctx->ptr) != hipSuccess) st = ALPACA_ERR_HIP;
+  delete ctx;
+  return st;
+}
+
+int alpaca_context_reserve(alpaca_context *ctx, int dim, int nblocks) {
+  if (!ctx || (dim != 2 && dim != 3) || nblocks < 1 || nblocks > kMaxBlocks)
+    return ALPACA_ERR_INVALID;
+  double *fl = nullptr;
+  size_t const n = dim == 2 ? flux_doubles<2>(nblocks) : flux_doubles<3>(nblocks);
+  return context_scratch(ctx, n * sizeof(double), &fl);
+}
+
+int alpaca_ctx_convective_fluxes(alpaca_context *ctx, const alpaca_case_desc *desc, int nblocks,
+                                 const double *d_avg, double *d_fluxes) {
   int st = validate(desc);
   if (st != ALPACA_OK) return st;
-  if (nblocks < 1 || nblocks > kMaxBlocks || !d_avg || !d_fluxes) return ALPACA_ERR_INVALID;
-  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!ctx || nblocks < 1 || nblocks > kMaxBlocks || !d_avg || !d_fluxes) return ALPACA_ERR_INVALID;
+  hipStream_t s = ctx->stream;
   double *fl = nullptr;
   if (desc->dim == 2) {
-    if ((st = workspace(flux_doubles<2>(nblocks) * sizeof(double), &fl)) != ALPACA_OK) return st;
+    if ((st = context_scratch(ctx, flux_doubles<2>(nblocks) * sizeof(double), &fl)) != ALPACA_OK)
+      return st;
     launch_faces<2>(desc, nblocks, d_avg, fl, s);
     size_t total = (size_t)nblocks * 2 * faces_per_dir<2>();
     hipLaunchKernelGGL(scatter_ref_fluxes<2>, dim3((unsigned)((total + kThreads - 1) / kThreads)),
                        dim3(kThreads), 0, s, fl, d_fluxes, nblocks);
   } else {
-    if ((st = workspace(flux_doubles<3>(nblocks) * sizeof(double), &fl)) != ALPACA_OK) return st;
+    if ((st = context_scratch(ctx, flux_doubles<3>(nblocks) * sizeof(double), &fl)) != ALPACA_OK)
+      return st;
     launch_faces<3>(desc, nblocks, d_avg, fl, s);
     size_t total = (size_t)nblocks * 3 * faces_per_dir<3>();
     hipLaunchKernelGGL(scatter_ref_fluxes<3>, dim3((unsigned)((total + kThreads - 1) / kThreads)),
@@ -782,17 +923,76 @@ int alpaca_convective_fluxes(const alpaca_case_desc *desc, int nblocks, const do
   return hip_status(hipGetLastError());
 }
 
+}  // extern "C"
+
+namespace alpaca {
+namespace {
+
+// Right-hand side of nblocks blocks with the face fluxes in fl: zero the
+// output blocks, faces, divergence, and (jump != null) the boundary-flux save.
+int rhs_with_scratch(const alpaca_case_desc *desc, int nblocks, const double *d_avg, double *d_rhs,
+                     double *d_jump, double *fl, hipStream_t s) {
+  size_t const n = alpaca_block_doubles(desc->dim) * (size_t)nblocks;
+  unsigned const groups = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 65536);
+  hipLaunchKernelGGL(zero_kernel, dim3(groups), dim3(kThreads), 0, s, d_rhs, n);
+  if (desc->dim == 2)
+    return stage_impl<2>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, fl, d_jump);
+  return stage_impl<3>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, fl, d_jump);
+}
+
+int rhs_context(alpaca_context *ctx, const alpaca_case_desc *desc, int nblocks,
+                const double *d_avg, double *d_rhs, double *d_jump) {
+  double *fl = nullptr;
+  size_t const n = desc->dim == 2 ? flux_doubles<2>(nblocks) : flux_doubles<3>(nblocks);
+  int st = context_scratch(ctx, n * sizeof(double), &fl);
+  if (st != ALPACA_OK) return st;
+  return rhs_with_scratch(desc, nblocks, d_avg, d_rhs, d_jump, fl, ctx->stream);
+}
+
+}  // namespace
+}  // namespace alpaca
+
+extern "C" {
+
+int alpaca_ctx_right_hand_side(alpaca_context *ctx, const alpaca_case_desc *desc, int nblocks,
+                               const double *d_avg, double *d_rhs) {
+  int st = validate(desc);
+  if (st != ALPACA_OK) return st;
+  if (!ctx || nblocks < 1 || nblocks > kMaxBlocks || !d_avg || !d_rhs || d_avg == d_rhs)
+    return ALPACA_ERR_INVALID;
+  return rhs_context(ctx, desc, nblocks, d_avg, d_rhs, nullptr);
+}
+
+int alpaca_ctx_right_hand_side_jump_fluxes(alpaca_context *ctx, const alpaca_case_desc *desc,
+                                           int nblocks, const double *d_avg, double *d_rhs,
+                                           double *d_jump_fluxes) {
+  int st = validate(desc);
+  if (st != ALPACA_OK) return st;
+  if (!ctx || nblocks < 1 || nblocks > kMaxBlocks || !d_avg || !d_rhs || !d_jump_fluxes ||
+      d_avg == d_rhs)
+    return ALPACA_ERR_INVALID;
+  return rhs_context(ctx, desc, nblocks, d_avg, d_rhs, d_jump_fluxes);
+}
+
+int alpaca_convective_fluxes(const alpaca_case_desc *desc, int nblocks, const double *d_avg,
+                             double *d_fluxes, void *stream) {
+  int st = validate(desc);
+  if (st != ALPACA_OK) return st;
+  if (nblocks < 1 || nblocks > kMaxBlocks || !d_avg || !d_fluxes) return ALPACA_ERR_INVALID;
+  alpaca_context *ctx = nullptr;
+  if ((st = stream_context(static_cast<hipStream_t>(stream), &ctx)) != ALPACA_OK) return st;
+  return alpaca_ctx_convective_fluxes(ctx, desc, nblocks, d_avg, d_fluxes);
+}
+
 int alpaca_right_hand_side(const alpaca_case_desc *desc, int nblocks, const double *d_avg,
                            double *d_rhs, void *stream) {
   int st = validate(desc);
   if (st != ALPACA_OK) return st;
   if (nblocks < 1 || nblocks > kMaxBlocks || !d_avg || !d_rhs || d_avg == d_rhs)
     return ALPACA_ERR_INVALID;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  size_t const bytes = alpaca_block_doubles(desc->dim) * sizeof(double) * (size_t)nblocks;
-  if (hipMemsetAsync(d_rhs, 0, bytes, s) != hipSuccess) return ALPACA_ERR_HIP;
-  if (desc->dim == 2) return stage_impl<2>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s);
-  return stage_impl<3>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s);
+  alpaca_context *ctx = nullptr;
+  if ((st = stream_context(static_cast<hipStream_t>(stream), &ctx)) != ALPACA_OK) return st;
+  return rhs_context(ctx, desc, nblocks, d_avg, d_rhs, nullptr);
 }
 
 int alpaca_right_hand_side_jump_fluxes(const alpaca_case_desc *desc, int nblocks,
@@ -802,12 +1002,9 @@ int alpaca_right_hand_side_jump_fluxes(const alpaca_case_desc *desc, int nblocks
   if (st != ALPACA_OK) return st;
   if (nblocks < 1 || nblocks > kMaxBlocks || !d_avg || !d_rhs || !d_jump_fluxes || d_avg == d_rhs)
     return ALPACA_ERR_INVALID;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  size_t const bytes = alpaca_block_doubles(desc->dim) * sizeof(double) * (size_t)nblocks;
-  if (hipMemsetAsync(d_rhs, 0, bytes, s) != hipSuccess) return ALPACA_ERR_HIP;
-  if (desc->dim == 2)
-    return stage_impl<2>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, d_jump_fluxes);
-  return stage_impl<3>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, d_jump_fluxes);
+  alpaca_context *ctx = nullptr;
+  if ((st = stream_context(static_cast<hipStream_t>(stream), &ctx)) != ALPACA_OK) return st;
+  return rhs_context(ctx, desc, nblocks, d_avg, d_rhs, d_jump_fluxes);
 }
 
 int alpaca_rk3_stage(const alpaca_case_desc *desc, int stage, const double *d_avg, double *d_init,
@@ -850,6 +1047,33 @@ int alpaca_fill_halos_table(int dim, double *d_blocks, int nblocks, const int *d
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (dim == 2) return halo_table_impl<2>(d_blocks, nblocks, d_neighbors, bc, s);
   return halo_table_impl<3>(d_blocks, nblocks, d_neighbors, bc, s);
+}
+
+int alpaca_fill_halos_list(int dim, double *d_blocks, int nblocks, const int *d_list, int n,
+                           const int *d_neighbors, const int *side_bc, int mode, void *stream) {
+  if ((dim != 2 && dim != 3) || !d_blocks || !side_bc || nblocks < 1 || nblocks > kMaxBlocks ||
+      n < 0 || n > kMaxBlocks || (mode != 0 && mode != 1))
+    return ALPACA_ERR_INVALID;
+  if (n == 0) return ALPACA_OK;
+  if (!d_list || !d_neighbors) return ALPACA_ERR_INVALID;
+  SideBc bc;
+  for (int i = 0; i < 6; ++i) {
+    bc.kind[i] = i < 2 * dim ? side_bc[i] : ALPACA_BC_ZERO_GRADIENT;
+    if (bc.kind[i] != ALPACA_BC_ZERO_GRADIENT && bc.kind[i] != ALPACA_BC_SYMMETRY &&
+        bc.kind[i] != ALPACA_BC_PERIODIC)
+      return ALPACA_ERR_INVALID;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  size_t const cells = (size_t)kTC * kTC * (dim == 3 ? kTC : 1);
+  dim3 grid((unsigned)(((size_t)n * cells + kThreads - 1) / kThreads));
+  ProfileScope prof(ALPACA_KERNEL_HALO, s, double(n) * cells);
+  if (dim == 2)
+    hipLaunchKernelGGL(halo_list_kernel<2>, grid, dim3(kThreads), 0, s, d_blocks, nblocks, d_list,
+                       n, d_neighbors, bc, mode);
+  else
+    hipLaunchKernelGGL(halo_list_kernel<3>, grid, dim3(kThreads), 0, s, d_blocks, nblocks, d_list,
+                       n, d_neighbors, bc, mode);
+  return hip_status(hipGetLastError());
 }
 
 int alpaca_timestep(const alpaca_case_desc *desc, const double *d_avg, alpaca_case_state *d_state,

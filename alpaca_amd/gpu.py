@@ -54,10 +54,23 @@ _SIGNATURES = {
     "alpaca_right_hand_side_jump_fluxes": ([ctypes.POINTER(CaseDesc), ctypes.c_int, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
                                            ctypes.c_int),
+    "alpaca_context_create": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "alpaca_context_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "alpaca_context_reserve": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "alpaca_ctx_convective_fluxes": ([ctypes.c_void_p, ctypes.POINTER(CaseDesc), ctypes.c_int,
+                                      ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_ctx_right_hand_side": ([ctypes.c_void_p, ctypes.POINTER(CaseDesc), ctypes.c_int,
+                                    ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_ctx_right_hand_side_jump_fluxes": ([ctypes.c_void_p, ctypes.POINTER(CaseDesc),
+                                                ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p], ctypes.c_int),
     "alpaca_rk3_stage": ([ctypes.POINTER(CaseDesc), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "alpaca_fill_halos_table": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                  ctypes.POINTER(ctypes.c_int), ctypes.c_void_p], ctypes.c_int),
+    "alpaca_fill_halos_list": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "alpaca_fill_halos": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_void_p],
                           ctypes.c_int),
     "alpaca_timestep": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_void_p,
@@ -72,6 +85,9 @@ _SIGNATURES = {
     "alpaca_ensemble_bytes": ([ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_size_t),
     "alpaca_ensemble_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint,
                                 ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "alpaca_ensemble_create_shared": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint,
+                                       ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)],
+                                      ctypes.c_int),
     "alpaca_ensemble_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "alpaca_ensemble_add_case": ([ctypes.c_void_p, ctypes.POINTER(CaseDesc), ctypes.c_void_p,
                                   ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int)],

@@ -114,6 +114,28 @@ int alpaca_profile_read(int kernel, int reset, double *total_ms, long long *laun
 /* Doubles per haloed block (NE * 24^dim). */
 size_t alpaca_block_doubles(int dim);
 
+/* Contexts of the per-block entries (i) and (ii).  The reference keeps its
+ * face-flux arrays on the stack of SpaceSolver::UpdateFluxes
+ * (space_solver.cpp:50-116), one set per call; here those faces go to device
+ * scratch, which a context owns together with the HIP stream its calls are
+ * queued on.  Use one context per host thread or stream: calls on one context
+ * are ordered by its stream and share its scratch, calls on different
+ * contexts never share scratch.  Scratch grows on demand without a device
+ * synchronisation (outgrown buffers are kept until alpaca_context_destroy,
+ * which synchronises the context's stream and frees them).  The
+ * stream-argument forms (alpaca_convective_fluxes, alpaca_right_hand_side,
+ * alpaca_right_hand_side_jump_fluxes) use an implicit context per (device,
+ * stream), kept until the process ends; prefer an explicit context for
+ * streams that are created and destroyed during the run. */
+typedef struct alpaca_context alpaca_context;
+/* Binds a new context to the current device and to `stream` (NULL = the
+ * default stream). */
+int alpaca_context_create(void *stream, alpaca_context **out);
+int alpaca_context_destroy(alpaca_context *ctx);
+/* Grows the context's scratch to what nblocks blocks of dimension dim need,
+ * so later calls up to that size allocate nothing. */
+int alpaca_context_reserve(alpaca_context *ctx, int dim, int nblocks);
+
 /* (i) Per-block convective fluxes.
  * Replaces ConvectiveTermSolver<D>::UpdateConvectiveFluxes
  * (convective_term_solver.h:67-81) for the selected family.
@@ -141,6 +163,17 @@ int alpaca_right_hand_side(const alpaca_case_desc *desc, int nblocks,
 int alpaca_right_hand_side_jump_fluxes(const alpaca_case_desc *desc, int nblocks,
                                        const double *d_avg, double *d_rhs,
                                        double *d_jump_fluxes, void *stream);
+
+/* (i) and (ii) on an explicit context (its stream and scratch); the same
+ * results, bit for bit, as the stream-argument forms above. */
+int alpaca_ctx_convective_fluxes(alpaca_context *ctx, const alpaca_case_desc *desc,
+                                 int nblocks, const double *d_avg, double *d_fluxes);
+int alpaca_ctx_right_hand_side(alpaca_context *ctx, const alpaca_case_desc *desc,
+                               int nblocks, const double *d_avg, double *d_rhs);
+int alpaca_ctx_right_hand_side_jump_fluxes(alpaca_context *ctx,
+                                           const alpaca_case_desc *desc, int nblocks,
+                                           const double *d_avg, double *d_rhs,
+                                           double *d_jump_fluxes);
 
 /* (iii) One SSP-RK3 stage on every block of a case: FillInitialBuffer,
  * UpdateFluxes, PrepareBufferForIntegration and IntegrateNode
@@ -185,6 +218,25 @@ int alpaca_fill_halos_table(int dim, double *d_blocks, int nblocks,
                             const int *d_neighbors, const int *side_bc,
                             void *stream);
 
+/* (iv) for a multi-level block pool: the halo update of the n blocks in
+ * d_list (device int32) on one level (InternalHaloManager::
+ * MaterialHaloUpdateOnLevel, internal_halo_manager.cpp:53-97, followed by the
+ * external conditions, halo_manager.cpp:89-123).  d_neighbors (device,
+ * int32[n][27] in 3D, [n][9] in 2D, row i for block d_list[i], slots as for
+ * alpaca_fill_halos_table): the neighbour's pool block, -1 where the offset
+ * leaves the domain (an external side), or -2 where no block of this level
+ * exists there (a level jump, InternalBoundaryType::JumpBoundaryLocal: the
+ * region is predicted from the parent by alpaca_mr_jump_halo_update).
+ * mode 0: internal copies and external conditions, leaving jump regions and
+ * every external-slab cell whose source lies in one; mode 1 (after the jump
+ * update): exactly those external-slab cells, from the block's own predicted
+ * halo cells, as the reference's external conditions over the full
+ * transverse extent read them.  List entries outside [0, nblocks) are
+ * skipped. */
+int alpaca_fill_halos_list(int dim, double *d_blocks, int nblocks, const int *d_list,
+                           int n, const int *d_neighbors, const int *side_bc, int mode,
+                           void *stream);
+
 /* (v) CFL time step of a case: ComputeTimestepSize
  * (modular_algorithm_assembler.cpp:1777-1997) for inviscid flow.
  * S = max over internal cells of sum_d(|u_d| + c); dt = cfl / (S / dx),
@@ -219,7 +271,8 @@ int alpaca_advance(const alpaca_case_desc *desc, double *d_u, double *d_s1,
  * job).  Each case keeps its own time step, end time and stop flags; one
  * alpaca_ensemble_step advances every running case by one macro step with
  * one persistent kernel per RK stage and per enabled class (flux family x
- * arithmetic), replayed from a captured hipGraph.  A case's results do not
+ * arithmetic; the classes of one stage are parallel branches of the step),
+ * replayed from a captured hipGraph.  A case's results do not
  * depend on what else runs: per-case reductions only, fixed arithmetic order.
  * The ensemble owns its device memory (allocated at create).
  *
@@ -243,6 +296,15 @@ enum {
 size_t alpaca_ensemble_bytes(int dim, int max_blocks, int max_cases);
 int alpaca_ensemble_create(int dim, int max_blocks, int max_cases, unsigned class_mask,
                            void *stream, alpaca_ensemble **out);
+/* A second ensemble, of either dimension, that admits its cases into the
+ * block pool of `peer` (and of every ensemble sharing it): one HBM budget
+ * for the 2D and the 3D cases of a serving loop.  The pool's unit is one 2D
+ * haloed block; a 3D block is exactly 30 units and takes a 30-aligned range.
+ * Each ensemble keeps its own stream, case slots, work lists and captured
+ * step; the pool is freed with the last ensemble that uses it. */
+int alpaca_ensemble_create_shared(alpaca_ensemble *peer, int dim, int max_cases,
+                                  unsigned class_mask, void *stream,
+                                  alpaca_ensemble **out);
 int alpaca_ensemble_destroy(alpaca_ensemble *e);
 /* Admits a case: copies its nb[0]*nb[1]*nb[2] haloed blocks (internal cells
  * are used) from d_blocks into the pool, fills its halos and computes its
