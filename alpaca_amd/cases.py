@@ -261,3 +261,18 @@ def blobs_shock_3d_torch(nb: int, seed: int, gamma: float = 1.4, nblobs: int = 3
         cons[3, i0:i1] = z
         cons[4, i0:i1] = z
     return cons
+
+
+def batched_cases(ncases: int = 512, seed: int = 0):
+    """The batched-ensemble workload: ncases independent 2D four-quadrant
+    problems, grid sizes 32^2/64^2/128^2/256^2 cells (2/4/8/16 blocks a side)
+    with weights 0.4/0.3/0.2/0.1, gamma ~ U[1.2, 1.67], states as
+    four_quadrant_2d (case index i).  Returns [(nb, gamma, state0)]."""
+    rng = SplitMix64(0xBA7C, seed)
+    specs = []
+    for i in range(ncases):
+        u = rng.uniform(0.0, 1.0)
+        nb = 2 if u < 0.4 else (4 if u < 0.7 else (8 if u < 0.9 else 16))
+        gamma = rng.uniform(1.2, 1.67)
+        specs.append((nb, gamma, four_quadrant_2d(nb, nb, seed=seed, gamma=gamma, case=i)))
+    return specs

@@ -124,10 +124,31 @@ void launch_batch(int tile_config, int groups, hipStream_t s, const double *avg,
 
 }  // namespace
 
+// The HBM block pool of one or two ensembles (a 2D and a 3D one admitted
+// against one budget).  Its unit is one 2D haloed block (4 x 24^2 doubles,
+// 18,432 B per buffer); a 3D haloed block (5 x 24^3 doubles) is exactly 30
+// units, so a 3D case takes a 30-aligned unit range and indexes the same
+// arrays in 3D blocks.  Three arrays (u, s1, s2) of `units` units each.
+constexpr int kUnitDoubles = 4 * 24 * 24;
+constexpr int kUnitsPerBlock3 = 30;
+static_assert(5 * 24 * 24 * 24 == kUnitsPerBlock3 * kUnitDoubles, "3D block = 30 units");
+
+struct BlockPool {
+  double *u = nullptr, *s1 = nullptr, *s2 = nullptr;
+  long long units = 0;
+  std::map<long long, long long> free_ranges;  // unit start -> length, coalesced
+  std::mutex mu;
+  int refs = 0;
+};
+
 struct alpaca_ensemble {
   int dim = 0;
   int max_blocks = 0, max_cases = 0;
+  int units_per_block = 1;
+  BlockPool *pool = nullptr;
   hipStream_t stream = nullptr;
+  hipStream_t aux[kClasses] = {};  // branch streams of the captured step
+  hipEvent_t fork[3] = {}, join[3][kClasses] = {};
   double *u = nullptr, *s1 = nullptr, *s2 = nullptr;
   CaseSlot *d_slots = nullptr;
   alpaca_case_state *d_states = nullptr;
@@ -139,7 +160,6 @@ struct alpaca_ensemble {
   std::vector<CaseSlot> h_slots;
   std::vector<alpaca_case_desc> h_desc;
   std::vector<int> h_active;
-  std::map<int, int> free_ranges;  // start -> length, coalesced
   int tile_config = 0;
   int groups_per_cu = 0;  // 0: default from occupancy
   int num_cus = 256;
@@ -154,34 +174,59 @@ int hip_ok(hipError_t e) { return e == hipSuccess ? ALPACA_OK : ALPACA_ERR_HIP; 
 
 size_t block_bytes(int dim) { return alpaca_block_doubles(dim) * sizeof(double); }
 
+// First fit of n blocks of e's dimension in the shared pool (unit ranges
+// aligned to e's block size); *start is in e's blocks.
 int alloc_range(alpaca_ensemble *e, int n, int *start) {
-  for (auto it = e->free_ranges.begin(); it != e->free_ranges.end(); ++it) {
-    if (it->second >= n) {
-      *start = it->first;
-      int const rest = it->second - n;
-      int const rs = it->first + n;
-      e->free_ranges.erase(it);
-      if (rest > 0) e->free_ranges[rs] = rest;
-      return ALPACA_OK;
-    }
+  BlockPool &p = *e->pool;
+  long long const upb = e->units_per_block;
+  long long const need = (long long)n * upb;
+  std::lock_guard<std::mutex> lock(p.mu);
+  for (auto it = p.free_ranges.begin(); it != p.free_ranges.end(); ++it) {
+    long long const rs = it->first, rl = it->second;
+    long long const a = (rs + upb - 1) / upb * upb;
+    if (a + need > rs + rl) continue;
+    p.free_ranges.erase(it);
+    if (a > rs) p.free_ranges[rs] = a - rs;
+    if (a + need < rs + rl) p.free_ranges[a + need] = rs + rl - (a + need);
+    *start = int(a / upb);
+    return ALPACA_OK;
   }
   return ALPACA_ERR_CAPACITY;
 }
 
 void free_range(alpaca_ensemble *e, int start, int n) {
-  auto it = e->free_ranges.emplace(start, n).first;
+  BlockPool &p = *e->pool;
+  long long const upb = e->units_per_block;
+  std::lock_guard<std::mutex> lock(p.mu);
+  auto it = p.free_ranges.emplace((long long)start * upb, (long long)n * upb).first;
   auto next = std::next(it);
-  if (next != e->free_ranges.end() && it->first + it->second == next->first) {
+  if (next != p.free_ranges.end() && it->first + it->second == next->first) {
     it->second += next->second;
-    e->free_ranges.erase(next);
+    p.free_ranges.erase(next);
   }
-  if (it != e->free_ranges.begin()) {
+  if (it != p.free_ranges.begin()) {
     auto prev = std::prev(it);
     if (prev->first + prev->second == it->first) {
       prev->second += it->second;
-      e->free_ranges.erase(it);
+      p.free_ranges.erase(it);
     }
   }
+}
+
+// Free blocks of e's dimension and the largest range of them.
+void pool_capacity(alpaca_ensemble *e, int *free_blocks, int *largest) {
+  BlockPool &p = *e->pool;
+  long long const upb = e->units_per_block;
+  std::lock_guard<std::mutex> lock(p.mu);
+  long long tot = 0, big = 0;
+  for (auto const &r : p.free_ranges) {
+    long long const a = (r.first + upb - 1) / upb * upb;
+    long long const nb = a < r.first + r.second ? (r.first + r.second - a) / upb : 0;
+    tot += nb;
+    big = std::max(big, nb);
+  }
+  *free_blocks = (int)std::min<long long>(tot, INT32_MAX);
+  *largest = (int)std::min<long long>(big, INT32_MAX);
 }
 
 int default_groups_per_cu(int dim, int tile_config) {
@@ -208,6 +253,34 @@ int upload_lists(alpaca_ensemble *e) {
   return ALPACA_OK;
 }
 
+// One class's persistent stage kernel on stream s.
+void launch_class(alpaca_ensemble *e, int c, int groups, hipStream_t s, int stage) {
+  double *const ins[3] = {e->u, e->s1, e->s2};
+  double *const outs[3] = {e->s1, e->s2, e->u};
+  int const flux = c / 3, mode = c % 3;
+  if (e->dim == 3) {
+    if (flux == 0)
+      launch_batch<3, 0>(e->tile_config, groups, s, ins[stage], e->u, outs[stage], e->d_slots,
+                         e->d_states, e->d_items[c], e->d_counts + c, stage, mode);
+    else
+      launch_batch<3, 1>(e->tile_config, groups, s, ins[stage], e->u, outs[stage], e->d_slots,
+                         e->d_states, e->d_items[c], e->d_counts + c, stage, mode);
+  } else {
+    if (flux == 0)
+      launch_batch<2, 0>(e->tile_config, groups, s, ins[stage], e->u, outs[stage], e->d_slots,
+                         e->d_states, e->d_items[c], e->d_counts + c, stage, mode);
+    else
+      launch_batch<2, 1>(e->tile_config, groups, s, ins[stage], e->u, outs[stage], e->d_slots,
+                         e->d_states, e->d_items[c], e->d_counts + c, stage, mode);
+  }
+}
+
+// The macro step as captured: dt, then per stage the persistent kernel of
+// every enabled class -- the classes of one stage as parallel branches (a
+// fork event on the step's stream, one branch stream per further class, a
+// join before the next stage), so one class's tail overlaps the others'
+// work -- then the per-case finish.  Classes touch disjoint cases, so the
+// branches never write the same block.
 int record_step(alpaca_ensemble *e) {
   int const g0 = e->groups_per_cu > 0 ? e->groups_per_cu : default_groups_per_cu(e->dim, e->tile_config);
   int groups = g0 * e->num_cus;
@@ -215,28 +288,22 @@ int record_step(alpaca_ensemble *e) {
   int const nc = e->max_cases;
   hipLaunchKernelGGL(ens_dt_kernel, dim3((nc + 255) / 256), dim3(256), 0, e->stream, e->d_slots,
                      e->d_states, nc);
-  double *const ins[3] = {e->u, e->s1, e->s2};
-  double *const outs[3] = {e->s1, e->s2, e->u};
+  int cls[kClasses];
+  int ncls = 0;
+  for (int c = 0; c < kClasses; ++c)
+    if (e->class_mask & (1u << c)) cls[ncls++] = c;
   for (int stage = 0; stage < 3; ++stage) {
-    for (int c = 0; c < kClasses; ++c) {
-      if (!(e->class_mask & (1u << c))) continue;
-      int const flux = c / 3, mode = c % 3;
-      if (e->dim == 3) {
-        if (flux == 0)
-          launch_batch<3, 0>(e->tile_config, groups, e->stream, ins[stage], e->u, outs[stage],
-                             e->d_slots, e->d_states, e->d_items[c], e->d_counts + c, stage, mode);
-        else
-          launch_batch<3, 1>(e->tile_config, groups, e->stream, ins[stage], e->u, outs[stage],
-                             e->d_slots, e->d_states, e->d_items[c], e->d_counts + c, stage, mode);
-      } else {
-        if (flux == 0)
-          launch_batch<2, 0>(e->tile_config, groups, e->stream, ins[stage], e->u, outs[stage],
-                             e->d_slots, e->d_states, e->d_items[c], e->d_counts + c, stage, mode);
-        else
-          launch_batch<2, 1>(e->tile_config, groups, e->stream, ins[stage], e->u, outs[stage],
-                             e->d_slots, e->d_states, e->d_items[c], e->d_counts + c, stage, mode);
-      }
+    if (ncls > 1 && hipEventRecord(e->fork[stage], e->stream) != hipSuccess) return ALPACA_ERR_HIP;
+    for (int i = 0; i < ncls; ++i) {
+      int const c = cls[i];
+      hipStream_t s = i == 0 ? e->stream : e->aux[c];
+      if (i > 0 && hipStreamWaitEvent(s, e->fork[stage], 0) != hipSuccess) return ALPACA_ERR_HIP;
+      launch_class(e, c, groups, s, stage);
+      if (i > 0 && hipEventRecord(e->join[stage][c], s) != hipSuccess) return ALPACA_ERR_HIP;
     }
+    for (int i = 1; i < ncls; ++i)
+      if (hipStreamWaitEvent(e->stream, e->join[stage][cls[i]], 0) != hipSuccess)
+        return ALPACA_ERR_HIP;
   }
   hipLaunchKernelGGL(ens_finish_kernel, dim3((nc + 255) / 256), dim3(256), 0, e->stream,
                      e->d_states, nc);
@@ -282,31 +349,55 @@ size_t alpaca_ensemble_bytes(int dim, int max_blocks, int max_cases) {
          (size_t)kClasses * max_blocks * sizeof(WorkItem) + kClasses * sizeof(int);
 }
 
-int alpaca_ensemble_create(int dim, int max_blocks, int max_cases, unsigned class_mask,
-                           void *stream, alpaca_ensemble **out) {
-  if (!out || (dim != 2 && dim != 3) || max_blocks < 1 || max_cases < 1 || class_mask == 0 ||
-      (class_mask >> kClasses) != 0 || max_blocks > kMaxBlocks || max_cases > kMaxBlocks)
-    return ALPACA_ERR_INVALID;
-  *out = nullptr;
-  auto *e = new (std::nothrow) alpaca_ensemble();
-  if (!e) return ALPACA_ERR_CAPACITY;
+}  // extern "C"
+
+namespace {
+
+void release_pool(BlockPool *p) {
+  if (!p) return;
+  bool last = false;
+  {
+    std::lock_guard<std::mutex> lock(p->mu);
+    last = --p->refs == 0;
+  }
+  if (!last) return;
+  (void)hipFree(p->u);
+  (void)hipFree(p->s1);
+  (void)hipFree(p->s2);
+  delete p;
+}
+
+// Everything of an ensemble but its pool: case slots, records, work lists of
+// the enabled classes, branch streams and events of the captured step.
+int ensemble_init(alpaca_ensemble *e, BlockPool *pool, int dim, int max_cases,
+                  unsigned class_mask, hipStream_t stream) {
   e->dim = dim;
-  e->max_blocks = max_blocks;
+  e->units_per_block = dim == 3 ? kUnitsPerBlock3 : 1;
+  e->pool = pool;
+  e->max_blocks = (int)std::min<long long>(pool->units / e->units_per_block, kMaxBlocks);
   e->max_cases = max_cases;
   e->class_mask = class_mask;
-  e->stream = static_cast<hipStream_t>(stream);
+  e->stream = stream;
+  e->u = pool->u;
+  e->s1 = pool->s1;
+  e->s2 = pool->s2;
   int dev = 0;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
     e->num_cus = prop.multiProcessorCount;
-  size_t const bb = block_bytes(dim) * (size_t)max_blocks;
-  bool ok = hipMalloc(&e->u, bb) == hipSuccess && hipMalloc(&e->s1, bb) == hipSuccess &&
-            hipMalloc(&e->s2, bb) == hipSuccess &&
-            hipMalloc(&e->d_slots, sizeof(CaseSlot) * max_cases) == hipSuccess &&
+  bool ok = hipMalloc(&e->d_slots, sizeof(CaseSlot) * max_cases) == hipSuccess &&
             hipMalloc(&e->d_states, sizeof(alpaca_case_state) * max_cases) == hipSuccess &&
             hipMalloc(&e->d_counts, sizeof(int) * kClasses) == hipSuccess;
   for (int c = 0; ok && c < kClasses; ++c)
-    ok = hipMalloc(&e->d_items[c], sizeof(WorkItem) * max_blocks) == hipSuccess;
+    if (class_mask & (1u << c))
+      ok = hipMalloc(&e->d_items[c], sizeof(WorkItem) * e->max_blocks) == hipSuccess &&
+           hipStreamCreateWithFlags(&e->aux[c], hipStreamNonBlocking) == hipSuccess;
+  for (int st = 0; ok && st < 3; ++st) {
+    ok = hipEventCreateWithFlags(&e->fork[st], hipEventDisableTiming) == hipSuccess;
+    for (int c = 0; ok && c < kClasses; ++c)
+      if (class_mask & (1u << c))
+        ok = hipEventCreateWithFlags(&e->join[st][c], hipEventDisableTiming) == hipSuccess;
+  }
   if (ok) {
     ok = hipMemsetAsync(e->d_states, 0, sizeof(alpaca_case_state) * max_cases, e->stream) ==
              hipSuccess &&
@@ -314,14 +405,64 @@ int alpaca_ensemble_create(int dim, int max_blocks, int max_cases, unsigned clas
          hipMemsetAsync(e->d_slots, 0, sizeof(CaseSlot) * max_cases, e->stream) == hipSuccess &&
          hipStreamSynchronize(e->stream) == hipSuccess;
   }
-  if (!ok) {
-    alpaca_ensemble_destroy(e);
-    return ALPACA_ERR_HIP;
-  }
+  if (!ok) return ALPACA_ERR_HIP;
   e->h_slots.assign(max_cases, CaseSlot{});
   e->h_desc.assign(max_cases, alpaca_case_desc{});
   e->h_active.assign(max_cases, 0);
-  e->free_ranges[0] = max_blocks;
+  return ALPACA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int alpaca_ensemble_create(int dim, int max_blocks, int max_cases, unsigned class_mask,
+                           void *stream, alpaca_ensemble **out) {
+  if (!out || (dim != 2 && dim != 3) || max_blocks < 1 || max_cases < 1 || class_mask == 0 ||
+      (class_mask >> kClasses) != 0 || max_blocks > kMaxBlocks || max_cases > kMaxBlocks)
+    return ALPACA_ERR_INVALID;
+  *out = nullptr;
+  auto *pool = new (std::nothrow) BlockPool();
+  auto *e = new (std::nothrow) alpaca_ensemble();
+  if (!pool || !e) {
+    delete pool;
+    delete e;
+    return ALPACA_ERR_CAPACITY;
+  }
+  pool->refs = 1;
+  pool->units = (long long)max_blocks * (dim == 3 ? kUnitsPerBlock3 : 1);
+  size_t const bb = sizeof(double) * kUnitDoubles * (size_t)pool->units;
+  e->pool = pool;
+  if (hipMalloc(&pool->u, bb) != hipSuccess || hipMalloc(&pool->s1, bb) != hipSuccess ||
+      hipMalloc(&pool->s2, bb) != hipSuccess ||
+      ensemble_init(e, pool, dim, max_cases, class_mask, static_cast<hipStream_t>(stream)) !=
+          ALPACA_OK) {
+    alpaca_ensemble_destroy(e);
+    return ALPACA_ERR_HIP;
+  }
+  pool->free_ranges[0] = pool->units;
+  *out = e;
+  return ALPACA_OK;
+}
+
+int alpaca_ensemble_create_shared(alpaca_ensemble *peer, int dim, int max_cases,
+                                  unsigned class_mask, void *stream, alpaca_ensemble **out) {
+  if (!out || !peer || (dim != 2 && dim != 3) || max_cases < 1 || class_mask == 0 ||
+      (class_mask >> kClasses) != 0 || max_cases > kMaxBlocks)
+    return ALPACA_ERR_INVALID;
+  *out = nullptr;
+  auto *e = new (std::nothrow) alpaca_ensemble();
+  if (!e) return ALPACA_ERR_CAPACITY;
+  {
+    std::lock_guard<std::mutex> lock(peer->pool->mu);
+    peer->pool->refs += 1;
+  }
+  e->pool = peer->pool;
+  if (ensemble_init(e, peer->pool, dim, max_cases, class_mask, static_cast<hipStream_t>(stream)) !=
+      ALPACA_OK) {
+    alpaca_ensemble_destroy(e);
+    return ALPACA_ERR_HIP;
+  }
   *out = e;
   return ALPACA_OK;
 }
@@ -330,13 +471,19 @@ int alpaca_ensemble_destroy(alpaca_ensemble *e) {
   if (!e) return ALPACA_ERR_INVALID;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   drop_graph(e);
-  (void)hipFree(e->u);
-  (void)hipFree(e->s1);
-  (void)hipFree(e->s2);
   (void)hipFree(e->d_slots);
   (void)hipFree(e->d_states);
   (void)hipFree(e->d_counts);
-  for (int c = 0; c < kClasses; ++c) (void)hipFree(e->d_items[c]);
+  for (int c = 0; c < kClasses; ++c) {
+    (void)hipFree(e->d_items[c]);
+    if (e->aux[c]) (void)hipStreamDestroy(e->aux[c]);
+  }
+  for (int st = 0; st < 3; ++st) {
+    if (e->fork[st]) (void)hipEventDestroy(e->fork[st]);
+    for (int c = 0; c < kClasses; ++c)
+      if (e->join[st][c]) (void)hipEventDestroy(e->join[st][c]);
+  }
+  release_pool(e->pool);
   delete e;
   return ALPACA_OK;
 }
@@ -478,10 +625,7 @@ int alpaca_ensemble_capacity(alpaca_ensemble *e, int *free_blocks, int *largest_
   if (!e || !free_blocks || !largest_range || !free_slots) return ALPACA_ERR_INVALID;
   std::lock_guard<std::mutex> lock(e->mu);
   int tot = 0, big = 0;
-  for (auto const &r : e->free_ranges) {
-    tot += r.second;
-    big = std::max(big, r.second);
-  }
+  pool_capacity(e, &tot, &big);
   int slots = 0;
   for (int a : e->h_active) slots += a ? 0 : 1;
   *free_blocks = tot;

@@ -23,7 +23,13 @@ def class_bit(flux: int, precision: int) -> int:
 
 
 class Ensemble:
-    def __init__(self, dim: int, max_blocks: int, max_cases: int, classes, stream=None):
+    """One ensemble; with ``share_with`` (another Ensemble) it admits its
+    cases into that ensemble's block pool (alpaca_ensemble_create_shared), so
+    a 2D and a 3D ensemble draw on one HBM budget; ``max_blocks`` is then
+    ignored.  The pool lives until the last ensemble using it is closed."""
+
+    def __init__(self, dim: int, max_blocks: int, max_cases: int, classes, stream=None,
+                 share_with: "Ensemble | None" = None):
         import torch
 
         self.torch = torch
@@ -37,9 +43,14 @@ class Ensemble:
         # non-legacy stream.
         self.stream = stream if stream is not None else torch.cuda.Stream()
         h = ctypes.c_void_p()
-        gpu.check(self.lib.alpaca_ensemble_create(dim, max_blocks, max_cases, mask,
-                                                  ctypes.c_void_p(self.stream.cuda_stream),
-                                                  ctypes.byref(h)), "ensemble_create")
+        sp = ctypes.c_void_p(self.stream.cuda_stream)
+        if share_with is not None:
+            gpu.check(self.lib.alpaca_ensemble_create_shared(share_with.h, dim, max_cases, mask,
+                                                             sp, ctypes.byref(h)),
+                      "ensemble_create_shared")
+        else:
+            gpu.check(self.lib.alpaca_ensemble_create(dim, max_blocks, max_cases, mask, sp,
+                                                      ctypes.byref(h)), "ensemble_create")
         self.h = h
         self.desc = {}
 

@@ -1,12 +1,13 @@
 """Continuous-batching serving loop over device-resident ensembles.
 
 Requests (independent simulations) arrive as a Poisson stream.  Each is
-admitted as soon as its blocks fit the HBM budget (a 2D and a 3D block pool,
-FIFO with backfill), stays resident until it reaches its end time, then its
-state is read back and its blocks are released.  Between polls both pools
-advance by a few macro steps, each on its own stream so their kernels overlap
-on the device; every running case takes its own CFL step, so a request's
-result does not depend on what else is running.
+admitted as soon as its blocks fit the HBM budget (one block pool shared by
+a 2D and a 3D ensemble, FIFO with backfill; a request larger than the whole
+pool is rejected on arrival), stays resident until it reaches its end time,
+then its state is read back and its blocks are released.  Between polls both
+ensembles advance by a few macro steps, each on its own stream so their
+kernels overlap on the device; every running case takes its own CFL step, so
+a request's result does not depend on what else is running.
 
 Reported: p50/p99 time-to-solution (arrival to result), sustained
 cell-updates/s, peak and mean HBM occupancy of the admitted state.
@@ -84,30 +85,83 @@ def req_cfl(req: Request) -> float:
     return 0.6 if req.dim == 2 else 0.5
 
 
-class Server:
-    """Two resident pools (2D, 3D) sized from an HBM budget."""
+class Admission:
+    """Host side of admission, with no device state: requests move from
+    `pending` (not yet arrived) to `waiting` (arrived) and are admitted FIFO
+    with backfill -- a request that does not fit yet does not block smaller
+    ones behind it.  A request larger than the pool can ever hold
+    (``max_blocks[dim]``) is rejected on arrival, so `waiting` only holds
+    requests that fit once enough cases finish."""
 
-    def __init__(self, budget_bytes: float, share_3d: float = 0.8, max_cases: int = 4096,
-                 steps_per_poll: int = 4, tuning=None):
+    def __init__(self, requests, max_blocks):
+        self.pending = deque(sorted(requests, key=lambda r: r.arrival))
+        self.waiting = deque()
+        self.rejected = []
+        self.max_blocks = dict(max_blocks)
+
+    def arrive(self, now: float):
+        while self.pending and self.pending[0].arrival <= now:
+            req = self.pending.popleft()
+            if req_blocks(req) > self.max_blocks.get(req.dim, 0):
+                self.rejected.append(req)
+            else:
+                self.waiting.append(req)
+
+    def admit(self, fits, admit):
+        """Calls admit(req) for every waiting request that fits(req), in
+        arrival order; returns how many were admitted."""
+        still = deque()
+        n = 0
+        while self.waiting:
+            req = self.waiting.popleft()
+            if fits(req):
+                admit(req)
+                n += 1
+            else:
+                still.append(req)
+        self.waiting = still
+        return n
+
+    def next_arrival(self):
+        return self.pending[0].arrival if self.pending else None
+
+    def stuck(self, live: int) -> bool:
+        """Nothing runs and nothing arrives, yet requests wait: with an empty
+        pool every request within max_blocks fits, so this only happens when
+        case slots or the pool are held elsewhere.  The loop must end then
+        instead of spinning."""
+        return live == 0 and not self.pending and bool(self.waiting)
+
+
+class Server:
+    """A 3D and a 2D ensemble admitting into one block pool sized from the HBM
+    budget (a 3D block takes the room of 30 2D blocks), each stepping on its
+    own stream so their kernels overlap on the device."""
+
+    def __init__(self, budget_bytes: float, max_cases: int = 4096, steps_per_poll: int = 4,
+                 tuning=None):
         import torch
 
         self.torch = torch
         classes = [(gpu.FLUX_HLLC, gpu.FP64), (gpu.FLUX_HLLC, gpu.MIXED)]
         self.block_bytes = {d: 3 * gpu.block_doubles(d) * 8 for d in (2, 3)}
-        budget = {3: budget_bytes * share_3d, 2: budget_bytes * (1.0 - share_3d)}
-        self.pools = {}
+        max3 = int(budget_bytes // self.block_bytes[3])
+        if max3 < 1:
+            raise ValueError("HBM budget below one 3D block")
+        self.pools = {3: Ensemble(3, max_blocks=max3, max_cases=max_cases, classes=classes)}
+        self.pools[2] = Ensemble(2, max_blocks=0, max_cases=max_cases, classes=classes,
+                                 share_with=self.pools[3])
+        # One pool: a 3D block is 30 2D blocks.
+        self.max_blocks = {3: max3, 2: 30 * max3}
         for d in (2, 3):
-            max_blocks = int(budget[d] // self.block_bytes[d])
-            self.pools[d] = Ensemble(d, max_blocks=max_blocks, max_cases=max_cases,
-                                     classes=classes)
             if tuning and d in tuning:
                 self.pools[d].set_tuning(*tuning[d])
         self.budget = budget_bytes
         self.steps_per_poll = steps_per_poll
 
     def close(self):
-        for p in self.pools.values():
-            p.close()
+        for d in (2, 3):  # the 2D ensemble shares the 3D one's pool
+            self.pools[d].close()
 
     def fits(self, req: Request) -> bool:
         free_blocks, largest, slots = self.pools[req.dim].capacity()
@@ -118,9 +172,10 @@ class Server:
         return pool.add_case(case_desc(req), initial_state(req), req.t_end, req_cfl(req))
 
     def serve(self, requests, keep_results=False, max_wall=None, log=None):
+        """Serve the stream; keep_results: True (every request's final state)
+        or a set of request ids."""
         torch = self.torch
-        pending = deque(sorted(requests, key=lambda r: r.arrival))
-        waiting = deque()
+        queue = Admission(requests, self.max_blocks)
         live = {2: {}, 3: {}}  # case id -> request
         done = []
         results = {}
@@ -129,30 +184,29 @@ class Server:
         updates = 0
         t0 = time.perf_counter()
         last_log = t0
-        while pending or waiting or live[2] or live[3]:
+
+        def admit(req):
+            nonlocal resident
+            cid = self.admit(req)
+            live[req.dim][cid] = (req, time.perf_counter() - t0)
+            resident += req_blocks(req) * self.block_bytes[req.dim]
+
+        while queue.pending or queue.waiting or live[2] or live[3]:
             now = time.perf_counter() - t0
             if max_wall is not None and now > max_wall:
                 break
-            while pending and pending[0].arrival <= now:
-                waiting.append(pending.popleft())
-            # FIFO admission with backfill: a request that does not fit yet
-            # does not block smaller ones behind it.
-            still = deque()
-            while waiting:
-                req = waiting.popleft()
-                if self.fits(req):
-                    cid = self.admit(req)
-                    live[req.dim][cid] = (req, time.perf_counter() - t0)
-                    resident += req_blocks(req) * self.block_bytes[req.dim]
-                else:
-                    still.append(req)
-            waiting = still
-            if not (live[2] or live[3]):
-                if pending:
-                    time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
+            queue.arrive(now)
+            queue.admit(self.fits, admit)
+            nlive = len(live[2]) + len(live[3])
+            if nlive == 0:
+                if queue.stuck(nlive):
+                    break
+                nxt = queue.next_arrival()
+                if nxt is not None:
+                    time.sleep(max(0.0, min(0.01, nxt - now)))
                 continue
             occ_samples.append(resident)
-            # Both pools advance concurrently on their own streams.
+            # Both ensembles advance concurrently on their own streams.
             for d in (2, 3):
                 if live[d]:
                     self.pools[d].step(self.steps_per_poll)
@@ -167,7 +221,7 @@ class Server:
                         req, t_admit = live[d].pop(cid)
                         cells = req_blocks(req) * (16 ** req.dim)
                         updates += cells * rec.steps
-                        if keep_results:
+                        if keep_results is True or (keep_results and req.rid in keep_results):
                             results[req.rid] = self.pools[d].download(cid)
                         self.pools[d].remove_case(cid)
                         resident -= req_blocks(req) * self.block_bytes[d]
@@ -179,14 +233,16 @@ class Server:
             if log and time.perf_counter() - last_log > 5.0:
                 last_log = time.perf_counter()
                 log(f"[serving] t={time.perf_counter() - t0:.1f}s done={len(done)} "
-                    f"live={len(live[2]) + len(live[3])} waiting={len(waiting)} "
-                    f"pending={len(pending)}")
+                    f"live={len(live[2]) + len(live[3])} waiting={len(queue.waiting)} "
+                    f"pending={len(queue.pending)}")
         wall = time.perf_counter() - t0
         torch.cuda.synchronize()
         lat = np.array([r["latency"] for r in done]) if done else np.array([0.0])
         stats = {
             "requests": len(requests),
             "completed": len(done),
+            "rejected": len(queue.rejected),
+            "unserved": len(queue.waiting) + len(queue.pending),
             "wall_s": wall,
             "cell_updates": updates,
             "cell_updates_per_s": updates / wall if wall > 0 else 0.0,
@@ -201,17 +257,57 @@ class Server:
         return stats, done, results
 
 
-def main_bench(args):
-    """bench.py --workload serving: one JSON line."""
+def _alone_fp64(req: Request):
+    """A request run by itself in FP64 (its twin for the mixed error)."""
+    twin = dataclasses.replace(req, precision=gpu.FP64)
+    ens = Ensemble(twin.dim, max_blocks=req_blocks(twin), max_cases=1,
+                   classes=[(gpu.FLUX_HLLC, gpu.FP64)])
+    cid = ens.add_case(case_desc(twin), initial_state(twin), twin.t_end, req_cfl(twin))
+    ens.run_until_done([cid], poll=8)
+    out = ens.download(cid)
+    ens.close()
+    return out
+
+
+def stream_summary(rate: float = 200.0, window: float = 2.0, seed: int = 0,
+                   budget: float = 200e9, twins: int = 8, log=None):
+    """Serve a Poisson stream of the serving mix and summarise it: latency,
+    sustained rate, HBM occupancy, and the L1 density error / mass and energy
+    drift of the first `twins` mixed requests against their FP64 twins (run
+    alone afterwards)."""
     from . import autotune
 
+    reqs = generate_requests(rate, window, seed=seed)
+    tuning = {d: autotune.lookup(d, gpu.FLUX_HLLC, gpu.FP64) for d in (2, 3)}
+    mixed = [r for r in reqs if r.precision == gpu.MIXED][:twins]
+    server = Server(budget, tuning=tuning)
+    stats, done, results = server.serve(reqs, keep_results={r.rid for r in mixed}, log=log)
+    server.close()
+    errs = []
+    for r in mixed:
+        if r.rid not in results:
+            continue
+        m, f = results[r.rid], _alone_fp64(r)
+        errs.append({"rid": r.rid, "dim": r.dim, "cells": req_blocks(r) * 16 ** r.dim,
+                     "l1_rho_err_vs_fp64": float(np.abs(m[0] - f[0]).sum() / np.abs(f[0]).sum()),
+                     "mass_rel_diff_vs_fp64": float(abs(m[0].sum() - f[0].sum()) / abs(f[0]).sum()),
+                     "energy_rel_diff_vs_fp64": float(abs(m[1].sum() - f[1].sum()) /
+                                                      abs(f[1]).sum())})
+    stats["mixed_vs_fp64_twins"] = errs
+    if errs:
+        stats["mixed_l1_rho_err_worst"] = max(e["l1_rho_err_vs_fp64"] for e in errs)
+    stats["workload"] = (f"Poisson request stream {rate:g} req/s for {window:g} s: 70% 2D "
+                         "four-quadrant 64^2-512^2, 30% 3D blob+shock 64^3-128^3, t_end "
+                         f"U[0.05,0.25], FP64/mixed 50/50, HBM budget {budget / 1e9:g} GB (one "
+                         "block pool for 2D and 3D), continuous batching")
+    return stats
+
+
+def main_bench(args):
+    """bench.py --workload serving: one JSON line."""
     rate = getattr(args, "rate", 200.0)
     window = getattr(args, "window", 2.0)
-    reqs = generate_requests(rate, window, seed=0)
-    tuning = {d: autotune.lookup(d, gpu.FLUX_HLLC, gpu.FP64) for d in (2, 3)}
-    server = Server(200e9, tuning=tuning)
-    stats, done, _ = server.serve(reqs, log=lambda m: print(m, flush=True))
-    server.close()
+    stats = stream_summary(rate, window, log=lambda m: print(m, flush=True))
     line = {
         "metric": ("cell-updates/s WENO5-HLLC-RK3, FP64 vs mixed at stated L1 err; "
                    "p50/p99 case latency"),
@@ -226,10 +322,7 @@ def main_bench(args):
         "vs_baseline": None,
         "dtype": "f64|f32+f64",
         "data": "synthetic",
-        "config": {"workload": (f"Poisson request stream {rate:g} req/s for {window:g} s: 70% 2D "
-                                "four-quadrant 64^2-512^2, 30% 3D blob+shock 64^3-128^3, t_end "
-                                "U[0.05,0.25], FP64/mixed 50/50, HBM budget 200 GB, continuous "
-                                "batching")},
+        "config": {"workload": stats["workload"]},
         "serving": stats,
     }
     print(json.dumps(line), flush=True)
