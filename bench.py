@@ -10,15 +10,21 @@ whole domain; a cell-update is one internal cell advanced one macro step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Prints one JSON line.  `roofline` covers the dominant kernel (the fused
-stage kernel) timed with HIP events on its launch stream during the timed
-region; `modes` repeats the workload in the FP64-with-FMA and mixed forms
-(with the mixed density error against FP64); `roe_fp64` runs it with Roe
-flux splitting in FP64; `mr_operators` times the multiresolution kernels
-(averaging, prediction, wavelet details) on 512 parents and their 4096
-children; `cpu_baseline` times the reference's own numerics
-(oracle/_ref, threads over blocks) on a bounded sample of the same workload
-on this host's cores.
+Prints one JSON line.  `roofline` covers the dominant kernel (the fused stage
+kernel) timed with HIP events on its launch stream during the timed region
+(its `from_profile` part holds the counter figures of the profiled run in
+profiles/, not measured here); `modes` repeats the workload in the
+FP64-with-FMA and mixed forms, each compared with FP64 after 100 steps
+(density L1/Linf error, mass/energy drift); `roe_fp64` runs it with Roe flux
+splitting in FP64; `mr_operators` times the multiresolution kernels on 512
+parents and their 4096 children; `c0_2d_128` runs the 2D 128^2 configuration
+to t_end in FP64 and mixed; `batched` runs 512 uneven 2D cases in one
+ensemble against one at a time (FP64 and mixed, p50/p99 time to solution,
+per-case mixed error); `serving` serves a 200 req/s Poisson stream of the
+serving mix for 1 s (p50/p99 time to solution, sustained rate, HBM
+occupancy, mixed error of sampled requests against FP64 twins);
+`cpu_baseline` times the reference's own numerics (oracle/_ref, threads over
+blocks) on a bounded sample of the same workload on this host's cores.
 """
 from __future__ import annotations
 
@@ -53,25 +59,47 @@ def cpu_threads():
     return len(os.sched_getaffinity(0))
 
 
-def cpu_baseline(flux: int, threads: int):
-    """Reference numerics (or the C restatement if the reference harness is
-    not built) on a 128^3 periodic sample of the same workload (the 8^3-block
-    form of the blob + shock state), 2 steps: about 20 core-seconds."""
+def host_core_counts():
+    """Cores this process may run on (sched_getaffinity) and the host's
+    logical CPUs (lscpu's CPU(s) line; os.cpu_count() if lscpu is absent)."""
+    import subprocess
+
+    host = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("CPU(s):"):
+                host = int(ln.split(":", 1)[1])
+                break
+    except (OSError, ValueError, subprocess.SubprocessError):
+        host = None
+    return len(os.sched_getaffinity(0)), host or os.cpu_count()
+
+
+def cpu_baseline(flux: int, threads: int, steps: int = 5):
+    """The reference's own numerics (oracle/_ref: its translation units,
+    compiled here; the C restatement if that is not built) on a 128^3
+    periodic sample of the same workload (the 8^3-block form of the blob +
+    shock state), `steps` macro steps, threads over blocks."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
     from alpaca_amd import cases
 
     nb = [8, 8, 8]
     state = cases.blobs_shock_3d(8, seed=0)
-    steps = 2
     which = "ref" if oracle_py.ref_available(3) else "oracle"
     t0 = time.perf_counter()
     oracle_py.run(3, nb, [1, 1, 1], flux, 0.5, 1e300, steps, state, cell_size=1.0 / 128,
                   threads=threads, which=which)
     dt = time.perf_counter() - t0
     cells = 128 ** 3
+    affinity, host = host_core_counts()
     return {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": threads,
             "kind": "reference" if which == "ref" else "port",
+            "cores_affinity": affinity, "cores_host_lscpu": host,
+            "binary": ("oracle/_ref/libalpaca_ref3d.so: the reference's translation units "
+                       "built from source by oracle/Makefile" if which == "ref"
+                       else "oracle/liboracle.so (C restatement)"),
             "sample": f"128^3 periodic blob+shock (8^3 blocks), {steps} steps, "
                       f"{'HLLC' if flux == 0 else 'Roe'} FP64, {threads} threads over blocks, "
                       f"{dt:.2f} s wall ({dt * threads:.0f} core-seconds)"}
@@ -88,8 +116,11 @@ def load_profile(kernel_tag: str):
         return {}
 
 
-def run_mode(args, flux, precision, nb, keep_state=False):
-    """Time args.steps macro steps of one arithmetic mode; returns a dict."""
+def run_mode(args, flux, precision, nb, keep_state=False, state_after=0):
+    """Time args.steps macro steps of one arithmetic mode; returns a dict.
+    With keep_state, the density and the equation sums are kept after
+    max(state_after, warmup + steps) steps (the untimed rest run after the
+    timed region)."""
     import torch
 
     from alpaca_amd import cases, gpu
@@ -131,6 +162,11 @@ def run_mode(args, flux, precision, nb, keep_state=False):
         "steps_total": rec.steps,
     }
     if keep_state:
+        rest = state_after - (args.warmup + args.steps)
+        if rest > 0:
+            case.advance(rest)
+            torch.cuda.synchronize()
+        out["compared_after_steps"] = case.record().steps
         out["rho"] = gpu.from_blocks(case.u, 3, case.nb)[0].clone()
         out["sums"] = gpu.from_blocks(case.u, 3, case.nb).sum(dim=(1, 2, 3)).cpu().tolist()
     del case
@@ -139,24 +175,16 @@ def run_mode(args, flux, precision, nb, keep_state=False):
 
 
 def batched_specs(ncases, seed=0):
-    """512 uneven 2D four-quadrant cases: 32^2/64^2/128^2/256^2 cells with
-    weights 0.4/0.3/0.2/0.1, gamma ~ U[1.2, 1.67], zero-gradient sides."""
-    from alpaca_amd import cases, gpu
+    """512 uneven 2D four-quadrant cases (cases.batched_cases)."""
+    from alpaca_amd import cases
 
-    rng = cases.SplitMix64(0xBA7C, seed)
-    specs = []
-    for i in range(ncases):
-        u = rng.uniform(0.0, 1.0)
-        nb = 2 if u < 0.4 else (4 if u < 0.7 else (8 if u < 0.9 else 16))
-        gamma = rng.uniform(1.2, 1.67)
-        st0 = cases.four_quadrant_2d(nb, nb, seed=seed, gamma=gamma, case=i)
-        specs.append((nb, gamma, st0))
-    return specs
+    return cases.batched_cases(ncases, seed)
 
 
-def run_batched(args):
+def batched_summary(args):
     """All cases in one ensemble (one persistent kernel per RK stage, graph
-    step) versus the same cases run one at a time on the same engine."""
+    step) versus the same cases run one at a time on the same engine; FP64 and
+    mixed, with the mixed density error against FP64 per case."""
     import numpy as np
     import torch
 
@@ -232,7 +260,19 @@ def run_batched(args):
     out["mixed"]["l1_rho_err_vs_fp64_max_over_cases"] = max(errs)
     out["mixed"]["l1_rho_err_vs_fp64_mean_over_cases"] = float(np.mean(errs))
     out["mixed"]["speedup_vs_fp64"] = out["mixed"]["cell_updates_per_s"] / out["fp64"]["cell_updates_per_s"]
-    cells = sum(nb * nb * 256 for nb, _, _ in specs)
+    out["cases"] = len(specs)
+    out["cells"] = sum(nb * nb * 256 for nb, _, _ in specs)
+    out["leaf_blocks"] = total_blocks
+    out["workload"] = (f"{len(specs)} uneven 2D four-quadrant cases (32^2..256^2, weights "
+                       ".4/.3/.2/.1, gamma U[1.2,1.67]), HLLC, CFL 0.6, t_end 0.2, one persistent "
+                       "kernel per RK stage, graph-captured step; latency = time to solution "
+                       "with all cases admitted at t=0")
+    return out
+
+
+def run_batched(args):
+    out = batched_summary(args)
+    specs_n, cells, total_blocks = out["cases"], out["cells"], out["leaf_blocks"]
     line = {
         "metric": METRIC,
         "value": out["fp64"]["cell_updates_per_s"],
@@ -246,14 +286,48 @@ def run_batched(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": (f"{len(specs)} uneven 2D four-quadrant cases "
-                                f"(32^2..256^2, weights .4/.3/.2/.1, gamma U[1.2,1.67]), "
-                                "HLLC, CFL 0.6, t_end 0.2, one persistent kernel per RK stage, "
-                                "graph-captured step"),
-                   "cases": len(specs), "cells": cells, "leaf_blocks": total_blocks},
+        "config": {"workload": out["workload"], "cases": specs_n, "cells": cells,
+                   "leaf_blocks": total_blocks},
         "batched": out,
     }
     print(json.dumps(line), flush=True)
+
+
+def c0_summary():
+    """The CPU-runnable configuration on the GPU: 2D four-quadrant (seed 0),
+    128^2 cells = 8x8 blocks, HLLC, CFL 0.6, t_end 0.2, zero-gradient sides,
+    FP64 and mixed: throughput and the mixed density error and mass/energy
+    difference against FP64 at t_end."""
+    import numpy as np
+    import torch
+
+    from alpaca_amd import cases, gpu
+
+    st0 = cases.four_quadrant_2d(8, 8, seed=0)
+    out = {"workload": "2D four-quadrant seed 0, 128^2 cells (8x8 blocks of 16^2), HLLC, CFL 0.6, "
+                       "t_end 0.2, zero-gradient sides"}
+    finals = {}
+    for name, prec in (("fp64", gpu.FP64), ("mixed", gpu.MIXED)):
+        desc = gpu.make_desc(2, [8, 8], [0, 0], flux=gpu.FLUX_HLLC, precision=prec)
+        case = gpu.GpuCase(desc, st0, cfl=0.6, t_end=0.2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        steps, _ = case.run(max_steps=100000)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        finals[name] = case.download()
+        out[name] = {"steps": steps, "wall_s": wall,
+                     "cell_updates_per_s": 128 * 128 * steps / wall,
+                     "time": case.record().time}
+    f, m = finals["fp64"], finals["mixed"]
+    out["mixed"]["l1_rho_err_vs_fp64"] = float(np.abs(m[0] - f[0]).sum() / np.abs(f[0]).sum())
+    out["mixed"]["linf_rho_err_vs_fp64"] = float(np.abs(m[0] - f[0]).max() / np.abs(f[0]).max())
+    out["mixed"]["mass_rel_diff_vs_fp64"] = float(abs(m[0].sum() - f[0].sum()) / abs(f[0]).sum())
+    out["mixed"]["energy_rel_diff_vs_fp64"] = float(abs(m[1].sum() - f[1].sum()) / abs(f[1]).sum())
+    out["note"] = ("64 blocks cannot fill 256 CUs: the rate is launch- and latency-bound; "
+                   "each step reads the case record back (the reference's ComputeLoop checks "
+                   "the end time every step)")
+    return out
 
 
 def run_accuracy(args):
@@ -431,6 +505,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-modes", action="store_true",
                     help="time only --precision (skip the other arithmetic modes)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the C0, batched and serving legs of the default line")
+    ap.add_argument("--compare-steps", type=int, default=100,
+                    help="uniform: step count after which the modes are compared with FP64")
+    ap.add_argument("--serving-window", type=float, default=1.0,
+                    help="arrival window (s) of the serving leg of the default line")
     ap.add_argument("--workload", choices=["uniform", "batched", "serving", "accuracy", "mr"],
                     default="uniform",
                     help="uniform: the 256^3 case (default); batched: 512 uneven 2D cases; "
@@ -480,7 +560,8 @@ def main():
         modes.insert(0, args.precision)
     results = {}
     for m in modes:
-        results[m] = run_mode(args, flux, m, nb, keep_state=not args.no_modes)
+        results[m] = run_mode(args, flux, m, nb, keep_state=not args.no_modes,
+                              state_after=args.compare_steps)
     head = results[args.precision]
     value = head["value"]
 
@@ -492,30 +573,41 @@ def main():
     variant = {"fp64": "exact::", "fp64-fma": "fast::", "mixed": "fast::"}[args.precision]
     rtype = "float" if args.precision == "mixed" else "double"
     prof = load_profile(tag)
+    bound = "valu_fp32" if args.precision == "mixed" else "valu_fp64"
     roofline = {
-        "bound": "mfma",
+        "bound": bound,
         "achieved": achieved / 1e12,
         "peak": peak / 1e12,
         "unit": "TFLOP/s",
         "frac": achieved / peak,
         "traffic": prof.get("hbm_bytes_per_launch"),
         "kernel": f"{variant}stage_kernel<3,{'HLLC' if flux == 0 else 'Roe'},{rtype},8,8,8,256>",
-        "note": (f"compute-bound VALU stencil (no contraction maps to MFMA; peak is the "
-                 f"{'FP32' if args.precision == 'mixed' else 'FP64'} vector figure); "
-                 f"{flop_cell_stage:.0f} algorithmic FLOP per cell and RK stage x "
-                 f"{head['cells_per_launch']:.0f} cells per launch, avg launch "
-                 f"{head['stage_avg_ms']:.3f} ms over {head['stage_launches']} launches"),
+        "note": (f"compute roofline (the contract's 'mfma' slot): a VALU stencil, no contraction "
+                 f"maps to MFMA, so the peak is the {'FP32' if args.precision == 'mixed' else 'FP64'} "
+                 f"vector figure; achieved = {flop_cell_stage:.0f} algorithmic FLOP per cell and "
+                 f"RK stage x {head['cells_per_launch']:.0f} cells per launch / avg launch "
+                 f"{head['stage_avg_ms']:.3f} ms over {head['stage_launches']} launches (HIP "
+                 f"events on the launch stream, this run); traffic = HBM bytes per launch from "
+                 f"the rocprofv3 FETCH_SIZE/WRITE_SIZE runs in profiles/"),
     }
-    if prof.get("hbm_bytes_per_launch"):
-        # HBM bytes per launch (scaled FETCH_SIZE + WRITE_SIZE) over this run's launch time.
-        roofline["hbm_achieved_GBs"] = prof["hbm_bytes_per_launch"] / avg_launch_s / 1e9
-        roofline["hbm_frac"] = roofline["hbm_achieved_GBs"] / 8000.0
-    if prof.get("hw_flop_per_launch"):
-        # Hardware FLOP per launch (SQ_INSTS_VALU_FLOPS_FP64 x 64, FMA = 2) over this run's launch time.
-        roofline["valu_flop_frac"] = prof["hw_flop_per_launch"] / avg_launch_s / peak
-    if prof.get("valu_busy"):
-        roofline["valu_busy"] = prof["valu_busy"]
-        roofline["valu_issue_frac"] = prof.get("valu_issue_frac")
+    if prof:
+        # Counter figures of the profiled run (profiles/traffic.json, from the
+        # rocprofv3 --pmc runs its summary in profiles/ gives): not measured in
+        # this run.  Rates are these per-launch figures over this run's launch time.
+        fp = {"source": "profiles/traffic.json (rocprofv3 --pmc)", "kernel": prof.get("kernel")}
+        if prof.get("hbm_bytes_per_launch"):
+            fp["hbm_bytes_per_launch"] = prof["hbm_bytes_per_launch"]
+            fp["algorithmic_bytes_per_launch"] = prof.get("algorithmic_bytes_per_launch")
+            fp["hbm_GBs_at_this_launch_time"] = prof["hbm_bytes_per_launch"] / avg_launch_s / 1e9
+            fp["hbm_frac"] = fp["hbm_GBs_at_this_launch_time"] / 8000.0
+        if prof.get("hw_flop_per_launch"):
+            fp["hw_flop_per_launch"] = prof["hw_flop_per_launch"]
+            fp["hw_valu_flop_frac_at_this_launch_time"] = (prof["hw_flop_per_launch"] /
+                                                           avg_launch_s / peak)
+        for k in ("valu_busy", "valu_issue_frac", "valu_instr_per_cell"):
+            if prof.get(k) is not None:
+                fp[k] = prof[k]
+        roofline["from_profile"] = fp
     line = {
         "metric": METRIC,
         "value": value,
@@ -557,7 +649,7 @@ def main():
                 summary[m]["mass_drift"] = abs(r["sums"][0] - ref["sums"][0]) / abs(ref["sums"][0])
                 summary[m]["energy_drift"] = abs(r["sums"][1] - ref["sums"][1]) / abs(ref["sums"][1])
                 summary[m]["speedup_vs_fp64"] = r["value"] / ref["value"]
-            summary["compared_after_steps"] = args.warmup + args.steps
+            summary["compared_after_steps"] = ref.get("compared_after_steps")
         line["modes"] = summary
         if flux == 0:
             # The same workload with Roe flux splitting in FP64: the 100-step
@@ -571,6 +663,19 @@ def main():
         # The multiresolution operators (averaging, prediction, details) of
         # the multi-level mode at the same cell count on level 1.
         line["mr_operators"] = run_mr(args)
+    if not (args.no_extra or args.no_modes):
+        # The other configurations at their stated sizes: C0 (2D 128^2), the
+        # 512-case batched ensemble and the Poisson serving stream.
+        print("[bench] c0", file=sys.stderr, flush=True)
+        line["c0_2d_128"] = c0_summary()
+        print("[bench] batched", file=sys.stderr, flush=True)
+        line["batched"] = batched_summary(args)
+        print("[bench] serving", file=sys.stderr, flush=True)
+        from alpaca_amd import serving
+
+        line["serving"] = serving.stream_summary(
+            args.rate, args.serving_window,
+            log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(flux, cpu_threads())
     print(json.dumps(line), flush=True)
