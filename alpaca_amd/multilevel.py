@@ -1,0 +1,664 @@
+"""Multi-level driver: multiresolution with local time stepping.
+
+The host side of ALPACA's default mode for the single-phase path: a tree of
+blocks over levels 0..maximum_level, every level's leaves advanced with their
+own number of micro steps, parents filled by averaging, halos across level
+jumps predicted from the parent, jump fluxes corrected for conservation and
+the mesh adapted by the wavelet analysis after each finished level.  It
+follows the reference's ModularAlgorithmAssembler -- CreateNewSimulation
+(modular_algorithm_assembler.cpp:286-538), Advance (:543-910),
+ComputeRightHandSide (:954-1008), Integrate (:1068-1098), SwapBuffers
+(:1044-1059), JumpFluxAdjustment (:1347-1647), ComputeTimestepSize
+(:1777-1997), Remesh/DetermineRemeshingNodes/RefineNode (:2301-2553),
+GetLevels (:2563-2572) -- with the Averager (averager.cpp:62-140), the halo
+manager (halo_manager.cpp:59-123, internal_halo_manager.cpp:53-97,
+communication_manager.cpp:81-251) and the topology rules it relies on
+(topology_manager.cpp:150-260, 363-390).
+
+The driver owns the topology (a dict of nodes keyed by (level, i, j, k)) and
+the schedule; every operation on cell data is one batched call of a backend
+over the nodes of a level.  ``GpuBackend`` runs them as gfx950 kernels of
+libalpaca_hip.so on pools of haloed blocks in HBM.  Buffers per node follow
+the reference's Block: Average, RightHandSide and Initial conservatives plus
+the jump-flux and jump-conservative surfaces; a node's buffers live at the
+same slot of five device pools.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import gpu, lts
+from . import multiresolution as mr
+
+PERIODIC = gpu.BC_PERIODIC
+RK_STAGES = 3
+
+
+def halo_locations(dim: int) -> list:
+    """CC::HBS(): the halo locations of a block (faces, then the sticks and
+    corners the dimension has), as BoundaryLocation indices."""
+    return [loc for loc in range(len(mr.LOCATIONS))
+            if dim == 3 or mr.location_direction(loc)[2] == 0]
+
+
+def face_locations(dim: int) -> list:
+    """CC::ANBS(): East, West, North, South (, Top, Bottom)."""
+    return list(range(2 * dim))
+
+
+def offset_slot(dim: int, off) -> int:
+    """Slot of an offset in a neighbour-table row (alpaca_fill_halos_list)."""
+    if dim == 3:
+        return ((off[0] + 1) * 3 + (off[1] + 1)) * 3 + (off[2] + 1)
+    return (off[0] + 1) * 3 + (off[1] + 1)
+
+
+class Topology:
+    """The tree of nodes: existence, leaves, parents, children and neighbours
+    over levels 0..maximum_level (the reference's TopologyManager and the id
+    arithmetic of id_information.cpp, with (level, i, j, k) keys instead of
+    bit-encoded ids).  Periodic axes wrap at every level."""
+
+    def __init__(self, dim: int, nb0, maximum_level: int, periodic):
+        self.dim = dim
+        self.nb0 = [int(x) for x in nb0] + [1] * (3 - len(nb0))
+        if dim == 2:
+            self.nb0[2] = 1
+        self.maximum_level = maximum_level
+        self.periodic = list(periodic) + [False] * (3 - len(periodic))
+        self.leaf = {}  # key -> bool
+
+    def extent(self, level: int, axis: int) -> int:
+        if axis >= self.dim:
+            return 1
+        return self.nb0[axis] << level
+
+    def neighbor(self, key, off):
+        """Key of the same-level position at offset `off`, or None outside
+        the domain (wrapped across periodic axes)."""
+        level = key[0]
+        q = [key[1], key[2], key[3]]
+        for a in range(self.dim):
+            v = q[a] + off[a]
+            n = self.extent(level, a)
+            if v < 0 or v >= n:
+                if not self.periodic[a]:
+                    return None
+                v %= n
+            q[a] = v
+        return (level, q[0], q[1], q[2])
+
+    def exists(self, key) -> bool:
+        return key in self.leaf
+
+    def is_leaf(self, key) -> bool:
+        return self.leaf[key]
+
+    @staticmethod
+    def parent(key):
+        return (key[0] - 1, key[1] >> 1, key[2] >> 1, key[3] >> 1)
+
+    @staticmethod
+    def position(key) -> int:
+        """PositionOfNodeAmongSiblings (id & 7): bit 0 x, bit 1 y, bit 2 z."""
+        return (key[1] & 1) | ((key[2] & 1) << 1) | ((key[3] & 1) << 2)
+
+    def children(self, key):
+        lv, i, j, k = key
+        zs = (0, 1) if self.dim == 3 else (0,)
+        out = []
+        for p in range(1 << self.dim):
+            dx, dy, dz = p & 1, (p >> 1) & 1, (p >> 2) & 1
+            if self.dim == 2 and dz:
+                continue
+            out.append((lv + 1, 2 * i + dx, 2 * j + dy, 2 * k + dz if zs != (0,) else 0))
+        return out
+
+    def ids_on_level(self, level: int):
+        return sorted(k for k in self.leaf if k[0] == level)
+
+    def leaves_on_level(self, level: int):
+        return sorted(k for k, lf in self.leaf.items() if lf and k[0] == level)
+
+    def leaves(self):
+        return sorted((k for k, lf in self.leaf.items() if lf), key=lambda k: k)
+
+    def current_maximum_level(self) -> int:
+        return max(k[0] for k in self.leaf)
+
+
+class MultiLevelSolver:
+    """One single-phase multi-level simulation.
+
+    side_bc: per side (-x, +x, -y, +y[, -z, +z]) ALPACA_BC_*; periodic axes
+    must be periodic on both sides.  initial(level, key) returns the
+    conservatives [NE][16][16][16|1] of a node's internal cells (prime states
+    at that level's cell centres converted as PrimeStatesToConservatives).
+    """
+
+    def __init__(self, backend, dim: int, nb0, node_size: float, maximum_level: int, side_bc,
+                 initial, cfl: float, t_end: float, epsilon_reference: float,
+                 level_of_epsilon_reference: int, flux: int = gpu.FLUX_HLLC,
+                 gamma: float = 1.4, background_pressure: float = 0.0):
+        self.be = backend
+        self.dim = dim
+        self.node_size = float(node_size)
+        self.maximum_level = int(maximum_level)
+        self.side_bc = list(side_bc)
+        periodic = [side_bc[2 * a] == PERIODIC for a in range(dim)]
+        for a in range(dim):
+            if (side_bc[2 * a] == PERIODIC) != (side_bc[2 * a + 1] == PERIODIC):
+                raise ValueError("a periodic axis must be periodic on both sides")
+        self.topo = Topology(dim, nb0, maximum_level, periodic)
+        self.initial = initial
+        self.cfl = float(cfl)
+        self.t_end = float(t_end)
+        self.flux = flux
+        self.gamma = gamma
+        self.b = background_pressure
+        self.thresholds = mr.ThresholdComputer(dim, maximum_level, level_of_epsilon_reference,
+                                               epsilon_reference).table(mr.MAX_LEVELS)
+        self.slot = {}
+        self.free = list(range(backend.capacity - 1, -1, -1))
+        self.micro: list = []  # micro time steps of the running macro step
+        self.current_run_time = 0.0
+        self.macro_steps = 0
+        self.all_levels = list(range(self.maximum_level + 1))
+        self.halo_locs = halo_locations(dim)
+        self.face_locs = face_locations(dim)
+
+    # ------------------------------------------------------------- helpers
+    def cell_size(self, level: int) -> float:
+        return self.node_size / float(1 << level) / gpu.IC
+
+    def _new_node(self, key, leaf=True):
+        if not self.free:
+            raise MemoryError("multi-level block pool exhausted")
+        s = self.free.pop()
+        self.slot[key] = s
+        self.topo.leaf[key] = leaf
+        self.be.clear(s)
+        return s
+
+    def _remove_node(self, key):
+        s = self.slot.pop(key)
+        del self.topo.leaf[key]
+        self.free.append(s)
+
+    def _slots(self, keys):
+        return [self.slot[k] for k in keys]
+
+    def _link(self, key):
+        return (self.slot[key], self.slot[self.topo.parent(key)], self.topo.position(key), key[0])
+
+    def _lts_timestep(self, stage, n, jump=False):
+        return self.be.lts_timestep(self.micro, n, stage, jump)
+
+    # -------------------------------------------------------- halo update
+    def _neighbor_table(self, keys):
+        """Rows for alpaca_fill_halos_list: neighbour slot, -1 outside the
+        domain, -2 for a level jump (no node of this level there)."""
+        dim = self.dim
+        offs = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1)
+                for dz in ((-1, 0, 1) if dim == 3 else (0,))]
+        table = np.empty((len(keys), len(offs)), dtype=np.int32)
+        for r, key in enumerate(keys):
+            for off in offs:
+                sl = offset_slot(dim, off)
+                if off == (0, 0, 0):
+                    table[r, sl] = self.slot[key]
+                    continue
+                nb = self.topo.neighbor(key, off)
+                if nb is None:
+                    table[r, sl] = -1
+                elif self.topo.exists(nb):
+                    table[r, sl] = self.slot[nb]
+                else:
+                    table[r, sl] = -2
+        return table
+
+    def halo_update_on_level(self, level: int, cut_jumps: bool):
+        """MaterialHaloUpdateOnLevel for the conservatives (the right-hand-side
+        buffers): same-level copies, then (unless cut_jumps) the jump halos
+        predicted from the parents, then the external conditions."""
+        keys = self.topo.ids_on_level(level)
+        if not keys:
+            return
+        table = self._neighbor_table(keys)
+        jump_links, jump_locs = [], []
+        if not cut_jumps:
+            for r, key in enumerate(keys):
+                for loc in self.halo_locs:
+                    d = mr.location_direction(loc)
+                    if table[r, offset_slot(self.dim, d)] == -2:
+                        jump_links.append(self._link(key))
+                        jump_locs.append(loc)
+        self.be.halo_update(self._slots(keys), table, jump_links, jump_locs, self.side_bc)
+
+    def halo_update(self, levels_ascending, cut_jumps=False):
+        """HaloManager::MaterialHaloUpdate (halo_manager.cpp:59-76)."""
+        levels = list(levels_ascending)
+        if cut_jumps and levels:
+            self.halo_update_on_level(levels.pop(0), True)
+        for level in levels:
+            self.halo_update_on_level(level, False)
+
+    # ----------------------------------------------------------- averaging
+    def average_material(self, child_levels_descending):
+        """Averager::AverageMaterial: every node of each child level (but 0)
+        averaged into its parent's right-hand-side buffer, finest first."""
+        for level in child_levels_descending:
+            if level == 0:
+                continue
+            keys = self.topo.ids_on_level(level)
+            if keys:
+                self.be.average([self._link(k) for k in keys])
+
+    # ---------------------------------------------------------- stage work
+    def compute_right_hand_side(self, levels, stage):
+        for level in levels:
+            keys = self.topo.leaves_on_level(level)
+            if not keys:
+                continue
+            slots = self._slots(keys)
+            if stage == 0:
+                self.be.fill_initial(slots)
+            self.be.right_hand_side(slots, self.cell_size(level))
+            self.be.prepare(slots, stage)
+
+    def integrate(self, levels, stage):
+        for level in levels:
+            n = 1 << (self.maximum_level - level)
+            leaves = self.topo.leaves_on_level(level)
+            if leaves:
+                self.be.integrate_node(self._slots(leaves), self._lts_timestep(stage, n, True),
+                                       self._lts_timestep(stage, n))
+            by_loc = {}
+            for key in self.topo.ids_on_level(level):
+                for loc in self.halo_locs:
+                    nb = self.topo.neighbor(key, mr.location_direction(loc))
+                    if nb is not None and not self.topo.exists(nb):
+                        by_loc.setdefault(loc, []).append(self.slot[key])
+            ts = self._lts_timestep(stage, n)
+            for loc, slots in by_loc.items():
+                self.be.integrate_range(slots, mr.halo_recv_range(self.dim, loc), ts)
+
+    def swap(self, levels):
+        slots = [self.slot[k] for lv in levels for k in self.topo.ids_on_level(lv)]
+        if slots:
+            self.be.swap(slots)
+
+    def jump_flux_adjustment(self, finished_levels_descending):
+        for level in finished_levels_descending:
+            if level == 0:
+                continue
+            keys = self.topo.ids_on_level(level)
+            if keys:
+                self.be.average_jump([self._link(k) for k in keys])
+        for level in finished_levels_descending[1:]:
+            leaves = self.topo.leaves_on_level(level)
+            if not leaves:
+                continue
+            entries = np.full((len(leaves), 7), -1, dtype=np.int32)
+            for r, key in enumerate(leaves):
+                entries[r, 0] = self.slot[key]
+                for loc in self.face_locs:
+                    nb = self.topo.neighbor(key, mr.location_direction(loc))
+                    if nb is not None and self.topo.exists(nb) and not self.topo.is_leaf(nb):
+                        entries[r, 1 + loc] = self.slot[nb]
+            self.be.jump_flux_adjustment(entries, 1.0 / self.cell_size(level))
+        slots = [self.slot[k] for lv in finished_levels_descending
+                 for k in self.topo.ids_on_level(lv)]
+        if slots:
+            self.be.reset_jump_conservatives(slots)
+
+    # ------------------------------------------------------------ remeshing
+    def determine_remeshing_nodes(self, parent_levels):
+        remove, refine = [], []
+        groups = []  # (children, [decision or None])
+        links = []
+        for lp in parent_levels:
+            for pkey in self.topo.ids_on_level(lp):
+                kids = [c for c in self.topo.children(pkey) if self.topo.exists(c)]
+                if not kids:
+                    continue
+                idx = []
+                for c in kids:
+                    if self.topo.is_leaf(c):
+                        idx.append(len(links))
+                        links.append(self._link(c))
+                    else:
+                        idx.append(None)
+                groups.append((kids, idx))
+        decisions = self.be.decisions(links, self.thresholds) if links else []
+        for kids, idx in groups:
+            dec = [mr.NEUTRAL if i is None else int(decisions[i]) for i in idx]
+            refine += [c for c, d in zip(kids, dec) if d == mr.REFINE]
+            if all(d == mr.COARSE for d in dec):
+                remove += kids
+        return remove, refine
+
+    def _coarsen(self, removed):
+        parents = sorted({self.topo.parent(k) for k in removed if self.topo.parent(k)[0] > 0})
+        for pkey in parents:
+            for c in self.topo.children(pkey):
+                if self.topo.exists(c):
+                    self._remove_node(c)
+            self.topo.leaf[pkey] = True
+
+    def refine_node(self, key):
+        """RefineNode: children created, their internal cells predicted from
+        the node's right-hand-side buffer."""
+        self.topo.leaf[key] = False
+        links = []
+        for c in self.topo.children(key):
+            self._new_node(c)
+            links.append(self._link(c))
+        return links
+
+    def remesh(self, levels_ascending):
+        cur_max = self.topo.current_maximum_level()
+        parent_levels = [lv for lv in levels_ascending if lv < cur_max]
+        remove, refine = self.determine_remeshing_nodes(parent_levels)
+        refine = [k for k in refine if self.topo.is_leaf(k) and k[0] != self.maximum_level]
+        links = []
+        for key in refine:
+            links += self.refine_node(key)
+        if links:
+            self.be.predict(links, (gpu.HS, gpu.IC) * self.dim)
+        self.halo_update(levels_ascending[1:])
+        self._coarsen([k for k in remove if k[0] != 1])
+
+    # ------------------------------------------------------------ time step
+    def compute_timestep(self) -> float:
+        run_time = self.current_run_time
+        for dt in self.micro:
+            run_time += dt
+        dx_finest = self.cell_size(self.maximum_level)
+        return self.be.timestep(self._slots(self.topo.leaves()), self.cfl, dx_finest, run_time,
+                                self.t_end)
+
+    # ------------------------------------------------------- initialisation
+    def create_new_simulation(self):
+        for level in self.all_levels:
+            if level > 0:
+                for pkey in self.topo.ids_on_level(level - 1):
+                    self.topo.leaf[pkey] = False
+                    for c in self.topo.children(pkey):
+                        self._new_node(c)
+            else:
+                nb0 = self.topo.nb0
+                for i in range(nb0[0]):
+                    for j in range(nb0[1]):
+                        for k in range(nb0[2]):
+                            self._new_node((0, i, j, k))
+            keys = self.topo.ids_on_level(level)
+            self.be.set_internal(self._slots(keys),
+                                 np.stack([self.initial(level, key) for key in keys]))
+            self.halo_update_on_level(level, False)
+            if level > 1:
+                remove, _ = self.determine_remeshing_nodes([level - 1])
+                self._coarsen(remove)
+        self.swap(self.all_levels)
+
+    # -------------------------------------------------------------- advance
+    def advance(self):
+        """One macro step: 2^maximum_level micro steps on the finest level."""
+        lmax = self.maximum_level
+        levels_desc = list(reversed(self.all_levels))
+        for timestep in range(1 << lmax):
+            self.micro.append(self.compute_timestep())
+            for stage in range(RK_STAGES):
+                self.compute_right_hand_side(levels_desc, stage)
+                self.average_material(levels_desc)
+                self.halo_update(self.all_levels)
+                levels_desc = lts.levels_to_update(timestep, lmax)
+                levels_asc = list(reversed(levels_desc))
+                self.integrate(levels_desc, stage)
+                self.average_material(levels_desc[:-1])
+                last = stage == RK_STAGES - 1
+                if last:
+                    self.jump_flux_adjustment(levels_desc)
+                self.halo_update(levels_asc, cut_jumps=True)
+                if last:
+                    self.remesh(levels_asc)
+                self.swap(levels_desc)
+        # ComputeLoop: ResetAllJumpBuffers, then FinishMacroTimestep.
+        self.be.reset_jump_all(list(self.slot.values()))
+        s = 0.0
+        for dt in self.micro:
+            s += dt
+        self.current_run_time += s
+        last_dt = self.micro[-1]
+        self.micro = []
+        self.macro_steps += 1
+        return last_dt
+
+    def run(self, max_macro_steps: int = 1 << 30):
+        """ComputeLoop (modular_algorithm_assembler.cpp:97-195): macro steps
+        until t_end, or until the last micro step falls below DBL_EPSILON."""
+        while self.current_run_time < self.t_end and self.macro_steps < max_macro_steps:
+            last = self.advance()
+            if 0.0 < last < 2.220446049250313e-16:
+                break
+        return self.macro_steps
+
+    # ---------------------------------------------------------------- output
+    def leaf_data(self):
+        """(keys, internal conservatives [n][NE][16][16][16|1]) of all leaves
+        from the Average buffers."""
+        keys = self.topo.leaves()
+        return keys, self.be.get_internal(self._slots(keys))
+
+    def leaf_cells(self):
+        """Cell centres, volumes and conservatives of every leaf cell, as the
+        finest-level output mesh lays them out (standard_finest_level_mesh_
+        generator.cpp:160-300): a leaf cell of level l spans dx_l per axis in
+        2D plus node_size / 2^l along z."""
+        keys, cons = self.leaf_data()
+        dim = self.dim
+        icz = gpu.IC if dim == 3 else 1
+        centres, vols, vals = [], [], []
+        for key, c in zip(keys, cons):
+            lv = key[0]
+            dx = self.cell_size(lv)
+            ii, jj, kk = np.meshgrid(np.arange(gpu.IC), np.arange(gpu.IC), np.arange(icz),
+                                     indexing="ij")
+            x = (key[1] * gpu.IC + ii + 0.5) * dx
+            y = (key[2] * gpu.IC + jj + 0.5) * dx
+            z = (key[3] * icz + kk + 0.5) * dx
+            centres.append(np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1))
+            dz = dx if dim == 3 else self.node_size / float(1 << lv)
+            vols.append(np.full(x.size, dx * dx * dz))
+            vals.append(c.reshape(dim + 2, -1).T)
+        return np.concatenate(centres), np.concatenate(vols), np.concatenate(vals)
+
+
+class GpuBackend:
+    """The driver's cell operations as gfx950 kernels on device pools."""
+
+    def __init__(self, dim: int, capacity: int, flux: int = gpu.FLUX_HLLC,
+                 precision: int = gpu.FP64, gamma: float = 1.4, background_pressure: float = 0.0):
+        import torch
+
+        self.torch = torch
+        self.lib = gpu.load_library()
+        self.dim = dim
+        self.capacity = int(capacity)
+        self.flux, self.precision = flux, precision
+        self.gamma, self.b = gamma, background_pressure
+        tz = gpu.TC if dim == 3 else 1
+        icz = gpu.IC if dim == 3 else 1
+        shape = (self.capacity, dim + 2, gpu.TC, gpu.TC, tz)
+        self.avg = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        self.rhs = torch.zeros_like(self.avg)
+        self.init = torch.zeros_like(self.avg)
+        sshape = (self.capacity, 6, dim + 2, gpu.IC, icz)
+        self.jflux = torch.zeros(sshape, dtype=torch.float64, device="cuda")
+        self.jcons = torch.zeros_like(self.jflux)
+        self.state = torch.zeros(ctypes.sizeof(gpu.CaseState), dtype=torch.uint8, device="cuda")
+        self.ctx = ctypes.c_void_p()
+        gpu.check(self.lib.alpaca_context_create(gpu._stream_ptr(), ctypes.byref(self.ctx)),
+                  "context_create")
+        self.lti = lts.LevelTimeIntegrator(dim)
+        self.mrop = mr.Multiresolution()
+
+    def close(self):
+        if self.ctx:
+            gpu.check(self.lib.alpaca_context_destroy(self.ctx), "context_destroy")
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- helpers
+    def _idx(self, slots):
+        return self.torch.as_tensor(np.asarray(slots, dtype=np.int64), device="cuda")
+
+    def _list(self, slots):
+        return self.torch.as_tensor(np.asarray(slots, dtype=np.int32), device="cuda")
+
+    def _links(self, links):
+        return self.torch.as_tensor(mr.make_links(links), device="cuda")
+
+    def _p(self, t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    def _desc(self, n, cell_size):
+        return gpu.make_desc(self.dim, [n, 1, 1], [0, 0, 0], flux=self.flux,
+                             precision=self.precision, gamma=self.gamma, b=self.b,
+                             cell_size=cell_size)
+
+    # -- node data
+    def clear(self, slot):
+        for t in (self.avg, self.rhs, self.init, self.jflux, self.jcons):
+            t[slot].zero_()
+
+    def set_internal(self, slots, cons):
+        """Internal cells of the right-hand-side buffers (ImposeInitialCondition)."""
+        h, i = gpu.HS, gpu.IC
+        src = self.torch.as_tensor(np.ascontiguousarray(cons), device="cuda")
+        idx = self._idx(slots)
+        if self.dim == 3:
+            self.rhs[idx, :, h:h + i, h:h + i, h:h + i] = src
+        else:
+            self.rhs[idx, :, h:h + i, h:h + i, :] = src
+
+    def get_internal(self, slots):
+        h, i = gpu.HS, gpu.IC
+        a = self.avg[self._idx(slots)]
+        if self.dim == 3:
+            return a[:, :, h:h + i, h:h + i, h:h + i].cpu().numpy()
+        return a[:, :, h:h + i, h:h + i, :].cpu().numpy()
+
+    def swap(self, slots):
+        idx = self._idx(slots)
+        a = self.avg.index_select(0, idx)
+        self.avg.index_copy_(0, idx, self.rhs.index_select(0, idx))
+        self.rhs.index_copy_(0, idx, a)
+
+    def fill_initial(self, slots):
+        self.lti.fill_initial_buffer(self.avg, self.init, self._list(slots), 0)
+
+    # -- stage operations
+    def right_hand_side(self, slots, cell_size):
+        idx = self._idx(slots)
+        n = len(slots)
+        a = self.avg.index_select(0, idx).contiguous()
+        r = self.torch.empty_like(a)
+        j = self.jflux.index_select(0, idx).contiguous()
+        desc = self._desc(n, cell_size)
+        gpu.check(self.lib.alpaca_ctx_right_hand_side_jump_fluxes(
+            self.ctx, ctypes.byref(desc), n, self._p(a), self._p(r), self._p(j)), "rhs")
+        self.rhs.index_copy_(0, idx, r)
+        self.jflux.index_copy_(0, idx, j)
+
+    def prepare(self, slots, stage):
+        if stage > 0:
+            self.lti.prepare_buffer_for_integration(self.avg, self.init, self._list(slots), stage)
+
+    def lts_timestep(self, micro, n, stage, jump):
+        self.lti.micro_timestep_sizes = list(micro)
+        return self.lti.timestep(stage, n, jump)
+
+    def integrate_node(self, slots, jump_timestep, timestep):
+        lst = self._list(slots)
+        nb = self.capacity
+        gpu.check(self.lib.alpaca_lts_integrate_jump_conservatives(
+            self.dim, self._p(self.jcons), self._p(self.jflux), nb, self._p(lst), len(slots),
+            jump_timestep, gpu._stream_ptr()), "lts_integrate_jump_conservatives")
+        gpu.check(self.lib.alpaca_lts_integrate(self.dim, self._p(self.avg), self._p(self.rhs), nb,
+                                                self._p(lst), len(slots), None, timestep,
+                                                gpu._stream_ptr()), "lts_integrate")
+
+    def integrate_range(self, slots, rng, timestep):
+        lst = self._list(slots)
+        r = list(rng) + [0, 1] * (3 - len(rng) // 2)
+        cr = (ctypes.c_int * 6)(*r)
+        gpu.check(self.lib.alpaca_lts_integrate(self.dim, self._p(self.avg), self._p(self.rhs),
+                                                self.capacity, self._p(lst), len(slots), cr,
+                                                timestep, gpu._stream_ptr()), "lts_integrate halo")
+
+    def average(self, links):
+        self.mrop.average(self.rhs, self.dim, self._links(links))
+
+    def average_jump(self, links):
+        self.mrop.average_jump_buffer(self.jcons, self.dim, self._links(links))
+
+    def halo_update(self, slots, table, jump_links, jump_locs, side_bc):
+        lst = self._list(slots)
+        tab = self.torch.as_tensor(np.ascontiguousarray(table, dtype=np.int32), device="cuda")
+        bc = (ctypes.c_int * 6)(*(list(side_bc) + [0] * (6 - len(side_bc))))
+        st = gpu._stream_ptr()
+        gpu.check(self.lib.alpaca_fill_halos_list(self.dim, self._p(self.rhs), self.capacity,
+                                                  self._p(lst), len(slots), self._p(tab), bc, 0,
+                                                  st), "fill_halos_list")
+        if jump_links:
+            locs = self.torch.as_tensor(np.asarray(jump_locs, dtype=np.int32), device="cuda")
+            self.mrop.jump_halo_update(self.rhs, self.dim, self._links(jump_links), locs)
+        gpu.check(self.lib.alpaca_fill_halos_list(self.dim, self._p(self.rhs), self.capacity,
+                                                  self._p(lst), len(slots), self._p(tab), bc, 1,
+                                                  st), "fill_halos_list (after jumps)")
+
+    def jump_flux_adjustment(self, entries, one_cell_size):
+        ent = self.torch.as_tensor(np.ascontiguousarray(entries, dtype=np.int32), device="cuda")
+        lts.jump_flux_adjustment(self.dim, self.rhs, self.jcons, ent, one_cell_size)
+
+    def reset_jump_conservatives(self, slots):
+        lts.reset_jump_conservatives(self.dim, self.jcons, self._list(slots))
+
+    def reset_jump_all(self, slots):
+        lst = self._list(slots)
+        lts.reset_jump_conservatives(self.dim, self.jcons, lst)
+        lts.reset_jump_conservatives(self.dim, self.jflux, lst)
+
+    def predict(self, links, rng):
+        self.mrop.prediction(self.rhs, self.dim, self._links(links), rng)
+
+    def decisions(self, links, thresholds):
+        m = mr.Multiresolution(norm=mr.LINF)
+        lk = self._links(links)
+        n = lk.shape[0]
+        det = self.torch.empty(n, dtype=self.torch.float64, device="cuda")
+        dec = self.torch.empty(n, dtype=self.torch.int32, device="cuda")
+        eps = (ctypes.c_double * len(thresholds))(*thresholds)
+        gpu.check(self.lib.alpaca_mr_child_needs_remeshing(
+            self.dim, self._p(self.rhs), self.capacity, self._p(lk), n, m.norm, eps,
+            len(thresholds), self._p(det), self._p(dec), gpu._stream_ptr()),
+            "mr_child_needs_remeshing")
+        return dec.cpu().numpy()
+
+    def timestep(self, leaf_slots, cfl, dx_finest, run_time, t_end):
+        a = self.avg.index_select(0, self._idx(leaf_slots)).contiguous()
+        desc = self._desc(len(leaf_slots), dx_finest)
+        st = gpu._stream_ptr()
+        sp = self._p(self.state)
+        gpu.check(self.lib.alpaca_case_state_init(sp, run_time, t_end, cfl, st), "state_init")
+        gpu.check(self.lib.alpaca_timestep(ctypes.byref(desc), self._p(a), sp, st), "timestep")
+        rec = gpu.CaseState.from_buffer_copy(self.state.cpu().numpy().tobytes())
+        return rec.dt

@@ -811,6 +811,30 @@ static void block_signal(run_t *r, int lo, int hi, void *arg) {
   }
 }
 
+/* The same signal maximum for one haloed block (used by the multi-level
+ * driver's checker, whose leaves have different cell sizes: the reference
+ * divides the maximum over all leaves by the finest cell size,
+ * modular_algorithm_assembler.cpp:1932). */
+double ora_block_signal(int dim, double gamma, double b, const double *avg) {
+  layout_t const L = make_layout(dim);
+  double s = 0.0;
+  for (int i = HS; i < HS + IC; ++i)
+    for (int j = HS; j < HS + IC; ++j)
+      for (int k = (dim == 3 ? HS : 0); k < (dim == 3 ? HS + IC : 1); ++k) {
+        double c[NEMAX] = {0.0};
+        load_cell(&L, avg, i * L.stride[0] + j * L.stride[1] + k, c);
+        prim_t p;
+        cons_to_prim(dim, gamma, b, c, &p);
+        double const cs = eos_sos(gamma, b, p.rho, p.p);
+        double vps[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < dim; ++d)
+          vps[d] = fabs(p.vel[d]) + cs;
+        double const sum = dim == 2 ? vps[0] + vps[1] : cms3(vps[0], vps[1], vps[2]);
+        s = smax(s, sum);
+      }
+  return s;
+}
+
 /* One RK stage on blocks [lo, hi): FillInitialBuffer, UpdateFluxes,
  * PrepareBufferForIntegration, IntegrateNode (time_integrator.h:98-113,
  * 312-340, 369-457; runge_kutta_3_TVD.h:39-59). */

@@ -66,6 +66,10 @@ void ora_block_rhs(int dim, int flux, double gamma, double b,
 int ora_run(const ora_grid *g, double cfl, double t_end, int max_steps,
             int threads, double *state, double *dts, double *t_final);
 
+/* ComputeTimestepSize's signal maximum over one haloed block's internal
+ * cells: max over cells of sum_d(|u_d| + c) (inviscid terms). */
+double ora_block_signal(int dim, double gamma, double b, const double *avg);
+
 /* Conservation sums over a domain state [NE][NX][NY][NZ]: out[e] = sum. */
 void ora_domain_sums(int dim, const int *ncell, const double *state,
                      double *out);
