@@ -127,13 +127,41 @@ inline alpaca_case_desc MakeDesc(int dim, int const nb[3], int const bc[3], int 
 // Volume forces stay untouched (no convective volume term for these
 // equations).  The flux family is fixed at construction, as the reference's
 // Concretize<ConvectiveTermSolvers> selection fixes it at compile time.
+// Owner of an alpaca_context (the stream and face-flux scratch of the
+// per-block entries); movable, not copyable.
+class Context {
+ public:
+  explicit Context(hipStream_t stream, int dim = 3, int reserve_blocks = 0) {
+    Check(alpaca_context_create(stream, &ctx_), "alpaca_context_create");
+    if (reserve_blocks > 0)
+      Check(alpaca_context_reserve(ctx_, dim, reserve_blocks), "alpaca_context_reserve");
+  }
+  Context(Context const &) = delete;
+  Context &operator=(Context const &) = delete;
+  Context(Context &&o) noexcept : ctx_(o.ctx_) { o.ctx_ = nullptr; }
+  Context &operator=(Context &&o) noexcept {
+    std::swap(ctx_, o.ctx_);
+    return *this;
+  }
+  ~Context() {
+    if (ctx_) (void)alpaca_context_destroy(ctx_);
+  }
+  alpaca_context *get() const { return ctx_; }
+
+ private:
+  alpaca_context *ctx_ = nullptr;
+};
+
+// The reference's SpaceSolver owns its convective solver by value, one per
+// thread of the rank; each GpuConvectiveTermSolver owns its context, so
+// solvers on different streams never share scratch.
 template <int DIM> class GpuConvectiveTermSolver {
   using G = Geometry<DIM>;
 
  public:
   GpuConvectiveTermSolver(int flux_family, double gamma, double background_pressure,
                           int precision = ALPACA_FP64, hipStream_t stream = nullptr)
-      : stream_(stream) {
+      : stream_(stream), ctx_(stream, DIM, 1) {
     int const nb[3] = {1, 1, 1};
     int const bc[3] = {ALPACA_BC_ZERO_GRADIENT, ALPACA_BC_ZERO_GRADIENT,
                        ALPACA_BC_ZERO_GRADIENT};
@@ -159,8 +187,9 @@ template <int DIM> class GpuConvectiveTermSolver {
                             hipMemcpyHostToDevice, stream_), "upload fluxes");
     CheckHip(hipMemcpyAsync(fluxes_.get<char>() + 2 * fb, &fluxes_z[0][0][0][0], fb,
                             hipMemcpyHostToDevice, stream_), "upload fluxes");
-    Check(alpaca_convective_fluxes(&d, 1, block_.get<double>(), fluxes_.get<double>(), stream_),
-          "alpaca_convective_fluxes");
+    Check(alpaca_ctx_convective_fluxes(ctx_.get(), &d, 1, block_.get<double>(),
+                                       fluxes_.get<double>()),
+          "alpaca_ctx_convective_fluxes");
     CheckHip(hipMemcpyAsync(&fluxes_x[0][0][0][0], fluxes_.get<char>(), fb,
                             hipMemcpyDeviceToHost, stream_), "download fluxes");
     CheckHip(hipMemcpyAsync(&fluxes_y[0][0][0][0], fluxes_.get<char>() + fb, fb,
@@ -173,6 +202,7 @@ template <int DIM> class GpuConvectiveTermSolver {
  private:
   alpaca_case_desc desc_;
   hipStream_t stream_;
+  Context ctx_;
   mutable DeviceBuffer block_, fluxes_;
 };
 

@@ -167,8 +167,6 @@ def write(path_without_extension, time, dim, nb, cell_size, density, velocity, p
     """Write ``<path>.h5`` and ``<path>.xdmf`` for one output time, and append
     the time's grid to ``<series>.xdmf`` when a time-series file is given.
     density, pressure: [NX][NY][NZ]; velocity: [dim][NX][NY][NZ]."""
-    lib = _lib()
-    h5 = path_without_extension + ".h5"
     vertex_ids, coords = finest_level_mesh(dim, nb, cell_size)
     ncells = vertex_ids.shape[0]
     rho = _block_cell_order(np.asarray(density, dtype=np.float64), dim, nb)
@@ -176,6 +174,71 @@ def write(path_without_extension, time, dim, nb, cell_size, density, velocity, p
     vel = np.zeros((ncells, 3, 1))
     for d in range(dim):
         vel[:, d, 0] = _block_cell_order(np.asarray(velocity[d], dtype=np.float64), dim, nb)
+    return write_cells(path_without_extension, time, vertex_ids, coords, rho, vel, p, series)
+
+
+def multilevel_mesh(dim, nb0, node_size, leaf_keys, current_maximum_level):
+    """(vertex_ids, vertex_coordinates) of the leaves of a multi-level tree on
+    the finest-level vertex lattice (StandardFinestLevelMeshGenerator::
+    DoComputeVertexIDs, standard_finest_level_mesh_generator.cpp:214-300): a
+    leaf (level, i, j, k) spans level_factor = 2^(current max - level)
+    lattice steps per cell, its cells k outer and i fastest; lattice index n
+    lies at n * dx_finest (2D: the cell of a level-l leaf is one of its own
+    cell sizes thick)."""
+    res = 1 << current_maximum_level
+    dx_f = node_size / res / IC
+    icz = IC if dim == 3 else 1
+    vx = nb0[0] * res * IC + 1
+    vy = nb0[1] * res * IC + 1
+    vz = (nb0[2] * res * IC if dim == 3 else res) + 1
+    k, j, i = np.meshgrid(np.arange(icz), np.arange(IC), np.arange(IC), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1),
+               (0, 1, 1)]
+    ids = []
+    for level, bx, by, bz in leaf_keys:
+        lf = res >> level
+        x = (bx * IC + i) * lf
+        y = (by * IC + j) * lf
+        z = (bz * icz + k) * lf
+        ids.append(np.stack([(x + a * lf) + (y + b * lf) * vx + (z + c * lf) * vx * vy
+                             for a, b, c in corners], axis=1))
+    vertex_ids = np.concatenate(ids).astype(np.uint64)
+    g = np.arange(vx * vy * vz, dtype=np.int64)
+    coords = np.stack([(g % vx) * dx_f, ((g // vx) % vy) * dx_f, (g // (vx * vy)) * dx_f],
+                      axis=1).astype(np.float64)
+    return vertex_ids, coords
+
+
+def write_multilevel(path_without_extension, time, solver, series=None):
+    """Output of a multi-level run (alpaca_amd.multilevel.MultiLevelSolver):
+    every leaf's cells with density, velocity and pressure from its Average
+    buffer (ConservativesToPrimeStates, prime_state_handler.h:279-370)."""
+    keys, cons = solver.leaf_data()
+    dim = solver.dim
+    gamma, b = solver.gamma, solver.b
+    vertex_ids, coords = multilevel_mesh(dim, solver.topo.nb0, solver.node_size, keys,
+                                         solver.topo.current_maximum_level())
+    # leaf blocks [n][NE][16][16][icz] -> cells per leaf, k outer and i fastest
+    c = np.asarray(cons, dtype=np.float64).transpose(0, 1, 4, 3, 2).reshape(len(keys), dim + 2, -1)
+    c = c.transpose(1, 0, 2).reshape(dim + 2, -1)
+    rho = c[0]
+    one_rho = 1.0 / rho
+    vel = np.zeros((rho.size, 3, 1))
+    for d in range(dim):
+        vel[:, d, 0] = c[2 + d] * one_rho
+    q = sum(c[2 + d] * c[2 + d] for d in range(dim))
+    p = -gamma * b + (gamma - 1.0) * (c[1] - 0.5 * q / rho)
+    return write_cells(path_without_extension, time, vertex_ids, coords, rho, vel, p, series)
+
+
+def write_cells(path_without_extension, time, vertex_ids, coords, rho, vel, p, series=None):
+    """The HDF5 file and its XDMF description from cell-ordered data:
+    vertex_ids [cells][8], coords [vertices][3], rho and p [cells],
+    vel [cells][3][1]."""
+    lib = _lib()
+    h5 = path_without_extension + ".h5"
+    ncells = vertex_ids.shape[0]
     f64, u64 = _type("H5T_NATIVE_DOUBLE_g"), _type("H5T_NATIVE_ULLONG_g")
     fid = _ok(lib.H5Fcreate(h5.encode(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), "H5Fcreate")
     try:
@@ -195,9 +258,9 @@ def write(path_without_extension, time, dim, nb, cell_size, density, velocity, p
         lib.H5Gclose(g)
         g = _ok(lib.H5Gcreate2(fid, b"cell_data", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
                 "H5Gcreate2")
-        _write_dataset(g, "density", rho.reshape(ncells, 1, 1), f64)
-        _write_dataset(g, "velocity", vel, f64)
-        _write_dataset(g, "pressure", p.reshape(ncells, 1, 1), f64)
+        _write_dataset(g, "density", np.ascontiguousarray(rho).reshape(ncells, 1, 1), f64)
+        _write_dataset(g, "velocity", np.ascontiguousarray(vel), f64)
+        _write_dataset(g, "pressure", np.ascontiguousarray(p).reshape(ncells, 1, 1), f64)
         lib.H5Gclose(g)
     finally:
         lib.H5Fclose(fid)

@@ -78,3 +78,29 @@ def test_topology_rules():
             counts[lv] += 1
     assert counts == {0: 1, 1: 2, 2: 4, 3: 8}
     assert lts.levels_to_update(7, 3) == [3, 2, 1, 0]
+
+
+def test_multilevel_output_read_by_reference_analysis(tmp_path):
+    """A maximum-level-2 SodX run written on the finest-level mesh (leaves of
+    levels 1 and 2) and read back with sod_analysis.py's recipe (cell centres
+    and volumes from the vertices): the testsuite's norms (CSV row 21)."""
+    from alpaca_amd import h5output
+    from test_h5output import HAVE_HDF5, _read_like_sod_analysis
+    import exact_riemann
+    if not HAVE_HDF5:
+        pytest.skip("libhdf5 not available")
+    be = oracle_ml.OracleBackend(2, ml_cases.capacity(2, 4, 2), flux=gpu.FLUX_HLLC)
+    sol = ml_cases.sod_solver(be, gpu.FLUX_HLLC, 2)
+    sol.create_new_simulation()
+    sol.run()
+    h5 = h5output.write_multilevel(str(tmp_path / "sod_ml"), sol.current_run_time, sol)
+    x, rho, u, p, vol, t = _read_like_sod_analysis(h5)
+    assert t == 0.2 and len(rho) == 256 * len(sol.topo.leaves())
+    ex = exact_riemann.sample((x - 0.5) / t, 1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4)
+    got_rho = np.sum(np.abs(rho / ex[0] - 1.0) * vol)
+    got_p = np.sum(np.abs(p / ex[2] - 1.0) * vol)
+    want = ml_cases.SOD_CSV[(gpu.FLUX_HLLC, 2)]
+    assert abs(got_rho / want[0] - 1.0) < 1e-6 and abs(got_p / want[1] - 1.0) < 1e-6
+    # the vertex extents give each leaf cell its own level's size
+    levels = {round(v ** (1 / 3) * 64 * 4) for v in vol}
+    assert levels <= {1, 2}
