@@ -445,6 +445,40 @@ class MultiLevelSolver:
                 break
         return self.macro_steps
 
+    # --------------------------------------------------------------- restart
+    def write_restart(self, path):
+        """Every node of every level with its haloed Average buffer and its
+        reference node id (RestartManager::WriteRestartFile,
+        restart_manager.cpp:300-470); call between macro steps."""
+        from . import restart
+
+        keys = sorted(self.slot)
+        ids = [restart.node_id(*k) for k in keys]
+        blocks = self.be.get_blocks(self._slots(keys))
+        return restart.write_nodes(path, self.current_run_time, self.dim, ids, blocks, self.gamma,
+                                   self.b, self.maximum_level)
+
+    def restore(self, path):
+        """RestoreSimulation + FinalizeSimulationRestart (restart_manager.cpp:
+        91-290, modular_algorithm_assembler.cpp:247-284): the nodes of the
+        file created with their conservatives in the right-hand-side buffers,
+        leaves where no child exists, then the swap of all levels."""
+        from . import restart
+
+        if self.slot:
+            raise ValueError("restore needs a solver that holds no nodes yet")
+        t, _, ids, cons = restart.read_nodes(path, self.dim, self.maximum_level)
+        keys = [restart.decode_node_id(int(i)) for i in ids]
+        keys = [(lv, i, j, k if self.dim == 3 else 0) for lv, i, j, k in keys]
+        for key in keys:
+            self._new_node(key)
+        for key in keys:
+            self.topo.leaf[key] = not any(self.topo.exists(c) for c in self.topo.children(key))
+        self.be.set_rhs_blocks(self._slots(keys), cons)
+        self.swap(self.all_levels)
+        self.current_run_time = float(t)
+        self.micro = []
+
     # ---------------------------------------------------------------- output
     def leaf_data(self):
         """(keys, internal conservatives [n][NE][16][16][16|1]) of all leaves
@@ -548,6 +582,14 @@ class GpuBackend:
             self.rhs[idx, :, h:h + i, h:h + i, h:h + i] = src
         else:
             self.rhs[idx, :, h:h + i, h:h + i, :] = src
+
+    def get_blocks(self, slots):
+        return self.avg[self._idx(slots)].cpu().numpy()
+
+    def set_rhs_blocks(self, slots, blocks):
+        src = self.torch.as_tensor(np.ascontiguousarray(blocks), device="cuda")
+        self.rhs.index_copy_(0, self._idx(slots),
+                             src.reshape((len(slots),) + tuple(self.rhs.shape[1:])))
 
     def get_internal(self, slots):
         h, i = gpu.HS, gpu.IC

@@ -68,6 +68,40 @@ def decode_level0_id(node_id: int):
     return tuple(b)
 
 
+def node_id(level: int, bx: int, by: int, bz: int) -> int:
+    """Id of the node at block position (bx, by, bz) of `level`: its level-0
+    ancestor's id followed by one sibling position (bit 0 x, bit 1 y, bit 2 z)
+    per level, IdsOfChildren = (parent << 3) + position (id_information.h:
+    150-192)."""
+    nid = level0_id(bx >> level, by >> level, bz >> level)
+    for m in range(level - 1, -1, -1):
+        pos = ((bx >> m) & 1) | (((by >> m) & 1) << 1) | (((bz >> m) & 1) << 2)
+        nid = (nid << 3) + pos
+    return nid
+
+
+def level_of_node(nid: int) -> int:
+    """LevelOfNode (id_information.h:59-67): the shadow bits cut, three bits
+    per level above the seed's 0b1010."""
+    w = int(nid) >> POSITION_BITS
+    level = 0
+    while w > 10:
+        w >>= 3
+        level += 1
+    return level
+
+
+def decode_node_id(nid: int):
+    """(level, bx, by, bz) of a node id."""
+    level = level_of_node(nid)
+    b = list(decode_level0_id(int(nid) >> (3 * level)))
+    for m in range(level - 1, -1, -1):
+        pos = (int(nid) >> (3 * m)) & 7
+        for d in range(3):
+            b[d] = (b[d] << 1) | ((pos >> d) & 1)
+    return (level, b[0], b[1], b[2])
+
+
 def creation_order(nb):
     """(package block index, node id) in the reference's level-0 creation
     order: x fastest, then y, then z (nb: 2 or 3 block counts)."""
@@ -135,7 +169,6 @@ def write(path, time, dim, nb, blocks, gamma, b=0.0, order=None):
     package's block order (bx * ny + by) * nz + bz; order: the (block index,
     node id) sequence to write, default the reference's creation order.
     Returns the path."""
-    lib = _lib()
     nb3 = _nb3(dim, nb)
     tz = TC if dim == 3 else 1
     blocks = np.asarray(blocks, dtype=np.float64).reshape(-1, dim + 2, TC, TC, tz)
@@ -145,7 +178,18 @@ def write(path, time, dim, nb, blocks, gamma, b=0.0, order=None):
         order = creation_order(nb3)
     idx = [k for k, _ in order]
     ids = np.array([i for _, i in order], dtype=np.uint64)
-    cons = np.ascontiguousarray(blocks[idx])
+    return write_nodes(path, time, dim, ids, blocks[idx], gamma, b, 0)
+
+
+def write_nodes(path, time, dim, ids, blocks, gamma, b=0.0, maximum_level=0):
+    """Write a restart file of the given nodes (ids) and their haloed Average
+    buffers [n][NE][24][24][TZ] (RestartManager::WriteRestartFile writes every
+    node of every level, restart_manager.cpp:406-431)."""
+    lib = _lib()
+    tz = TC if dim == 3 else 1
+    ids = np.asarray(ids, dtype=np.uint64)
+    cons = np.ascontiguousarray(np.asarray(blocks, dtype=np.float64).reshape(-1, dim + 2, TC, TC, tz))
+    idx = range(cons.shape[0])
     # halo cells that no halo update reaches (edges and corners in 3D) hold
     # zeros on the device; their prime states are written as zeros
     with np.errstate(divide="ignore", invalid="ignore"):
@@ -176,7 +220,7 @@ def write(path, time, dim, nb, blocks, gamma, b=0.0, order=None):
         _attr(g, "Dimensions", dim, u32, ctypes.c_uint)
         _attr(g, "InternalCells", IC, u32, ctypes.c_uint)
         _attr(g, "HaloSize", HS, u32, ctypes.c_uint)
-        _attr(g, "MaximumLevel", 0, u32, ctypes.c_uint)
+        _attr(g, "MaximumLevel", int(maximum_level), u32, ctypes.c_uint)
         for name in ("LengthReference", "VelocityReference", "DensityReference",
                      "TemperatureReference"):
             _attr(g, name, 1.0, f64, ctypes.c_double)
@@ -216,12 +260,11 @@ def _read_attr(fid, name, h5name, ctype):
     return v.value
 
 
-def read(path, dim=None):
-    """Read a restart file: (time, dim, nb, blocks [n][NE][24][24][TZ] in the
-    package's block order).  Like RestoreSimulation's consistency checks it
-    refuses another dimension (when dim is given), block size, halo width,
-    maximum level or non-unit references, and what this path cannot continue:
-    several materials, interface blocks, an incomplete level-0 grid."""
+def read_nodes(path, dim=None, maximum_level=0):
+    """Read a restart file of any level: (time, dim, ids, blocks [n][NE][24][24][TZ]
+    in file order).  Refuses another dimension (when given), block size, halo
+    width, maximum level or non-unit references, several materials and
+    interface blocks (RestoreSimulation's checks, restart_manager.cpp:91-170)."""
     lib = _lib()
     fid = _ok(lib.H5Fopen(str(path).encode(), h5output.H5F_ACC_RDONLY, H5P_DEFAULT), "H5Fopen")
     try:
@@ -233,8 +276,8 @@ def read(path, dim=None):
             raise RestartError("restart file has another number of internal cells")
         if _read_attr(fid, "HaloSize", u32, ctypes.c_uint) != HS:
             raise RestartError("restart file has another halo size")
-        if _read_attr(fid, "MaximumLevel", u32, ctypes.c_uint) != 0:
-            raise RestartError("restart file has refined levels (maximum level above 0)")
+        if _read_attr(fid, "MaximumLevel", u32, ctypes.c_uint) != maximum_level:
+            raise RestartError("restart file has another maximum level")
         for name in ("LengthReference", "VelocityReference", "DensityReference",
                      "TemperatureReference"):
             if _read_attr(fid, name, f64, ctypes.c_double) != 1.0:
@@ -252,6 +295,14 @@ def read(path, dim=None):
     tz = TC if fdim == 3 else 1
     if cons.shape[1:] != (fdim + 2, TC, TC, tz) or cons.shape[0] != ids.shape[0]:
         raise RestartError(f"Conservatives has shape {cons.shape}")
+    return t, int(fdim), ids, cons
+
+
+def read(path, dim=None):
+    """Read a level-0 restart file: (time, dim, nb, blocks [n][NE][24][24][TZ]
+    in the package's block order).  Besides read_nodes' checks it refuses an
+    incomplete level-0 grid."""
+    t, fdim, ids, cons = read_nodes(path, dim, 0)
     pos = [decode_level0_id(i) for i in ids]
     nb3 = [max(p[d] for p in pos) + 1 for d in range(3)]
     if len(set(pos)) != len(pos) or len(pos) != nb3[0] * nb3[1] * nb3[2]:

@@ -93,6 +93,13 @@ class OracleBackend:
         for s, c in zip(slots, cons):
             self.rhs[s][self._internal()] = c
 
+    def get_blocks(self, slots):
+        return np.stack([self.avg[s] for s in slots])
+
+    def set_rhs_blocks(self, slots, blocks):
+        for s, b in zip(slots, blocks):
+            self.rhs[s] = b
+
     def get_internal(self, slots):
         return np.stack([self.avg[s][self._internal()] for s in slots])
 

@@ -67,3 +67,21 @@ def test_periodic_gpu_equals_oracle_and_conserves(torch, dim, lmax, steps):
     drift = np.abs(s1 - s0)[:2] / a0[:2]
     print(f"  drift mass/energy {drift}", flush=True)
     assert np.all(drift <= 1e-13)
+
+
+def test_restart_written_by_cpu_continues_on_gpu(torch, tmp_path):
+    """A multi-level snapshot written by the CPU backend's run, restored by
+    the GPU backend: both continue to the same tree and bits."""
+    print("[multilevel restart]", flush=True)
+    cap = ml_cases.capacity(2, 4, 2)
+    o = ml_cases.sod_solver(oracle_ml.OracleBackend(2, cap), gpu.FLUX_ROE, 2)
+    o.create_new_simulation()
+    for _ in range(8):
+        o.advance()
+    path = o.write_restart(str(tmp_path / "ml.h5"))
+    g = ml_cases.sod_solver(multilevel.GpuBackend(2, cap, flux=gpu.FLUX_ROE), gpu.FLUX_ROE, 2)
+    g.restore(path)
+    for n in range(4):
+        o.advance()
+        g.advance()
+        _same(g, o, f"after restore, step {n + 1}")
