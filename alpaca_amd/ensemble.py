@@ -53,6 +53,7 @@ class Ensemble:
                                                       ctypes.byref(h)), "ensemble_create")
         self.h = h
         self.desc = {}
+        self._staged = []  # admission sources in flight on the stream
 
     @staticmethod
     def bytes_needed(dim, max_blocks, max_cases):
@@ -69,16 +70,22 @@ class Ensemble:
         except Exception:
             pass
 
-    def add_case(self, desc: gpu.CaseDesc, state0, t_end: float, cfl: float) -> int:
-        """Admit a case from its domain conservatives [NE][NX][NY][NZ]."""
+    def add_case(self, desc: gpu.CaseDesc, state0, t_end: float, cfl: float,
+                 sync: bool = True) -> int:
+        """Admit a case from its domain conservatives [NE][NX][NY][NZ].  The
+        copy into the pool is ordered on the ensemble's stream; with
+        sync=False the call returns at once and the staged source is kept
+        alive until the next states() (which synchronises the stream)."""
         torch = self.torch
         nb = [desc.nb[0], desc.nb[1], desc.nb[2] if desc.dim == 3 else 1]
+        with torch.cuda.stream(self.stream):
+            if torch.is_tensor(state0):
+                g = state0.to(device="cuda", dtype=torch.float64)
+            else:
+                g = torch.as_tensor(np.ascontiguousarray(state0), device="cuda")
+            blocks = gpu.to_blocks(g, desc.dim, nb).contiguous()
         if torch.is_tensor(state0):
-            g = state0.to(device="cuda", dtype=torch.float64)
-        else:
-            g = torch.as_tensor(np.ascontiguousarray(state0), device="cuda")
-        blocks = gpu.to_blocks(g, desc.dim, nb).contiguous()
-        self.stream.wait_stream(torch.cuda.current_stream())
+            self.stream.wait_stream(torch.cuda.current_stream())
         cid = ctypes.c_int()
         st = self.lib.alpaca_ensemble_add_case(self.h, ctypes.byref(desc),
                                                ctypes.c_void_p(blocks.data_ptr()), t_end, cfl,
@@ -86,8 +93,10 @@ class Ensemble:
         if st == 5:
             raise MemoryError("ensemble capacity exhausted")
         gpu.check(st, "ensemble_add_case")
-        # The copy into the pool is stream-ordered; keep the source alive until done.
-        self.stream.synchronize()
+        if sync:
+            self.stream.synchronize()
+        else:
+            self._staged.append(blocks)
         self.desc[cid.value] = (desc, nb)
         return cid.value
 
@@ -109,6 +118,7 @@ class Ensemble:
         n = self.max_cases if n is None else n
         arr = (gpu.CaseState * n)()
         gpu.check(self.lib.alpaca_ensemble_read_states(self.h, arr, n), "ensemble_read_states")
+        self._staged.clear()  # the stream has drained: staged sources are consumed
         return list(arr)
 
     def case_blocks(self, case_id: int):

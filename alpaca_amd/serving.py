@@ -169,7 +169,9 @@ class Server:
 
     def admit(self, req: Request) -> int:
         pool = self.pools[req.dim]
-        return pool.add_case(case_desc(req), initial_state(req), req.t_end, req_cfl(req))
+        with self.torch.cuda.stream(pool.stream):
+            st0 = initial_state(req)
+        return pool.add_case(case_desc(req), st0, req.t_end, req_cfl(req), sync=False)
 
     def serve(self, requests, keep_results=False, max_wall=None, log=None):
         """Serve the stream; keep_results: True (every request's final state)
