@@ -636,8 +636,12 @@ def main():
     if len(results) > 1:
         summary = {}
         for m, r in results.items():
+            pk = PEAK_FP32 if m == "mixed" else PEAK_FP64
             summary[m] = {"cell_updates_per_s": r["value"], "ms_per_step": r["ms_per_step"],
-                          "stage_avg_ms": r["stage_avg_ms"]}
+                          "stage_avg_ms": r["stage_avg_ms"],
+                          "roofline_frac": (r["cells_per_launch"] * flop_cell_stage /
+                                            (r["stage_avg_ms"] / 1e3) / pk),
+                          "roofline_peak": "fp32 vector" if m == "mixed" else "fp64 vector"}
         ref = results.get("fp64")
         if ref is not None and "rho" in ref:
             for m, r in results.items():
