@@ -481,8 +481,8 @@ __global__ __launch_bounds__(kThreads) void signal_kernel(const double *__restri
 __global__ void dt_kernel(alpaca_case_state *state, double cell_size) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double const s = __longlong_as_double((long long)state->smax);
-  double const rate = s / cell_size;
-  double dt = rate == 0.0 ? 1.7976931348623157e308 : state->cfl / rate;
+  double const rate = exact::fdiv(s, cell_size);
+  double dt = rate == 0.0 ? 1.7976931348623157e308 : exact::fdiv(state->cfl, rate);
   if (state->time + dt > state->t_end) dt = exact::smax(state->t_end - state->time, 0.0);
   state->dt = dt;
   state->smax = 0;
@@ -494,6 +494,24 @@ __global__ void dt_kernel(alpaca_case_state *state, double cell_size) {
 __global__ __launch_bounds__(kThreads) void zero_kernel(double *p, size_t n) {
   size_t const stride = (size_t)gridDim.x * kThreads;
   for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < n; t += stride) p[t] = 0.0;
+}
+
+// The exact form's FP64 division and square root as the face kernels
+// evaluate them, elementwise (alpaca_exact_fp64_ops): op 0 the division of
+// the physical quantities, op 1 the WENO5 weight division (its in-range core),
+// op 2 the square root.
+template <int OP>
+__global__ __launch_bounds__(kThreads) void exact_ops_kernel(const double *__restrict__ a,
+                                                             const double *__restrict__ b,
+                                                             double *__restrict__ out, int n) {
+  int const i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  if constexpr (OP == 0)
+    out[i] = exact::fdiv(a[i], b[i]);
+  else if constexpr (OP == 1)
+    out[i] = exact::weno_div(a[i], b[i]);
+  else
+    out[i] = exact::dsqrt(a[i]);
 }
 
 // FinishMacroTimestep (time_integrator.h:244-248).
@@ -639,7 +657,7 @@ int validate(const alpaca_case_desc *d) {
 int case_blocks(const alpaca_case_desc *d) { return d->nb[0] * d->nb[1] * (d->dim == 3 ? d->nb[2] : 1); }
 
 template <int DIM>
-size_t flux_doubles(int nblocks) {
+constexpr size_t flux_doubles(int nblocks) {
   return (size_t)nblocks * DIM * Dims<DIM>::NE * faces_per_dir<DIM>();
 }
 
@@ -1005,6 +1023,22 @@ int alpaca_right_hand_side_jump_fluxes(const alpaca_case_desc *desc, int nblocks
   alpaca_context *ctx = nullptr;
   if ((st = stream_context(static_cast<hipStream_t>(stream), &ctx)) != ALPACA_OK) return st;
   return rhs_context(ctx, desc, nblocks, d_avg, d_rhs, d_jump_fluxes);
+}
+
+int alpaca_exact_fp64_ops(int op, const double *d_a, const double *d_b, double *d_out, int n,
+                          void *stream) {
+  if (op < 0 || op > 2 || n < 0 || n > kMaxBlocks * 64 || !d_a || !d_out || (op < 2 && !d_b))
+    return ALPACA_ERR_INVALID;
+  if (n == 0) return ALPACA_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  dim3 const grid((unsigned)((n + kThreads - 1) / kThreads));
+  if (op == 0)
+    hipLaunchKernelGGL(exact_ops_kernel<0>, grid, dim3(kThreads), 0, s, d_a, d_b, d_out, n);
+  else if (op == 1)
+    hipLaunchKernelGGL(exact_ops_kernel<1>, grid, dim3(kThreads), 0, s, d_a, d_b, d_out, n);
+  else
+    hipLaunchKernelGGL(exact_ops_kernel<2>, grid, dim3(kThreads), 0, s, d_a, d_a, d_out, n);
+  return hip_status(hipGetLastError());
 }
 
 int alpaca_rk3_stage(const alpaca_case_desc *desc, int stage, const double *d_avg, double *d_init,

@@ -54,8 +54,8 @@ __global__ void ens_dt_kernel(const CaseSlot *slots, alpaca_case_state *states, 
     return;
   }
   double const sig = __longlong_as_double((long long)s.smax);
-  double const rate = sig / slots[i].cell_size;
-  double dt = rate == 0.0 ? 1.7976931348623157e308 : s.cfl / rate;
+  double const rate = exact::fdiv(sig, slots[i].cell_size);
+  double dt = rate == 0.0 ? 1.7976931348623157e308 : exact::fdiv(s.cfl, rate);
   if (s.time + dt > s.t_end) dt = exact::smax(s.t_end - s.time, 0.0);
   s.dt = dt;
   s.smax = 0;

@@ -75,6 +75,25 @@ template <int DIM, int DIR> __device__ __forceinline__ constexpr int seq(int q) 
 
 constexpr int round_up64(int n) { return (n + 63) / 64 * 64; }
 
+// Reads of one haloed block through a raw buffer resource: the block's base
+// address lives in scalar registers, a thread passes one byte offset in a
+// VGPR and the field/stencil part as a scalar offset, so a read costs no
+// 64-bit address arithmetic.  The block pointer must be uniform over the
+// wavefront.  num_records is the block's size in bytes; every offset the
+// kernels form stays inside it, and voffset and soffset are never negative.
+template <int DIM> struct BlockReader {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ explicit BlockReader(const double *blk)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(blk), (short)0,
+                                            (int)(Dims<DIM>::BLOCK * sizeof(double)),
+                                            0x00020000)) {}
+  __device__ __forceinline__ double operator()(int voffset_bytes, int soffset_bytes) const {
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    u2 const v = __builtin_amdgcn_raw_buffer_load_b64(r, voffset_bytes, soffset_bytes, 0);
+    return __builtin_bit_cast(double, v);
+  }
+};
+
 // Logical workgroup index with consecutive logical groups kept on one XCD:
 // the hardware deals workgroups round-robin over the 8 XCDs, so group g runs
 // on XCD g % 8.  Remapping makes each XCD sweep a contiguous range of blocks,

@@ -37,6 +37,48 @@ __device__ __forceinline__ double nr_sqrt64(double x) {
   return x == 0.0 ? 0.0 : g;
 }
 
+// Divisors whose reciprocal lies within about 2^-100 (relative) of a rounding
+// midpoint: b = 2^e (2 - k 2^-52) for small odd k, whose significand has its
+// top 20 bits set (a density just below a power of two after round-off is
+// one).  For them the two Newton steps of the hardware division sequence can
+// leave the reciprocal one ulp from RN(1/b), and a quotient that also lies
+// within about 2^-105 of a midpoint then rounds the wrong way (0x1.6666666666663p-2
+// / 0x1.ffffffffffffbp-4 gives 2.8 instead of x86's 2.8000000000000003).
+// Other divisors' reciprocals come out of the sequence correctly rounded, and
+// with a correctly rounded reciprocal its final residual correction rounds the
+// quotient correctly.
+__device__ __forceinline__ bool recip_hard(double b) {
+  return ((unsigned)((unsigned long long)__double_as_longlong(b) >> 32) & 0xFFFFFu) == 0xFFFFFu;
+}
+
+// The correctly rounded quotient from q, a quotient within one ulp of a/b:
+// of q and its neighbour toward a/b, the one whose residual a - b q' is
+// smaller (both residuals are exact in an FMA when |a| >= 2^-960; the distance
+// to a/b is |residual| / |b| for both).  A tie cannot occur: no quotient of
+// two doubles lies exactly on a midpoint.
+__device__ __forceinline__ double div_round(double a, double b, double q) {
+  if (!(__builtin_fabs(q) <= 1.7976931348623157e308) || q == 0.0 ||
+      !(__builtin_fabs(a) >= 0x1p-960))
+    return q;
+  double const r = __builtin_fma(-b, q, a);
+  if (r == 0.0) return q;
+  bool const above = (r > 0.0) == (b > 0.0);  // a/b > q
+  long long const qi = __double_as_longlong(q);
+  double const q2 = __longlong_as_double(above == (q > 0.0) ? qi + 1 : qi - 1);
+  double const r2 = __builtin_fma(-b, q2, a);
+  return __builtin_fabs(r2) < __builtin_fabs(r) ? q2 : q;
+}
+
+// The WENO5 weight divisions c / x in the exact form: x = (s + 1e-6)^2 in
+// [1e-12, 2^700) and c in {0.1, 0.6, 0.3} (and 1 / (a1 + a2 + a3) >= 2^-704),
+// where the hardware division sequence scales nothing and fixes up nothing,
+// so it is exactly its core nr_div64; the rare divisor with a hard reciprocal
+// takes the exact rounding decision.
+__device__ __forceinline__ double weno_div(double c, double x) {
+  double const q = nr_div64(c, x);
+  return __builtin_expect(recip_hard(x), 0) ? div_round(c, x, q) : q;
+}
+
 // Division and square root: the compiler's FP64 sequences (what a/b and
 // sqrt(x) compile to) in the exact namespace; Newton-Raphson FP64 in the fast
 // namespace; hardware reciprocal and square root (about 1 ulp) for the FP32
@@ -46,8 +88,12 @@ template <typename R> __device__ __forceinline__ R fdiv(R a, R b) {
     return a * __builtin_amdgcn_rcpf(b);
   else if constexpr (kFastArith)
     return nr_div64(a, b);
-  else
-    return a / b;
+  else {
+    // The compiler's sequence, correctly rounded also for the divisors with a
+    // hard reciprocal.
+    double const q = a / b;
+    return __builtin_expect(recip_hard(b), 0) ? div_round(a, b, q) : q;
+  }
 }
 template <typename R> __device__ __forceinline__ R dsqrt(R x) {
   if constexpr (std::is_same<R, float>::value) {
@@ -210,15 +256,15 @@ template <typename R> __device__ __forceinline__ R weno5(R v1, R v2, R v3, R v4,
     auto hi = [](double v) { return (unsigned)((unsigned long long)__double_as_longlong(v) >> 32); };
     unsigned const hmax = __builtin_elementwise_max(__builtin_elementwise_max(hi(x1), hi(x2)), hi(x3));
     if (__builtin_expect(hmax < 0x6BB00000u, 1)) {  // 2^700
-      a1 = nr_div64(0.1, x1);
-      a2 = nr_div64(0.6, x2);
-      a3 = nr_div64(0.3, x3);
-      one_a_sum = nr_div64(1.0, a1 + a2 + a3);
+      a1 = weno_div(0.1, x1);
+      a2 = weno_div(0.6, x2);
+      a3 = weno_div(0.3, x3);
+      one_a_sum = weno_div(1.0, a1 + a2 + a3);
     } else {
-      a1 = 0.1 / x1;
-      a2 = 0.6 / x2;
-      a3 = 0.3 / x3;
-      one_a_sum = 1.0 / (a1 + a2 + a3);
+      a1 = fdiv(0.1, x1);
+      a2 = fdiv(0.6, x2);
+      a3 = fdiv(0.3, x3);
+      one_a_sum = fdiv(1.0, a1 + a2 + a3);
     }
   } else {
     a1 = fdiv(R(0.1), x1);
@@ -501,14 +547,107 @@ __device__ __forceinline__ void advection(const double *c, const Prim<double> &p
   a[kMomX + DIR] += pd.p;
 }
 
+// FluxSplittingScheme::ComputeFluxes + ComputeAdvection for one face in the
+// exact FP64 form (flux_splitting_scheme.cpp:42-76, 185-315), added into f.
+// Every value is formed by the reference's operations in the reference's
+// order; only the time at which each is formed differs, which without
+// contraction changes no bit.  To keep two waves per SIMD the face is
+// processed in a fixed sequence with few values live: the eigensystem of
+// cells i and i+1, then the advection of the six stencil cells one cell at a
+// time (kept in registers), then field by field the characteristic
+// projections and WENO5, with the conservatives read again from the block for
+// each field (the L1 and L2 serve the re-reads).  The reads go through a
+// buffer resource (no address arithmetic).  Each group of reads (and each
+// field's row of L) takes its inputs through an empty asm statement that
+// also consumes the previous group's last result, so the compiler neither
+// merges the re-reads nor hoists one field's work above the previous one's:
+// left to itself it forms the products of all fields at once and needs
+// about 340 registers (one wave per SIMD).
+template <int DIM, int DIR>
+__device__ __forceinline__ void roe_split_flux_exact(double gamma, double b,
+                                                     const double *__restrict__ blk, int off,
+                                                     double *f) {
+  constexpr int NE = DIM + 2;
+  constexpr int S = axis_stride<DIM, DIR>();
+  BlockReader<DIM> const rd(blk);
+  // Byte offset of stencil cell 0 (cell i-2, inside the haloed block).
+  int const off0 = (off - 2 * S) * (int)sizeof(double);
+  auto at = [&rd](int o, int m, int e) {
+    return rd(o, (e * Dims<DIM>::FIELD + m * S) * (int)sizeof(double));
+  };
+  Eig<NE, double> ev;
+  {
+    double Ui[NE], Un[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      Ui[e] = at(off0, 2, e);
+      Un[e] = at(off0, 3, e);
+    }
+    CellQ<double> qt, qn;
+    cell_quantities<DIM, double>(gamma, b, Ui, qt);
+    cell_quantities<DIM, double>(gamma, b, Un, qn);
+    roe_eigensystem<DIM, DIR, double, true>(gamma, qt, qn, ev);
+  }
+  double dep = ev.lam[0];
+  double adv[6][NE];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    int o = off0;
+    asm volatile("" : "+v"(o) : "v"(dep));
+    double U[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) U[e] = at(o, m, e);
+    Prim<double> P;
+    cons_to_prim<DIM, double>(gamma, b, U, P);
+    advection<DIM, DIR>(U, P, adv[m]);
+    dep = adv[m][NE - 1];
+  }
+  double chf[NE];
+#pragma unroll
+  for (int n = 0; n < NE; ++n) {
+    // The field's row of L and its eigenvalue pass through the asm too, so
+    // no product of this field is formed before the previous field is done.
+    int o = off0;
+    double lam = ev.lam[n], Lr[5];
+#pragma unroll
+    for (int l = 0; l < NE; ++l) Lr[l] = ev.L[n][l];
+    if constexpr (NE == 5)
+      asm volatile("" : "+v"(o), "+v"(lam), "+v"(Lr[0]), "+v"(Lr[1]), "+v"(Lr[2]), "+v"(Lr[3]),
+                   "+v"(Lr[4]) : "v"(dep));
+    else
+      asm volatile("" : "+v"(o), "+v"(lam), "+v"(Lr[0]), "+v"(Lr[1]), "+v"(Lr[2]), "+v"(Lr[3])
+                   : "v"(dep));
+    double pos[6], neg[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      double sp = 0.0, sn = 0.0;
+#pragma unroll
+      for (int q = 0; q < NE; ++q) {
+        int const l = seq<DIM, DIR>(q);
+        double const uct = at(o, m, l) * Lr[l];
+        double const act = adv[m][l] * Lr[l];
+        sp += (act + lam * uct);
+        sn += (act - lam * uct);
+      }
+      pos[m] = sp;
+      neg[m] = sn;
+    }
+    chf[n] = 0.5 * (weno_left(pos) + weno_right(neg));
+    dep = chf[n];
+  }
+  roe_right_eigenvectors<DIM, DIR, double, true>(ev);
+  double phys[NE];
+  to_physical<DIM, DIR>(chf, ev.Rm, phys);
+#pragma unroll
+  for (int e = 0; e < NE; ++e) f[e] += phys[e];
+}
+
 // FluxSplittingScheme::ComputeFluxes + ComputeAdvection for one face
-// (flux_splitting_scheme.cpp:42-76, 185-315), added into f.  The exact FP64
-// kernel keeps the reference's order -- every stencil cell's advection, then
-// field by field the projections and WENO5 -- while the contracting kernels
-// walk the stencil cell by cell (load, advection, projection onto every
-// field), which keeps fewer values live: 2 waves per SIMD instead of 1 for
-// the FP64-with-FMA kernel, 3 instead of 2 for the mixed one.  Each
-// projection is the same expression either way.
+// (flux_splitting_scheme.cpp:42-76, 185-315), added into f, in the
+// contracting forms (FP64 with FMA, mixed).  The stencil is walked cell by
+// cell -- load, advection, projection onto every field -- which keeps few
+// values live: 2 waves per SIMD for the FP64-with-FMA kernel, 3 for the mixed
+// one.  Each projection is the expression of the exact form.
 template <int DIM, int DIR, typename R>
 __device__ __forceinline__ void roe_split_flux(double gamma64, double b64,
                                                const double *__restrict__ base, R *f) {
@@ -524,38 +663,7 @@ __device__ __forceinline__ void roe_split_flux(double gamma64, double b64,
   CellQ<R> qt, qn;
   double adv2[NE];  // advection of cell i (FP64; mixed form only)
   R chf[NE];
-  if constexpr (std::is_same<R, double>::value && !kFastArith) {
-    double U[6][NE], adv[6][NE];
-    Prim<double> P[6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) load(m, U[m]);
-#pragma unroll
-    for (int m = 0; m < 6; ++m) cons_to_prim<DIM, double>(gamma64, b64, U[m], P[m]);
-    cell_quantities<DIM, R>(gamma64, b64, U[2], qt);
-    cell_quantities<DIM, R>(gamma64, b64, U[3], qn);
-    roe_eigensystem<DIM, DIR, R, false>(gamma, qt, qn, ev);
-#pragma unroll
-    for (int m = 0; m < 6; ++m) advection<DIM, DIR>(U[m], P[m], adv[m]);
-#pragma unroll
-    for (int n = 0; n < NE; ++n) {
-      R pos[6], neg[6];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        R sp = R(0.0), sn = R(0.0);
-#pragma unroll
-        for (int q = 0; q < NE; ++q) {
-          int const l = seq<DIM, DIR>(q);
-          R const uct = U[m][l] * ev.L[n][l];
-          R const act = adv[m][l] * ev.L[n][l];
-          sp += (act + ev.lam[n] * uct);
-          sn += (act - ev.lam[n] * uct);
-        }
-        pos[m] = sp;
-        neg[m] = sn;
-      }
-      chf[n] = R(0.5) * (weno_left(pos) + weno_right(neg));
-    }
-  } else {
+  {
     double Ui[NE], Un[NE];
     load(2, Ui);
     load(3, Un);
@@ -632,8 +740,9 @@ __device__ __forceinline__ void roe_split_flux(double gamma64, double b64,
 // Writes f (zero when the reference skips the face).
 template <int DIM, int DIR, int FLUX, typename R>
 __device__ __forceinline__ void face_flux(double gamma64, double b64,
-                                          const double *__restrict__ base, R *f) {
+                                          const double *__restrict__ blk, int off, R *f) {
   constexpr int NE = DIM + 2;
+  const double *__restrict__ base = blk + off;
   R const gamma = R(gamma64), b = R(b64);
   // Cell i-2+m of the stencil: base points at cell i's first field.
   auto load = [base](int m, double *c) {
@@ -703,6 +812,8 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
     hllc<DIM, DIR>(gamma, b, cl, cr, pl, pr, hf);
 #pragma unroll
     for (int e = 0; e < NE; ++e) f[e] += hf[e];
+  } else if constexpr (std::is_same<R, double>::value && !kFastArith) {
+    roe_split_flux_exact<DIM, DIR>(gamma64, b64, blk, off, f);
   } else {
     roe_split_flux<DIM, DIR, R>(gamma64, b64, base, f);
   }
@@ -714,5 +825,5 @@ template <int DIM, int DIR, int FLUX, typename R>
 __device__ __forceinline__ void stencil_flux(const double *__restrict__ blk, int i, int j, int k,
                                              double gamma, double b, R *f) {
   using D = Dims<DIM>;
-  face_flux<DIM, DIR, FLUX, R>(gamma, b, blk + i * D::SX + j * D::SY + k, f);
+  face_flux<DIM, DIR, FLUX, R>(gamma, b, blk, i * D::SX + j * D::SY + k, f);
 }

@@ -215,7 +215,13 @@ __device__ __forceinline__ void stage_tile(const TileJob &j, int blk_local, int 
 // spills 34-64 registers under the same hold and runs slower with it, so it
 // and every other variant keep the default.
 template <int DIM, int FLUX, typename R>
-constexpr int kMinWaves = (DIM == 2 && FLUX == 0 && !kFastArith && std::is_same<R, double>::value) ? 3 : 1;
+constexpr int kMinWaves = (DIM == 2 && FLUX == 0 && !kFastArith && std::is_same<R, double>::value) ? 3
+                          : (FLUX == 1 && !kFastArith && std::is_same<R, double>::value) ? 2 : 1;
+
+// The persistent batch kernel holds only the exact flux-splitting variant (to
+// 2 waves per SIMD, which its field-by-field face allows).
+template <int FLUX, typename R>
+constexpr int kBatchMinWaves = (FLUX == 1 && !kFastArith && std::is_same<R, double>::value) ? 2 : 1;
 
 // Single case: one workgroup per (block, tile).
 template <int DIM, int FLUX, typename R, int TX, int TY, int TZ, int NT>
@@ -255,7 +261,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kMinWaves<DI
 // split into 8 contiguous ranges, one per XCD (group g runs on XCD g % 8),
 // so a case's blocks share an L2.  Units of stopped cases are skipped.
 template <int DIM, int FLUX, typename R, int TX, int TY, int TZ, int NT>
-__global__ __launch_bounds__(NT) void batch_stage_kernel(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kBatchMinWaves<FLUX, R>, 8))) void batch_stage_kernel(
     const double *__restrict__ avg, const double *init, double *out, const CaseSlot *slots,
     alpaca_case_state *states, const WorkItem *items, const int *n_items, int stage) {
   using T = TileShape<DIM, TX, TY, TZ>;

@@ -142,6 +142,11 @@ class MultiLevelSolver:
                  initial, cfl: float, t_end: float, epsilon_reference: float,
                  level_of_epsilon_reference: int, flux: int = gpu.FLUX_HLLC,
                  gamma: float = 1.4, background_pressure: float = 0.0):
+        for name, want in (("flux", flux), ("gamma", gamma), ("b", background_pressure),
+                           ("dim", dim)):
+            have = getattr(backend, name, want)
+            if have != want:
+                raise ValueError(f"the backend's {name} is {have}, the solver's {want}")
         self.be = backend
         self.dim = dim
         self.node_size = float(node_size)

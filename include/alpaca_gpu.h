@@ -175,6 +175,15 @@ int alpaca_ctx_right_hand_side_jump_fluxes(alpaca_context *ctx,
                                            const double *d_avg, double *d_rhs,
                                            double *d_jump_fluxes);
 
+/* Verification hook, no counterpart in the reference: the exact form's FP64
+ * operations as its face kernels evaluate them, elementwise over n values on
+ * `stream`.  op 0: d_out = d_a / d_b (the division of physical quantities);
+ * op 1: the WENO5 weight division d_a / d_b (its core for divisors in
+ * [1e-12, 2^700)); op 2: d_out = sqrt(d_a) (d_b unused).  Tests compare the
+ * results with the x86 reference build's correctly rounded operations. */
+int alpaca_exact_fp64_ops(int op, const double *d_a, const double *d_b, double *d_out, int n,
+                          void *stream);
+
 /* (iii) One SSP-RK3 stage on every block of a case: FillInitialBuffer,
  * UpdateFluxes, PrepareBufferForIntegration and IntegrateNode
  * (time_integrator.h:98-113, 312-457; runge_kutta_3_TVD.h:39-59):

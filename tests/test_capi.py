@@ -127,3 +127,14 @@ def test_grid_neighbour_table():
             for k in range(t.shape[0]):
                 pos = [k // (nb3[1] * nb3[2]), (k // nb3[2]) % nb3[1], k % nb3[2]]
                 assert (t[k, slot] < 0) == (pos[ax] == 0)
+
+
+def test_exact_ops_refuse_bad_arguments_before_any_device_call():
+    """The exact FP64 operations hook refuses bad arguments with
+    ALPACA_ERR_INVALID before any HIP call."""
+    lib = gpu.load_library()
+    p, q = ctypes.c_void_p(256), ctypes.c_void_p(512)
+    assert lib.alpaca_exact_fp64_ops(3, p, p, q, 10, None) == 1
+    assert lib.alpaca_exact_fp64_ops(0, p, None, q, 10, None) == 1
+    assert lib.alpaca_exact_fp64_ops(0, p, p, q, -1, None) == 1
+    assert lib.alpaca_exact_fp64_ops(2, p, None, q, 0, None) == 0

@@ -74,7 +74,7 @@ def test_restart_written_by_cpu_continues_on_gpu(torch, tmp_path):
     the GPU backend: both continue to the same tree and bits."""
     print("[multilevel restart]", flush=True)
     cap = ml_cases.capacity(2, 4, 2)
-    o = ml_cases.sod_solver(oracle_ml.OracleBackend(2, cap), gpu.FLUX_ROE, 2)
+    o = ml_cases.sod_solver(oracle_ml.OracleBackend(2, cap, flux=gpu.FLUX_ROE), gpu.FLUX_ROE, 2)
     o.create_new_simulation()
     for _ in range(8):
         o.advance()

@@ -149,3 +149,16 @@ def test_multilevel_restart_continuation_bitexact(tmp_path):
     k2, c2 = again.leaf_data()
     assert k1 == k2 and np.array_equal(c1, c2)
     assert again.current_run_time == sol.current_run_time
+
+
+def test_solver_refuses_a_backend_of_another_flux_or_gas():
+    """The backend computes the right-hand side, so its flux family and gas
+    must be the solver's: a mismatch is refused instead of silently running
+    the backend's."""
+    cap = ml_cases.capacity(2, 4, 2)
+    with pytest.raises(ValueError, match="flux"):
+        ml_cases.sod_solver(oracle_ml.OracleBackend(2, cap), gpu.FLUX_ROE, 2)
+    with pytest.raises(ValueError, match="gamma"):
+        multilevel.MultiLevelSolver(oracle_ml.OracleBackend(2, cap, gamma=1.67), 2, [4, 1],
+                                    ml_cases.SOD_NODE, 2, [gpu.BC_SYMMETRY] * 4,
+                                    ml_cases.sod_initial, 0.6, 0.2, 1e-3, 2)
