@@ -573,6 +573,9 @@ struct ProfileSlot {
 };
 std::mutex g_prof_mutex;
 bool g_prof_on = false;
+// Set while an entry point that may not allocate runs on this thread:
+// ProfileScope then records no events (hipEventCreate allocates).
+thread_local bool g_prof_suppressed = false;
 ProfileSlot g_prof[ALPACA_KERNEL_COUNT];
 
 struct ProfileScope {
@@ -580,7 +583,7 @@ struct ProfileScope {
   hipStream_t stream;
   hipEvent_t start = nullptr, stop = nullptr;
   ProfileScope(int k, hipStream_t s, double units) : kernel(k), stream(s) {
-    if (!g_prof_on) return;
+    if (!g_prof_on || g_prof_suppressed) return;
     if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) {
       start = stop = nullptr;
       return;
@@ -943,6 +946,22 @@ int alpaca_ctx_convective_fluxes(alpaca_context *ctx, const alpaca_case_desc *de
 
 }  // extern "C"
 
+// Face-flux scratch of alpaca_right_hand_side_noalloc: one array per GPU,
+// part of the library's code object, so each GPU the library runs on holds
+// it (64 MiB) until the process ends.  Host code never passes this name to a
+// kernel (on the host it is only a stand-in for the symbol): it uses the
+// device address that noalloc_scratch() obtains per device.
+constexpr size_t kNoallocScratchDoubles = ALPACA_RHS_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double scratch[kNoallocScratchDoubles];
+static_assert(ALPACA_RHS_NOALLOC_MAX_BLOCKS_3D * alpaca::flux_doubles<3>(1) <= kNoallocScratchDoubles &&
+                  (ALPACA_RHS_NOALLOC_MAX_BLOCKS_3D + 1) * alpaca::flux_doubles<3>(1) >
+                      kNoallocScratchDoubles,
+              "3D block limit of the no-allocation scratch");
+static_assert(ALPACA_RHS_NOALLOC_MAX_BLOCKS_2D * alpaca::flux_doubles<2>(1) <= kNoallocScratchDoubles &&
+                  (ALPACA_RHS_NOALLOC_MAX_BLOCKS_2D + 1) * alpaca::flux_doubles<2>(1) >
+                      kNoallocScratchDoubles,
+              "2D block limit of the no-allocation scratch");
+
 namespace alpaca {
 namespace {
 
@@ -965,6 +984,26 @@ int rhs_context(alpaca_context *ctx, const alpaca_case_desc *desc, int nblocks,
   int st = context_scratch(ctx, n * sizeof(double), &fl);
   if (st != ALPACA_OK) return st;
   return rhs_with_scratch(desc, nblocks, d_avg, d_rhs, d_jump, fl, ctx->stream);
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (hipGetSymbolAddress with that device current) and kept.
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+double *g_noalloc_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(double **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return ALPACA_ERR_HIP;
+  if (dev < 0 || dev >= kMaxDevices) return ALPACA_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (!g_noalloc_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return ALPACA_ERR_HIP;
+    g_noalloc_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_noalloc_ptr[dev];
+  return ALPACA_OK;
 }
 
 }  // namespace
@@ -1023,6 +1062,21 @@ int alpaca_right_hand_side_jump_fluxes(const alpaca_case_desc *desc, int nblocks
   alpaca_context *ctx = nullptr;
   if ((st = stream_context(static_cast<hipStream_t>(stream), &ctx)) != ALPACA_OK) return st;
   return rhs_context(ctx, desc, nblocks, d_avg, d_rhs, d_jump_fluxes);
+}
+
+int alpaca_right_hand_side_noalloc(const alpaca_case_desc *desc, int nblocks, const double *d_avg,
+                                   double *d_rhs, void *stream) {
+  int st = validate(desc);
+  if (st != ALPACA_OK) return st;
+  if (nblocks < 1 || !d_avg || !d_rhs || d_avg == d_rhs) return ALPACA_ERR_INVALID;
+  if (nblocks > (desc->dim == 2 ? ALPACA_RHS_NOALLOC_MAX_BLOCKS_2D : ALPACA_RHS_NOALLOC_MAX_BLOCKS_3D))
+    return ALPACA_ERR_CAPACITY;
+  double *fl = nullptr;
+  if ((st = noalloc_scratch(&fl)) != ALPACA_OK) return st;
+  g_prof_suppressed = true;
+  st = rhs_with_scratch(desc, nblocks, d_avg, d_rhs, nullptr, fl, static_cast<hipStream_t>(stream));
+  g_prof_suppressed = false;
+  return st;
 }
 
 int alpaca_exact_fp64_ops(int op, const double *d_a, const double *d_b, double *d_out, int n,

@@ -54,6 +54,8 @@ _SIGNATURES = {
     "alpaca_right_hand_side_jump_fluxes": ([ctypes.POINTER(CaseDesc), ctypes.c_int, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
                                            ctypes.c_int),
+    "alpaca_right_hand_side_noalloc": ([ctypes.POINTER(CaseDesc), ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "alpaca_exact_fp64_ops": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "alpaca_context_create": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),

@@ -175,6 +175,24 @@ int alpaca_ctx_right_hand_side_jump_fluxes(alpaca_context *ctx,
                                            const double *d_avg, double *d_rhs,
                                            double *d_jump_fluxes);
 
+/* (ii) without allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same results, bit for bit, as
+ * alpaca_right_hand_side; the face fluxes go to a fixed 64 MiB scratch array
+ * that the library holds on each GPU it runs on until the process ends, so
+ * the call makes no hipMalloc/hipFree and no other allocating call (the
+ * profiling hooks record nothing for it).  At most
+ * ALPACA_RHS_NOALLOC_MAX_BLOCKS_3D / _2D blocks per call; larger calls return
+ * ALPACA_ERR_CAPACITY and launch nothing.  Every call on one GPU shares that
+ * GPU's array, which keeps the last call's face fluxes (each call writes all
+ * the faces it reads): calls that overlap on one GPU (different streams, or
+ * host threads without ordering) race. */
+#define ALPACA_RHS_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define ALPACA_RHS_NOALLOC_MAX_BLOCKS_3D 128
+#define ALPACA_RHS_NOALLOC_MAX_BLOCKS_2D 3855
+int alpaca_right_hand_side_noalloc(const alpaca_case_desc *desc, int nblocks,
+                                   const double *d_avg, double *d_rhs, void *stream);
+
 /* Verification hook, no counterpart in the reference: the exact form's FP64
  * operations as its face kernels evaluate them, elementwise over n values on
  * `stream`.  op 0: d_out = d_a / d_b (the division of physical quantities);

@@ -129,11 +129,22 @@ def test_grid_neighbour_table():
                 assert (t[k, slot] < 0) == (pos[ax] == 0)
 
 
-def test_exact_ops_refuse_bad_arguments_before_any_device_call():
-    """The exact FP64 operations hook refuses bad arguments with
-    ALPACA_ERR_INVALID before any HIP call."""
+def test_noalloc_and_exact_ops_refuse_before_any_device_call():
+    """The no-allocation right-hand side refuses block counts above its fixed
+    scratch (128 3D / 3855 2D blocks) with ALPACA_ERR_CAPACITY, and bad
+    arguments with ALPACA_ERR_INVALID, before any HIP call; so does the exact
+    FP64 operations hook."""
     lib = gpu.load_library()
     p, q = ctypes.c_void_p(256), ctypes.c_void_p(512)
+    d3 = gpu.make_desc(3, [1, 1, 1], [0, 0, 0], cell_size=1.0 / 128)
+    d2 = gpu.make_desc(2, [1, 1], [0, 0], cell_size=1.0 / 128)
+    assert lib.alpaca_right_hand_side_noalloc(ctypes.byref(d3), 129, p, q, None) == 5
+    assert lib.alpaca_right_hand_side_noalloc(ctypes.byref(d2), 3856, p, q, None) == 5
+    assert lib.alpaca_right_hand_side_noalloc(ctypes.byref(d3), 0, p, q, None) == 1
+    assert lib.alpaca_right_hand_side_noalloc(ctypes.byref(d3), 1, p, p, None) == 1
+    bad = gpu.make_desc(3, [1, 1, 1], [0, 0, 0])
+    bad.dim = 5
+    assert lib.alpaca_right_hand_side_noalloc(ctypes.byref(bad), 1, p, q, None) == 1
     assert lib.alpaca_exact_fp64_ops(3, p, p, q, 10, None) == 1
     assert lib.alpaca_exact_fp64_ops(0, p, None, q, 10, None) == 1
     assert lib.alpaca_exact_fp64_ops(0, p, p, q, -1, None) == 1
