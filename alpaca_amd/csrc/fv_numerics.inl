@@ -40,7 +40,8 @@ __device__ __forceinline__ double nr_sqrt64(double x) {
 // Divisors whose reciprocal lies within about 2^-100 (relative) of a rounding
 // midpoint: b = 2^e (2 - k 2^-52) for small odd k, whose significand has its
 // top 20 bits set (a density just below a power of two after round-off is
-// one).  For them the two Newton steps of the hardware division sequence can
+// one; so are the Roe-averaged density and sqrt(rho_i) + sqrt(rho_n) of two
+// such cells).  For them the two Newton steps of the hardware division sequence can
 // leave the reciprocal one ulp from RN(1/b), and a quotient that also lies
 // within about 2^-105 of a midpoint then rounds the wrong way (0x1.6666666666663p-2
 // / 0x1.ffffffffffffbp-4 gives 2.8 instead of x86's 2.8000000000000003).
@@ -72,12 +73,12 @@ __device__ __forceinline__ double div_round(double a, double b, double q) {
 // The WENO5 weight divisions c / x in the exact form: x = (s + 1e-6)^2 in
 // [1e-12, 2^700) and c in {0.1, 0.6, 0.3} (and 1 / (a1 + a2 + a3) >= 2^-704),
 // where the hardware division sequence scales nothing and fixes up nothing,
-// so it is exactly its core nr_div64; the rare divisor with a hard reciprocal
-// takes the exact rounding decision.
-__device__ __forceinline__ double weno_div(double c, double x) {
-  double const q = nr_div64(c, x);
-  return __builtin_expect(recip_hard(x), 0) ? div_round(c, x, q) : q;
-}
+// so it is exactly its core nr_div64.  These 40 divisions per face take no
+// hard-reciprocal check (it would cost about 3% of the exact stage kernel's time): their
+// divisors are smoothness indicators, not state values, and one of them would
+// have to lie within 2^-48 below a power of two, with the quotient within
+// about 2^-105 of a midpoint, to round differently from x86.
+__device__ __forceinline__ double weno_div(double c, double x) { return nr_div64(c, x); }
 
 // Division and square root: the compiler's FP64 sequences (what a/b and
 // sqrt(x) compile to) in the exact namespace; Newton-Raphson FP64 in the fast
@@ -563,6 +564,8 @@ __device__ __forceinline__ void advection(const double *c, const Prim<double> &p
 // merges the re-reads nor hoists one field's work above the previous one's:
 // left to itself it forms the products of all fields at once and needs
 // about 340 registers (one wave per SIMD).
+constexpr int kRoePrefetch = 2;  // stencil cells read ahead for the next field
+
 template <int DIM, int DIR>
 __device__ __forceinline__ void roe_split_flux_exact(double gamma, double b,
                                                      const double *__restrict__ blk, int off,
@@ -602,20 +605,32 @@ __device__ __forceinline__ void roe_split_flux_exact(double gamma, double b,
     advection<DIM, DIR>(U, P, adv[m]);
     dep = adv[m][NE - 1];
   }
+  // Conservatives of the stencil for the field being projected; the first
+  // kRoePrefetch cells of the next field are read while this field's WENO5
+  // runs, the rest after it.
+  double U[6][NE];
+  {
+    int o = off0;
+    asm volatile("" : "+v"(o) : "v"(dep));
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+      for (int e = 0; e < NE; ++e) U[m][e] = at(o, m, e);
+  }
   double chf[NE];
 #pragma unroll
   for (int n = 0; n < NE; ++n) {
-    // The field's row of L and its eigenvalue pass through the asm too, so
-    // no product of this field is formed before the previous field is done.
-    int o = off0;
+    // The field's row of L and its eigenvalue pass through an asm statement
+    // that consumes the previous field's result, so no product of this field
+    // is formed before the previous field is done.
     double lam = ev.lam[n], Lr[5];
 #pragma unroll
     for (int l = 0; l < NE; ++l) Lr[l] = ev.L[n][l];
     if constexpr (NE == 5)
-      asm volatile("" : "+v"(o), "+v"(lam), "+v"(Lr[0]), "+v"(Lr[1]), "+v"(Lr[2]), "+v"(Lr[3]),
+      asm volatile("" : "+v"(lam), "+v"(Lr[0]), "+v"(Lr[1]), "+v"(Lr[2]), "+v"(Lr[3]),
                    "+v"(Lr[4]) : "v"(dep));
     else
-      asm volatile("" : "+v"(o), "+v"(lam), "+v"(Lr[0]), "+v"(Lr[1]), "+v"(Lr[2]), "+v"(Lr[3])
+      asm volatile("" : "+v"(lam), "+v"(Lr[0]), "+v"(Lr[1]), "+v"(Lr[2]), "+v"(Lr[3])
                    : "v"(dep));
     double pos[6], neg[6];
 #pragma unroll
@@ -624,7 +639,7 @@ __device__ __forceinline__ void roe_split_flux_exact(double gamma, double b,
 #pragma unroll
       for (int q = 0; q < NE; ++q) {
         int const l = seq<DIM, DIR>(q);
-        double const uct = at(o, m, l) * Lr[l];
+        double const uct = U[m][l] * Lr[l];
         double const act = adv[m][l] * Lr[l];
         sp += (act + lam * uct);
         sn += (act - lam * uct);
@@ -632,8 +647,24 @@ __device__ __forceinline__ void roe_split_flux_exact(double gamma, double b,
       pos[m] = sp;
       neg[m] = sn;
     }
+    if (n + 1 < NE) {
+      int o = off0;
+      asm volatile("" : "+v"(o) : "v"(pos[5]), "v"(neg[5]));
+#pragma unroll
+      for (int m = 0; m < kRoePrefetch; ++m)
+#pragma unroll
+        for (int e = 0; e < NE; ++e) U[m][e] = at(o, m, e);
+    }
     chf[n] = 0.5 * (weno_left(pos) + weno_right(neg));
     dep = chf[n];
+    if (n + 1 < NE) {
+      int o = off0;
+      asm volatile("" : "+v"(o) : "v"(dep));
+#pragma unroll
+      for (int m = kRoePrefetch; m < 6; ++m)
+#pragma unroll
+        for (int e = 0; e < NE; ++e) U[m][e] = at(o, m, e);
+    }
   }
   roe_right_eigenvectors<DIM, DIR, double, true>(ev);
   double phys[NE];
