@@ -3,16 +3,14 @@
 Each input (copies of testsuite/InputFiles/**, see golden/inputs/SOURCES.txt)
 is read by alpaca_amd.inputfile, advanced to its end time on the GPU in FP64
 and compared with the oracle advancing the same initial state with the same
-sides: maximum relative error <= 1e-10 (the FP64 bound), same step count and
-time steps to 1e-10; whether the run is bit-exact is printed.  DetailedSod is
-also checked against the testsuite's level-0 error norms
-(2D_DetailedSod_errors_reference.csv rows 2 and 20).
+sides: the same bits in every cell, the same step count and the same time
+steps.  DetailedSod is also checked against the testsuite's level-0 error
+norms (2D_DetailedSod_errors_reference.csv rows 2 and 20).
 
-Bit-exactness is not asserted here: gfx950's FP64 division (reciprocal, two
-Newton steps, one residual correction) is not correctly rounded when the
-quotient lies within ~2^-105 of a rounding midpoint, which decimal data such
-as Sod's (0.35 / 0.125 with round-off) reaches; x86 division is.  Such a
-1-ulp difference stays at the 1e-15 level.
+Sod's decimal data reach FP64 divisions whose quotient lies within ~2^-105
+of a rounding midpoint by a divisor just below a power of two (0.35 / 0.125
+with round-off); the exact form rounds those as x86 does, so these runs are
+bit-exact too.
 """
 import os
 
@@ -61,9 +59,9 @@ def test_input_file_runs_match_oracle(torch, name, dim, flux):
     print(f"  steps={res['steps']} t={res['time']} max rel err {err:.3e} "
           f"bit-exact {np.array_equal(got, want)}", flush=True)
     assert res["steps"] == len(want_dts)
-    assert np.max(np.abs(res["dts"] - want_dts) / want_dts) <= 1e-10
-    assert abs(res["time"] - tf) <= 1e-12 * tf and tf == case.end_time
-    assert err <= 1e-10
+    assert np.array_equal(np.asarray(res["dts"]), np.asarray(want_dts))
+    assert res["time"] == tf and tf == case.end_time
+    assert np.array_equal(got, want), f"max rel err {err:.3e}"
     if name.startswith("DetailedSod"):
         rho_l1, p_l1 = {gpu.FLUX_ROE: (1.00420788e-04, 1.13784806e-04),
                         gpu.FLUX_HLLC: (1.02588746e-04, 1.12368183e-04)}[flux]
