@@ -49,3 +49,30 @@ def test_unfittable_requests_do_not_spin():
     q2.arrive(1.0)
     assert q2.admit(lambda r: False, lambda r: None) == 0  # e.g. no case slot free
     assert q2.stuck(0) and not q2.stuck(1)
+
+
+def test_shortest_first_with_a_live_cell_cap():
+    """order='shortest' admits the smallest estimated work first and stops at
+    the live-cell cap (but never leaves the device idle); a request past
+    max_wait goes first whatever its size."""
+    big = Request(0, 0.0, 3, 8, 0.25, gpu.FP64, seed=0)     # 2M cells, long
+    mid = Request(1, 0.1, 2, 32, 0.25, gpu.FP64, seed=1)    # 262K cells
+    small = Request(2, 0.2, 2, 4, 0.05, gpu.FP64, seed=2)   # 4K cells, short
+    assert serving.req_work(small) < serving.req_work(mid) < serving.req_work(big)
+    q = Admission([big, mid, small], {2: 10 ** 6, 3: 10 ** 5}, order="shortest",
+                  live_cells_cap=300_000)
+    q.arrive(1.0)
+    got = []
+    assert q.admit(lambda r: True, lambda r: got.append(r.rid), live_cells=0, now=1.0) == 2
+    assert got == [2, 1]  # the cap (300K cells) holds the 2M-cell request back
+    assert [r.rid for r in q.waiting] == [0]
+    # with nothing live the first candidate is admitted even above the cap
+    assert q.admit(lambda r: True, lambda r: got.append(r.rid), live_cells=0, now=1.0) == 1
+    assert got == [2, 1, 0]
+    # aging: past max_wait the big request overtakes smaller ones
+    q2 = Admission([big, small], {2: 10 ** 6, 3: 10 ** 5}, order="shortest",
+                   live_cells_cap=10 ** 9, max_wait=5.0)
+    q2.arrive(1.0)
+    order = []
+    q2.admit(lambda r: True, lambda r: order.append(r.rid), now=10.0)
+    assert order == [0, 2]
