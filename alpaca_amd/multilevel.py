@@ -69,6 +69,17 @@ class Topology:
         self.maximum_level = maximum_level
         self.periodic = list(periodic) + [False] * (3 - len(periodic))
         self.leaf = {}  # key -> bool
+        # Bumped by every change of the tree (nodes or leaf flags): the
+        # solver's per-level tables are rebuilt only when it moved.
+        self.version = 0
+
+    def set_leaf(self, key, flag: bool):
+        self.leaf[key] = flag
+        self.version += 1
+
+    def remove(self, key):
+        del self.leaf[key]
+        self.version += 1
 
     def extent(self, level: int, axis: int) -> int:
         if axis >= self.dim:
@@ -173,6 +184,8 @@ class MultiLevelSolver:
         self.all_levels = list(range(self.maximum_level + 1))
         self.halo_locs = halo_locations(dim)
         self.face_locs = face_locations(dim)
+        self._loc_dir = {loc: mr.location_direction(loc) for loc in range(len(mr.LOCATIONS))}
+        self._cache = {}
 
     # ------------------------------------------------------------- helpers
     def cell_size(self, level: int) -> float:
@@ -183,13 +196,13 @@ class MultiLevelSolver:
             raise MemoryError("multi-level block pool exhausted")
         s = self.free.pop()
         self.slot[key] = s
-        self.topo.leaf[key] = leaf
+        self.topo.set_leaf(key, leaf)
         self.be.clear(s)
         return s
 
     def _remove_node(self, key):
         s = self.slot.pop(key)
-        del self.topo.leaf[key]
+        self.topo.remove(key)
         self.free.append(s)
 
     def _slots(self, keys):
@@ -201,46 +214,90 @@ class MultiLevelSolver:
     def _lts_timestep(self, stage, n, jump=False):
         return self.be.lts_timestep(self.micro, n, stage, jump)
 
+    def _cached(self, kind, level, build):
+        """A per-level table (node lists, neighbour tables, link lists) built
+        once per version of the tree."""
+        ent = self._cache.get((kind, level))
+        if ent is not None and ent[0] == self.topo.version:
+            return ent[1]
+        val = build()
+        self._cache[(kind, level)] = (self.topo.version, val)
+        return val
+
+    def _ids(self, level):
+        return self._cached("ids", level, lambda: self.topo.ids_on_level(level))
+
+    def _leaves(self, level):
+        return self._cached("leaves", level, lambda: self.topo.leaves_on_level(level))
+
     # -------------------------------------------------------- halo update
+    def _grid(self, level):
+        """Dense view of a level: slot of the node at each (i, j, k) (-1 where
+        there is none) and its leaf flag, as arrays over the level's extent."""
+        def build():
+            shape = tuple(self.topo.extent(level, a) for a in range(3))
+            slots = np.full(shape, -1, dtype=np.int32)
+            leaf = np.zeros(shape, dtype=bool)
+            for key in self._ids(level):
+                slots[key[1], key[2], key[3]] = self.slot[key]
+                leaf[key[1], key[2], key[3]] = self.topo.leaf[key]
+            return slots, leaf
+        return self._cached("grid", level, build)
+
+    def _neighbors(self, level, keys, offs):
+        """For the nodes `keys` of a level and same-level offsets `offs`
+        (m x 3): (slot of the neighbour or -1 where no node exists, inside
+        the domain, neighbour is a leaf), each an n x m array (the
+        reference's neighbour search, topology_manager.cpp:363-390, with
+        periodic wrap)."""
+        slots, leaf = self._grid(level)
+        k = np.array([key[1:] for key in keys], dtype=np.int64).reshape(-1, 1, 3)
+        o = np.asarray(offs, dtype=np.int64).reshape(1, -1, 3)
+        q = k + o
+        ext = np.array(slots.shape, dtype=np.int64)
+        inside = np.ones(q.shape[:2], dtype=bool)
+        for a in range(3):
+            out = (q[..., a] < 0) | (q[..., a] >= ext[a])
+            if a < self.dim and self.topo.periodic[a]:
+                q[..., a] %= ext[a]
+            else:
+                inside &= ~out
+                q[..., a] = np.clip(q[..., a], 0, ext[a] - 1)
+        nb = np.where(inside, slots[q[..., 0], q[..., 1], q[..., 2]], -1)
+        nleaf = inside & leaf[q[..., 0], q[..., 1], q[..., 2]]
+        return nb, inside, nleaf
+
     def _neighbor_table(self, keys):
         """Rows for alpaca_fill_halos_list: neighbour slot, -1 outside the
         domain, -2 for a level jump (no node of this level there)."""
         dim = self.dim
         offs = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1)
                 for dz in ((-1, 0, 1) if dim == 3 else (0,))]
-        table = np.empty((len(keys), len(offs)), dtype=np.int32)
-        for r, key in enumerate(keys):
-            for off in offs:
-                sl = offset_slot(dim, off)
-                if off == (0, 0, 0):
-                    table[r, sl] = self.slot[key]
-                    continue
-                nb = self.topo.neighbor(key, off)
-                if nb is None:
-                    table[r, sl] = -1
-                elif self.topo.exists(nb):
-                    table[r, sl] = self.slot[nb]
-                else:
-                    table[r, sl] = -2
-        return table
+        nb, inside, _ = self._neighbors(keys[0][0], keys, offs)
+        table = np.where(inside, np.where(nb >= 0, nb, -2), -1).astype(np.int32)
+        # columns follow offset_slot's order, which is the order of offs
+        return np.ascontiguousarray(table)
 
     def halo_update_on_level(self, level: int, cut_jumps: bool):
         """MaterialHaloUpdateOnLevel for the conservatives (the right-hand-side
         buffers): same-level copies, then (unless cut_jumps) the jump halos
         predicted from the parents, then the external conditions."""
-        keys = self.topo.ids_on_level(level)
+        keys = self._ids(level)
         if not keys:
             return
-        table = self._neighbor_table(keys)
-        jump_links, jump_locs = [], []
-        if not cut_jumps:
-            for r, key in enumerate(keys):
-                for loc in self.halo_locs:
-                    d = mr.location_direction(loc)
-                    if table[r, offset_slot(self.dim, d)] == -2:
-                        jump_links.append(self._link(key))
-                        jump_locs.append(loc)
-        self.be.halo_update(self._slots(keys), table, jump_links, jump_locs, self.side_bc)
+
+        def build():
+            table = self._neighbor_table(keys)
+            cols = [offset_slot(self.dim, self._loc_dir[loc]) for loc in self.halo_locs]
+            rows, which = np.nonzero(table[:, cols] == -2)
+            jump_links = [self._link(keys[r]) for r in rows.tolist()]
+            jump_locs = [self.halo_locs[c] for c in which.tolist()]
+            return self._slots(keys), table, jump_links, jump_locs
+
+        slots, table, jump_links, jump_locs = self._cached("halo", level, build)
+        if cut_jumps:
+            jump_links, jump_locs = [], []
+        self.be.halo_update(slots, table, jump_links, jump_locs, self.side_bc)
 
     def halo_update(self, levels_ascending, cut_jumps=False):
         """HaloManager::MaterialHaloUpdate (halo_manager.cpp:59-76)."""
@@ -257,17 +314,17 @@ class MultiLevelSolver:
         for level in child_levels_descending:
             if level == 0:
                 continue
-            keys = self.topo.ids_on_level(level)
-            if keys:
-                self.be.average([self._link(k) for k in keys])
+            links = self._cached("links", level,
+                                 lambda: [self._link(k) for k in self._ids(level)])
+            if links:
+                self.be.average(links)
 
     # ---------------------------------------------------------- stage work
     def compute_right_hand_side(self, levels, stage):
         for level in levels:
-            keys = self.topo.leaves_on_level(level)
-            if not keys:
+            slots = self._cached("leaf_slots", level, lambda: self._slots(self._leaves(level)))
+            if not slots:
                 continue
-            slots = self._slots(keys)
             if stage == 0:
                 self.be.fill_initial(slots)
             self.be.right_hand_side(slots, self.cell_size(level))
@@ -276,22 +333,34 @@ class MultiLevelSolver:
     def integrate(self, levels, stage):
         for level in levels:
             n = 1 << (self.maximum_level - level)
-            leaves = self.topo.leaves_on_level(level)
-            if leaves:
-                self.be.integrate_node(self._slots(leaves), self._lts_timestep(stage, n, True),
+            slots = self._cached("leaf_slots", level, lambda: self._slots(self._leaves(level)))
+            if slots:
+                self.be.integrate_node(slots, self._lts_timestep(stage, n, True),
                                        self._lts_timestep(stage, n))
-            by_loc = {}
-            for key in self.topo.ids_on_level(level):
-                for loc in self.halo_locs:
-                    nb = self.topo.neighbor(key, mr.location_direction(loc))
-                    if nb is not None and not self.topo.exists(nb):
-                        by_loc.setdefault(loc, []).append(self.slot[key])
+
+            def build():
+                keys = self._ids(level)
+                by_loc = {}
+                if not keys:
+                    return by_loc
+                nb, inside, _ = self._neighbors(level, keys,
+                                                [self._loc_dir[loc] for loc in self.halo_locs])
+                missing = inside & (nb < 0)
+                slots = self._slots(keys)
+                for c, loc in enumerate(self.halo_locs):
+                    rows = np.nonzero(missing[:, c])[0]
+                    if rows.size:
+                        by_loc[loc] = [slots[r] for r in rows.tolist()]
+                return by_loc
+
+            by_loc = self._cached("jump_halos", level, build)
             ts = self._lts_timestep(stage, n)
             for loc, slots in by_loc.items():
                 self.be.integrate_range(slots, mr.halo_recv_range(self.dim, loc), ts)
 
     def swap(self, levels):
-        slots = [self.slot[k] for lv in levels for k in self.topo.ids_on_level(lv)]
+        slots = [s for lv in levels for s in self._cached("slots", lv,
+                                                            lambda: self._slots(self._ids(lv)))]
         if slots:
             self.be.swap(slots)
 
@@ -299,23 +368,29 @@ class MultiLevelSolver:
         for level in finished_levels_descending:
             if level == 0:
                 continue
-            keys = self.topo.ids_on_level(level)
-            if keys:
-                self.be.average_jump([self._link(k) for k in keys])
+            links = self._cached("links", level,
+                                 lambda: [self._link(k) for k in self._ids(level)])
+            if links:
+                self.be.average_jump(links)
         for level in finished_levels_descending[1:]:
-            leaves = self.topo.leaves_on_level(level)
+            leaves = self._leaves(level)
             if not leaves:
                 continue
-            entries = np.full((len(leaves), 7), -1, dtype=np.int32)
-            for r, key in enumerate(leaves):
-                entries[r, 0] = self.slot[key]
-                for loc in self.face_locs:
-                    nb = self.topo.neighbor(key, mr.location_direction(loc))
-                    if nb is not None and self.topo.exists(nb) and not self.topo.is_leaf(nb):
-                        entries[r, 1 + loc] = self.slot[nb]
+
+            def build():
+                entries = np.full((len(leaves), 7), -1, dtype=np.int32)
+                entries[:, 0] = self._slots(leaves)
+                nb, inside, nleaf = self._neighbors(level, leaves,
+                                                    [self._loc_dir[loc] for loc in self.face_locs])
+                parent_of_finer = inside & (nb >= 0) & ~nleaf
+                for c, loc in enumerate(self.face_locs):
+                    entries[:, 1 + loc] = np.where(parent_of_finer[:, c], nb[:, c], -1)
+                return entries
+
+            entries = self._cached("jfa", level, build)
             self.be.jump_flux_adjustment(entries, 1.0 / self.cell_size(level))
-        slots = [self.slot[k] for lv in finished_levels_descending
-                 for k in self.topo.ids_on_level(lv)]
+        slots = [s for lv in finished_levels_descending
+                 for s in self._cached("slots", lv, lambda: self._slots(self._ids(lv)))]
         if slots:
             self.be.reset_jump_conservatives(slots)
 
@@ -351,12 +426,12 @@ class MultiLevelSolver:
             for c in self.topo.children(pkey):
                 if self.topo.exists(c):
                     self._remove_node(c)
-            self.topo.leaf[pkey] = True
+            self.topo.set_leaf(pkey, True)
 
     def refine_node(self, key):
         """RefineNode: children created, their internal cells predicted from
         the node's right-hand-side buffer."""
-        self.topo.leaf[key] = False
+        self.topo.set_leaf(key, False)
         links = []
         for c in self.topo.children(key):
             self._new_node(c)
@@ -382,15 +457,15 @@ class MultiLevelSolver:
         for dt in self.micro:
             run_time += dt
         dx_finest = self.cell_size(self.maximum_level)
-        return self.be.timestep(self._slots(self.topo.leaves()), self.cfl, dx_finest, run_time,
-                                self.t_end)
+        slots = self._cached("all_leaf_slots", -1, lambda: self._slots(self.topo.leaves()))
+        return self.be.timestep(slots, self.cfl, dx_finest, run_time, self.t_end)
 
     # ------------------------------------------------------- initialisation
     def create_new_simulation(self):
         for level in self.all_levels:
             if level > 0:
                 for pkey in self.topo.ids_on_level(level - 1):
-                    self.topo.leaf[pkey] = False
+                    self.topo.set_leaf(pkey, False)
                     for c in self.topo.children(pkey):
                         self._new_node(c)
             else:
@@ -478,7 +553,7 @@ class MultiLevelSolver:
         for key in keys:
             self._new_node(key)
         for key in keys:
-            self.topo.leaf[key] = not any(self.topo.exists(c) for c in self.topo.children(key))
+            self.topo.set_leaf(key, not any(self.topo.exists(c) for c in self.topo.children(key)))
         self.be.set_rhs_blocks(self._slots(keys), cons)
         self.swap(self.all_levels)
         self.current_run_time = float(t)
@@ -543,6 +618,7 @@ class GpuBackend:
                   "context_create")
         self.lti = lts.LevelTimeIntegrator(dim)
         self.mrop = mr.Multiresolution()
+        self._memo = {}
 
     def close(self):
         if self.ctx:
@@ -556,14 +632,29 @@ class GpuBackend:
             pass
 
     # -- helpers
+    def _device(self, obj, kind, make):
+        """Device copy of a host table the solver keeps between calls (its
+        per-level tables live until the tree changes): uploaded once per
+        object.  The memo holds the object too, so its id cannot be reused
+        while the entry exists."""
+        key = (id(obj), kind)
+        hit = self._memo.get(key)
+        if hit is not None and hit[0] is obj:
+            return hit[1]
+        if len(self._memo) > 4096:
+            self._memo.clear()
+        t = self.torch.as_tensor(make(obj), device="cuda")
+        self._memo[key] = (obj, t)
+        return t
+
     def _idx(self, slots):
-        return self.torch.as_tensor(np.asarray(slots, dtype=np.int64), device="cuda")
+        return self._device(slots, "i64", lambda o: np.asarray(o, dtype=np.int64))
 
     def _list(self, slots):
-        return self.torch.as_tensor(np.asarray(slots, dtype=np.int32), device="cuda")
+        return self._device(slots, "i32", lambda o: np.asarray(o, dtype=np.int32))
 
     def _links(self, links):
-        return self.torch.as_tensor(mr.make_links(links), device="cuda")
+        return self._device(links, "links", mr.make_links)
 
     def _p(self, t):
         return ctypes.c_void_p(t.data_ptr())
@@ -659,21 +750,21 @@ class GpuBackend:
 
     def halo_update(self, slots, table, jump_links, jump_locs, side_bc):
         lst = self._list(slots)
-        tab = self.torch.as_tensor(np.ascontiguousarray(table, dtype=np.int32), device="cuda")
+        tab = self._device(table, "table", lambda o: np.ascontiguousarray(o, dtype=np.int32))
         bc = (ctypes.c_int * 6)(*(list(side_bc) + [0] * (6 - len(side_bc))))
         st = gpu._stream_ptr()
         gpu.check(self.lib.alpaca_fill_halos_list(self.dim, self._p(self.rhs), self.capacity,
                                                   self._p(lst), len(slots), self._p(tab), bc, 0,
                                                   st), "fill_halos_list")
         if jump_links:
-            locs = self.torch.as_tensor(np.asarray(jump_locs, dtype=np.int32), device="cuda")
+            locs = self._device(jump_locs, "i32", lambda o: np.asarray(o, dtype=np.int32))
             self.mrop.jump_halo_update(self.rhs, self.dim, self._links(jump_links), locs)
         gpu.check(self.lib.alpaca_fill_halos_list(self.dim, self._p(self.rhs), self.capacity,
                                                   self._p(lst), len(slots), self._p(tab), bc, 1,
                                                   st), "fill_halos_list (after jumps)")
 
     def jump_flux_adjustment(self, entries, one_cell_size):
-        ent = self.torch.as_tensor(np.ascontiguousarray(entries, dtype=np.int32), device="cuda")
+        ent = self._device(entries, "table", lambda o: np.ascontiguousarray(o, dtype=np.int32))
         lts.jump_flux_adjustment(self.dim, self.rhs, self.jcons, ent, one_cell_size)
 
     def reset_jump_conservatives(self, slots):
