@@ -276,3 +276,28 @@ def batched_cases(ncases: int = 512, seed: int = 0):
         gamma = rng.uniform(1.2, 1.67)
         specs.append((nb, gamma, four_quadrant_2d(nb, nb, seed=seed, gamma=gamma, case=i)))
     return specs
+
+
+def blob_node_initial(dim: int, node_size: float, gamma: float = 1.4):
+    """Initial condition of a multi-level node for the periodic blob case: a
+    dense, high-pressure blob (radius^2 < 0.02 around (0.4, 0.55, 0.25)) in a
+    uniform flow (u, v) = (0.5, 0.3).  Returns initial(level, key) giving the
+    conservatives [NE][16][16][16|1] of node key = (level, i, j, k) at its
+    cell centres, the form the multi-level driver takes."""
+    ic = 16
+
+    def initial(level, key):
+        dx = node_size / (1 << level) / ic
+        ax = [(key[a + 1] * ic + np.arange(ic) + 0.5) * dx for a in range(dim)]
+        g = np.meshgrid(*ax, indexing="ij")
+        centre = (0.4, 0.55, 0.25)
+        r2 = sum((g[a] - centre[a]) ** 2 for a in range(dim))
+        inside = r2 < 0.02
+        rho = 1.0 + 2.0 * inside
+        p = 1.0 + 4.0 * inside
+        vel = [0.5 * np.ones_like(rho), 0.3 * np.ones_like(rho)] + (
+            [np.zeros_like(rho)] if dim == 3 else [])
+        c = primes_to_conservatives(dim, gamma, 0.0, rho, vel, p)
+        return c[..., None] if dim == 2 else c
+    return initial
+

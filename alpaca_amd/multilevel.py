@@ -191,14 +191,19 @@ class MultiLevelSolver:
     def cell_size(self, level: int) -> float:
         return self.node_size / float(1 << level) / gpu.IC
 
-    def _new_node(self, key, leaf=True):
-        if not self.free:
+    def _new_nodes(self, keys, leaf=True):
+        """Nodes with zeroed buffers (one batched clear of their slots)."""
+        if len(self.free) < len(keys):
             raise MemoryError("multi-level block pool exhausted")
-        s = self.free.pop()
-        self.slot[key] = s
-        self.topo.set_leaf(key, leaf)
-        self.be.clear(s)
-        return s
+        slots = []
+        for key in keys:
+            s = self.free.pop()
+            self.slot[key] = s
+            self.topo.set_leaf(key, leaf)
+            slots.append(s)
+        if slots:
+            self.be.clear(slots)
+        return slots
 
     def _remove_node(self, key):
         s = self.slot.pop(key)
@@ -432,11 +437,9 @@ class MultiLevelSolver:
         """RefineNode: children created, their internal cells predicted from
         the node's right-hand-side buffer."""
         self.topo.set_leaf(key, False)
-        links = []
-        for c in self.topo.children(key):
-            self._new_node(c)
-            links.append(self._link(c))
-        return links
+        kids = self.topo.children(key)
+        self._new_nodes(kids)
+        return [self._link(c) for c in kids]
 
     def remesh(self, levels_ascending):
         cur_max = self.topo.current_maximum_level()
@@ -464,16 +467,15 @@ class MultiLevelSolver:
     def create_new_simulation(self):
         for level in self.all_levels:
             if level > 0:
+                kids = []
                 for pkey in self.topo.ids_on_level(level - 1):
                     self.topo.set_leaf(pkey, False)
-                    for c in self.topo.children(pkey):
-                        self._new_node(c)
+                    kids += self.topo.children(pkey)
+                self._new_nodes(kids)
             else:
                 nb0 = self.topo.nb0
-                for i in range(nb0[0]):
-                    for j in range(nb0[1]):
-                        for k in range(nb0[2]):
-                            self._new_node((0, i, j, k))
+                self._new_nodes([(0, i, j, k) for i in range(nb0[0]) for j in range(nb0[1])
+                                 for k in range(nb0[2])])
             keys = self.topo.ids_on_level(level)
             self.be.set_internal(self._slots(keys),
                                  np.stack([self.initial(level, key) for key in keys]))
@@ -550,8 +552,7 @@ class MultiLevelSolver:
         t, _, ids, cons = restart.read_nodes(path, self.dim, self.maximum_level)
         keys = [restart.decode_node_id(int(i)) for i in ids]
         keys = [(lv, i, j, k if self.dim == 3 else 0) for lv, i, j, k in keys]
-        for key in keys:
-            self._new_node(key)
+        self._new_nodes(keys)
         for key in keys:
             self.topo.set_leaf(key, not any(self.topo.exists(c) for c in self.topo.children(key)))
         self.be.set_rhs_blocks(self._slots(keys), cons)
@@ -665,9 +666,10 @@ class GpuBackend:
                              cell_size=cell_size)
 
     # -- node data
-    def clear(self, slot):
+    def clear(self, slots):
+        idx = self.torch.as_tensor(np.asarray(slots, dtype=np.int64), device="cuda")
         for t in (self.avg, self.rhs, self.init, self.jflux, self.jcons):
-            t[slot].zero_()
+            t.index_fill_(0, idx, 0.0)
 
     def set_internal(self, slots, cons):
         """Internal cells of the right-hand-side buffers (ImposeInitialCondition)."""

@@ -479,6 +479,53 @@ def run_mr(args, nparents=512):
                         "face slabs of every child", "kernels": res}
 
 
+def run_multilevel(warmup: int = 1, steps: int = 3):
+    """The multi-level mode on the GPU backend: 3D periodic blob, 4^3 level-0
+    nodes of size 0.25, maximum level 2 (multiresolution with local time
+    stepping, remeshing every micro step), HLLC FP64.  A leaf of level l
+    advances 2^l times per macro step; the rate counts those leaf
+    cell-updates."""
+    import time
+
+    import torch
+
+    from alpaca_amd import cases, gpu, multilevel
+
+    dim, lmax, nb0 = 3, 2, [4, 4, 4]
+    cap = 64 * (1 + 8 + 64)
+    be = multilevel.GpuBackend(dim, cap, flux=gpu.FLUX_HLLC)
+    sol = multilevel.MultiLevelSolver(be, dim, nb0, 0.25, lmax, [gpu.BC_PERIODIC] * 6,
+                                      cases.blob_node_initial(dim, 0.25), 0.5, 10.0, 1e-3, lmax)
+    t0 = time.perf_counter()
+    sol.create_new_simulation()
+    torch.cuda.synchronize()
+    create_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        sol.advance()
+    torch.cuda.synchronize()
+    updates = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lv = [len(sol.topo.leaves_on_level(level)) for level in range(lmax + 1)]
+        updates += sum(n * 16 ** 3 * (1 << level) for level, n in enumerate(lv))
+        sol.advance()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    leaves = [len(sol.topo.leaves_on_level(level)) for level in range(lmax + 1)]
+    out = {"workload": ("3D periodic blob, 4^3 level-0 nodes, maximum level 2 (MR + local time "
+                        "stepping, remeshing each micro step), HLLC FP64, GPU backend"),
+           "create_s": create_s, "macro_steps": steps, "macro_step_ms": wall * 1e3 / steps,
+           "leaves_per_level_after": leaves,
+           "leaf_cell_updates_per_s": updates / wall,
+           "finest_uniform_cells": 64 * 4 ** 3 * 16 ** 3,
+           "note": ("leaf cells are a fraction of the finest level's uniform cells; the rate "
+                    "counts every leaf's micro steps")}
+    be.close()
+    del be, sol
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_copy_probe():
     """A plain device copy of known size, for scaling the HBM counters."""
     import torch
@@ -511,7 +558,8 @@ def main():
                     help="uniform: step count after which the modes are compared with FP64")
     ap.add_argument("--serving-window", type=float, default=1.0,
                     help="arrival window (s) of the serving leg of the default line")
-    ap.add_argument("--workload", choices=["uniform", "batched", "serving", "accuracy", "mr"],
+    ap.add_argument("--workload", choices=["uniform", "batched", "serving", "accuracy", "mr",
+                                           "multilevel"],
                     default="uniform",
                     help="uniform: the 256^3 case (default); batched: 512 uneven 2D cases; "
                          "serving: Poisson request stream")
@@ -545,6 +593,9 @@ def main():
         return
     if args.workload == "mr":
         print(json.dumps(run_mr(args)), flush=True)
+        return
+    if args.workload == "multilevel":
+        print(json.dumps(run_multilevel()), flush=True)
         return
     if args.workload == "serving":
         from alpaca_amd import serving
@@ -665,8 +716,11 @@ def main():
                                 "roofline_frac": (roe["cells_per_launch"] * FLOP_PER_CELL_STAGE[(3, 1)]
                                                   / (roe["stage_avg_ms"] / 1e3) / PEAK_FP64)}
         # The multiresolution operators (averaging, prediction, details) of
-        # the multi-level mode at the same cell count on level 1.
+        # the multi-level mode at the same cell count on level 1, and a
+        # multi-level run through the driver.
         line["mr_operators"] = run_mr(args)
+        print("[bench] multilevel", file=sys.stderr, flush=True)
+        line["multilevel"] = run_multilevel()
     if not (args.no_extra or args.no_modes):
         # The other configurations at their stated sizes: C0 (2D 128^2), the
         # 512-case batched ensemble and the Poisson serving stream.

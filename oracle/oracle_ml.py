@@ -80,9 +80,10 @@ class OracleBackend:
         pass
 
     # -- node data
-    def clear(self, slot):
-        for t in (self.avg, self.rhs, self.init, self.jflux, self.jcons):
-            t[slot] = 0.0
+    def clear(self, slots):
+        for slot in slots:
+            for t in (self.avg, self.rhs, self.init, self.jflux, self.jcons):
+                t[slot] = 0.0
 
     def _internal(self):
         if self.dim == 3:

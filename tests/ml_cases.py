@@ -57,20 +57,7 @@ def sod_norms(sol):
 
 def blob_initial(dim, node_size):
     """Periodic case: a dense, high-pressure blob in a uniform flow."""
-    def initial(level, key):
-        dx = node_size / (1 << level) / gpu.IC
-        ax = [(key[a + 1] * gpu.IC + np.arange(gpu.IC) + 0.5) * dx for a in range(dim)]
-        g = np.meshgrid(*ax, indexing="ij")
-        centre = (0.4, 0.55, 0.25)
-        r2 = sum((g[a] - centre[a]) ** 2 for a in range(dim))
-        inside = r2 < 0.02
-        rho = 1.0 + 2.0 * inside
-        p = 1.0 + 4.0 * inside
-        vel = [0.5 * np.ones_like(rho), 0.3 * np.ones_like(rho)] + (
-            [np.zeros_like(rho)] if dim == 3 else [])
-        c = cases.primes_to_conservatives(dim, 1.4, 0.0, rho, vel, p)
-        return c[..., None] if dim == 2 else c
-    return initial
+    return cases.blob_node_initial(dim, node_size)
 
 
 def blob_solver(backend, dim, lmax, flux=gpu.FLUX_HLLC):
