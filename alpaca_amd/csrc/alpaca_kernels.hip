@@ -51,6 +51,20 @@ struct CaseArgs {
   double gamma, b, cell_size;
 };
 
+// Which block of device memory the i-th block of a per-block launch is: the
+// i-th block of a contiguous array (list == nullptr), or list[i] of a pool of
+// `capacity` blocks, an entry outside [0, capacity) being skipped (-1).  The
+// face-flux scratch stays indexed by i.
+struct BlockList {
+  const int *list = nullptr;
+  int capacity = 0;
+  __device__ __forceinline__ int block(int i) const {
+    if (!list) return i;
+    int const b = list[i];
+    return (b >= 0 && b < capacity) ? b : -1;
+  }
+};
+
 // ---------------------------------------------------------------- faces ---
 // Output layout per (block, dir): [NE][n0][n1][n2] with n_dir = 17 and the
 // other extents 16 (1 for z in 2D): face index a along DIR is the face
@@ -79,7 +93,7 @@ template <int DIM, int DIR, int FLUX, int MODE>
 __global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__restrict__ in,
                                                              double *__restrict__ flux_out,
                                                              int nblocks, double gamma,
-                                                             double b) {
+                                                             double b, BlockList list) {
   using D = Dims<DIM>;
   constexpr int NE = D::NE;
   constexpr int NF = faces_per_dir<DIM>();
@@ -90,6 +104,8 @@ __global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__res
   int const blk = g / groups_per_block;
   int const f = (g % groups_per_block) * kThreads + threadIdx.x;
   if (blk >= nblocks || f >= NF) return;
+  int const mem = list.block(blk);
+  if (mem < 0) return;
   int const a2 = f % n2;
   int const a1 = (f / n2) % n1;
   int const a0 = f / (n2 * n1);
@@ -98,7 +114,7 @@ __global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__res
   int const j = a1 + (DIR == 1 ? kHS - 1 : kHS);
   int const k = DIM == 3 ? a2 + (DIR == 2 ? kHS - 1 : kHS) : 0;
   double fl[NE];
-  mode_stencil_flux<DIM, DIR, FLUX, MODE>(in + (size_t)blk * D::BLOCK, i, j, k, gamma, b, fl);
+  mode_stencil_flux<DIM, DIR, FLUX, MODE>(in + (size_t)mem * D::BLOCK, i, j, k, gamma, b, fl);
   double *o = flux_out + ((size_t)blk * DIM + DIR) * NE * NF + f;
 #pragma unroll
   for (int e = 0; e < NE; ++e) o[e * NF] = fl[e];
@@ -113,7 +129,7 @@ __global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__res
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void jump_flux_kernel(const double *__restrict__ flux,
                                                              double *__restrict__ jump,
-                                                             int nblocks) {
+                                                             int nblocks, BlockList list) {
   using D = Dims<DIM>;
   constexpr int NE = D::NE;
   constexpr int NF = faces_per_dir<DIM>();
@@ -122,6 +138,8 @@ __global__ __launch_bounds__(kThreads) void jump_flux_kernel(const double *__res
   size_t const t = (size_t)blockIdx.x * kThreads + threadIdx.x;
   if (t >= (size_t)nblocks * 6 * FACE) return;
   int const blk = int(t / (6 * FACE));
+  int const mem = list.block(blk);
+  if (mem < 0) return;
   int const loc = int((t / FACE) % 6);
   int const r = int(t % FACE);
   int const e = r / (kIC * ICZ), i = (r / ICZ) % kIC, j = r % ICZ;
@@ -139,7 +157,7 @@ __global__ __launch_bounds__(kThreads) void jump_flux_kernel(const double *__res
         v = fz[(i * kIC + j) * (kIC + 1) + (loc == 4 ? kIC : 0)];  // Top / Bottom
       }
   }
-  jump[t] += v;
+  jump[(size_t)mem * 6 * FACE + t % (6 * FACE)] += v;
 }
 
 // ------------------------------------------------------------- combine ---
@@ -159,7 +177,7 @@ __host__ __device__ inline StageCoef stage_coef(int stage) {
 template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) void combine_kernel(
     const double *__restrict__ flux, const double *__restrict__ avg, const double *init,
-    double *out, int nblocks, int stage, double inv_dx, alpaca_case_state *state) {
+    double *out, int nblocks, int stage, double inv_dx, alpaca_case_state *state, BlockList list) {
   using D = Dims<DIM>;
   constexpr int NE = D::NE;
   constexpr int NF = faces_per_dir<DIM>();
@@ -170,6 +188,8 @@ __global__ __launch_bounds__(kThreads) void combine_kernel(
   int const blk = g / groups_per_block;
   int const c = (g % groups_per_block) * kThreads + threadIdx.x;
   if (blk >= nblocks || c >= cells) return;
+  int const mem = list.block(blk);
+  if (mem < 0) return;
   int const kk = c % ICZ;
   int const jj = (c / ICZ) % kIC;
   int const ii = c / (ICZ * kIC);
@@ -183,7 +203,7 @@ __global__ __launch_bounds__(kThreads) void combine_kernel(
   int const y_hi = (ii * (kIC + 1) + jj + 1) * ICZ + kk;
   int const z_lo = (ii * kIC + jj) * (kIC + 1) + kk;     // fz [16][16][17]
   int const z_hi = z_lo + 1;
-  size_t const cell = (size_t)blk * D::BLOCK + (ii + kHS) * D::SX + (jj + kHS) * D::SY +
+  size_t const cell = (size_t)mem * D::BLOCK + (ii + kHS) * D::SX + (jj + kHS) * D::SY +
                       (DIM == 3 ? kk + kHS : 0);
   double dt = 0.0;
   StageCoef co{1.0, 0.0, 1.0};
@@ -438,17 +458,19 @@ __global__ __launch_bounds__(kThreads) void halo_list_kernel(double *blocks, int
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void signal_kernel(const double *__restrict__ in, int nblocks,
                                                           double gamma, double b,
-                                                          alpaca_case_state *state) {
+                                                          alpaca_case_state *state, BlockList list) {
   using D = Dims<DIM>;
   constexpr int NE = D::NE;
   constexpr int ICZ = D::ICZ;
   __shared__ double part[kThreads / 64];
   int const blk = blockIdx.x;
   if (blk >= nblocks) return;
+  int const mem = list.block(blk);  // uniform over the workgroup
+  if (mem < 0) return;
   double s = 0.0;
   for (int c = threadIdx.x; c < D::CELLS; c += kThreads) {
     int const kk = c % ICZ, jj = (c / ICZ) % kIC, ii = c / (ICZ * kIC);
-    const double *p = in + (size_t)blk * D::BLOCK + (ii + kHS) * D::SX + (jj + kHS) * D::SY +
+    const double *p = in + (size_t)mem * D::BLOCK + (ii + kHS) * D::SX + (jj + kHS) * D::SY +
                       (DIM == 3 ? kk + kHS : 0);
     double u[NE];
 #pragma unroll
@@ -494,6 +516,45 @@ __global__ void dt_kernel(alpaca_case_state *state, double cell_size) {
 __global__ __launch_bounds__(kThreads) void zero_kernel(double *p, size_t n) {
   size_t const stride = (size_t)gridDim.x * kThreads;
   for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < n; t += stride) p[t] = 0.0;
+}
+
+// The listed blocks of a pool, 16 bytes per thread and step (grid-stride; a
+// haloed block is an even number of doubles, so every pair is 16-byte
+// aligned): OP 0 zeroes a's blocks, OP 1 exchanges the blocks of a and b
+// (SwapConservativeBuffersForNode, buffer_operations_material.h:83-91), OP 2
+// copies a's blocks into b's.
+template <int OP>
+__global__ __launch_bounds__(kThreads) void listed_blocks_kernel(double *a, double *b,
+                                                                 size_t block_doubles,
+                                                                 BlockList list, int n) {
+  size_t const pairs = block_doubles / 2;
+  size_t const total = (size_t)n * pairs;
+  size_t const stride = (size_t)gridDim.x * kThreads;
+  for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < total; t += stride) {
+    int const mem = list.block(int(t / pairs));
+    if (mem < 0) continue;
+    size_t const o = (size_t)mem * pairs + t % pairs;
+    double2 *pa = reinterpret_cast<double2 *>(a) + o;
+    if constexpr (OP == 0) {
+      *pa = double2{0.0, 0.0};
+    } else if constexpr (OP == 2) {
+      reinterpret_cast<double2 *>(b)[o] = *pa;
+    } else {
+      double2 *pb = reinterpret_cast<double2 *>(b) + o;
+      double2 const va = *pa;
+      double2 const vb = *pb;
+      *pa = vb;
+      *pb = va;
+    }
+  }
+}
+
+template <int OP>
+void launch_listed_blocks(double *a, double *b, int dim, BlockList list, int n, hipStream_t s) {
+  size_t const bd = alpaca_block_doubles(dim);
+  size_t const groups = ((size_t)n * (bd / 2) + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL((listed_blocks_kernel<OP>), dim3((unsigned)std::min<size_t>(groups, 65536)),
+                     dim3(kThreads), 0, s, a, b, bd, list, n);
 }
 
 // The exact form's FP64 division and square root as the face kernels
@@ -666,46 +727,47 @@ constexpr size_t flux_doubles(int nblocks) {
 
 template <int DIM, int DIR, int FLUX>
 void launch_faces_dir(const alpaca_case_desc *d, int nblocks, const double *in, double *fl,
-                      hipStream_t s) {
+                      hipStream_t s, BlockList list) {
   constexpr int NF = faces_per_dir<DIM>();
   constexpr int gpb = (NF + kThreads - 1) / kThreads;
   dim3 grid((unsigned)(nblocks * gpb));
   ProfileScope prof(ALPACA_KERNEL_FACE_FLUX, s, double(NF) * nblocks);
   if (d->precision == ALPACA_MIXED)
     hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, kModeMixed>), grid, dim3(kThreads), 0, s,
-                       in, fl, nblocks, d->gamma, d->background_pressure);
+                       in, fl, nblocks, d->gamma, d->background_pressure, list);
   else if (d->precision == ALPACA_FP64_FMA)
     hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, kModeFma>), grid, dim3(kThreads), 0, s,
-                       in, fl, nblocks, d->gamma, d->background_pressure);
+                       in, fl, nblocks, d->gamma, d->background_pressure, list);
   else
     hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, kModeExact>), grid, dim3(kThreads), 0, s,
-                       in, fl, nblocks, d->gamma, d->background_pressure);
+                       in, fl, nblocks, d->gamma, d->background_pressure, list);
 }
 
 template <int DIM, int FLUX>
 void launch_faces_flux(const alpaca_case_desc *d, int nblocks, const double *in, double *fl,
-                       hipStream_t s) {
-  launch_faces_dir<DIM, 0, FLUX>(d, nblocks, in, fl, s);
-  launch_faces_dir<DIM, 1, FLUX>(d, nblocks, in, fl, s);
-  if constexpr (DIM == 3) launch_faces_dir<DIM, 2, FLUX>(d, nblocks, in, fl, s);
+                       hipStream_t s, BlockList list) {
+  launch_faces_dir<DIM, 0, FLUX>(d, nblocks, in, fl, s, list);
+  launch_faces_dir<DIM, 1, FLUX>(d, nblocks, in, fl, s, list);
+  if constexpr (DIM == 3) launch_faces_dir<DIM, 2, FLUX>(d, nblocks, in, fl, s, list);
 }
 
 template <int DIM>
 void launch_faces(const alpaca_case_desc *d, int nblocks, const double *in, double *fl,
-                  hipStream_t s) {
+                  hipStream_t s, BlockList list = {}) {
   if (d->flux == ALPACA_FLUX_HLLC)
-    launch_faces_flux<DIM, ALPACA_FLUX_HLLC>(d, nblocks, in, fl, s);
+    launch_faces_flux<DIM, ALPACA_FLUX_HLLC>(d, nblocks, in, fl, s, list);
   else
-    launch_faces_flux<DIM, ALPACA_FLUX_ROE>(d, nblocks, in, fl, s);
+    launch_faces_flux<DIM, ALPACA_FLUX_ROE>(d, nblocks, in, fl, s, list);
 }
 
 // Faces into the scratch fl (flux_doubles<DIM>(nblocks) doubles), then the
-// divergence (mode 1) or divergence + RK combination (mode 0) into out.
+// divergence (mode 1) or divergence + RK combination (mode 0) into out.  With
+// a block list, avg, init, out and jump are pools indexed through it.
 template <int DIM>
 int stage_impl(const alpaca_case_desc *d, int nblocks, int stage, int mode, const double *avg,
                double *init, double *out, alpaca_case_state *state, hipStream_t s, double *fl,
-               double *jump = nullptr) {
-  launch_faces<DIM>(d, nblocks, avg, fl, s);
+               double *jump = nullptr, BlockList list = {}) {
+  launch_faces<DIM>(d, nblocks, avg, fl, s, list);
   constexpr int cells = Dims<DIM>::CELLS;
   constexpr int gpb = (cells + kThreads - 1) / kThreads;
   dim3 grid((unsigned)(nblocks * gpb));
@@ -715,15 +777,15 @@ int stage_impl(const alpaca_case_desc *d, int nblocks, int stage, int mode, cons
     double *ini = (init && init != avg) ? init : nullptr;
     if (stage > 0) ini = init;
     hipLaunchKernelGGL((combine_kernel<DIM, 0>), grid, dim3(kThreads), 0, s, fl, avg, ini, out,
-                       nblocks, stage, inv_dx, state);
+                       nblocks, stage, inv_dx, state, list);
   } else {
     hipLaunchKernelGGL((combine_kernel<DIM, 1>), grid, dim3(kThreads), 0, s, fl, avg, nullptr,
-                       out, nblocks, 0, inv_dx, state);
+                       out, nblocks, 0, inv_dx, state, list);
   }
   if (jump) {
     size_t const total = (size_t)nblocks * 6 * Dims<DIM>::NE * kIC * Dims<DIM>::ICZ;
     hipLaunchKernelGGL((jump_flux_kernel<DIM>), dim3((unsigned)((total + kThreads - 1) / kThreads)),
-                       dim3(kThreads), 0, s, fl, jump, nblocks);
+                       dim3(kThreads), 0, s, fl, jump, nblocks, list);
   }
   return hip_status(hipGetLastError());
 }
@@ -823,11 +885,10 @@ int halo_table_impl(double *blocks, int nblocks, const int *nbr, const SideBc &b
 
 template <int DIM>
 int timestep_impl(const alpaca_case_desc *d, const double *avg, alpaca_case_state *state,
-                  hipStream_t s) {
-  int const nblocks = case_blocks(d);
+                  hipStream_t s, int nblocks, BlockList list = {}) {
   ProfileScope prof(ALPACA_KERNEL_TIMESTEP, s, double(Dims<DIM>::CELLS) * nblocks);
   hipLaunchKernelGGL((signal_kernel<DIM>), dim3(nblocks), dim3(kThreads), 0, s, avg, nblocks,
-                     d->gamma, d->background_pressure, state);
+                     d->gamma, d->background_pressure, state, list);
   hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(64), 0, s, state, d->cell_size);
   return hip_status(hipGetLastError());
 }
@@ -968,22 +1029,26 @@ namespace {
 // Right-hand side of nblocks blocks with the face fluxes in fl: zero the
 // output blocks, faces, divergence, and (jump != null) the boundary-flux save.
 int rhs_with_scratch(const alpaca_case_desc *desc, int nblocks, const double *d_avg, double *d_rhs,
-                     double *d_jump, double *fl, hipStream_t s) {
-  size_t const n = alpaca_block_doubles(desc->dim) * (size_t)nblocks;
-  unsigned const groups = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 65536);
-  hipLaunchKernelGGL(zero_kernel, dim3(groups), dim3(kThreads), 0, s, d_rhs, n);
+                     double *d_jump, double *fl, hipStream_t s, BlockList list = {}) {
+  if (list.list) {
+    launch_listed_blocks<0>(d_rhs, nullptr, desc->dim, list, nblocks, s);
+  } else {
+    size_t const n = alpaca_block_doubles(desc->dim) * (size_t)nblocks;
+    unsigned const groups = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 65536);
+    hipLaunchKernelGGL(zero_kernel, dim3(groups), dim3(kThreads), 0, s, d_rhs, n);
+  }
   if (desc->dim == 2)
-    return stage_impl<2>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, fl, d_jump);
-  return stage_impl<3>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, fl, d_jump);
+    return stage_impl<2>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, fl, d_jump, list);
+  return stage_impl<3>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, fl, d_jump, list);
 }
 
 int rhs_context(alpaca_context *ctx, const alpaca_case_desc *desc, int nblocks,
-                const double *d_avg, double *d_rhs, double *d_jump) {
+                const double *d_avg, double *d_rhs, double *d_jump, BlockList list = {}) {
   double *fl = nullptr;
   size_t const n = desc->dim == 2 ? flux_doubles<2>(nblocks) : flux_doubles<3>(nblocks);
   int st = context_scratch(ctx, n * sizeof(double), &fl);
   if (st != ALPACA_OK) return st;
-  return rhs_with_scratch(desc, nblocks, d_avg, d_rhs, d_jump, fl, ctx->stream);
+  return rhs_with_scratch(desc, nblocks, d_avg, d_rhs, d_jump, fl, ctx->stream, list);
 }
 
 // Device address of `scratch` on the current device, looked up once per
@@ -1029,6 +1094,44 @@ int alpaca_ctx_right_hand_side_jump_fluxes(alpaca_context *ctx, const alpaca_cas
       d_avg == d_rhs)
     return ALPACA_ERR_INVALID;
   return rhs_context(ctx, desc, nblocks, d_avg, d_rhs, d_jump_fluxes);
+}
+
+int alpaca_ctx_right_hand_side_jump_fluxes_list(alpaca_context *ctx, const alpaca_case_desc *desc,
+                                                int capacity, const int *d_list, int n,
+                                                const double *d_avg, double *d_rhs,
+                                                double *d_jump_fluxes) {
+  int st = validate(desc);
+  if (st != ALPACA_OK) return st;
+  if (!ctx || capacity < 1 || capacity > kMaxBlocks || n < 0 || n > kMaxBlocks || !d_avg ||
+      !d_rhs || !d_jump_fluxes || d_avg == d_rhs)
+    return ALPACA_ERR_INVALID;
+  if (n == 0) return ALPACA_OK;
+  if (!d_list) return ALPACA_ERR_INVALID;
+  return rhs_context(ctx, desc, n, d_avg, d_rhs, d_jump_fluxes, BlockList{d_list, capacity});
+}
+
+int alpaca_swap_blocks_list(int dim, double *d_a, double *d_b, int capacity, const int *d_list,
+                            int n, void *stream) {
+  if ((dim != 2 && dim != 3) || !d_a || !d_b || d_a == d_b || capacity < 1 ||
+      capacity > kMaxBlocks || n < 0 || n > kMaxBlocks)
+    return ALPACA_ERR_INVALID;
+  if (n == 0) return ALPACA_OK;
+  if (!d_list) return ALPACA_ERR_INVALID;
+  launch_listed_blocks<1>(d_a, d_b, dim, BlockList{d_list, capacity}, n,
+                          static_cast<hipStream_t>(stream));
+  return hip_status(hipGetLastError());
+}
+
+int alpaca_copy_blocks_list(int dim, const double *d_src, double *d_dst, int capacity,
+                            const int *d_list, int n, void *stream) {
+  if ((dim != 2 && dim != 3) || !d_src || !d_dst || d_src == d_dst || capacity < 1 ||
+      capacity > kMaxBlocks || n < 0 || n > kMaxBlocks)
+    return ALPACA_ERR_INVALID;
+  if (n == 0) return ALPACA_OK;
+  if (!d_list) return ALPACA_ERR_INVALID;
+  launch_listed_blocks<2>(const_cast<double *>(d_src), d_dst, dim, BlockList{d_list, capacity}, n,
+                          static_cast<hipStream_t>(stream));
+  return hip_status(hipGetLastError());
 }
 
 int alpaca_convective_fluxes(const alpaca_case_desc *desc, int nblocks, const double *d_avg,
@@ -1170,8 +1273,21 @@ int alpaca_timestep(const alpaca_case_desc *desc, const double *d_avg, alpaca_ca
   if (st != ALPACA_OK) return st;
   if (!d_avg || !d_state) return ALPACA_ERR_INVALID;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (desc->dim == 2) return timestep_impl<2>(desc, d_avg, d_state, s);
-  return timestep_impl<3>(desc, d_avg, d_state, s);
+  if (desc->dim == 2) return timestep_impl<2>(desc, d_avg, d_state, s, case_blocks(desc));
+  return timestep_impl<3>(desc, d_avg, d_state, s, case_blocks(desc));
+}
+
+int alpaca_timestep_list(const alpaca_case_desc *desc, const double *d_avg, int capacity,
+                         const int *d_list, int n, alpaca_case_state *d_state, void *stream) {
+  int st = validate(desc);
+  if (st != ALPACA_OK) return st;
+  if (!d_avg || !d_state || !d_list || capacity < 1 || capacity > kMaxBlocks || n < 1 ||
+      n > kMaxBlocks)
+    return ALPACA_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  BlockList const list{d_list, capacity};
+  if (desc->dim == 2) return timestep_impl<2>(desc, d_avg, d_state, s, n, list);
+  return timestep_impl<3>(desc, d_avg, d_state, s, n, list);
 }
 
 int alpaca_finish_step(alpaca_case_state *d_state, void *stream) {

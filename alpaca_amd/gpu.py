@@ -68,6 +68,14 @@ _SIGNATURES = {
     "alpaca_ctx_right_hand_side_jump_fluxes": ([ctypes.c_void_p, ctypes.POINTER(CaseDesc),
                                                 ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_void_p], ctypes.c_int),
+    "alpaca_ctx_right_hand_side_jump_fluxes_list": ([ctypes.c_void_p, ctypes.POINTER(CaseDesc),
+                                                     ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                                     ctypes.c_void_p, ctypes.c_void_p,
+                                                     ctypes.c_void_p], ctypes.c_int),
+    "alpaca_swap_blocks_list": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_copy_blocks_list": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "alpaca_rk3_stage": ([ctypes.POINTER(CaseDesc), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "alpaca_fill_halos_table": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
@@ -79,6 +87,9 @@ _SIGNATURES = {
                           ctypes.c_int),
     "alpaca_timestep": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_void_p,
                          ctypes.c_void_p], ctypes.c_int),
+    "alpaca_timestep_list": ([ctypes.POINTER(CaseDesc), ctypes.c_void_p, ctypes.c_int,
+                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
+                             ctypes.c_int),
     "alpaca_finish_step": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "alpaca_case_state_init": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_void_p], ctypes.c_int),

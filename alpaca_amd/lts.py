@@ -70,20 +70,20 @@ class LevelTimeIntegrator:
         return out.value
 
     # -- per-node operations over a list of pool blocks
-    def fill_initial_buffer(self, average, initial, block_list, stage: int):
+    def fill_initial_buffer(self, average, initial, block_list, stage: int, stream=None):
         """FillInitialBuffer (time_integrator.h:369-406), single phase: the
         Average buffers of the listed blocks copied to Initial at stage 0."""
         if stage != 0:
             return
-        torch = gpu._torch()
         nb = _pool(average, self.dim)
         if _pool(initial, self.dim) != nb:
             raise gpu.AlpacaError("average and initial pools differ in size")
-        per = gpu.block_doubles(self.dim)
-        a = average.view(nb, per)
-        i = initial.view(nb, per)
-        idx = block_list.to(torch.long)
-        i.index_copy_(0, idx, a.index_select(0, idx))
+        n = _list(block_list)
+        lib = gpu.load_library()
+        gpu.check(lib.alpaca_copy_blocks_list(self.dim, ctypes.c_void_p(average.data_ptr()),
+                                              ctypes.c_void_p(initial.data_ptr()), nb,
+                                              ctypes.c_void_p(block_list.data_ptr()), n,
+                                              _stream(stream)), "copy_blocks_list")
 
     def prepare_buffer_for_integration(self, average, initial, block_list, stage: int, stream=None):
         lib = gpu.load_library()

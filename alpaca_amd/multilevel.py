@@ -297,12 +297,20 @@ class MultiLevelSolver:
             rows, which = np.nonzero(table[:, cols] == -2)
             jump_links = [self._link(keys[r]) for r in rows.tolist()]
             jump_locs = [self.halo_locs[c] for c in which.tolist()]
-            return self._slots(keys), table, jump_links, jump_locs
+            slots = self._slots(keys)
+            # External-slab cells whose source lies in a jump region (written
+            # after the jump update) exist only in blocks with both a jump
+            # neighbour (-2) and an offset leaving the domain (-1).
+            second = None
+            jrows = np.nonzero((table == -2).any(axis=1) & (table == -1).any(axis=1))[0]
+            if jrows.size:
+                second = ([slots[r] for r in jrows.tolist()], np.ascontiguousarray(table[jrows]))
+            return slots, table, jump_links, jump_locs, second
 
-        slots, table, jump_links, jump_locs = self._cached("halo", level, build)
+        slots, table, jump_links, jump_locs, second = self._cached("halo", level, build)
         if cut_jumps:
             jump_links, jump_locs = [], []
-        self.be.halo_update(slots, table, jump_links, jump_locs, self.side_bc)
+        self.be.halo_update(slots, table, jump_links, jump_locs, self.side_bc, second)
 
     def halo_update(self, levels_ascending, cut_jumps=False):
         """HaloManager::MaterialHaloUpdate (halo_manager.cpp:59-76)."""
@@ -614,8 +622,12 @@ class GpuBackend:
         self.jflux = torch.zeros(sshape, dtype=torch.float64, device="cuda")
         self.jcons = torch.zeros_like(self.jflux)
         self.state = torch.zeros(ctypes.sizeof(gpu.CaseState), dtype=torch.uint8, device="cuda")
+        # Every operation is queued on the stream current at construction (the
+        # context's stream too); torch work of the backend runs under it.
+        self.stream = torch.cuda.current_stream()
+        self.st = ctypes.c_void_p(self.stream.cuda_stream)
         self.ctx = ctypes.c_void_p()
-        gpu.check(self.lib.alpaca_context_create(gpu._stream_ptr(), ctypes.byref(self.ctx)),
+        gpu.check(self.lib.alpaca_context_create(self.st, ctypes.byref(self.ctx)),
                   "context_create")
         self.lti = lts.LevelTimeIntegrator(dim)
         self.mrop = mr.Multiresolution()
@@ -644,9 +656,17 @@ class GpuBackend:
             return hit[1]
         if len(self._memo) > 4096:
             self._memo.clear()
-        t = self.torch.as_tensor(make(obj), device="cuda")
+        t = self._upload(make(obj))
         self._memo[key] = (obj, t)
         return t
+
+    def _upload(self, arr):
+        """Host array to the device without waiting for the queued work: a
+        pinned staging copy, then an asynchronous copy on the backend's
+        stream (torch keeps the staging block until that copy is done)."""
+        host = self.torch.from_numpy(np.ascontiguousarray(arr))
+        with self.torch.cuda.stream(self.stream):
+            return host.pin_memory().to("cuda", non_blocking=True)
 
     def _idx(self, slots):
         return self._device(slots, "i64", lambda o: np.asarray(o, dtype=np.int64))
@@ -667,60 +687,62 @@ class GpuBackend:
 
     # -- node data
     def clear(self, slots):
-        idx = self.torch.as_tensor(np.asarray(slots, dtype=np.int64), device="cuda")
-        for t in (self.avg, self.rhs, self.init, self.jflux, self.jcons):
-            t.index_fill_(0, idx, 0.0)
+        idx = self._upload(np.asarray(slots, dtype=np.int64))
+        with self.torch.cuda.stream(self.stream):
+            for t in (self.avg, self.rhs, self.init, self.jflux, self.jcons):
+                t.index_fill_(0, idx, 0.0)
 
     def set_internal(self, slots, cons):
         """Internal cells of the right-hand-side buffers (ImposeInitialCondition)."""
         h, i = gpu.HS, gpu.IC
-        src = self.torch.as_tensor(np.ascontiguousarray(cons), device="cuda")
+        src = self._upload(cons)
         idx = self._idx(slots)
-        if self.dim == 3:
-            self.rhs[idx, :, h:h + i, h:h + i, h:h + i] = src
-        else:
-            self.rhs[idx, :, h:h + i, h:h + i, :] = src
+        with self.torch.cuda.stream(self.stream):
+            if self.dim == 3:
+                self.rhs[idx, :, h:h + i, h:h + i, h:h + i] = src
+            else:
+                self.rhs[idx, :, h:h + i, h:h + i, :] = src
 
     def get_blocks(self, slots):
-        return self.avg[self._idx(slots)].cpu().numpy()
+        with self.torch.cuda.stream(self.stream):
+            return self.avg[self._idx(slots)].cpu().numpy()
 
     def set_rhs_blocks(self, slots, blocks):
-        src = self.torch.as_tensor(np.ascontiguousarray(blocks), device="cuda")
-        self.rhs.index_copy_(0, self._idx(slots),
-                             src.reshape((len(slots),) + tuple(self.rhs.shape[1:])))
+        src = self._upload(blocks)
+        with self.torch.cuda.stream(self.stream):
+            self.rhs.index_copy_(0, self._idx(slots),
+                                 src.reshape((len(slots),) + tuple(self.rhs.shape[1:])))
 
     def get_internal(self, slots):
         h, i = gpu.HS, gpu.IC
-        a = self.avg[self._idx(slots)]
-        if self.dim == 3:
-            return a[:, :, h:h + i, h:h + i, h:h + i].cpu().numpy()
-        return a[:, :, h:h + i, h:h + i, :].cpu().numpy()
+        with self.torch.cuda.stream(self.stream):
+            a = self.avg[self._idx(slots)]
+            if self.dim == 3:
+                return a[:, :, h:h + i, h:h + i, h:h + i].cpu().numpy()
+            return a[:, :, h:h + i, h:h + i, :].cpu().numpy()
 
     def swap(self, slots):
-        idx = self._idx(slots)
-        a = self.avg.index_select(0, idx)
-        self.avg.index_copy_(0, idx, self.rhs.index_select(0, idx))
-        self.rhs.index_copy_(0, idx, a)
+        lst = self._list(slots)
+        gpu.check(self.lib.alpaca_swap_blocks_list(self.dim, self._p(self.avg), self._p(self.rhs),
+                                                   self.capacity, self._p(lst), len(slots),
+                                                   self.st), "swap_blocks_list")
 
     def fill_initial(self, slots):
-        self.lti.fill_initial_buffer(self.avg, self.init, self._list(slots), 0)
+        self.lti.fill_initial_buffer(self.avg, self.init, self._list(slots), 0, self.st)
 
     # -- stage operations
     def right_hand_side(self, slots, cell_size):
-        idx = self._idx(slots)
+        lst = self._list(slots)
         n = len(slots)
-        a = self.avg.index_select(0, idx).contiguous()
-        r = self.torch.empty_like(a)
-        j = self.jflux.index_select(0, idx).contiguous()
         desc = self._desc(n, cell_size)
-        gpu.check(self.lib.alpaca_ctx_right_hand_side_jump_fluxes(
-            self.ctx, ctypes.byref(desc), n, self._p(a), self._p(r), self._p(j)), "rhs")
-        self.rhs.index_copy_(0, idx, r)
-        self.jflux.index_copy_(0, idx, j)
+        gpu.check(self.lib.alpaca_ctx_right_hand_side_jump_fluxes_list(
+            self.ctx, ctypes.byref(desc), self.capacity, self._p(lst), n, self._p(self.avg),
+            self._p(self.rhs), self._p(self.jflux)), "rhs")
 
     def prepare(self, slots, stage):
         if stage > 0:
-            self.lti.prepare_buffer_for_integration(self.avg, self.init, self._list(slots), stage)
+            self.lti.prepare_buffer_for_integration(self.avg, self.init, self._list(slots), stage,
+                                                    self.st)
 
     def lts_timestep(self, micro, n, stage, jump):
         self.lti.micro_timestep_sizes = list(micro)
@@ -731,10 +753,10 @@ class GpuBackend:
         nb = self.capacity
         gpu.check(self.lib.alpaca_lts_integrate_jump_conservatives(
             self.dim, self._p(self.jcons), self._p(self.jflux), nb, self._p(lst), len(slots),
-            jump_timestep, gpu._stream_ptr()), "lts_integrate_jump_conservatives")
+            jump_timestep, self.st), "lts_integrate_jump_conservatives")
         gpu.check(self.lib.alpaca_lts_integrate(self.dim, self._p(self.avg), self._p(self.rhs), nb,
                                                 self._p(lst), len(slots), None, timestep,
-                                                gpu._stream_ptr()), "lts_integrate")
+                                                self.st), "lts_integrate")
 
     def integrate_range(self, slots, rng, timestep):
         lst = self._list(slots)
@@ -742,43 +764,50 @@ class GpuBackend:
         cr = (ctypes.c_int * 6)(*r)
         gpu.check(self.lib.alpaca_lts_integrate(self.dim, self._p(self.avg), self._p(self.rhs),
                                                 self.capacity, self._p(lst), len(slots), cr,
-                                                timestep, gpu._stream_ptr()), "lts_integrate halo")
+                                                timestep, self.st), "lts_integrate halo")
 
     def average(self, links):
-        self.mrop.average(self.rhs, self.dim, self._links(links))
+        self.mrop.average(self.rhs, self.dim, self._links(links), self.st)
 
     def average_jump(self, links):
-        self.mrop.average_jump_buffer(self.jcons, self.dim, self._links(links))
+        self.mrop.average_jump_buffer(self.jcons, self.dim, self._links(links), self.st)
 
-    def halo_update(self, slots, table, jump_links, jump_locs, side_bc):
+    def halo_update(self, slots, table, jump_links, jump_locs, side_bc, second=None):
+        """second: (slots, table rows) of the blocks the pass after the jump
+        update can write (None: no such block)."""
         lst = self._list(slots)
         tab = self._device(table, "table", lambda o: np.ascontiguousarray(o, dtype=np.int32))
         bc = (ctypes.c_int * 6)(*(list(side_bc) + [0] * (6 - len(side_bc))))
-        st = gpu._stream_ptr()
+        st = self.st
         gpu.check(self.lib.alpaca_fill_halos_list(self.dim, self._p(self.rhs), self.capacity,
                                                   self._p(lst), len(slots), self._p(tab), bc, 0,
                                                   st), "fill_halos_list")
         if jump_links:
             locs = self._device(jump_locs, "i32", lambda o: np.asarray(o, dtype=np.int32))
-            self.mrop.jump_halo_update(self.rhs, self.dim, self._links(jump_links), locs)
-        gpu.check(self.lib.alpaca_fill_halos_list(self.dim, self._p(self.rhs), self.capacity,
-                                                  self._p(lst), len(slots), self._p(tab), bc, 1,
-                                                  st), "fill_halos_list (after jumps)")
+            self.mrop.jump_halo_update(self.rhs, self.dim, self._links(jump_links), locs, self.st)
+        if second is not None:
+            lst2 = self._list(second[0])
+            tab2 = self._device(second[1], "table",
+                                lambda o: np.ascontiguousarray(o, dtype=np.int32))
+            gpu.check(self.lib.alpaca_fill_halos_list(self.dim, self._p(self.rhs), self.capacity,
+                                                      self._p(lst2), len(second[0]),
+                                                      self._p(tab2), bc, 1, st),
+                      "fill_halos_list (after jumps)")
 
     def jump_flux_adjustment(self, entries, one_cell_size):
         ent = self._device(entries, "table", lambda o: np.ascontiguousarray(o, dtype=np.int32))
-        lts.jump_flux_adjustment(self.dim, self.rhs, self.jcons, ent, one_cell_size)
+        lts.jump_flux_adjustment(self.dim, self.rhs, self.jcons, ent, one_cell_size, self.st)
 
     def reset_jump_conservatives(self, slots):
-        lts.reset_jump_conservatives(self.dim, self.jcons, self._list(slots))
+        lts.reset_jump_conservatives(self.dim, self.jcons, self._list(slots), self.st)
 
     def reset_jump_all(self, slots):
         lst = self._list(slots)
-        lts.reset_jump_conservatives(self.dim, self.jcons, lst)
-        lts.reset_jump_conservatives(self.dim, self.jflux, lst)
+        lts.reset_jump_conservatives(self.dim, self.jcons, lst, self.st)
+        lts.reset_jump_conservatives(self.dim, self.jflux, lst, self.st)
 
     def predict(self, links, rng):
-        self.mrop.prediction(self.rhs, self.dim, self._links(links), rng)
+        self.mrop.prediction(self.rhs, self.dim, self._links(links), rng, self.st)
 
     def decisions(self, links, thresholds):
         m = mr.Multiresolution(norm=mr.LINF)
@@ -789,16 +818,21 @@ class GpuBackend:
         eps = (ctypes.c_double * len(thresholds))(*thresholds)
         gpu.check(self.lib.alpaca_mr_child_needs_remeshing(
             self.dim, self._p(self.rhs), self.capacity, self._p(lk), n, m.norm, eps,
-            len(thresholds), self._p(det), self._p(dec), gpu._stream_ptr()),
+            len(thresholds), self._p(det), self._p(dec), self.st),
             "mr_child_needs_remeshing")
-        return dec.cpu().numpy()
+        with self.torch.cuda.stream(self.stream):
+            return dec.cpu().numpy()
 
     def timestep(self, leaf_slots, cfl, dx_finest, run_time, t_end):
-        a = self.avg.index_select(0, self._idx(leaf_slots)).contiguous()
+        lst = self._list(leaf_slots)
         desc = self._desc(len(leaf_slots), dx_finest)
-        st = gpu._stream_ptr()
+        st = self.st
         sp = self._p(self.state)
         gpu.check(self.lib.alpaca_case_state_init(sp, run_time, t_end, cfl, st), "state_init")
-        gpu.check(self.lib.alpaca_timestep(ctypes.byref(desc), self._p(a), sp, st), "timestep")
-        rec = gpu.CaseState.from_buffer_copy(self.state.cpu().numpy().tobytes())
+        gpu.check(self.lib.alpaca_timestep_list(ctypes.byref(desc), self._p(self.avg),
+                                                self.capacity, self._p(lst), len(leaf_slots), sp,
+                                                st), "timestep")
+        with self.torch.cuda.stream(self.stream):
+            raw = self.state.cpu().numpy().tobytes()
+        rec = gpu.CaseState.from_buffer_copy(raw)
         return rec.dt

@@ -175,6 +175,32 @@ int alpaca_ctx_right_hand_side_jump_fluxes(alpaca_context *ctx,
                                            const double *d_avg, double *d_rhs,
                                            double *d_jump_fluxes);
 
+/* (ii) with the boundary-flux save over the listed leaves of a node pool:
+ * the loop of ModularAlgorithmAssembler::ComputeRightHandSide
+ * (modular_algorithm_assembler.cpp:990-1006) calling SpaceSolver::UpdateFluxes
+ * for each leaf of a level.  d_avg, d_rhs and d_jump_fluxes are pools of
+ * `capacity` haloed blocks / SurfaceBuffers; the n distinct device entries of
+ * d_list name the blocks, each treated as alpaca_ctx_right_hand_side_
+ * jump_fluxes treats its blocks (the same bits), with no copy of the pool.
+ * Entries outside [0, capacity) are skipped. */
+int alpaca_ctx_right_hand_side_jump_fluxes_list(alpaca_context *ctx,
+                                                const alpaca_case_desc *desc, int capacity,
+                                                const int *d_list, int n, const double *d_avg,
+                                                double *d_rhs, double *d_jump_fluxes);
+
+/* SwapBuffers (modular_algorithm_assembler.cpp:1044-1059) of a pool: the
+ * haloed blocks d_list[i] of d_a and d_b exchanged
+ * (SwapConservativeBuffersForNode, buffer_operations_material.h:83-91); n
+ * distinct device entries, those outside [0, capacity) skipped. */
+int alpaca_swap_blocks_list(int dim, double *d_a, double *d_b, int capacity, const int *d_list,
+                            int n, void *stream);
+
+/* FillInitialBuffer (time_integrator.h:369-406) of a pool, single phase: the
+ * haloed blocks d_list[i] of d_src copied into d_dst (the Average buffers
+ * into the Initial ones at stage 0); entries as for alpaca_swap_blocks_list. */
+int alpaca_copy_blocks_list(int dim, const double *d_src, double *d_dst, int capacity,
+                            const int *d_list, int n, void *stream);
+
 /* (ii) without allocation.  No counterpart in the reference: for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  The same results, bit for bit, as
@@ -270,6 +296,14 @@ int alpaca_fill_halos_list(int dim, double *d_blocks, int nblocks, const int *d_
  * clamped to t_end - time.  Writes d_state->dt (and smax). */
 int alpaca_timestep(const alpaca_case_desc *desc, const double *d_avg,
                     alpaca_case_state *d_state, void *stream);
+
+/* (v) over the n listed leaves of a pool of `capacity` haloed blocks (every
+ * leaf of a multi-level tree: ComputeTimestepSize's loop over
+ * tree_.Leaves(), modular_algorithm_assembler.cpp:1797-1869); desc gives the
+ * dimension, material and finest cell size (its nb is checked as for any
+ * case, not used).  Entries outside [0, capacity) are skipped. */
+int alpaca_timestep_list(const alpaca_case_desc *desc, const double *d_avg, int capacity,
+                         const int *d_list, int n, alpaca_case_state *d_state, void *stream);
 
 /* TimeIntegrator::FinishMacroTimestep (time_integrator.h:244-248) on the
  * device record: time += dt, steps += 1.  alpaca_advance does this itself;

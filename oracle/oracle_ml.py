@@ -158,7 +158,10 @@ class OracleBackend:
             self.rhs[child][e] = oracle_py.mr_prediction(self.dim, self.rhs[parent][e],
                                                          self.rhs[child][e], pos, rng)
 
-    def halo_update(self, slots, table, jump_links, jump_locs, side_bc):
+    def halo_update(self, slots, table, jump_links, jump_locs, side_bc, second=None):
+        # `second` (the blocks a device pass after the jump update can write)
+        # is a device-side launch restriction; this restatement visits every
+        # block.
         dim = self.dim
         locs = [loc for loc in range(26) if dim == 3 or _direction(loc)[2] == 0]
         # same-level copies

@@ -149,3 +149,42 @@ def test_noalloc_and_exact_ops_refuse_before_any_device_call():
     assert lib.alpaca_exact_fp64_ops(0, p, None, q, 10, None) == 1
     assert lib.alpaca_exact_fp64_ops(0, p, p, q, -1, None) == 1
     assert lib.alpaca_exact_fp64_ops(2, p, None, q, 0, None) == 0
+
+
+def test_pool_list_entries_refuse_before_any_device_call():
+    """The block-list forms over a node pool (right-hand side with the jump
+    fluxes, buffer swap and copy, CFL reduction) refuse bad arguments with
+    ALPACA_ERR_INVALID, and an empty list is done without a device call."""
+    lib = gpu.load_library()
+    p, q, r, lst = (ctypes.c_void_p(v) for v in (256, 512, 768, 1024))
+    ctx = ctypes.c_void_p(2048)  # never dereferenced on these paths
+    d3 = gpu.make_desc(3, [4, 1, 1], [0, 0, 0], cell_size=1.0 / 64)
+    rhs = lib.alpaca_ctx_right_hand_side_jump_fluxes_list
+    assert rhs(None, ctypes.byref(d3), 8, lst, 4, p, q, r) == 1      # no context
+    assert rhs(ctx, ctypes.byref(d3), 0, lst, 4, p, q, r) == 1       # no pool
+    assert rhs(ctx, ctypes.byref(d3), 1 << 25, lst, 4, p, q, r) == 1
+    assert rhs(ctx, ctypes.byref(d3), 8, None, 4, p, q, r) == 1      # no list
+    assert rhs(ctx, ctypes.byref(d3), 8, lst, -1, p, q, r) == 1
+    assert rhs(ctx, ctypes.byref(d3), 8, lst, 4, p, p, r) == 1       # avg is rhs
+    assert rhs(ctx, ctypes.byref(d3), 8, lst, 4, p, q, None) == 1
+    assert rhs(ctx, ctypes.byref(d3), 8, None, 0, p, q, r) == 0      # nothing to do
+    swap = lib.alpaca_swap_blocks_list
+    assert swap(4, p, q, 8, lst, 4, None) == 1
+    assert swap(3, p, p, 8, lst, 4, None) == 1
+    assert swap(3, p, q, 0, lst, 4, None) == 1
+    assert swap(3, p, q, 8, None, 4, None) == 1
+    assert swap(3, p, q, 8, lst, 1 << 25, None) == 1
+    assert swap(2, p, q, 8, None, 0, None) == 0
+    copy = lib.alpaca_copy_blocks_list
+    assert copy(3, p, p, 8, lst, 4, None) == 1
+    assert copy(3, None, q, 8, lst, 4, None) == 1
+    assert copy(3, p, q, 8, lst, -1, None) == 1
+    assert copy(3, p, q, 8, None, 0, None) == 0
+    ts = lib.alpaca_timestep_list
+    assert ts(ctypes.byref(d3), p, 8, None, 4, q, None) == 1
+    assert ts(ctypes.byref(d3), p, 8, lst, 0, q, None) == 1
+    assert ts(ctypes.byref(d3), p, 0, lst, 4, q, None) == 1
+    assert ts(ctypes.byref(d3), p, 8, lst, 4, None, None) == 1
+    bad = gpu.make_desc(3, [4, 1, 1], [0, 0, 0])
+    bad.gamma = 1.0
+    assert ts(ctypes.byref(bad), p, 8, lst, 4, q, None) == 1

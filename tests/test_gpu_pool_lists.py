@@ -1,0 +1,150 @@
+"""Block-list forms over a node pool (the multi-level solver's storage): the
+right-hand side with the jump-flux save, the buffer swap and copy and the
+CFL reduction over listed blocks give the same bits as the contiguous forms on
+the same blocks gathered, skip entries outside the pool and leave every
+unlisted block untouched."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from alpaca_amd import cases, gpu
+
+pytestmark = pytest.mark.gpu
+
+CAP = 12
+LIST = [7, 2, 9, 0, 11]
+SKIPPED = [-1, CAP]  # outside the pool: skipped by the kernels
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available()
+    gpu.load_library()
+    return _t
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _pool(torch, dim, seed):
+    host = np.stack([cases.random_block(dim, seed + s, smooth=(s % 3 == 1)) for s in range(CAP)])
+    return torch.as_tensor(host, device="cuda").contiguous()
+
+
+def _surfaces(torch, dim, seed):
+    rng = np.random.default_rng(seed)
+    shape = (CAP, 6, dim + 2, 16, 16 if dim == 3 else 1)
+    return torch.as_tensor(rng.standard_normal(shape), device="cuda").contiguous()
+
+
+def _same_bits(a, b):
+    return np.array_equal(a.cpu().numpy().view(np.int64), b.cpu().numpy().view(np.int64))
+
+
+@pytest.mark.parametrize("dim,flux", [(2, gpu.FLUX_HLLC), (3, gpu.FLUX_HLLC), (3, gpu.FLUX_ROE)])
+def test_rhs_list_equals_contiguous_form(torch, dim, flux):
+    print(f"[pool rhs list] dim={dim} flux={flux}", flush=True)
+    lib = gpu.load_library()
+    ctx = ctypes.c_void_p()
+    gpu.check(lib.alpaca_context_create(gpu._stream_ptr(), ctypes.byref(ctx)), "context_create")
+    try:
+        avg = _pool(torch, dim, 40 + dim)
+        rhs0 = _pool(torch, dim, 90 + dim)  # stale contents: overwritten only where listed
+        jf0 = _surfaces(torch, dim, 7)
+        rhs, jf = rhs0.clone(), jf0.clone()
+        entries = LIST[:2] + SKIPPED[:1] + LIST[2:] + SKIPPED[1:]
+        lst = torch.tensor(entries, dtype=torch.int32, device="cuda")
+        desc = gpu.make_desc(dim, [len(entries), 1, 1], [0, 0, 0], flux=flux, cell_size=1.0 / 64)
+        gpu.check(lib.alpaca_ctx_right_hand_side_jump_fluxes_list(
+            ctx, ctypes.byref(desc), CAP, _p(lst), len(entries), _p(avg), _p(rhs), _p(jf)),
+            "rhs list")
+        idx = torch.tensor(LIST, dtype=torch.int64, device="cuda")
+        a = avg.index_select(0, idx).contiguous()
+        r = torch.empty_like(a)
+        j = jf0.index_select(0, idx).contiguous()
+        d2 = gpu.make_desc(dim, [len(LIST), 1, 1], [0, 0, 0], flux=flux, cell_size=1.0 / 64)
+        gpu.check(lib.alpaca_ctx_right_hand_side_jump_fluxes(
+            ctx, ctypes.byref(d2), len(LIST), _p(a), _p(r), _p(j)), "rhs contiguous")
+        torch.cuda.synchronize()
+        assert _same_bits(rhs.index_select(0, idx), r)
+        assert _same_bits(jf.index_select(0, idx), j)
+        rest = torch.tensor([b for b in range(CAP) if b not in LIST], dtype=torch.int64,
+                            device="cuda")
+        assert _same_bits(rhs.index_select(0, rest), rhs0.index_select(0, rest))
+        assert _same_bits(jf.index_select(0, rest), jf0.index_select(0, rest))
+        assert torch.equal(avg, _pool(torch, dim, 40 + dim))
+    finally:
+        gpu.check(lib.alpaca_context_destroy(ctx), "context_destroy")
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_swap_list_exchanges_listed_blocks_only(torch, dim):
+    print(f"[pool swap list] dim={dim}", flush=True)
+    lib = gpu.load_library()
+    a0, b0 = _pool(torch, dim, 3), _pool(torch, dim, 300)
+    a, b = a0.clone(), b0.clone()
+    entries = SKIPPED[:1] + LIST + SKIPPED[1:]
+    lst = torch.tensor(entries, dtype=torch.int32, device="cuda")
+    gpu.check(lib.alpaca_swap_blocks_list(dim, _p(a), _p(b), CAP, _p(lst), len(entries),
+                                          gpu._stream_ptr()), "swap list")
+    torch.cuda.synchronize()
+    for k in range(CAP):
+        if k in LIST:
+            assert torch.equal(a[k], b0[k]) and torch.equal(b[k], a0[k])
+        else:
+            assert torch.equal(a[k], a0[k]) and torch.equal(b[k], b0[k])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_copy_list_copies_listed_blocks_only(torch, dim):
+    print(f"[pool copy list] dim={dim}", flush=True)
+    lib = gpu.load_library()
+    a0, b0 = _pool(torch, dim, 7), _pool(torch, dim, 700)
+    a, b = a0.clone(), b0.clone()
+    entries = LIST + SKIPPED
+    lst = torch.tensor(entries, dtype=torch.int32, device="cuda")
+    gpu.check(lib.alpaca_copy_blocks_list(dim, _p(a), _p(b), CAP, _p(lst), len(entries),
+                                          gpu._stream_ptr()), "copy list")
+    torch.cuda.synchronize()
+    assert torch.equal(a, a0)
+    for k in range(CAP):
+        assert torch.equal(b[k], a0[k] if k in LIST else b0[k])
+
+
+def _dt(torch, lib, call):
+    state = torch.zeros(ctypes.sizeof(gpu.CaseState), dtype=torch.uint8, device="cuda")
+    st = gpu._stream_ptr()
+    gpu.check(lib.alpaca_case_state_init(_p(state), 0.0, 1.0, 0.6, st), "state_init")
+    gpu.check(call(_p(state), st), "timestep")
+    torch.cuda.synchronize()
+    rec = gpu.CaseState.from_buffer_copy(state.cpu().numpy().tobytes())
+    return rec.dt
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_timestep_list_equals_contiguous_form(torch, dim):
+    print(f"[pool timestep list] dim={dim}", flush=True)
+    lib = gpu.load_library()
+    avg = _pool(torch, dim, 500)
+    # one listed block with a much hotter state (energy x100: sound speed about
+    # x10), so that it decides dt
+    fast = LIST[3]
+    avg[fast, 1] *= 100.0
+    entries = LIST + SKIPPED
+    lst = torch.tensor(entries, dtype=torch.int32, device="cuda")
+    desc = gpu.make_desc(dim, [len(entries), 1, 1], [0, 0, 0], cell_size=1.0 / 64)
+    dt_list = _dt(torch, lib, lambda sp, st: lib.alpaca_timestep_list(
+        ctypes.byref(desc), _p(avg), CAP, _p(lst), len(entries), sp, st))
+    a = avg.index_select(0, torch.tensor(LIST, dtype=torch.int64, device="cuda")).contiguous()
+    d2 = gpu.make_desc(dim, [len(LIST), 1, 1], [0, 0, 0], cell_size=1.0 / 64)
+    dt_cont = _dt(torch, lib, lambda sp, st: lib.alpaca_timestep(ctypes.byref(d2), _p(a), sp, st))
+    assert dt_list == dt_cont
+    d_slow = gpu.make_desc(dim, [len(LIST) - 1, 1, 1], [0, 0, 0], cell_size=1.0 / 64)
+    slow = [b for b in LIST if b != fast]
+    a_slow = avg.index_select(0, torch.tensor(slow, dtype=torch.int64, device="cuda")).contiguous()
+    dt_slow = _dt(torch, lib, lambda sp, st: lib.alpaca_timestep(ctypes.byref(d_slow),
+                                                                 _p(a_slow), sp, st))
+    assert dt_list < dt_slow  # the fast listed block was seen
