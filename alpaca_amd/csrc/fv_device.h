@@ -33,6 +33,17 @@ constexpr int kTC = 24;  // cells per side with halo
 
 enum : int { kMass = 0, kEnergy = 1, kMomX = 2 };
 
+namespace {
+// Per-axis direction of each BoundaryLocation, in the enum's order
+// (boundary_specifications.h:56-86): +1 East/North/Top, -1 West/South/Bottom,
+// 0 along an axis the location does not name.
+__constant__ signed char kLocationDir[26][3] = {
+    {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},    {0, 0, -1},  {0, 1, -1},
+    {0, -1, -1}, {0, 1, 1},   {0, -1, 1},  {1, 0, -1},  {-1, 0, -1},  {1, 0, 1},   {-1, 0, 1},
+    {1, 1, 0},   {-1, 1, 0},  {1, -1, 0},  {-1, -1, 0}, {1, 1, 1},    {1, 1, -1},  {1, -1, 1},
+    {1, -1, -1}, {-1, 1, 1},  {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+}  // namespace
+
 template <int DIM> struct Dims {
   static constexpr int NE = DIM + 2;             // equations
   static constexpr int TZ = DIM == 3 ? kTC : 1;  // halo cells along z

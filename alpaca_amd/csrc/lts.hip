@@ -16,6 +16,8 @@
 // operation order; the translation unit is built without contraction.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fv_device.h"
 
 namespace alpaca {
@@ -59,6 +61,52 @@ __global__ __launch_bounds__(kThreads) void integrate_kernel(const double *u_old
     int const k = box.start[2] + r % box.count[2];
     size_t const c = (size_t)b * D::BLOCK + (size_t)e * D::FIELD + i * D::SX + j * D::SY + k;
     u_new[c] = u_old[c] + timestep * u_new[c];
+  }
+}
+
+// IntegrateJumpHalos (time_integrator.h:281-301) for every (node, location)
+// pair of a level at once: entries[i] = {pool block, BoundaryLocation}, and
+// the location's halo slab (GetStartIndicesHaloRecv / GetHaloSize,
+// communication_types.h:76-132,190-197: per axis [IC+HS, TC), [0, HS) or
+// [HS, HS+IC) for direction +1, -1, 0) takes u_new = u_old + timestep * u_new.
+// The slabs of distinct locations are disjoint, so the launch order of the
+// reference's per-location calls does not matter.  One workgroup per entry
+// (grid-stride); entries with a block outside the pool, a location outside
+// the enum or, in 2D, a Top/Bottom part are skipped.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void jump_halo_integrate_kernel(
+    const double *u_old, double *u_new, int nblocks, const int *__restrict__ entries, int n,
+    double timestep) {
+  using D = Dims<DIM>;
+  for (int q = blockIdx.x; q < n; q += gridDim.x) {
+    int const b = entries[2 * q];
+    int const loc = entries[2 * q + 1];
+    if (!block_ok(b, nblocks) || loc < 0 || loc >= 26) continue;  // uniform over the group
+    Box box;
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      int const d = kLocationDir[loc][a];
+      if (a >= DIM) {
+        ok = ok && d == 0;
+        box.start[a] = 0;
+        box.count[a] = 1;
+      } else {
+        box.start[a] = d > 0 ? kIC + kHS : (d < 0 ? 0 : kHS);
+        box.count[a] = d != 0 ? kHS : kIC;
+      }
+    }
+    if (!ok) continue;
+    int const cells = box.count[0] * box.count[1] * box.count[2];
+    for (int t = threadIdx.x; t < D::NE * cells; t += kThreads) {
+      int const r = t % cells;
+      int const e = t / cells;
+      int const i = box.start[0] + r / (box.count[1] * box.count[2]);
+      int const j = box.start[1] + (r / box.count[2]) % box.count[1];
+      int const k = box.start[2] + r % box.count[2];
+      size_t const c = (size_t)b * D::BLOCK + (size_t)e * D::FIELD + i * D::SX + j * D::SY + k;
+      u_new[c] = u_old[c] + timestep * u_new[c];
+    }
   }
 }
 
@@ -259,6 +307,22 @@ int alpaca_lts_integrate(int dim, const double *d_old, double *d_new, int nblock
       hipLaunchKernelGGL((lts::integrate_kernel<3, true>), grid, block, 0, s, d_old, d_new, nblocks,
                          d_list, n, box, timestep);
   }
+  return lts::hip_status(hipGetLastError());
+}
+
+int alpaca_lts_integrate_jump_halos(int dim, const double *d_old, double *d_new, int nblocks,
+                                    const int *d_entries, int n, double timestep, void *stream) {
+  if (lts::bad_sizes(dim, nblocks, n) || !d_old || !d_new || (n > 0 && !d_entries))
+    return ALPACA_ERR_INVALID;
+  if (n == 0) return ALPACA_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  dim3 const grid((unsigned)std::min(n, 65536));
+  if (dim == 2)
+    hipLaunchKernelGGL((lts::jump_halo_integrate_kernel<2>), grid, dim3(lts::kThreads), 0, s, d_old,
+                       d_new, nblocks, d_entries, n, timestep);
+  else
+    hipLaunchKernelGGL((lts::jump_halo_integrate_kernel<3>), grid, dim3(lts::kThreads), 0, s, d_old,
+                       d_new, nblocks, d_entries, n, timestep);
   return lts::hip_status(hipGetLastError());
 }
 

@@ -356,15 +356,6 @@ __global__ __launch_bounds__(kStencilThreads) __attribute__((amdgpu_waves_per_eu
   predict_range<DIM>(s, blocks, l, e, r);
 }
 
-// Per-axis direction of each BoundaryLocation, in the enum's order
-// (boundary_specifications.h:56-86): +1 East/North/Top, -1 West/South/Bottom,
-// 0 along an axis the location does not name.
-__constant__ signed char kLocationDir[26][3] = {
-    {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},    {0, 0, -1},  {0, 1, -1},
-    {0, -1, -1}, {0, 1, 1},   {0, -1, 1},  {1, 0, -1},  {-1, 0, -1},  {1, 0, 1},   {-1, 0, 1},
-    {1, 1, 0},   {-1, 1, 0},  {1, -1, 0},  {-1, -1, 0}, {1, 1, 1},    {1, 1, -1},  {1, -1, 1},
-    {1, -1, -1}, {-1, 1, 1},  {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
-
 // The halo slab of a location (CommunicationTypes::GetStartIndicesHaloRecv and
 // GetHaloSize, communication_types.h:76-132,190-197): per axis, direction +1
 // gives the cells [IC+HS, TC), -1 gives [0, HS), 0 the internal cells

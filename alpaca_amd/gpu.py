@@ -135,6 +135,9 @@ _SIGNATURES = {
     "alpaca_lts_integrate": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                               ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                               ctypes.c_double, ctypes.c_void_p], ctypes.c_int),
+    "alpaca_lts_integrate_jump_halos": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_void_p], ctypes.c_int),
     "alpaca_lts_integrate_jump_conservatives": ([ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                  ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                                  ctypes.c_double, ctypes.c_void_p], ctypes.c_int),

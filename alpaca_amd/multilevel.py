@@ -367,9 +367,8 @@ class MultiLevelSolver:
                 return by_loc
 
             by_loc = self._cached("jump_halos", level, build)
-            ts = self._lts_timestep(stage, n)
-            for loc, slots in by_loc.items():
-                self.be.integrate_range(slots, mr.halo_recv_range(self.dim, loc), ts)
+            if by_loc:
+                self.be.integrate_jump_halos(by_loc, self._lts_timestep(stage, n))
 
     def swap(self, levels):
         slots = [s for lv in levels for s in self._cached("slots", lv,
@@ -758,13 +757,14 @@ class GpuBackend:
                                                 self._p(lst), len(slots), None, timestep,
                                                 self.st), "lts_integrate")
 
-    def integrate_range(self, slots, rng, timestep):
-        lst = self._list(slots)
-        r = list(rng) + [0, 1] * (3 - len(rng) // 2)
-        cr = (ctypes.c_int * 6)(*r)
-        gpu.check(self.lib.alpaca_lts_integrate(self.dim, self._p(self.avg), self._p(self.rhs),
-                                                self.capacity, self._p(lst), len(slots), cr,
-                                                timestep, self.st), "lts_integrate halo")
+    def integrate_jump_halos(self, by_loc, timestep):
+        """IntegrateJumpHalos for every (node, location) of a level at once;
+        by_loc maps a BoundaryLocation to the pool blocks with a jump there."""
+        ent = self._device(by_loc, "entries", lambda o: np.array(
+            [(s, loc) for loc, slots in o.items() for s in slots], dtype=np.int32).reshape(-1, 2))
+        gpu.check(self.lib.alpaca_lts_integrate_jump_halos(
+            self.dim, self._p(self.avg), self._p(self.rhs), self.capacity, self._p(ent),
+            ent.shape[0], timestep, self.st), "lts_integrate_jump_halos")
 
     def average(self, links):
         self.mrop.average(self.rhs, self.dim, self._links(links), self.st)

@@ -514,6 +514,18 @@ int alpaca_lts_integrate(int dim, const double *d_old, double *d_new, int nblock
                          const int *d_list, int n, const int *range, double timestep,
                          void *stream);
 
+/* IntegrateJumpHalos (time_integrator.h:281-301) over the jump halos of a
+ * level, the loop over (node, location) pairs of Integrate
+ * (modular_algorithm_assembler.cpp:1084-1096), in one call: d_entries
+ * (device, int32[n][2]) = {pool block, BoundaryLocation (East = 0 ...
+ * WestSouthBottom = 25)}; the location's halo slab (GetStartIndicesHaloRecv /
+ * GetHaloSize, communication_types.h:76-132,190-197) takes d_new = d_old +
+ * timestep * d_new.  The slabs of different locations are disjoint.  Entries
+ * with a block outside [0, nblocks), a location outside the enum or, in 2D,
+ * one with a Top/Bottom part are skipped. */
+int alpaca_lts_integrate_jump_halos(int dim, const double *d_old, double *d_new, int nblocks,
+                                    const int *d_entries, int n, double timestep, void *stream);
+
 /* IntegrateJumpConservatives (time_integrator.h:64-84) on jump surfaces in
  * SurfaceBuffer layout ([nblocks][6][NE][16][16|1]): d_cons += timestep *
  * d_flux, then d_flux = 0, on the faces the dimension has (4 in 2D, 6 in

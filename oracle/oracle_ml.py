@@ -144,6 +144,16 @@ class OracleBackend:
         for s in slots:
             self.rhs[s] = oracle_py.lts_integrate(self.dim, self.avg[s], self.rhs[s], r, timestep)
 
+    def integrate_jump_halos(self, by_loc, timestep):
+        """IntegrateJumpHalos (time_integrator.h:281-301) per location of the
+        level's jumps (modular_algorithm_assembler.cpp:1084-1096), the halo
+        slab from GetStartIndicesHaloRecv / GetHaloSize."""
+        for loc, slots in by_loc.items():
+            rng = []
+            for sl in _recv(self.dim, _direction(loc))[:self.dim]:
+                rng += [sl.start, sl.stop - sl.start]
+            self.integrate_range(slots, rng, timestep)
+
     def average(self, links):
         for child, parent, pos, _ in links:
             self.rhs[parent] = oracle_py.mr_average(self.dim, self.rhs[child], self.rhs[parent], pos)

@@ -153,8 +153,9 @@ def test_noalloc_and_exact_ops_refuse_before_any_device_call():
 
 def test_pool_list_entries_refuse_before_any_device_call():
     """The block-list forms over a node pool (right-hand side with the jump
-    fluxes, buffer swap and copy, CFL reduction) refuse bad arguments with
-    ALPACA_ERR_INVALID, and an empty list is done without a device call."""
+    fluxes, buffer swap and copy, CFL reduction, jump-halo integration)
+    refuse bad arguments with ALPACA_ERR_INVALID, and an empty list is done
+    without a device call."""
     lib = gpu.load_library()
     p, q, r, lst = (ctypes.c_void_p(v) for v in (256, 512, 768, 1024))
     ctx = ctypes.c_void_p(2048)  # never dereferenced on these paths
@@ -180,6 +181,12 @@ def test_pool_list_entries_refuse_before_any_device_call():
     assert copy(3, None, q, 8, lst, 4, None) == 1
     assert copy(3, p, q, 8, lst, -1, None) == 1
     assert copy(3, p, q, 8, None, 0, None) == 0
+    jh = lib.alpaca_lts_integrate_jump_halos
+    assert jh(4, p, q, 8, lst, 4, 1e-3, None) == 1
+    assert jh(3, None, q, 8, lst, 4, 1e-3, None) == 1
+    assert jh(3, p, q, 0, lst, 4, 1e-3, None) == 1
+    assert jh(3, p, q, 8, None, 4, 1e-3, None) == 1
+    assert jh(3, p, q, 8, None, 0, 1e-3, None) == 0
     ts = lib.alpaca_timestep_list
     assert ts(ctypes.byref(d3), p, 8, None, 4, q, None) == 1
     assert ts(ctypes.byref(d3), p, 8, lst, 0, q, None) == 1

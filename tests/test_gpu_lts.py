@@ -160,3 +160,34 @@ def test_jump_flux_adjustment_and_reset_bitexact(torch, dim):
         assert np.all(after[b, :2 * dim] == 0.0)
         assert np.array_equal(after[b, 2 * dim:], got_j[b, 2 * dim:])
     assert np.array_equal(after[1], got_j[1])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_jump_halos_of_a_level_in_one_call(torch, dim):
+    """alpaca_lts_integrate_jump_halos over (block, location) entries equals
+    the oracle's per-location halo integration (IntegrateJumpHalos with
+    GetStartIndicesHaloRecv / GetHaloSize), bit for bit; entries with a block
+    outside the pool or (2D) a Top/Bottom location are skipped."""
+    import ctypes
+    from alpaca_amd import multiresolution as mr
+    print(f"[lts jump halos] dim={dim}", flush=True)
+    nb = 6
+    host, dev = _pools(torch, dim, nb, 77 + dim)
+    locs = [loc for loc in range(26) if dim == 3 or mr.location_direction(loc)[2] == 0]
+    rng = np.random.default_rng(dim)
+    entries = [(int(b), loc) for b in (4, 0, 5) for loc in rng.permutation(locs)[:7].tolist()]
+    entries += [(nb + 1, locs[0]), (2, 26)] + ([(3, 4)] if dim == 2 else [])
+    ent = torch.as_tensor(np.array(entries, dtype=np.int32), device="cuda")
+    ts = 3.25e-4
+    lib = gpu.load_library()
+    gpu.check(lib.alpaca_lts_integrate_jump_halos(
+        dim, ctypes.c_void_p(dev["avg"].data_ptr()), ctypes.c_void_p(dev["rhs"].data_ptr()), nb,
+        ctypes.c_void_p(ent.data_ptr()), len(entries), ts, gpu._stream_ptr()), "jump halos")
+    torch.cuda.synchronize()
+    got = dev["rhs"].cpu().numpy()
+    want = host["rhs"].copy()
+    for b, loc in entries[:21]:
+        r = mr.halo_recv_range(dim, loc)
+        want[b] = O.lts_integrate(dim, host["avg"][b], want[b], tuple(r) + (0, 1) * (3 - dim), ts)
+    assert np.array_equal(got.view(np.int64), want.view(np.int64))
+    assert torch.equal(dev["avg"].cpu(), torch.as_tensor(host["avg"]))
