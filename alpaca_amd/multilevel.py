@@ -216,6 +216,24 @@ class MultiLevelSolver:
     def _link(self, key):
         return (self.slot[key], self.slot[self.topo.parent(key)], self.topo.position(key), key[0])
 
+    def _level_links(self, level):
+        """The links (child slot, parent slot, position among siblings,
+        level) of every node of a level above 0, in the order of _ids, as an
+        int32 [n][4] array (from the dense level grids)."""
+        def build():
+            keys = self._ids(level)
+            if level == 0 or not keys:
+                return np.zeros((0, 4), dtype=np.int32)
+            k = np.asarray(keys, dtype=np.int64)
+            slots, _ = self._grid(level)
+            pslots, _ = self._grid(level - 1)
+            child = slots[k[:, 1], k[:, 2], k[:, 3]]
+            parent = pslots[k[:, 1] >> 1, k[:, 2] >> 1, k[:, 3] >> 1]
+            pos = (k[:, 1] & 1) | ((k[:, 2] & 1) << 1) | ((k[:, 3] & 1) << 2)
+            return np.ascontiguousarray(
+                np.stack([child, parent, pos, k[:, 0]], axis=1).astype(np.int32))
+        return self._cached("level_links", level, build)
+
     def _lts_timestep(self, stage, n, jump=False):
         return self.be.lts_timestep(self.micro, n, stage, jump)
 
@@ -295,7 +313,7 @@ class MultiLevelSolver:
             table = self._neighbor_table(keys)
             cols = [offset_slot(self.dim, self._loc_dir[loc]) for loc in self.halo_locs]
             rows, which = np.nonzero(table[:, cols] == -2)
-            jump_links = [self._link(keys[r]) for r in rows.tolist()]
+            jump_links = self._level_links(level)[rows] if rows.size else []
             jump_locs = [self.halo_locs[c] for c in which.tolist()]
             slots = self._slots(keys)
             # External-slab cells whose source lies in a jump region (written
@@ -327,9 +345,8 @@ class MultiLevelSolver:
         for level in child_levels_descending:
             if level == 0:
                 continue
-            links = self._cached("links", level,
-                                 lambda: [self._link(k) for k in self._ids(level)])
-            if links:
+            links = self._level_links(level)
+            if len(links):
                 self.be.average(links)
 
     # ---------------------------------------------------------- stage work
@@ -380,9 +397,8 @@ class MultiLevelSolver:
         for level in finished_levels_descending:
             if level == 0:
                 continue
-            links = self._cached("links", level,
-                                 lambda: [self._link(k) for k in self._ids(level)])
-            if links:
+            links = self._level_links(level)
+            if len(links):
                 self.be.average_jump(links)
         for level in finished_levels_descending[1:]:
             leaves = self._leaves(level)
@@ -408,28 +424,41 @@ class MultiLevelSolver:
 
     # ------------------------------------------------------------ remeshing
     def determine_remeshing_nodes(self, parent_levels):
+        """The children of every node of the parent levels, per parent (in
+        key order) and position: leaf children get a remeshing decision (one
+        batched call), the others count as neutral; children to refine, and
+        the whole family of a parent whose children are all to be coarsened
+        (DetermineRemeshingNodes, modular_algorithm_assembler.cpp:2426-2532)."""
         remove, refine = [], []
-        groups = []  # (children, [decision or None])
+        fams = []  # (child keys, leaf flags) per parent level, in family order
         links = []
         for lp in parent_levels:
-            for pkey in self.topo.ids_on_level(lp):
-                kids = [c for c in self.topo.children(pkey) if self.topo.exists(c)]
-                if not kids:
-                    continue
-                idx = []
-                for c in kids:
-                    if self.topo.is_leaf(c):
-                        idx.append(len(links))
-                        links.append(self._link(c))
-                    else:
-                        idx.append(None)
-                groups.append((kids, idx))
-        decisions = self.be.decisions(links, self.thresholds) if links else []
-        for kids, idx in groups:
-            dec = [mr.NEUTRAL if i is None else int(decisions[i]) for i in idx]
-            refine += [c for c, d in zip(kids, dec) if d == mr.REFINE]
-            if all(d == mr.COARSE for d in dec):
-                remove += kids
+            lk = self._level_links(lp + 1)
+            if not len(lk):
+                continue
+            k = np.asarray(self._ids(lp + 1), dtype=np.int64)
+            order = np.lexsort((lk[:, 2], k[:, 3] >> 1, k[:, 2] >> 1, k[:, 1] >> 1))
+            k = k[order]
+            _, leafgrid = self._grid(lp + 1)
+            leaf = leafgrid[k[:, 1], k[:, 2], k[:, 3]]
+            fams.append((k, leaf))
+            links.append(lk[order][leaf])
+        links = np.concatenate(links) if links else np.zeros((0, 4), dtype=np.int32)
+        decisions = np.asarray(self.be.decisions(links, self.thresholds)) if len(links) else []
+        off = 0
+        for k, leaf in fams:
+            dec = np.full(len(k), mr.NEUTRAL, dtype=np.int64)
+            n = int(leaf.sum())
+            dec[leaf] = decisions[off:off + n]
+            off += n
+            refine += [tuple(x) for x in k[dec == mr.REFINE].tolist()]
+            parent = k[:, 1:] >> 1
+            start = np.ones(len(k), dtype=bool)
+            start[1:] = np.any(parent[1:] != parent[:-1], axis=1)
+            starts = np.nonzero(start)[0]
+            all_coarse = np.logical_and.reduceat(dec == mr.COARSE, starts)
+            fam = np.cumsum(start) - 1
+            remove += [tuple(x) for x in k[all_coarse[fam]].tolist()]
         return remove, refine
 
     def _coarsen(self, removed):
@@ -782,7 +811,7 @@ class GpuBackend:
         gpu.check(self.lib.alpaca_fill_halos_list(self.dim, self._p(self.rhs), self.capacity,
                                                   self._p(lst), len(slots), self._p(tab), bc, 0,
                                                   st), "fill_halos_list")
-        if jump_links:
+        if len(jump_links):
             locs = self._device(jump_locs, "i32", lambda o: np.asarray(o, dtype=np.int32))
             self.mrop.jump_halo_update(self.rhs, self.dim, self._links(jump_links), locs, self.st)
         if second is not None:

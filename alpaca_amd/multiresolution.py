@@ -102,7 +102,8 @@ def halo_recv_range(dim: int, location: int) -> tuple:
 
 def make_links(rows) -> np.ndarray:
     """int32 [n][4] links from (child, parent, position, level) rows."""
-    a = np.asarray(list(rows), dtype=np.int32).reshape(-1, 4)
+    a = np.asarray(rows if isinstance(rows, np.ndarray) else list(rows),
+                   dtype=np.int32).reshape(-1, 4)
     return np.ascontiguousarray(a)
 
 

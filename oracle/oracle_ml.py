@@ -35,6 +35,12 @@ def _direction(loc):
                  for hi, lo in (("East", "West"), ("North", "South"), ("Top", "Bottom")))
 
 
+def _rows(links):
+    """Link rows (child, parent, position, level) as Python ints, from a list
+    of tuples or an int array."""
+    return [tuple(int(v) for v in r) for r in links]
+
+
 def _slot(dim, d):
     return ((d[0] + 1) * 3 + (d[1] + 1)) * 3 + (d[2] + 1) if dim == 3 else (d[0] + 1) * 3 + (d[1] + 1)
 
@@ -155,11 +161,11 @@ class OracleBackend:
             self.integrate_range(slots, rng, timestep)
 
     def average(self, links):
-        for child, parent, pos, _ in links:
+        for child, parent, pos, _ in _rows(links):
             self.rhs[parent] = oracle_py.mr_average(self.dim, self.rhs[child], self.rhs[parent], pos)
 
     def average_jump(self, links):
-        for child, parent, pos, _ in links:
+        for child, parent, pos, _ in _rows(links):
             self.jcons[parent] = oracle_py.mr_average_jump(self.dim, self.jcons[child],
                                                            self.jcons[parent], pos)
 
@@ -183,7 +189,7 @@ class OracleBackend:
                     self.rhs[s][(slice(None),) + _recv(dim, d)] = \
                         self.rhs[nb][(slice(None),) + _send(dim, d)]
         # level jumps: prediction from the parent over the halo range
-        for (child, parent, pos, _), loc in zip(jump_links, jump_locs):
+        for (child, parent, pos, _), loc in zip(_rows(jump_links), jump_locs):
             d = _direction(loc)
             rng = []
             for a in range(3):
@@ -237,12 +243,12 @@ class OracleBackend:
 
     def predict(self, links, rng):
         r = list(rng) + [0, 1] * (3 - len(rng) // 2)
-        for child, parent, pos, _ in links:
+        for child, parent, pos, _ in _rows(links):
             self._predict_range(child, parent, pos, r)
 
     def decisions(self, links, thresholds):
         out = []
-        for child, parent, pos, level in links:
+        for child, parent, pos, level in _rows(links):
             det = oracle_py.mr_detail_linf(self.dim, self.rhs[parent], self.rhs[child], pos)
             out.append(oracle_py.mr_decision(det, thresholds[level]))
         return np.array(out, dtype=np.int32)
