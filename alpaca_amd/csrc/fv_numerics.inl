@@ -793,9 +793,16 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
 #pragma unroll
   for (int e = 0; e < NE; ++e) f[e] = R(0.0);
   if constexpr (FLUX == 0) {
+    // The stencil through a buffer resource on the block: one 32-bit byte
+    // offset per cell, the field as a scalar offset.
+    BlockReader<DIM> const rd(blk);
     double Ud[6][NE];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) load(m, Ud[m]);
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+      for (int e = 0; e < NE; ++e)
+        Ud[m][e] = rd((off + (m - 2) * axis_stride<DIM, DIR>()) * (int)sizeof(double),
+                      e * Dims<DIM>::FIELD * (int)sizeof(double));
     R U[6][NE];
 #pragma unroll
     for (int m = 0; m < 6; ++m)
