@@ -34,14 +34,17 @@ constexpr int kThreads = 256;         // averaging
 constexpr int kStencilThreads = 192;  // prediction and details: 1728 = 9 x 192 parent cells
 constexpr int kMaxLevels = 32;
 
-// LDS layout of a parent stencil window: [16][16][16] doubles in 3D, [16][16]
-// in 2D, whatever part of it a range fills, so that every stencil offset is a
-// compile-time constant.
-template <int DIM> struct Lds {
-  static constexpr int SX = DIM == 3 ? 256 : 16;
-  static constexpr int SY = DIM == 3 ? 16 : 1;
-  static constexpr int SIZE = 16 * SX;
+// LDS layout of a parent stencil window: [W][W][W] doubles in 3D, [W][W] in
+// 2D, whatever part of it a range fills, so that every stencil offset is a
+// compile-time constant.  W = 16 holds any window; W = 12 the windows of
+// ranges of at most 8 predicted parent cells per axis (jump halo slabs, the
+// internal cells).
+template <int DIM, int W> struct LdsW {
+  static constexpr int SX = DIM == 3 ? W * W : W;
+  static constexpr int SY = DIM == 3 ? W : 1;
+  static constexpr int SIZE = W * SX;
 };
+template <int DIM> using Lds = LdsW<DIM, 16>;
 
 // ConsistencyManagedSum, three and four values (mathematical_functions.h:79-91,
 // 115-124): 0.5 (min + max) of the three pairings s1, s2, s3, where std::min
@@ -199,10 +202,10 @@ struct Range {
 // window (Lds layout): out[dx | dy << 1 | dz << 2].  In 2D the
 // z terms are the reference's zero-initialised Qz, Qxz, Qyz, Qxyz, and enter
 // the sums as such (their signs matter for signed zeros).
-template <int DIM>
+template <int DIM, typename L = Lds<DIM>>
 __device__ __forceinline__ void predict_cell(const double *__restrict__ s, int o,
                                              double out[1 << DIM]) {
-  constexpr int sx = Lds<DIM>::SX, sy = Lds<DIM>::SY;
+  constexpr int sx = L::SX, sy = L::SY;
   constexpr double c0 = -22.0 / 128.0, c1 = 3.0 / 128.0;
   constexpr double c00 = c0 * c0, c01 = c0 * c1, c10 = c01, c11 = c1 * c1;
   constexpr double c000 = c00 * c0, c001 = c00 * c1, c010 = c001, c011 = c0 * c11;
@@ -264,8 +267,8 @@ template <int DIM> struct Window {
   }
   // LDS index of predicted parent cell (a, b, c) (counted from the range's
   // first predicted cell).
-  __device__ static int centre(int a, int b, int c) {
-    return (a + 2) * Lds<DIM>::SX + (b + 2) * Lds<DIM>::SY + (DIM == 3 ? c + 2 : 0);
+  template <typename L = Lds<DIM>> __device__ static int centre(int a, int b, int c) {
+    return (a + 2) * L::SX + (b + 2) * L::SY + (DIM == 3 ? c + 2 : 0);
   }
 };
 
@@ -338,6 +341,88 @@ __device__ __forceinline__ void predict_range(double *s, double *__restrict__ bl
   }
 }
 
+// Predicts range r (at most 8 predicted parent cells per axis) of every
+// equation of child l.child from its parent, in one workgroup: while the
+// children of equation e are predicted from one LDS window, the parent
+// window of equation e + 1 is loaded into registers and then written to the
+// other window, so the loads overlap the stencil arithmetic.  One barrier
+// per equation; the caller's early exits are uniform over the workgroup.
+template <int DIM> using SmallLds = LdsW<DIM, 12>;
+
+template <int DIM>
+__device__ __forceinline__ void predict_range_equations(double *s, double *__restrict__ blocks,
+                                                        const alpaca_mr_link &l, const Range &r) {
+  using D = Dims<DIM>;
+  using L = SmallLds<DIM>;
+  constexpr int K = 12;  // window cells per axis at most
+  static_assert(K * K <= kStencilThreads, "one thread per window column");
+  Window<DIM> const win(r, l.position);
+  const double *parent = blocks + (size_t)l.parent * D::BLOCK;
+  double *child = blocks + (size_t)l.child * D::BLOCK;
+  // Thread (b, c) of the window's w1 x w2 plane moves the column a = 0..w0-1:
+  // field offset g + a SX, LDS offset q + a L::SX.
+  int const plane = win.w[1] * win.w[2];
+  bool const mover = int(threadIdx.x) < plane;
+  int const b = mover ? int(threadIdx.x) / win.w[2] : 0, c = mover ? int(threadIdx.x) % win.w[2] : 0;
+  int const g = win.p0[0] * D::SX + (win.p0[1] + b) * D::SY + (win.p0[2] + c);
+  int const q = b * L::SY + c;
+  int const w0 = win.w[0];
+  double pre[K];
+  auto load = [&](int e) {
+    if (!mover) return;
+    const double *src = parent + (size_t)e * D::FIELD + g;
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+      if (a < w0) pre[a] = src[a * D::SX];
+  };
+  auto put = [&](double *w) {
+    if (!mover) return;
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+      if (a < w0) w[q + a * L::SX] = pre[a];
+  };
+  load(0);
+  put(s);
+  __syncthreads();
+  int const nz = DIM == 3 ? r.count[2] : 1;
+  int const tasks = r.count[0] * r.count[1] * nz;
+#pragma unroll 1
+  for (int e = 0; e < D::NE; ++e) {
+    if (e + 1 < D::NE) load(e + 1);
+    const double *cur = s + (e & 1) * L::SIZE;
+    double *out = child + (size_t)e * D::FIELD;
+    for (int t = threadIdx.x; t < tasks; t += kStencilThreads) {
+      int const a = t / (r.count[1] * nz), b = (t / nz) % r.count[1], c = t % nz;
+      double v[1 << DIM];
+      predict_cell<DIM, L>(cur, Window<DIM>::template centre<L>(a, b, c), v);
+      int const ci = r.start[0] + 2 * a, cj = r.start[1] + 2 * b;
+      int const ck = DIM == 3 ? r.start[2] + 2 * c : 0;
+      store_children<DIM>(out + ci * D::SX + cj * D::SY + ck, v);
+    }
+    // The other window was last read by equation e - 1, before the previous
+    // barrier.
+    if (e + 1 < D::NE) put(s + ((e + 1) & 1) * L::SIZE);
+    __syncthreads();
+  }
+}
+
+// Ranges whose window fits the 12-cell layout: at most 8 predicted parent
+// cells per axis.
+__host__ __device__ inline bool small_window(const Range &r) {
+  return r.count[0] <= 8 && r.count[1] <= 8 && r.count[2] <= 8;
+}
+
+// prediction_kernel<DIM, false> over ranges with small windows: one
+// workgroup per link, every equation.
+template <int DIM>
+__global__ __launch_bounds__(kStencilThreads) void prediction_small_kernel(
+    double *__restrict__ blocks, int nblocks, const alpaca_mr_link *__restrict__ links, Range r) {
+  __shared__ double s[2 * SmallLds<DIM>::SIZE];
+  alpaca_mr_link const l = links[blockIdx.x];
+  if (!link_ok<DIM>(l, nblocks)) return;
+  predict_range_equations<DIM>(s, blocks, l, r);
+}
+
 template <int DIM, bool FULL>
 __global__ __launch_bounds__(kStencilThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void prediction_kernel(
     double *__restrict__ blocks, int nblocks, const alpaca_mr_link *__restrict__ links, Range r) {
@@ -380,19 +465,17 @@ template <int DIM> __device__ __forceinline__ bool location_range(int loc, Range
 
 // UpdateMaterialJumpNoMpi (internal_halo_manager.cpp:231-262) for a batch:
 // link i's child gets the halo slab of location locations[i] predicted from
-// its parent, one workgroup per (link, equation).
+// its parent (every equation), one workgroup per link.
 template <int DIM>
-__global__ __launch_bounds__(kStencilThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void jump_halo_kernel(
+__global__ __launch_bounds__(kStencilThreads) void jump_halo_kernel(
     double *__restrict__ blocks, int nblocks, const alpaca_mr_link *__restrict__ links,
     const int *__restrict__ locations) {
-  using D = Dims<DIM>;
-  __shared__ double s[Lds<DIM>::SIZE];
-  int const e = blockIdx.x % D::NE;
-  int const i = blockIdx.x / D::NE;
+  __shared__ double s[2 * SmallLds<DIM>::SIZE];
+  int const i = blockIdx.x;
   alpaca_mr_link const l = links[i];
   Range r;
   if (!link_ok<DIM>(l, nblocks) || !location_range<DIM>(locations[i], r)) return;
-  predict_range<DIM>(s, blocks, l, e, r);
+  predict_range_equations<DIM>(s, blocks, l, r);
 }
 
 // ------------------------------------------------------------------ details
@@ -540,7 +623,10 @@ template <int DIM> int prediction_impl(double *blocks, int nblocks, const alpaca
   bool const full = r.start[0] == 0 && r.start[1] == 0 && r.start[2] == 0 &&
                     r.count[0] == kTC / 2 && r.count[1] == kTC / 2 &&
                     r.count[2] == (DIM == 3 ? kTC / 2 : 1);
-  if (full)
+  if (!full && small_window(r))
+    hipLaunchKernelGGL((prediction_small_kernel<DIM>), dim3((unsigned)nlinks),
+                       dim3(kStencilThreads), 0, s, blocks, nblocks, links, r);
+  else if (full)
     hipLaunchKernelGGL((prediction_kernel<DIM, true>), grid, dim3(kStencilThreads), 0, s, blocks,
                        nblocks, links, r);
   else
@@ -551,7 +637,7 @@ template <int DIM> int prediction_impl(double *blocks, int nblocks, const alpaca
 
 template <int DIM> int jump_halo_impl(double *blocks, int nblocks, const alpaca_mr_link *links,
                                       const int *locations, int nlinks, hipStream_t s) {
-  dim3 grid((unsigned)nlinks * Dims<DIM>::NE);
+  dim3 grid((unsigned)nlinks);
   hipLaunchKernelGGL((jump_halo_kernel<DIM>), grid, dim3(kStencilThreads), 0, s, blocks, nblocks,
                      links, locations);
   return hip_status(hipGetLastError());
