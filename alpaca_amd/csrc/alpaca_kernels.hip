@@ -51,10 +51,9 @@ struct CaseArgs {
   double gamma, b, cell_size;
 };
 
-// Which block of device memory the i-th block of a per-block launch is: the
-// i-th block of a contiguous array (list == nullptr), or list[i] of a pool of
-// `capacity` blocks, an entry outside [0, capacity) being skipped (-1).  The
-// face-flux scratch stays indexed by i.
+// Which block of device memory the i-th block of a launch is: the i-th block
+// of a contiguous array (list == nullptr), or list[i] of a pool of `capacity`
+// blocks, an entry outside [0, capacity) being skipped (-1).
 struct BlockList {
   const int *list = nullptr;
   int capacity = 0;
@@ -93,7 +92,7 @@ template <int DIM, int DIR, int FLUX, int MODE>
 __global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__restrict__ in,
                                                              double *__restrict__ flux_out,
                                                              int nblocks, double gamma,
-                                                             double b, BlockList list) {
+                                                             double b) {
   using D = Dims<DIM>;
   constexpr int NE = D::NE;
   constexpr int NF = faces_per_dir<DIM>();
@@ -104,8 +103,6 @@ __global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__res
   int const blk = g / groups_per_block;
   int const f = (g % groups_per_block) * kThreads + threadIdx.x;
   if (blk >= nblocks || f >= NF) return;
-  int const mem = list.block(blk);
-  if (mem < 0) return;
   int const a2 = f % n2;
   int const a1 = (f / n2) % n1;
   int const a0 = f / (n2 * n1);
@@ -114,7 +111,7 @@ __global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__res
   int const j = a1 + (DIR == 1 ? kHS - 1 : kHS);
   int const k = DIM == 3 ? a2 + (DIR == 2 ? kHS - 1 : kHS) : 0;
   double fl[NE];
-  mode_stencil_flux<DIM, DIR, FLUX, MODE>(in + (size_t)mem * D::BLOCK, i, j, k, gamma, b, fl);
+  mode_stencil_flux<DIM, DIR, FLUX, MODE>(in + (size_t)blk * D::BLOCK, i, j, k, gamma, b, fl);
   double *o = flux_out + ((size_t)blk * DIM + DIR) * NE * NF + f;
 #pragma unroll
   for (int e = 0; e < NE; ++e) o[e * NF] = fl[e];
@@ -129,7 +126,7 @@ __global__ __launch_bounds__(kThreads) void face_flux_kernel(const double *__res
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void jump_flux_kernel(const double *__restrict__ flux,
                                                              double *__restrict__ jump,
-                                                             int nblocks, BlockList list) {
+                                                             int nblocks) {
   using D = Dims<DIM>;
   constexpr int NE = D::NE;
   constexpr int NF = faces_per_dir<DIM>();
@@ -138,8 +135,6 @@ __global__ __launch_bounds__(kThreads) void jump_flux_kernel(const double *__res
   size_t const t = (size_t)blockIdx.x * kThreads + threadIdx.x;
   if (t >= (size_t)nblocks * 6 * FACE) return;
   int const blk = int(t / (6 * FACE));
-  int const mem = list.block(blk);
-  if (mem < 0) return;
   int const loc = int((t / FACE) % 6);
   int const r = int(t % FACE);
   int const e = r / (kIC * ICZ), i = (r / ICZ) % kIC, j = r % ICZ;
@@ -157,7 +152,7 @@ __global__ __launch_bounds__(kThreads) void jump_flux_kernel(const double *__res
         v = fz[(i * kIC + j) * (kIC + 1) + (loc == 4 ? kIC : 0)];  // Top / Bottom
       }
   }
-  jump[(size_t)mem * 6 * FACE + t % (6 * FACE)] += v;
+  jump[t] += v;
 }
 
 // ------------------------------------------------------------- combine ---
@@ -177,7 +172,7 @@ __host__ __device__ inline StageCoef stage_coef(int stage) {
 template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) void combine_kernel(
     const double *__restrict__ flux, const double *__restrict__ avg, const double *init,
-    double *out, int nblocks, int stage, double inv_dx, alpaca_case_state *state, BlockList list) {
+    double *out, int nblocks, int stage, double inv_dx, alpaca_case_state *state) {
   using D = Dims<DIM>;
   constexpr int NE = D::NE;
   constexpr int NF = faces_per_dir<DIM>();
@@ -188,8 +183,6 @@ __global__ __launch_bounds__(kThreads) void combine_kernel(
   int const blk = g / groups_per_block;
   int const c = (g % groups_per_block) * kThreads + threadIdx.x;
   if (blk >= nblocks || c >= cells) return;
-  int const mem = list.block(blk);
-  if (mem < 0) return;
   int const kk = c % ICZ;
   int const jj = (c / ICZ) % kIC;
   int const ii = c / (ICZ * kIC);
@@ -203,7 +196,7 @@ __global__ __launch_bounds__(kThreads) void combine_kernel(
   int const y_hi = (ii * (kIC + 1) + jj + 1) * ICZ + kk;
   int const z_lo = (ii * kIC + jj) * (kIC + 1) + kk;     // fz [16][16][17]
   int const z_hi = z_lo + 1;
-  size_t const cell = (size_t)mem * D::BLOCK + (ii + kHS) * D::SX + (jj + kHS) * D::SY +
+  size_t const cell = (size_t)blk * D::BLOCK + (ii + kHS) * D::SX + (jj + kHS) * D::SY +
                       (DIM == 3 ? kk + kHS : 0);
   double dt = 0.0;
   StageCoef co{1.0, 0.0, 1.0};
@@ -520,7 +513,7 @@ __global__ __launch_bounds__(kThreads) void zero_kernel(double *p, size_t n) {
 
 // The listed blocks of a pool, 16 bytes per thread and step (grid-stride; a
 // haloed block is an even number of doubles, so every pair is 16-byte
-// aligned): OP 0 zeroes a's blocks, OP 1 exchanges the blocks of a and b
+// aligned): OP 1 exchanges the blocks of a and b
 // (SwapConservativeBuffersForNode, buffer_operations_material.h:83-91), OP 2
 // copies a's blocks into b's.
 template <int OP>
@@ -535,9 +528,7 @@ __global__ __launch_bounds__(kThreads) void listed_blocks_kernel(double *a, doub
     if (mem < 0) continue;
     size_t const o = (size_t)mem * pairs + t % pairs;
     double2 *pa = reinterpret_cast<double2 *>(a) + o;
-    if constexpr (OP == 0) {
-      *pa = double2{0.0, 0.0};
-    } else if constexpr (OP == 2) {
+    if constexpr (OP == 2) {
       reinterpret_cast<double2 *>(b)[o] = *pa;
     } else {
       double2 *pb = reinterpret_cast<double2 *>(b) + o;
@@ -727,47 +718,46 @@ constexpr size_t flux_doubles(int nblocks) {
 
 template <int DIM, int DIR, int FLUX>
 void launch_faces_dir(const alpaca_case_desc *d, int nblocks, const double *in, double *fl,
-                      hipStream_t s, BlockList list) {
+                      hipStream_t s) {
   constexpr int NF = faces_per_dir<DIM>();
   constexpr int gpb = (NF + kThreads - 1) / kThreads;
   dim3 grid((unsigned)(nblocks * gpb));
   ProfileScope prof(ALPACA_KERNEL_FACE_FLUX, s, double(NF) * nblocks);
   if (d->precision == ALPACA_MIXED)
     hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, kModeMixed>), grid, dim3(kThreads), 0, s,
-                       in, fl, nblocks, d->gamma, d->background_pressure, list);
+                       in, fl, nblocks, d->gamma, d->background_pressure);
   else if (d->precision == ALPACA_FP64_FMA)
     hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, kModeFma>), grid, dim3(kThreads), 0, s,
-                       in, fl, nblocks, d->gamma, d->background_pressure, list);
+                       in, fl, nblocks, d->gamma, d->background_pressure);
   else
     hipLaunchKernelGGL((face_flux_kernel<DIM, DIR, FLUX, kModeExact>), grid, dim3(kThreads), 0, s,
-                       in, fl, nblocks, d->gamma, d->background_pressure, list);
+                       in, fl, nblocks, d->gamma, d->background_pressure);
 }
 
 template <int DIM, int FLUX>
 void launch_faces_flux(const alpaca_case_desc *d, int nblocks, const double *in, double *fl,
-                       hipStream_t s, BlockList list) {
-  launch_faces_dir<DIM, 0, FLUX>(d, nblocks, in, fl, s, list);
-  launch_faces_dir<DIM, 1, FLUX>(d, nblocks, in, fl, s, list);
-  if constexpr (DIM == 3) launch_faces_dir<DIM, 2, FLUX>(d, nblocks, in, fl, s, list);
+                       hipStream_t s) {
+  launch_faces_dir<DIM, 0, FLUX>(d, nblocks, in, fl, s);
+  launch_faces_dir<DIM, 1, FLUX>(d, nblocks, in, fl, s);
+  if constexpr (DIM == 3) launch_faces_dir<DIM, 2, FLUX>(d, nblocks, in, fl, s);
 }
 
 template <int DIM>
 void launch_faces(const alpaca_case_desc *d, int nblocks, const double *in, double *fl,
-                  hipStream_t s, BlockList list = {}) {
+                  hipStream_t s) {
   if (d->flux == ALPACA_FLUX_HLLC)
-    launch_faces_flux<DIM, ALPACA_FLUX_HLLC>(d, nblocks, in, fl, s, list);
+    launch_faces_flux<DIM, ALPACA_FLUX_HLLC>(d, nblocks, in, fl, s);
   else
-    launch_faces_flux<DIM, ALPACA_FLUX_ROE>(d, nblocks, in, fl, s, list);
+    launch_faces_flux<DIM, ALPACA_FLUX_ROE>(d, nblocks, in, fl, s);
 }
 
 // Faces into the scratch fl (flux_doubles<DIM>(nblocks) doubles), then the
-// divergence (mode 1) or divergence + RK combination (mode 0) into out.  With
-// a block list, avg, init, out and jump are pools indexed through it.
+// divergence (mode 1) or divergence + RK combination (mode 0) into out.
 template <int DIM>
 int stage_impl(const alpaca_case_desc *d, int nblocks, int stage, int mode, const double *avg,
                double *init, double *out, alpaca_case_state *state, hipStream_t s, double *fl,
-               double *jump = nullptr, BlockList list = {}) {
-  launch_faces<DIM>(d, nblocks, avg, fl, s, list);
+               double *jump = nullptr) {
+  launch_faces<DIM>(d, nblocks, avg, fl, s);
   constexpr int cells = Dims<DIM>::CELLS;
   constexpr int gpb = (cells + kThreads - 1) / kThreads;
   dim3 grid((unsigned)(nblocks * gpb));
@@ -777,15 +767,15 @@ int stage_impl(const alpaca_case_desc *d, int nblocks, int stage, int mode, cons
     double *ini = (init && init != avg) ? init : nullptr;
     if (stage > 0) ini = init;
     hipLaunchKernelGGL((combine_kernel<DIM, 0>), grid, dim3(kThreads), 0, s, fl, avg, ini, out,
-                       nblocks, stage, inv_dx, state, list);
+                       nblocks, stage, inv_dx, state);
   } else {
     hipLaunchKernelGGL((combine_kernel<DIM, 1>), grid, dim3(kThreads), 0, s, fl, avg, nullptr,
-                       out, nblocks, 0, inv_dx, state, list);
+                       out, nblocks, 0, inv_dx, state);
   }
   if (jump) {
     size_t const total = (size_t)nblocks * 6 * Dims<DIM>::NE * kIC * Dims<DIM>::ICZ;
     hipLaunchKernelGGL((jump_flux_kernel<DIM>), dim3((unsigned)((total + kThreads - 1) / kThreads)),
-                       dim3(kThreads), 0, s, fl, jump, nblocks, list);
+                       dim3(kThreads), 0, s, fl, jump, nblocks);
   }
   return hip_status(hipGetLastError());
 }
@@ -836,6 +826,52 @@ void launch_stage_fused(const alpaca_case_desc *d, int stage, const double *avg,
                                                        state, a);
   } else {
     launch_stage_kernel<2, FLUX, MODE, 16, 16, 1, 128>(dim3(nblocks), s, avg, init, out, state, a);
+  }
+}
+
+// Right-hand side with the jump-flux save over listed pool blocks: the fused
+// tile kernel (faces in LDS), tile shapes as for the fused stage kernel.
+template <int DIM, int FLUX, int MODE>
+void launch_rhs_tiles(const alpaca_case_desc *d, const double *avg, double *rhs, double *jump,
+                      const int *list, int n, int capacity, hipStream_t s) {
+  constexpr int TX = DIM == 3 ? 8 : 16, TY = DIM == 3 ? 8 : 16, TZ = DIM == 3 ? 8 : 1;
+  constexpr int NT = DIM == 3 ? 256 : 128;
+  constexpr int tiles = exact::TileShape<DIM, TX, TY, TZ>::TILES;
+  dim3 const grid((unsigned)((size_t)n * tiles));
+  double const inv_dx = 1.0 / d->cell_size;
+  ProfileScope prof(ALPACA_KERNEL_STAGE, s, double(Dims<DIM>::CELLS) * n);
+  if constexpr (MODE == kModeExact)
+    hipLaunchKernelGGL((exact::rhs_tile_kernel<DIM, FLUX, double, TX, TY, TZ, NT>), grid, dim3(NT),
+                       0, s, avg, rhs, jump, list, n, capacity, d->gamma, d->background_pressure,
+                       inv_dx);
+  else if constexpr (MODE == kModeFma)
+    hipLaunchKernelGGL((fast::rhs_tile_kernel<DIM, FLUX, double, TX, TY, TZ, NT>), grid, dim3(NT),
+                       0, s, avg, rhs, jump, list, n, capacity, d->gamma, d->background_pressure,
+                       inv_dx);
+  else
+    hipLaunchKernelGGL((fast::rhs_tile_kernel<DIM, FLUX, float, TX, TY, TZ, NT>), grid, dim3(NT),
+                       0, s, avg, rhs, jump, list, n, capacity, d->gamma, d->background_pressure,
+                       inv_dx);
+}
+
+template <int DIM>
+void rhs_tiles(const alpaca_case_desc *d, const double *avg, double *rhs, double *jump,
+               const int *list, int n, int capacity, hipStream_t s) {
+  int const p = d->precision;
+  if (d->flux == ALPACA_FLUX_HLLC) {
+    if (p == ALPACA_MIXED)
+      launch_rhs_tiles<DIM, ALPACA_FLUX_HLLC, kModeMixed>(d, avg, rhs, jump, list, n, capacity, s);
+    else if (p == ALPACA_FP64_FMA)
+      launch_rhs_tiles<DIM, ALPACA_FLUX_HLLC, kModeFma>(d, avg, rhs, jump, list, n, capacity, s);
+    else
+      launch_rhs_tiles<DIM, ALPACA_FLUX_HLLC, kModeExact>(d, avg, rhs, jump, list, n, capacity, s);
+  } else {
+    if (p == ALPACA_MIXED)
+      launch_rhs_tiles<DIM, ALPACA_FLUX_ROE, kModeMixed>(d, avg, rhs, jump, list, n, capacity, s);
+    else if (p == ALPACA_FP64_FMA)
+      launch_rhs_tiles<DIM, ALPACA_FLUX_ROE, kModeFma>(d, avg, rhs, jump, list, n, capacity, s);
+    else
+      launch_rhs_tiles<DIM, ALPACA_FLUX_ROE, kModeExact>(d, avg, rhs, jump, list, n, capacity, s);
   }
 }
 
@@ -1029,26 +1065,22 @@ namespace {
 // Right-hand side of nblocks blocks with the face fluxes in fl: zero the
 // output blocks, faces, divergence, and (jump != null) the boundary-flux save.
 int rhs_with_scratch(const alpaca_case_desc *desc, int nblocks, const double *d_avg, double *d_rhs,
-                     double *d_jump, double *fl, hipStream_t s, BlockList list = {}) {
-  if (list.list) {
-    launch_listed_blocks<0>(d_rhs, nullptr, desc->dim, list, nblocks, s);
-  } else {
-    size_t const n = alpaca_block_doubles(desc->dim) * (size_t)nblocks;
-    unsigned const groups = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 65536);
-    hipLaunchKernelGGL(zero_kernel, dim3(groups), dim3(kThreads), 0, s, d_rhs, n);
-  }
+                     double *d_jump, double *fl, hipStream_t s) {
+  size_t const n = alpaca_block_doubles(desc->dim) * (size_t)nblocks;
+  unsigned const groups = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 65536);
+  hipLaunchKernelGGL(zero_kernel, dim3(groups), dim3(kThreads), 0, s, d_rhs, n);
   if (desc->dim == 2)
-    return stage_impl<2>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, fl, d_jump, list);
-  return stage_impl<3>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, fl, d_jump, list);
+    return stage_impl<2>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, fl, d_jump);
+  return stage_impl<3>(desc, nblocks, 0, 1, d_avg, nullptr, d_rhs, nullptr, s, fl, d_jump);
 }
 
 int rhs_context(alpaca_context *ctx, const alpaca_case_desc *desc, int nblocks,
-                const double *d_avg, double *d_rhs, double *d_jump, BlockList list = {}) {
+                const double *d_avg, double *d_rhs, double *d_jump) {
   double *fl = nullptr;
   size_t const n = desc->dim == 2 ? flux_doubles<2>(nblocks) : flux_doubles<3>(nblocks);
   int st = context_scratch(ctx, n * sizeof(double), &fl);
   if (st != ALPACA_OK) return st;
-  return rhs_with_scratch(desc, nblocks, d_avg, d_rhs, d_jump, fl, ctx->stream, list);
+  return rhs_with_scratch(desc, nblocks, d_avg, d_rhs, d_jump, fl, ctx->stream);
 }
 
 // Device address of `scratch` on the current device, looked up once per
@@ -1107,7 +1139,11 @@ int alpaca_ctx_right_hand_side_jump_fluxes_list(alpaca_context *ctx, const alpac
     return ALPACA_ERR_INVALID;
   if (n == 0) return ALPACA_OK;
   if (!d_list) return ALPACA_ERR_INVALID;
-  return rhs_context(ctx, desc, n, d_avg, d_rhs, d_jump_fluxes, BlockList{d_list, capacity});
+  if (desc->dim == 2)
+    rhs_tiles<2>(desc, d_avg, d_rhs, d_jump_fluxes, d_list, n, capacity, ctx->stream);
+  else
+    rhs_tiles<3>(desc, d_avg, d_rhs, d_jump_fluxes, d_list, n, capacity, ctx->stream);
+  return hip_status(hipGetLastError());
 }
 
 int alpaca_swap_blocks_list(int dim, double *d_a, double *d_b, int capacity, const int *d_list,

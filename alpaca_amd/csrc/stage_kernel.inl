@@ -91,6 +91,77 @@ template <int DIM, int TX, int TY, int TZ> struct TileShape {
   static constexpr int LDS = (DIM + 2) * (NFX + NFY + NFZ);  // reals
 };
 
+// Face fluxes of one tile (first cell x0, y0, z0 of block bb) into LDS:
+// every face of the tile, a flat loop whose per-direction ranges are padded
+// to 64 so each wavefront works on a single direction.  JUMP: a face on the
+// block's boundary is also added into the block's jump-flux surfaces
+// ([6][NE][IC][ICZ]: East, West, North, South, Top, Bottom; the boundary-flux
+// save of SpaceSolver::UpdateFluxes, space_solver.cpp:160-182).
+template <int DIM, int FLUX, typename R, int TX, int TY, int TZ, int NT, bool JUMP>
+__device__ __forceinline__ void tile_faces(const double *__restrict__ bb, int x0, int y0, int z0,
+                                           double gamma, double b, R *lds, double *jump) {
+  using D = Dims<DIM>;
+  using T = TileShape<DIM, TX, TY, TZ>;
+  constexpr int NE = D::NE;
+  constexpr int TZe = T::TZe;
+  constexpr int NFX = T::NFX, NFY = T::NFY, NFZ = T::NFZ;
+  constexpr int PX = T::PX, PY = T::PY, PZ = T::PZ;
+  constexpr int ICZ = D::ICZ;
+  R *FX = lds;
+  R *FY = lds + NE * NFX;
+  R *FZ = FY + NE * NFY;
+  int const zh = DIM == 3 ? kHS : 0;  // halo offset along z
+  // Adds a boundary face's flux into surface `loc` at entry (a, c).
+  auto save = [jump](int loc, int a, int c, const R *fl) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) jump[((loc * NE + e) * kIC + a) * ICZ + c] += double(fl[e]);
+  };
+  for (int f = threadIdx.x; f < PX + PY + PZ; f += NT) {
+    R fl[NE];
+    if (f < PX) {
+      if (f < NFX) {
+        int const k = f % TZe, jj = (f / TZe) % TY, ia = f / (TZe * TY);
+        stencil_flux<DIM, 0, FLUX, R>(bb, x0 + ia + kHS - 1, y0 + jj + kHS, z0 + k + zh, gamma, b,
+                                      fl);
+#pragma unroll
+        for (int e = 0; e < NE; ++e) FX[e * NFX + f] = fl[e];
+        if constexpr (JUMP) {
+          int const gx = x0 + ia;
+          if (gx == 0 || gx == kIC) save(gx == kIC ? 0 : 1, y0 + jj, z0 + k, fl);
+        }
+      }
+    } else if (f < PX + PY) {
+      int const r = f - PX;
+      if (r < NFY) {
+        int const k = r % TZe, ja = (r / TZe) % (TY + 1), i = r / (TZe * (TY + 1));
+        stencil_flux<DIM, 1, FLUX, R>(bb, x0 + i + kHS, y0 + ja + kHS - 1, z0 + k + zh, gamma, b,
+                                      fl);
+#pragma unroll
+        for (int e = 0; e < NE; ++e) FY[e * NFY + r] = fl[e];
+        if constexpr (JUMP) {
+          int const gy = y0 + ja;
+          if (gy == 0 || gy == kIC) save(gy == kIC ? 2 : 3, x0 + i, z0 + k, fl);
+        }
+      }
+    } else {
+      if constexpr (DIM == 3) {
+        int const r = f - PX - PY;
+        if (r < NFZ) {
+          int const ka = r % (TZ + 1), jj = (r / (TZ + 1)) % TY, i = r / ((TZ + 1) * TY);
+          stencil_flux<DIM, 2, FLUX, R>(bb, x0 + i + kHS, y0 + jj + kHS, z0 + ka + kHS - 1, gamma,
+                                        b, fl);
+#pragma unroll
+          for (int e = 0; e < NE; ++e) FZ[e * NFZ + r] = fl[e];
+          if constexpr (JUMP) {
+            int const gz = z0 + ka;
+            if (gz == 0 || gz == kIC) save(gz == kIC ? 4 : 5, x0 + i, y0 + jj, fl);
+          }
+        }
+      }
+    }
+  }
+}
+
 // One tile of one block: faces into LDS, barrier, divergence + RK + halo
 // push (+ CFL signal).  blk_local is the block's index within its case.
 // All threads of the workgroup must call it (it contains a barrier).
@@ -101,7 +172,6 @@ __device__ __forceinline__ void stage_tile(const TileJob &j, int blk_local, int 
   constexpr int NE = D::NE;
   constexpr int TZe = T::TZe, NTY = T::NTY, NTZ = T::NTZ;
   constexpr int NFX = T::NFX, NFY = T::NFY, NFZ = T::NFZ;
-  constexpr int PX = T::PX, PY = T::PY, PZ = T::PZ;
   R *FX = lds;
   R *FY = lds + NE * NFX;
   R *FZ = FY + NE * NFY;
@@ -110,41 +180,9 @@ __device__ __forceinline__ void stage_tile(const TileJob &j, int blk_local, int 
   int const y0 = ((tile / NTZ) % NTY) * TY;
   int const x0 = (tile / (NTZ * NTY)) * TX;
   const double *bb = j.avg + (size_t)blk * D::BLOCK;
-  int const zh = DIM == 3 ? kHS : 0;  // halo offset along z
 
   // 1. Face fluxes of the tile into LDS.
-  for (int f = threadIdx.x; f < PX + PY + PZ; f += NT) {
-    R fl[NE];
-    if (f < PX) {
-      if (f < NFX) {
-        int const k = f % TZe, jj = (f / TZe) % TY, ia = f / (TZe * TY);
-        stencil_flux<DIM, 0, FLUX, R>(bb, x0 + ia + kHS - 1, y0 + jj + kHS, z0 + k + zh, j.gamma,
-                                      j.b, fl);
-#pragma unroll
-        for (int e = 0; e < NE; ++e) FX[e * NFX + f] = fl[e];
-      }
-    } else if (f < PX + PY) {
-      int const r = f - PX;
-      if (r < NFY) {
-        int const k = r % TZe, ja = (r / TZe) % (TY + 1), i = r / (TZe * (TY + 1));
-        stencil_flux<DIM, 1, FLUX, R>(bb, x0 + i + kHS, y0 + ja + kHS - 1, z0 + k + zh, j.gamma,
-                                      j.b, fl);
-#pragma unroll
-        for (int e = 0; e < NE; ++e) FY[e * NFY + r] = fl[e];
-      }
-    } else {
-      if constexpr (DIM == 3) {
-        int const r = f - PX - PY;
-        if (r < NFZ) {
-          int const ka = r % (TZ + 1), jj = (r / (TZ + 1)) % TY, i = r / ((TZ + 1) * TY);
-          stencil_flux<DIM, 2, FLUX, R>(bb, x0 + i + kHS, y0 + jj + kHS, z0 + ka + kHS - 1,
-                                        j.gamma, j.b, fl);
-#pragma unroll
-          for (int e = 0; e < NE; ++e) FZ[e * NFZ + r] = fl[e];
-        }
-      }
-    }
-  }
+  tile_faces<DIM, FLUX, R, TX, TY, TZ, NT, false>(bb, x0, y0, z0, j.gamma, j.b, lds, nullptr);
   __syncthreads();
 
   // 2. Divergence + RK combination + halo push (+ CFL signal).
@@ -252,6 +290,79 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kMinWaves<DI
   j.stage = a.stage;
   j.signal = a.signal;
   stage_tile<DIM, FLUX, R, TX, TY, TZ, NT>(j, blk, g % T::TILES, lds);
+}
+
+// Right-hand side of listed blocks of a pool (SpaceSolver::UpdateFluxes in
+// full, space_solver.cpp:50-182), one workgroup per (list entry, tile):
+// the tile's faces into LDS, those on the block's boundary also added into
+// its jump-flux surfaces; after the barrier each internal cell's flux
+// divergence (ConsistencyManagedSum, space_solver.cpp:131-153) is written to
+// the block's right-hand-side buffer, and the tile's share of the block's
+// halo cells is set to zero (the whole output block is written, as by the
+// unfused per-block entry).  In 2D the Top and Bottom surfaces receive += 0.0,
+// as from the reference's zeroed z face array.  List entries outside
+// [0, capacity) are skipped (uniformly over the workgroup).
+template <int DIM, int FLUX, typename R, int TX, int TY, int TZ, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kMinWaves<DIM, FLUX, R>, 8))) void rhs_tile_kernel(
+    const double *__restrict__ avg, double *__restrict__ rhs, double *__restrict__ jump,
+    const int *__restrict__ list, int n, int capacity, double gamma, double b, double inv_dx) {
+  using D = Dims<DIM>;
+  using T = TileShape<DIM, TX, TY, TZ>;
+  constexpr int NE = D::NE;
+  constexpr int TZe = T::TZe, NTY = T::NTY, NTZ = T::NTZ;
+  constexpr int NFX = T::NFX, NFY = T::NFY, NFZ = T::NFZ;
+  constexpr int ICZ = D::ICZ;
+  constexpr int SURF = 6 * NE * kIC * ICZ;  // doubles per SurfaceBuffer
+  __shared__ R lds[T::LDS];
+  unsigned const g = xcd_remap_group(blockIdx.x, gridDim.x);
+  int const item = int(g / T::TILES), tile = int(g % T::TILES);
+  if (item >= n) return;
+  int const blk = list[item];
+  if (blk < 0 || blk >= capacity) return;  // uniform over the workgroup
+  int const z0 = DIM == 3 ? (tile % NTZ) * TZ : 0;
+  int const y0 = ((tile / NTZ) % NTY) * TY;
+  int const x0 = (tile / (NTZ * NTY)) * TX;
+  double *jb = jump + (size_t)blk * SURF;
+  tile_faces<DIM, FLUX, R, TX, TY, TZ, NT, true>(avg + (size_t)blk * D::BLOCK, x0, y0, z0, gamma,
+                                                 b, lds, jb);
+  if constexpr (DIM == 2) {
+    for (int t = threadIdx.x; t < 2 * NE * kIC; t += NT) jb[4 * NE * kIC + t] += 0.0;
+  }
+  __syncthreads();
+  const R *FX = lds;
+  const R *FY = lds + NE * NFX;
+  const R *FZ = FY + NE * NFY;
+  double *out = rhs + (size_t)blk * D::BLOCK;
+  for (int c = threadIdx.x; c < T::CELLS; c += NT) {
+    int const kk = c % TZe, jj = (c / TZe) % TY, ii = c / (TZe * TY);
+    int const cell = (x0 + ii + kHS) * D::SX + (y0 + jj + kHS) * D::SY +
+                     (DIM == 3 ? z0 + kk + kHS : 0);
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      double const dX = double(FX[e * NFX + (ii * TY + jj) * TZe + kk]) -
+                        double(FX[e * NFX + ((ii + 1) * TY + jj) * TZe + kk]);
+      double const dY = double(FY[e * NFY + (ii * (TY + 1) + jj) * TZe + kk]) -
+                        double(FY[e * NFY + (ii * (TY + 1) + jj + 1) * TZe + kk]);
+      double dZ = 0.0;
+      if constexpr (DIM == 3)
+        dZ = double(FZ[e * NFZ + (ii * TY + jj) * (TZ + 1) + kk]) -
+             double(FZ[e * NFZ + (ii * TY + jj) * (TZ + 1) + kk + 1]);
+      out[e * D::FIELD + cell] = 0.0 + (0.0 + dams<DIM>(dX, dY, dZ) * inv_dx);
+    }
+  }
+  // The tile's share of the halo cells: positions [tile * FIELD / TILES, ...).
+  constexpr int SHARE = D::FIELD / T::TILES;
+  static_assert(SHARE * T::TILES == D::FIELD, "halo shares cover the block");
+  for (int q = threadIdx.x; q < SHARE; q += NT) {
+    int const r = tile * SHARE + q;
+    int const i = r / D::SX, j = (r / D::SY) % kTC, k = DIM == 3 ? r % kTC : kHS;
+    bool const inside = i >= kHS && i < kHS + kIC && j >= kHS && j < kHS + kIC &&
+                        k >= kHS && k < kHS + kIC;
+    if (!inside) {
+#pragma unroll
+      for (int e = 0; e < NE; ++e) out[e * D::FIELD + r] = 0.0;
+    }
+  }
 }
 
 // Every tile of every listed block of every case of an ensemble, one

@@ -181,8 +181,10 @@ int alpaca_ctx_right_hand_side_jump_fluxes(alpaca_context *ctx,
  * for each leaf of a level.  d_avg, d_rhs and d_jump_fluxes are pools of
  * `capacity` haloed blocks / SurfaceBuffers; the n distinct device entries of
  * d_list name the blocks, each treated as alpaca_ctx_right_hand_side_
- * jump_fluxes treats its blocks (the same bits), with no copy of the pool.
- * Entries outside [0, capacity) are skipped. */
+ * jump_fluxes treats its blocks (the same bits, in every arithmetic form), in
+ * one launch with the face fluxes in LDS: no copy of the pool and no
+ * face-flux scratch (the context gives the stream).  Entries outside
+ * [0, capacity) are skipped. */
 int alpaca_ctx_right_hand_side_jump_fluxes_list(alpaca_context *ctx,
                                                 const alpaca_case_desc *desc, int capacity,
                                                 const int *d_list, int n, const double *d_avg,

@@ -44,9 +44,14 @@ def _same_bits(a, b):
     return np.array_equal(a.cpu().numpy().view(np.int64), b.cpu().numpy().view(np.int64))
 
 
-@pytest.mark.parametrize("dim,flux", [(2, gpu.FLUX_HLLC), (3, gpu.FLUX_HLLC), (3, gpu.FLUX_ROE)])
-def test_rhs_list_equals_contiguous_form(torch, dim, flux):
-    print(f"[pool rhs list] dim={dim} flux={flux}", flush=True)
+@pytest.mark.parametrize("dim,flux,prec", [
+    (2, gpu.FLUX_HLLC, gpu.FP64), (3, gpu.FLUX_HLLC, gpu.FP64), (2, gpu.FLUX_ROE, gpu.FP64),
+    (3, gpu.FLUX_ROE, gpu.FP64), (3, gpu.FLUX_HLLC, gpu.FP64_FMA), (3, gpu.FLUX_ROE, gpu.FP64_FMA),
+    (2, gpu.FLUX_HLLC, gpu.MIXED), (3, gpu.FLUX_HLLC, gpu.MIXED), (3, gpu.FLUX_ROE, gpu.MIXED)])
+def test_rhs_list_equals_contiguous_form(torch, dim, flux, prec):
+    """The fused tile kernel over listed pool blocks gives the bits of the
+    unfused per-block entry on the gathered blocks, in every arithmetic form."""
+    print(f"[pool rhs list] dim={dim} flux={flux} precision={prec}", flush=True)
     lib = gpu.load_library()
     ctx = ctypes.c_void_p()
     gpu.check(lib.alpaca_context_create(gpu._stream_ptr(), ctypes.byref(ctx)), "context_create")
@@ -57,7 +62,8 @@ def test_rhs_list_equals_contiguous_form(torch, dim, flux):
         rhs, jf = rhs0.clone(), jf0.clone()
         entries = LIST[:2] + SKIPPED[:1] + LIST[2:] + SKIPPED[1:]
         lst = torch.tensor(entries, dtype=torch.int32, device="cuda")
-        desc = gpu.make_desc(dim, [len(entries), 1, 1], [0, 0, 0], flux=flux, cell_size=1.0 / 64)
+        desc = gpu.make_desc(dim, [len(entries), 1, 1], [0, 0, 0], flux=flux, precision=prec,
+                             cell_size=1.0 / 64)
         gpu.check(lib.alpaca_ctx_right_hand_side_jump_fluxes_list(
             ctx, ctypes.byref(desc), CAP, _p(lst), len(entries), _p(avg), _p(rhs), _p(jf)),
             "rhs list")
@@ -65,7 +71,8 @@ def test_rhs_list_equals_contiguous_form(torch, dim, flux):
         a = avg.index_select(0, idx).contiguous()
         r = torch.empty_like(a)
         j = jf0.index_select(0, idx).contiguous()
-        d2 = gpu.make_desc(dim, [len(LIST), 1, 1], [0, 0, 0], flux=flux, cell_size=1.0 / 64)
+        d2 = gpu.make_desc(dim, [len(LIST), 1, 1], [0, 0, 0], flux=flux, precision=prec,
+                           cell_size=1.0 / 64)
         gpu.check(lib.alpaca_ctx_right_hand_side_jump_fluxes(
             ctx, ctypes.byref(d2), len(LIST), _p(a), _p(r), _p(j)), "rhs contiguous")
         torch.cuda.synchronize()
