@@ -493,8 +493,9 @@ __global__ __launch_bounds__(kThreads) void signal_kernel(const double *__restri
 
 // dt = CFL / (S / dx), end-time clamp, minimum-step health check
 // (modular_algorithm_assembler.cpp:1932-1982; ComputeLoop :137-140).
-__global__ void dt_kernel(alpaca_case_state *state, double cell_size) {
+__global__ void dt_kernel(alpaca_case_state *state, double cell_size, int guard = 0) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (guard && (state->flags & kFlagDone)) return;
   double const s = __longlong_as_double((long long)state->smax);
   double const rate = exact::fdiv(s, cell_size);
   double dt = rate == 0.0 ? 1.7976931348623157e308 : exact::fdiv(state->cfl, rate);
@@ -567,10 +568,15 @@ __global__ __launch_bounds__(kThreads) void exact_ops_kernel(const double *__res
 }
 
 // FinishMacroTimestep (time_integrator.h:244-248).
-__global__ void finish_step_kernel(alpaca_case_state *state) {
+// TimeIntegrator::FinishMacroTimestep (time := time + dt) and ComputeLoop's
+// end test (modular_algorithm_assembler.cpp:137-140): the done flag once the
+// time is no longer below t_end.  guard: a record already done is left alone.
+__global__ void finish_step_kernel(alpaca_case_state *state, int guard = 0) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (guard && (state->flags & kFlagDone)) return;
   state->time += 0.0 + state->dt;
   state->steps += 1;
+  if (!(state->time < state->t_end)) state->flags |= kFlagDone;
 }
 
 __global__ void state_init_kernel(alpaca_case_state *state, double time, double t_end, double cfl) {
@@ -802,7 +808,7 @@ void launch_stage_kernel(dim3 grid, hipStream_t s, const double *avg, const doub
 template <int DIM, int FLUX, int MODE>
 void launch_stage_fused(const alpaca_case_desc *d, int stage, const double *avg,
                         const double *init, double *out, double *init_copy,
-                        alpaca_case_state *state, int signal, hipStream_t s) {
+                        alpaca_case_state *state, int signal, hipStream_t s, bool guard) {
   int const nblocks = case_blocks(d);
   StageArgs a;
   a.nb[0] = d->nb[0];
@@ -817,6 +823,7 @@ void launch_stage_fused(const alpaca_case_desc *d, int stage, const double *avg,
   a.nblocks = nblocks;
   a.stage = stage;
   a.signal = signal;
+  a.guard = guard;
   a.init_copy = init_copy;
   ProfileScope prof(ALPACA_KERNEL_STAGE, s, double(Dims<DIM>::CELLS) * nblocks);
   if constexpr (DIM == 3) {
@@ -878,22 +885,22 @@ void rhs_tiles(const alpaca_case_desc *d, const double *avg, double *rhs, double
 template <int DIM>
 void stage_fused(const alpaca_case_desc *d, int stage, const double *avg, const double *init,
                  double *out, double *init_copy, alpaca_case_state *state, int signal,
-                 hipStream_t s) {
+                 hipStream_t s, bool guard = false) {
   int const p = d->precision;
   if (d->flux == ALPACA_FLUX_HLLC) {
     if (p == ALPACA_MIXED)
-      launch_stage_fused<DIM, ALPACA_FLUX_HLLC, kModeMixed>(d, stage, avg, init, out, init_copy, state, signal, s);
+      launch_stage_fused<DIM, ALPACA_FLUX_HLLC, kModeMixed>(d, stage, avg, init, out, init_copy, state, signal, s, guard);
     else if (p == ALPACA_FP64_FMA)
-      launch_stage_fused<DIM, ALPACA_FLUX_HLLC, kModeFma>(d, stage, avg, init, out, init_copy, state, signal, s);
+      launch_stage_fused<DIM, ALPACA_FLUX_HLLC, kModeFma>(d, stage, avg, init, out, init_copy, state, signal, s, guard);
     else
-      launch_stage_fused<DIM, ALPACA_FLUX_HLLC, kModeExact>(d, stage, avg, init, out, init_copy, state, signal, s);
+      launch_stage_fused<DIM, ALPACA_FLUX_HLLC, kModeExact>(d, stage, avg, init, out, init_copy, state, signal, s, guard);
   } else {
     if (p == ALPACA_MIXED)
-      launch_stage_fused<DIM, ALPACA_FLUX_ROE, kModeMixed>(d, stage, avg, init, out, init_copy, state, signal, s);
+      launch_stage_fused<DIM, ALPACA_FLUX_ROE, kModeMixed>(d, stage, avg, init, out, init_copy, state, signal, s, guard);
     else if (p == ALPACA_FP64_FMA)
-      launch_stage_fused<DIM, ALPACA_FLUX_ROE, kModeFma>(d, stage, avg, init, out, init_copy, state, signal, s);
+      launch_stage_fused<DIM, ALPACA_FLUX_ROE, kModeFma>(d, stage, avg, init, out, init_copy, state, signal, s, guard);
     else
-      launch_stage_fused<DIM, ALPACA_FLUX_ROE, kModeExact>(d, stage, avg, init, out, init_copy, state, signal, s);
+      launch_stage_fused<DIM, ALPACA_FLUX_ROE, kModeExact>(d, stage, avg, init, out, init_copy, state, signal, s, guard);
   }
 }
 
@@ -921,11 +928,11 @@ int halo_table_impl(double *blocks, int nblocks, const int *nbr, const SideBc &b
 
 template <int DIM>
 int timestep_impl(const alpaca_case_desc *d, const double *avg, alpaca_case_state *state,
-                  hipStream_t s, int nblocks, BlockList list = {}) {
+                  hipStream_t s, int nblocks, BlockList list = {}, int guard = 0) {
   ProfileScope prof(ALPACA_KERNEL_TIMESTEP, s, double(Dims<DIM>::CELLS) * nblocks);
   hipLaunchKernelGGL((signal_kernel<DIM>), dim3(nblocks), dim3(kThreads), 0, s, avg, nblocks,
                      d->gamma, d->background_pressure, state, list);
-  hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(64), 0, s, state, d->cell_size);
+  hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(64), 0, s, state, d->cell_size, guard);
   return hip_status(hipGetLastError());
 }
 
@@ -1351,20 +1358,26 @@ int alpaca_advance(const alpaca_case_desc *desc, double *d_u, double *d_s1, doub
   hipStream_t s = static_cast<hipStream_t>(stream);
   // The first step reduces the CFL signal of d_u; later steps take it from the
   // last stage of the step before.
-  if (nsteps > 0 && (st = alpaca_timestep(desc, d_u, d_state, stream)) != ALPACA_OK) return st;
+  if (nsteps > 0) {
+    st = desc->dim == 2 ? timestep_impl<2>(desc, d_u, d_state, s, case_blocks(desc), {}, 1)
+                        : timestep_impl<3>(desc, d_u, d_state, s, case_blocks(desc), {}, 1);
+    if (st != ALPACA_OK) return st;
+  }
+  // Steps once the record is done (its time reached t_end) do nothing: a run
+  // can be advanced in chunks of steps with one read-back per chunk.
   for (int n = 0; n < nsteps; ++n) {
-    if (n > 0) hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(64), 0, s, d_state, desc->cell_size);
+    if (n > 0) hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(64), 0, s, d_state, desc->cell_size, 1);
     bool const signal = n + 1 < nsteps;
     if (desc->dim == 2) {
-      stage_fused<2>(desc, 0, d_u, nullptr, d_s1, nullptr, d_state, 0, s);
-      stage_fused<2>(desc, 1, d_s1, d_u, d_s2, nullptr, d_state, 0, s);
-      stage_fused<2>(desc, 2, d_s2, d_u, d_u, nullptr, d_state, signal, s);
+      stage_fused<2>(desc, 0, d_u, nullptr, d_s1, nullptr, d_state, 0, s, true);
+      stage_fused<2>(desc, 1, d_s1, d_u, d_s2, nullptr, d_state, 0, s, true);
+      stage_fused<2>(desc, 2, d_s2, d_u, d_u, nullptr, d_state, signal, s, true);
     } else {
-      stage_fused<3>(desc, 0, d_u, nullptr, d_s1, nullptr, d_state, 0, s);
-      stage_fused<3>(desc, 1, d_s1, d_u, d_s2, nullptr, d_state, 0, s);
-      stage_fused<3>(desc, 2, d_s2, d_u, d_u, nullptr, d_state, signal, s);
+      stage_fused<3>(desc, 0, d_u, nullptr, d_s1, nullptr, d_state, 0, s, true);
+      stage_fused<3>(desc, 1, d_s1, d_u, d_s2, nullptr, d_state, 0, s, true);
+      stage_fused<3>(desc, 2, d_s2, d_u, d_u, nullptr, d_state, signal, s, true);
     }
-    hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, s, d_state);
+    hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, s, d_state, 1);
   }
   return hip_status(hipGetLastError());
 }

@@ -126,6 +126,7 @@ struct StageArgs {
   int stage;
   int signal;         // reduce the CFL signal of the new state into state->smax
   double *init_copy;  // stage 0: also copy the Average buffer here (or null)
+  int guard;          // do nothing once the case record has reached its end time
 };
 
 // One tile of work for the fused stage: the case's buffers (block 0 of the

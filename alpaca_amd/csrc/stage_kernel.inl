@@ -272,6 +272,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kMinWaves<DI
   unsigned const g = xcd_remap_group(blockIdx.x, gridDim.x);
   int const blk = g / T::TILES;
   if (blk >= a.nblocks) return;  // uniform over the workgroup
+  if (a.guard && (state->flags & kFlagDone)) return;
   TileJob j;
   j.avg = avg;
   j.init = init;

@@ -339,18 +339,20 @@ class GpuCase:
         return CaseState.from_buffer_copy(raw)
 
     def run(self, max_steps: int, sync_every: int = 1, on_step=None):
-        """ComputeLoop: advance while time < t_end (read back each step);
-        on_step(record) runs after each macro step, where ComputeLoop writes
-        restart snapshots and output."""
+        """ComputeLoop: advance while time < t_end; on_step(record) runs after
+        each macro step, where ComputeLoop writes restart snapshots and
+        output.  The record is read back every sync_every steps (1 with
+        on_step): the device skips the steps of a chunk that follow the end
+        time, and dts then holds the time step of each chunk's last step."""
+        chunk = 1 if on_step is not None else max(1, int(sync_every))
         steps = 0
         dts = []
-        while steps < max_steps:
+        rec = self.record()
+        while steps < max_steps and rec.time < self.t_end:
+            before = rec.steps
+            self.advance(min(chunk, max_steps - steps))
             rec = self.record()
-            if not rec.time < self.t_end:
-                break
-            self.advance(1)
-            steps += 1
-            rec = self.record()
+            steps += rec.steps - before
             dts.append(rec.dt)
             if rec.flags & 1:
                 raise AlpacaError("non-finite state")

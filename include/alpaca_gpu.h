@@ -84,7 +84,8 @@ typedef struct alpaca_case_state {
   double dt;       /* time step of the running macro step */
   uint64_t smax;   /* max over internal cells of sum_d(|u_d|+c), as IEEE bits */
   int32_t steps;   /* completed macro steps */
-  int32_t flags;   /* bit 0: non-finite state seen; bit 1: dt unhealthy */
+  int32_t flags;   /* bit 0: non-finite state seen; bit 1: dt unhealthy;
+                      bit 2: time reached t_end (set by the end of a step) */
 } alpaca_case_state;
 
 const char *alpaca_status_string(int status);
@@ -323,7 +324,10 @@ int alpaca_case_state_init(alpaca_case_state *d_state, double time,
  * d_s1 and d_s2 are scratch blocks of the same size: stage 0 writes u1 to
  * d_s1, stage 1 writes u2 to d_s2 (d_u is the initial buffer), stage 2 writes
  * u^(n+1) over d_u.  The CFL signal of each new state is reduced in the last
- * stage, so only the first of the nsteps runs a separate reduction. */
+ * stage, so only the first of the nsteps runs a separate reduction.  As
+ * ComputeLoop stops at the end time (modular_algorithm_assembler.cpp:
+ * 137-140), a step whose record already reached t_end (flags bit 2) does
+ * nothing: a run can be advanced in chunks with one read-back per chunk. */
 int alpaca_advance(const alpaca_case_desc *desc, double *d_u, double *d_s1,
                    double *d_s2, alpaca_case_state *d_state, int nsteps,
                    void *stream);

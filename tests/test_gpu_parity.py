@@ -332,3 +332,25 @@ def test_edge_blocks_bitexact(torch, dim, flux):
             want_rhs = oracle_py.block_rhs(dim, flux, 1.4, 0.0, host[bi], 1.0 / 128)
             g = got_rhs[:, bi * 16:(bi + 1) * 16, :, :icz]
             assert np.array_equal(g, want_rhs, equal_nan=True), kind
+
+
+@pytest.mark.parametrize("prec", [gpu.FP64, gpu.MIXED])
+def test_chunked_run_stops_at_end_time(torch, prec):
+    """Advancing in chunks of 7 or 1000 steps with one read-back per chunk
+    (the device skips the steps after t_end; 1000 exceeds the run's step
+    count) gives the per-step run's state bits, step count and final time."""
+    print(f"[chunked run] precision={prec}", flush=True)
+    st0 = cases.four_quadrant_2d(6, 4, seed=3)
+    desc = gpu.make_desc(2, [6, 4], [0, 0], flux=gpu.FLUX_HLLC, precision=prec)
+    out = {}
+    for every in (1, 7, 1000):
+        case = gpu.GpuCase(desc, st0, cfl=0.6, t_end=0.2)
+        steps, _ = case.run(max_steps=100000, sync_every=every)
+        rec = case.record()
+        out[every] = (case.download(), steps, rec.time, rec.steps, rec.flags)
+    a, sa, ta, ra, fa = out[1]
+    assert sa == ra < 1000 and not ta < 0.2 and fa == 4
+    for every in (7, 1000):
+        b, sb, tb, rb, fb = out[every]
+        assert sb == rb == sa and tb == ta and fb == fa
+        assert np.array_equal(a.view(np.int64), b.view(np.int64))
