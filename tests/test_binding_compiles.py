@@ -1,6 +1,7 @@
-"""The reference-side binding INTEGRATION.md shows (section 2: a CRTP
-ConvectiveTermSolver<GpuFiniteVolumeScheme> over alpaca_host.h) compiles
-against the reference's own headers (convective_term_solver.h:67-81 and what
+"""The reference-side binding -- the CRTP ConvectiveTermSolver<GpuFiniteVolume-
+Scheme> over alpaca_host.h that a maintainer adds to the reference, kept as
+the one `cpp` block defining GpuFiniteVolumeScheme in the integration notes
+at the repository root -- compiles against the reference's own headers (convective_term_solver.h:67-81 and what
 it includes), for 2D and 3D, with the reference's class template explicitly
 instantiated so every call path to UpdateImplementation is checked.  Needs
 the reference tree; syntax-only, output under a temporary directory."""
@@ -20,7 +21,7 @@ def _binding_source():
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     blocks = re.findall(r"```cpp\n(.*?)```", text, re.S)
     found = [b for b in blocks if "class GpuFiniteVolumeScheme" in b]
-    assert len(found) == 1, "INTEGRATION.md must show the GpuFiniteVolumeScheme binding once"
+    assert len(found) == 1, "the integration notes must show the GpuFiniteVolumeScheme binding once"
     return found[0]
 
 
