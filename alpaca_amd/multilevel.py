@@ -750,10 +750,18 @@ class GpuBackend:
             return a[:, :, h:h + i, h:h + i, :].cpu().numpy()
 
     def swap(self, slots):
+        """SwapBuffers as a one-way copy of the new state (RightHandSide) into
+        the Average buffers: what the exchange leaves in the RightHandSide
+        buffers is never read before it is overwritten (a leaf's by its next
+        right-hand side, any other node's internal cells by the averaging of
+        its children and its halo cells by the halo update that follows), so
+        the copy gives the reference's results at half the traffic of the
+        exchange.  The CPU backend exchanges, and the two backends' states are
+        compared bit for bit."""
         lst = self._list(slots)
-        gpu.check(self.lib.alpaca_swap_blocks_list(self.dim, self._p(self.avg), self._p(self.rhs),
+        gpu.check(self.lib.alpaca_copy_blocks_list(self.dim, self._p(self.rhs), self._p(self.avg),
                                                    self.capacity, self._p(lst), len(slots),
-                                                   self.st), "swap_blocks_list")
+                                                   self.st), "copy_blocks_list")
 
     def fill_initial(self, slots):
         self.lti.fill_initial_buffer(self.avg, self.init, self._list(slots), 0, self.st)

@@ -85,3 +85,43 @@ def test_restart_written_by_cpu_continues_on_gpu(torch, tmp_path):
         o.advance()
         g.advance()
         _same(g, o, f"after restore, step {n + 1}")
+
+
+class _ExchangingBackend(multilevel.GpuBackend):
+    """The GPU backend with SwapBuffers as the reference's exchange of the
+    Average and RightHandSide buffers (the backend itself copies)."""
+
+    def swap(self, slots):
+        lst = self._list(slots)
+        gpu.check(self.lib.alpaca_swap_blocks_list(self.dim, self._p(self.avg), self._p(self.rhs),
+                                                   self.capacity, self._p(lst), len(slots),
+                                                   self.st), "swap_blocks_list")
+
+
+@pytest.mark.parametrize("dim,lmax,nb0,bc,steps", [
+    (3, 3, [2, 2, 2], gpu.BC_ZERO_GRADIENT, 3),
+    (2, 4, [4, 4], gpu.BC_ZERO_GRADIENT, 4),
+    (2, 3, [4, 2], gpu.BC_SYMMETRY, 6)])
+def test_swap_as_copy_equals_exchange(torch, dim, lmax, nb0, bc, steps):
+    """The backend's one-way copy for SwapBuffers leaves every Average
+    buffer with the bits of the reference's exchange, over trees with level
+    jumps at the domain sides, maximum levels 3 and 4."""
+    from alpaca_amd import cases
+    print(f"[swap as copy] dim={dim} lmax={lmax}", flush=True)
+    node = 0.25 if dim == 3 else 0.5
+    cap = int(np.prod(nb0)) * sum((1 << dim) ** lv for lv in range(lmax + 1))
+    out = []
+    for cls in (multilevel.GpuBackend, _ExchangingBackend):
+        be = cls(dim, cap, flux=gpu.FLUX_HLLC)
+        sol = multilevel.MultiLevelSolver(be, dim, nb0, node, lmax, [bc] * (2 * dim),
+                                          cases.blob_node_initial(dim, node), 0.5, 10.0, 1e-3,
+                                          lmax)
+        sol.create_new_simulation()
+        for _ in range(steps):
+            sol.advance()
+        keys = sorted(sol.slot)
+        out.append((keys, be.get_blocks(sol._slots(keys)), sol.current_run_time))
+        be.close()
+    (ka, a, ta), (kb, b, tb) = out
+    assert ka == kb and ta == tb and len(ka) > int(np.prod(nb0))
+    assert np.array_equal(a.view(np.int64), b.view(np.int64))
