@@ -301,6 +301,11 @@ class MultiLevelSolver:
         # columns follow offset_slot's order, which is the order of offs
         return np.ascontiguousarray(table)
 
+    def _level_table(self, level):
+        """The neighbour table of a level's nodes (in _ids order), kept until
+        the tree changes."""
+        return self._cached("table", level, lambda: self._neighbor_table(self._ids(level)))
+
     def halo_update_on_level(self, level: int, cut_jumps: bool):
         """MaterialHaloUpdateOnLevel for the conservatives (the right-hand-side
         buffers): same-level copies, then (unless cut_jumps) the jump halos
@@ -310,7 +315,7 @@ class MultiLevelSolver:
             return
 
         def build():
-            table = self._neighbor_table(keys)
+            table = self._level_table(level)
             cols = [offset_slot(self.dim, self._loc_dir[loc]) for loc in self.halo_locs]
             rows, which = np.nonzero(table[:, cols] == -2)
             jump_links = self._level_links(level)[rows] if rows.size else []
@@ -373,9 +378,9 @@ class MultiLevelSolver:
                 by_loc = {}
                 if not keys:
                     return by_loc
-                nb, inside, _ = self._neighbors(level, keys,
-                                                [self._loc_dir[loc] for loc in self.halo_locs])
-                missing = inside & (nb < 0)
+                # a jump at a location: no node of this level there (-2)
+                cols = [offset_slot(self.dim, self._loc_dir[loc]) for loc in self.halo_locs]
+                missing = self._level_table(level)[:, cols] == -2
                 slots = self._slots(keys)
                 for c, loc in enumerate(self.halo_locs):
                     rows = np.nonzero(missing[:, c])[0]
