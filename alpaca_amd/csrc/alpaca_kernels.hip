@@ -803,8 +803,12 @@ void launch_stage_kernel(dim3 grid, hipStream_t s, const double *avg, const doub
                        s, avg, init, out, state, a);
 }
 
+// 2D cases with fewer leaf blocks than this run on 8x8 tiles.
+constexpr int kSmallCaseBlocks = 256;
+
 // Tile shape of the fused stage kernel: 8x8x8 cells and 256 threads in 3D,
-// the whole 16x16 block and 128 threads in 2D.
+// the whole 16x16 block and 128 threads in 2D (8x8 and 256 threads for a case
+// of fewer than kSmallCaseBlocks blocks).
 template <int DIM, int FLUX, int MODE>
 void launch_stage_fused(const alpaca_case_desc *d, int stage, const double *avg,
                         const double *init, double *out, double *init_copy,
@@ -831,8 +835,13 @@ void launch_stage_fused(const alpaca_case_desc *d, int stage, const double *avg,
     constexpr int tiles = (kIC / TX) * (kIC / TY) * (kIC / TZ);
     launch_stage_kernel<3, FLUX, MODE, TX, TY, TZ, NT>(dim3(nblocks * tiles), s, avg, init, out,
                                                        state, a);
-  } else {
+  } else if (nblocks >= kSmallCaseBlocks) {
     launch_stage_kernel<2, FLUX, MODE, 16, 16, 1, 128>(dim3(nblocks), s, avg, init, out, state, a);
+  } else {
+    // Fewer blocks than CUs: four 8x8 tiles per block spread the case over
+    // four times as many workgroups, and 256 threads give every thread at
+    // most one of a tile's faces (the same faces and sums per cell).
+    launch_stage_kernel<2, FLUX, MODE, 8, 8, 1, 256>(dim3(nblocks * 4), s, avg, init, out, state, a);
   }
 }
 
