@@ -487,11 +487,14 @@ class MultiLevelSolver:
         parent_levels = [lv for lv in levels_ascending if lv < cur_max]
         remove, refine = self.determine_remeshing_nodes(parent_levels)
         refine = [k for k in refine if self.topo.is_leaf(k) and k[0] != self.maximum_level]
-        links = []
-        for key in refine:
-            links += self.refine_node(key)
-        if links:
-            self.be.predict(links, (gpu.HS, gpu.IC) * self.dim)
+        if refine:
+            # RefineNode of every listed leaf at once: the same nodes and slots
+            # as one call per leaf, one batched clear and one prediction.
+            for key in refine:
+                self.topo.set_leaf(key, False)
+            kids = [c for key in refine for c in self.topo.children(key)]
+            self._new_nodes(kids)
+            self.be.predict([self._link(c) for c in kids], (gpu.HS, gpu.IC) * self.dim)
         self.halo_update(levels_ascending[1:])
         self._coarsen([k for k in remove if k[0] != 1])
 
