@@ -298,6 +298,31 @@ template <int NE, typename R> struct Eig {
   bool ok;
 };
 
+// Structural zeros of the Roe eigenvectors (eigendecomposition.h:238-466):
+// L rows 1 and 2 (the shear waves) hold only the mass and their own
+// transverse momentum; R leaves out the shear columns of the mass, normal
+// momentum and other transverse momentum rows.  The FP64-with-FMA form skips
+// these products (0 * x changes a sum only for a non-finite x, which that
+// form does not track); the exact form forms them, as the reference does,
+// and the mixed form keeps its measured accuracy by keeping them too.
+template <int DIM, int DIR> __device__ __forceinline__ constexpr bool left_zero(int n, int c) {
+  constexpr int Mp = kMomX + DIR, M1 = kMomX + Minor<DIR>::m1, M2 = kMomX + Minor<DIR>::m2;
+  if (n == 1) return c != kMass && c != M1;
+  if (DIM == 3 && n == 2) return c != kMass && c != M2;
+  (void)Mp;
+  return false;
+}
+template <int DIM, int DIR> __device__ __forceinline__ constexpr bool right_zero(int r, int c) {
+  constexpr int Mp = kMomX + DIR, M1 = kMomX + Minor<DIR>::m1, M2 = kMomX + Minor<DIR>::m2;
+  bool const shear = c >= 1 && c < DIM;  // columns 1 .. DIM-1
+  if (!shear) return false;
+  if (r == kMass || r == Mp) return true;
+  if (r == M1) return c != 1;
+  if (DIM == 3 && r == M2) return c != 2;
+  return false;
+}
+template <typename R> constexpr bool kSkipZeros = kFastArith && std::is_same<R, double>::value;
+
 // The exact FP64 finite-volume (HLLC) kernels build the right eigenvectors
 // only once the characteristic values are formed, so the 19 entries are not
 // held in registers meanwhile (it saves 20 VGPRs in 3D); without contraction
@@ -460,7 +485,8 @@ __device__ __forceinline__ void to_physical(const R *ch, const R (&Rm)[DIM + 2][
   for (int l = 0; l < NE; ++l) {
     R s = R(0.0);
 #pragma unroll
-    for (int n = 1; n < NE - 1; ++n) s += ch[n] * Rm[l][n];
+    for (int n = 1; n < NE - 1; ++n)
+      if (!(kSkipZeros<R> && right_zero<DIM, DIR>(l, n))) s += ch[n] * Rm[l][n];
     s += (ch[0] * Rm[l][0] + ch[NE - 1] * Rm[l][NE - 1]);
     out[l] = s;
   }
@@ -738,6 +764,7 @@ __device__ __forceinline__ void roe_split_flux(double gamma64, double b64,
 #pragma unroll
         for (int q = 0; q < NE; ++q) {
           int const l = seq<DIM, DIR>(q);
+          if (kSkipZeros<R> && left_zero<DIM, DIR>(n, l)) continue;
           R const uct = Uc[l] * ev.L[n][l];
           R const act = adv[l] * ev.L[n][l];
           sp += (act + ev.lam[n] * uct);
@@ -822,7 +849,7 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
 #pragma unroll
         for (int q = 0; q < NE; ++q) {
           int const l = seq<DIM, DIR>(q);
-          s += ev.L[n][l] * U[m][l];
+          if (!(kSkipZeros<R> && left_zero<DIM, DIR>(n, l))) s += ev.L[n][l] * U[m][l];
         }
         uc[m] = s;
       }
