@@ -229,8 +229,10 @@ __device__ __forceinline__ void cell_quantities(double gamma, double b, const do
   q.H = R(H);
 }
 
-// WENO5::ApplyImplementation (weno5.h:88-158) on ordered v1..v5.
-template <typename R> __device__ __forceinline__ R weno5(R v1, R v2, R v3, R v4, R v5) {
+// WENO5::ApplyImplementation (weno5.h:88-158) on ordered v1..v5.  ONE_DIV
+// (FP64 with FMA only) forms the weights with one division, below.
+template <typename R, bool ONE_DIV = false>
+__device__ __forceinline__ R weno5(R v1, R v2, R v3, R v4, R v5) {
   R const c_s1 = R(13.0 / 12.0), c_s2 = R(0.25), eps = R(1.0e-6);
   R const s11 = R(1.0) * v1 + R(-2.0) * v2 + R(1.0) * v3;
   R const s12 = R(1.0) * v1 + R(-4.0) * v2 + R(3.0) * v3;
@@ -244,7 +246,25 @@ template <typename R> __device__ __forceinline__ R weno5(R v1, R v2, R v3, R v4,
   R const x1 = (s1 + eps) * (s1 + eps);
   R const x2 = (s2 + eps) * (s2 + eps);
   R const x3 = (s3 + eps) * (s3 + eps);
+  R const p1 = R(2.0 / 6.0) * v1 + R(-7.0 / 6.0) * v2 + R(11.0 / 6.0) * v3;
+  R const p2 = R(-1.0 / 6.0) * v2 + R(5.0 / 6.0) * v3 + R(2.0 / 6.0) * v4;
+  R const p3 = R(2.0 / 6.0) * v3 + R(5.0 / 6.0) * v4 + R(-1.0 / 6.0) * v5;
   R a1, a2, a3, one_a_sum;
+  if constexpr (ONE_DIV) {
+    static_assert(kFastArith && std::is_same<R, double>::value, "FP64 with FMA only");
+    // FP64 with FMA: the weights d_k / x_k over the common denominator
+    // x1 x2 x3, so that w_k = b_k / (b1 + b2 + b3) with b_k = d_k x_i x_j and
+    // one division replaces four.  While every x_k (>= 1e-12) is below 2^200
+    // the b_k lie in [1e-25, 2^400); a weighted sum that still overflows
+    // (stencil values beyond about 2^620), and NaN or infinite x_k, take the
+    // four divisions below.
+    auto hi = [](double v) { return (unsigned)((unsigned long long)__double_as_longlong(v) >> 32); };
+    unsigned const hmax = __builtin_elementwise_max(__builtin_elementwise_max(hi(x1), hi(x2)), hi(x3));
+    double const b1 = 0.1 * x2 * x3, b2 = 0.6 * x1 * x3, b3 = 0.3 * x1 * x2;
+    double const num = b1 * p1 + b2 * p2 + b3 * p3;
+    if (__builtin_expect(hmax < 0x4C700000u && __builtin_fabs(num) <= 1.7976931348623157e308, 1))
+      return nr_div64(num, b1 + b2 + b3);
+  }
   if constexpr (!kFastArith && std::is_same<R, double>::value) {
     // x_k = (s_k + 1e-6)^2 >= 1e-12 (s_k >= 0 or NaN).  While every x_k is
     // below 2^700 the hardware's correctly rounded division of 0.1, 0.6, 0.3
@@ -276,17 +296,19 @@ template <typename R> __device__ __forceinline__ R weno5(R v1, R v2, R v3, R v4,
   R const w1 = a1 * one_a_sum;
   R const w2 = a2 * one_a_sum;
   R const w3 = a3 * one_a_sum;
-  return w1 * (R(2.0 / 6.0) * v1 + R(-7.0 / 6.0) * v2 + R(11.0 / 6.0) * v3) +
-         w2 * (R(-1.0 / 6.0) * v2 + R(5.0 / 6.0) * v3 + R(2.0 / 6.0) * v4) +
-         w3 * (R(2.0 / 6.0) * v3 + R(5.0 / 6.0) * v4 + R(-1.0 / 6.0) * v5);
+  return w1 * p1 + w2 * p2 + w3 * p3;
 }
 // UpwindLeft {0,+1} and UpwindRight {1,-1} (apply_wrapper.h:48-58).
-template <typename R> __device__ __forceinline__ R weno_left(const R *a) {
-  return weno5(a[0], a[1], a[2], a[3], a[4]);
+template <typename R, bool ONE_DIV = false> __device__ __forceinline__ R weno_left(const R *a) {
+  return weno5<R, ONE_DIV>(a[0], a[1], a[2], a[3], a[4]);
 }
-template <typename R> __device__ __forceinline__ R weno_right(const R *a) {
-  return weno5(a[5], a[4], a[3], a[2], a[1]);
+template <typename R, bool ONE_DIV = false> __device__ __forceinline__ R weno_right(const R *a) {
+  return weno5<R, ONE_DIV>(a[5], a[4], a[3], a[2], a[1]);
 }
+// The finite-volume face of the FP64-with-FMA form reconstructs with one
+// division per WENO5 (the flux-splitting face keeps four: there the branch
+// to the four-division path costs it its second wave per SIMD).
+template <typename R> constexpr bool kWenoOneDiv = kFastArith && std::is_same<R, double>::value;
 
 template <int NE, typename R> struct Eig {
   R L[NE][NE];
@@ -853,8 +875,8 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
         }
         uc[m] = s;
       }
-      minus[n] = weno_left(uc);
-      plus[n] = weno_right(uc);
+      minus[n] = weno_left<R, kWenoOneDiv<R>>(uc);
+      plus[n] = weno_right<R, kWenoOneDiv<R>>(uc);
     }
     R cl[NE], cr[NE];
     roe_right_eigenvectors<DIM, DIR>(ev);
