@@ -827,6 +827,90 @@ __device__ __forceinline__ void roe_split_flux(double gamma64, double b64,
   }
 }
 
+// The finite-volume face (FiniteVolumeScheme::ComputeFluxes,
+// finite_volume_scheme.cpp:160-251, with CharacteristicStateReconstruction,
+// characteristic_state_reconstruction.h:60-134) from its stencil Ud (FP64
+// conservatives of cells i-2..i+3 along DIR).  Writes f (zero when the
+// reference skips the face).
+template <int DIM, int DIR, typename R>
+__device__ __forceinline__ void fv_face(double gamma64, double b64, const double (&Ud)[6][DIM + 2],
+                                        R *f) {
+  constexpr int NE = DIM + 2;
+  R const gamma = R(gamma64), b = R(b64);
+  // Mixed form: the stencil enters the FP32 reconstruction as differences
+  // from cell i (taken in FP64), and cell i is added back in FP64 after the
+  // back-transform.  Exact arithmetic gives the same flux: the WENO5
+  // smoothness indicators see only differences and its candidate stencils
+  // sum to one, so WENO(v - c) = WENO(v) - c, and R L = I.  In FP32 it keeps
+  // the rounding error proportional to the local variation instead of the
+  // state's magnitude.
+  constexpr bool kShift = std::is_same<R, float>::value;
+  Eig<NE, R> ev;
+#pragma unroll
+  for (int e = 0; e < NE; ++e) f[e] = R(0.0);
+  R U[6][NE];
+#pragma unroll
+  for (int m = 0; m < 6; ++m)
+#pragma unroll
+    for (int e = 0; e < NE; ++e) U[m][e] = kShift ? R(Ud[m][e] - Ud[2][e]) : R(Ud[m][e]);
+  CellQ<R> qt, qn;
+  cell_quantities<DIM, R>(gamma64, b64, Ud[2], qt);
+  cell_quantities<DIM, R>(gamma64, b64, Ud[3], qn);
+  roe_eigensystem<DIM, DIR>(gamma, qt, qn, ev);
+  R minus[NE], plus[NE];
+#pragma unroll
+  for (int n = 0; n < NE; ++n) {
+    R uc[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      R s = R(0.0);
+#pragma unroll
+      for (int q = 0; q < NE; ++q) {
+        int const l = seq<DIM, DIR>(q);
+        if (!(kSkipZeros<R> && left_zero<DIM, DIR>(n, l))) s += ev.L[n][l] * U[m][l];
+      }
+      uc[m] = s;
+    }
+    minus[n] = weno_left<R, kWenoOneDiv<R>>(uc);
+    plus[n] = weno_right<R, kWenoOneDiv<R>>(uc);
+  }
+  R cl[NE], cr[NE];
+  roe_right_eigenvectors<DIM, DIR>(ev);
+  to_physical<DIM, DIR>(minus, ev.Rm, cl);
+  to_physical<DIM, DIR>(plus, ev.Rm, cr);
+  if constexpr (kShift) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      cl[e] = R(double(cl[e]) + Ud[2][e]);
+      cr[e] = R(double(cr[e]) + Ud[2][e]);
+    }
+  }
+  R const eps = R(2.220446049250313080847263336181640625e-16);  // DBL_EPSILON
+  if (cl[kMass] <= eps || cr[kMass] <= eps) return;
+  Prim<R> pl, pr;
+  cons_to_prim<DIM>(gamma, b, cl, pl);
+  cons_to_prim<DIM>(gamma, b, cr, pr);
+  if (pl.p <= -b || pr.p <= -b) return;
+  R hf[NE];
+  hllc<DIM, DIR>(gamma, b, cl, cr, pl, pr, hf);
+#pragma unroll
+  for (int e = 0; e < NE; ++e) f[e] += hf[e];
+}
+
+// The six-cell stencil of a face: cells off + (m - 2) * stride, m = 0..5,
+// through the block's buffer resource (one 32-bit byte offset per cell, the
+// field as a scalar offset).
+template <int DIM>
+__device__ __forceinline__ void load_stencil(const BlockReader<DIM> &rd, int off, int stride,
+                                             double (&Ud)[6][DIM + 2]) {
+#pragma unroll
+  for (int m = 0; m < 6; ++m)
+#pragma unroll
+    for (int e = 0; e < DIM + 2; ++e)
+      Ud[m][e] = rd((off + (m - 2) * stride) * (int)sizeof(double),
+                    e * Dims<DIM>::FIELD * (int)sizeof(double));
+}
+
 // One face flux from the six-cell stencil Ud[m] (FP64 conservatives of cells
 // i-2..i+3 along DIR); reconstruction and Riemann arithmetic in R.
 // FV:  FiniteVolumeScheme::ComputeFluxes (finite_volume_scheme.cpp:160-251)
@@ -839,82 +923,13 @@ __device__ __forceinline__ void face_flux(double gamma64, double b64,
                                           const double *__restrict__ blk, int off, R *f) {
   constexpr int NE = DIM + 2;
   const double *__restrict__ base = blk + off;
-  R const gamma = R(gamma64), b = R(b64);
-  // Cell i-2+m of the stencil: base points at cell i's first field.
-  auto load = [base](int m, double *c) {
-#pragma unroll
-    for (int e = 0; e < NE; ++e)
-      c[e] = base[e * Dims<DIM>::FIELD + (m - 2) * axis_stride<DIM, DIR>()];
-  };
-  // Mixed form: the stencil enters the FP32 reconstruction as differences
-  // from cell i (taken in FP64), and cell i is added back in FP64 after the
-  // back-transform.  Exact arithmetic gives the same flux: the WENO5
-  // smoothness indicators see only differences and its candidate stencils
-  // sum to one, so WENO(v - c) = WENO(v) - c, and R L = I.  In FP32 it keeps
-  // the rounding error proportional to the local variation instead of the
-  // state's magnitude.
-  constexpr bool kShift = std::is_same<R, float>::value;
-  Eig<NE, R> ev;
 #pragma unroll
   for (int e = 0; e < NE; ++e) f[e] = R(0.0);
   if constexpr (FLUX == 0) {
-    // The stencil through a buffer resource on the block: one 32-bit byte
-    // offset per cell, the field as a scalar offset.
     BlockReader<DIM> const rd(blk);
     double Ud[6][NE];
-#pragma unroll
-    for (int m = 0; m < 6; ++m)
-#pragma unroll
-      for (int e = 0; e < NE; ++e)
-        Ud[m][e] = rd((off + (m - 2) * axis_stride<DIM, DIR>()) * (int)sizeof(double),
-                      e * Dims<DIM>::FIELD * (int)sizeof(double));
-    R U[6][NE];
-#pragma unroll
-    for (int m = 0; m < 6; ++m)
-#pragma unroll
-      for (int e = 0; e < NE; ++e) U[m][e] = kShift ? R(Ud[m][e] - Ud[2][e]) : R(Ud[m][e]);
-    CellQ<R> qt, qn;
-    cell_quantities<DIM, R>(gamma64, b64, Ud[2], qt);
-    cell_quantities<DIM, R>(gamma64, b64, Ud[3], qn);
-    roe_eigensystem<DIM, DIR>(gamma, qt, qn, ev);
-    R minus[NE], plus[NE];
-#pragma unroll
-    for (int n = 0; n < NE; ++n) {
-      R uc[6];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        R s = R(0.0);
-#pragma unroll
-        for (int q = 0; q < NE; ++q) {
-          int const l = seq<DIM, DIR>(q);
-          if (!(kSkipZeros<R> && left_zero<DIM, DIR>(n, l))) s += ev.L[n][l] * U[m][l];
-        }
-        uc[m] = s;
-      }
-      minus[n] = weno_left<R, kWenoOneDiv<R>>(uc);
-      plus[n] = weno_right<R, kWenoOneDiv<R>>(uc);
-    }
-    R cl[NE], cr[NE];
-    roe_right_eigenvectors<DIM, DIR>(ev);
-    to_physical<DIM, DIR>(minus, ev.Rm, cl);
-    to_physical<DIM, DIR>(plus, ev.Rm, cr);
-    if constexpr (kShift) {
-#pragma unroll
-      for (int e = 0; e < NE; ++e) {
-        cl[e] = R(double(cl[e]) + Ud[2][e]);
-        cr[e] = R(double(cr[e]) + Ud[2][e]);
-      }
-    }
-    R const eps = R(2.220446049250313080847263336181640625e-16);  // DBL_EPSILON
-    if (cl[kMass] <= eps || cr[kMass] <= eps) return;
-    Prim<R> pl, pr;
-    cons_to_prim<DIM>(gamma, b, cl, pl);
-    cons_to_prim<DIM>(gamma, b, cr, pr);
-    if (pl.p <= -b || pr.p <= -b) return;
-    R hf[NE];
-    hllc<DIM, DIR>(gamma, b, cl, cr, pl, pr, hf);
-#pragma unroll
-    for (int e = 0; e < NE; ++e) f[e] += hf[e];
+    load_stencil<DIM>(rd, off, axis_stride<DIM, DIR>(), Ud);
+    fv_face<DIM, DIR, R>(gamma64, b64, Ud, f);
   } else if constexpr (std::is_same<R, double>::value && !kFastArith) {
     roe_split_flux_exact<DIM, DIR>(gamma64, b64, blk, off, f);
   } else {
