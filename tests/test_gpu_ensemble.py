@@ -128,10 +128,13 @@ def test_ensemble_continuous_admission_and_cancel(torch):
 
 @pytest.mark.parametrize("tile", [0, 1])
 def test_ensemble_3d_mixed_classes(torch, tile):
-    """3D cases of both flux families and two arithmetic modes in one pool."""
+    """3D cases of both flux families and all three arithmetic modes in one
+    pool: each equals its case run alone; the FP64 ones equal the single-case
+    kernel bit for bit, the FP64-with-FMA one is within 1e-10 of it."""
     print(f"[ensemble 3d] tile={tile}", flush=True)
     st0 = cases.blobs_shock_3d(2, seed=8)
-    combos = [(gpu.FLUX_HLLC, gpu.FP64), (gpu.FLUX_ROE, gpu.FP64), (gpu.FLUX_HLLC, gpu.MIXED)]
+    combos = [(gpu.FLUX_HLLC, gpu.FP64), (gpu.FLUX_ROE, gpu.FP64), (gpu.FLUX_HLLC, gpu.MIXED),
+              (gpu.FLUX_HLLC, gpu.FP64_FMA)]
     ens = Ensemble(3, max_blocks=40, max_cases=4, classes=combos)
     ens.set_tuning(tile)
     ids = []
@@ -149,5 +152,13 @@ def test_ensemble_3d_mixed_classes(torch, tile):
             single = gpu.GpuCase(d, st0, cfl=0.5)
             single.advance(12)
             assert np.array_equal(ens.download(cid), single.download())
+        if d.precision == gpu.FP64_FMA:
+            exact = gpu.make_desc(3, [2, 2, 2], [1, 1, 1], flux=d.flux, precision=gpu.FP64)
+            single = gpu.GpuCase(exact, st0, cfl=0.5)
+            single.advance(12)
+            want = single.download()
+            err = float(np.max(np.abs(ens.download(cid) - want)) / np.max(np.abs(want)))
+            print(f"  fp64-fma in the pool vs exact single case: {err:.3e}")
+            assert err <= 1e-10
         solo.close()
     ens.close()

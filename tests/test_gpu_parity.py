@@ -274,6 +274,25 @@ def test_fp64_fma_within_bound(torch):
         assert err <= REL_TOL
 
 
+@pytest.mark.parametrize("nb", [[8, 8], [16, 16]])
+def test_fp64_fma_within_bound_2d(torch, nb):
+    """FP64 with FMA contraction in 2D, both single-case tilings (8x8 tiles
+    below 256 blocks, 16x16 tiles from 256 blocks): four-quadrant problem,
+    zero-gradient sides, max error <= 1e-10 of the field maximum."""
+    steps = 100 if nb[0] == 8 else 30
+    print(f"[fp64-fma 2d] nb={nb} steps={steps}", flush=True)
+    state0 = cases.four_quadrant_2d(nb[0], nb[1], seed=0)
+    for flux in (gpu.FLUX_HLLC, gpu.FLUX_ROE):
+        desc = gpu.make_desc(2, nb, [0, 0], flux=flux, precision=gpu.FP64_FMA)
+        case = gpu.GpuCase(desc, state0, cfl=0.6)
+        case.advance(steps)
+        got = case.download()
+        want, _, _ = oracle_py.run(2, nb, [0, 0], flux, 0.6, 1e300, steps, state0)
+        err = _maxrel(got, want)
+        print(f"  flux={flux} max rel err={err:.3e}")
+        assert err <= REL_TOL
+
+
 MIXED_L1_RHO_TOL = 1e-6  # L1 density error of the mixed form vs the FP64 oracle
 
 
