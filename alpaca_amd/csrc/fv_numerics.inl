@@ -270,16 +270,21 @@ __device__ __forceinline__ R weno5(R v1, R v2, R v3, R v4, R v5) {
     static_assert(kFastArith && std::is_same<R, double>::value, "FP64 with FMA only");
     // FP64 with FMA: the weights d_k / x_k over the common denominator
     // x1 x2 x3, so that w_k = b_k / (b1 + b2 + b3) with b_k = d_k x_i x_j and
-    // one division replaces four.  While every x_k (>= 1e-12) is below 2^200
-    // the b_k lie in [1e-25, 2^400); a weighted sum that still overflows
-    // (stencil values beyond about 2^620), and NaN or infinite x_k, take the
-    // four divisions below.
-    auto hi = [](double v) { return (unsigned)((unsigned long long)__double_as_longlong(v) >> 32); };
-    unsigned const hmax = __builtin_elementwise_max(__builtin_elementwise_max(hi(x1), hi(x2)), hi(x3));
-    double const b1 = 0.1 * x2 * x3, b2 = 0.6 * x1 * x3, b3 = 0.3 * x1 * x2;
-    double const num = b1 * p1 + b2 * p2 + b3 * p3;
-    if (__builtin_expect(hmax < 0x4C700000u && __builtin_fabs(num) <= 1.7976931348623157e308, 1))
-      return fast_div64(num, b1 + b2 + b3);
+    // one division replaces four.  Each x_k (>= 1e-12) is first clamped to at
+    // most 2^1000 (an infinite one then gets a weight about 1e-289 of the
+    // others', as its d_k / x_k = 0 would; a NaN one still makes the result
+    // NaN through its stencil polynomial), and all three are scaled by 2^-e,
+    // e the exponent of the largest, into (2^-1041, 1), so the b_k stay at
+    // most 0.6 and their sum positive.  Scaling by a power of two changes no
+    // significand, so for x_k in the normal range the quotient has the bits
+    // of the unscaled one.
+    double const c1 = __builtin_fmin(x1, 0x1p1000), c2 = __builtin_fmin(x2, 0x1p1000),
+                 c3 = __builtin_fmin(x3, 0x1p1000);
+    int const e = __builtin_amdgcn_frexp_exp(__builtin_fmax(__builtin_fmax(c1, c2), c3));
+    double const y1 = __builtin_amdgcn_ldexp(c1, -e), y2 = __builtin_amdgcn_ldexp(c2, -e),
+                 y3 = __builtin_amdgcn_ldexp(c3, -e);
+    double const b1 = 0.1 * y2 * y3, b2 = 0.6 * y1 * y3, b3 = 0.3 * y1 * y2;
+    return fast_div64(b1 * p1 + b2 * p2 + b3 * p3, b1 + b2 + b3);
   }
   if constexpr (!kFastArith && std::is_same<R, double>::value) {
     // x_k = (s_k + 1e-6)^2 >= 1e-12 (s_k >= 0 or NaN).  While every x_k is
@@ -321,9 +326,7 @@ template <typename R, bool ONE_DIV = false> __device__ __forceinline__ R weno_le
 template <typename R, bool ONE_DIV = false> __device__ __forceinline__ R weno_right(const R *a) {
   return weno5<R, ONE_DIV>(a[5], a[4], a[3], a[2], a[1]);
 }
-// The finite-volume face of the FP64-with-FMA form reconstructs with one
-// division per WENO5 (the flux-splitting face keeps four: there the branch
-// to the four-division path costs it its second wave per SIMD).
+// The FP64-with-FMA form reconstructs with one division per WENO5.
 template <typename R> constexpr bool kWenoOneDiv = kFastArith && std::is_same<R, double>::value;
 
 template <int NE, typename R> struct Eig {
@@ -813,7 +816,9 @@ __device__ __forceinline__ void roe_split_flux(double gamma64, double b64,
       }
     }
 #pragma unroll
-    for (int n = 0; n < NE; ++n) chf[n] = R(0.5) * (weno_left(pos[n]) + weno_right(neg[n]));
+    for (int n = 0; n < NE; ++n)
+      chf[n] = R(0.5) * (weno_left<R, kWenoOneDiv<R>>(pos[n]) +
+                         weno_right<R, kWenoOneDiv<R>>(neg[n]));
   }
   R phys[NE];
   to_physical<DIM, DIR>(chf, ev.Rm, phys);

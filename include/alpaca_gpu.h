@@ -51,11 +51,10 @@ enum { ALPACA_FLUX_HLLC = 0, ALPACA_FLUX_ROE = 1 };
  *                    build (no FMA) bit for bit;
  *   ALPACA_FP64_FMA  FP64 with fused multiply-add contraction, Newton-Raphson
  *                    division and square root (within 1 ulp), WENO5 weights
- *                    over a common denominator on the finite-volume face and
- *                    the eigenvectors' zero entries left out of the sums (a
- *                    NaN or infinity can then stay out of a sum it would
- *                    reach in the reference); within 1e-10 of the exact form
- *                    after 100 steps;
+ *                    over a common denominator and the eigenvectors' zero
+ *                    entries left out of the sums (a NaN or infinity can then
+ *                    stay out of a sum it would reach in the reference);
+ *                    within 1e-10 of the exact form after 100 steps;
  *   ALPACA_MIXED     FP32 reconstruction and Riemann fluxes (hardware
  *                    reciprocal and square root), FP64 pressure recovery
  *                    p = (gamma-1)(E - |m|^2/(2 rho)), FP64 flux divergence,
