@@ -566,10 +566,10 @@ def main():
     ap.add_argument("--serving-window", type=float, default=1.0,
                     help="arrival window (s) of the serving leg of the default line")
     ap.add_argument("--workload", choices=["uniform", "batched", "serving", "accuracy", "mr",
-                                           "multilevel"],
+                                           "multilevel", "c0"],
                     default="uniform",
                     help="uniform: the 256^3 case (default); batched: 512 uneven 2D cases; "
-                         "serving: Poisson request stream")
+                         "serving: Poisson request stream; c0: the 2D 128^2 case to t_end")
     ap.add_argument("--cases", type=int, default=512, help="batched: number of cases")
     ap.add_argument("--poll", type=int, default=8, help="batched: steps between polls")
     ap.add_argument("--tune", action="store_true", help="batched: autotune the tile shape")
@@ -603,6 +603,9 @@ def main():
         return
     if args.workload == "multilevel":
         print(json.dumps(run_multilevel()), flush=True)
+        return
+    if args.workload == "c0":
+        print(json.dumps(c0_summary()), flush=True)
         return
     if args.workload == "serving":
         from alpaca_amd import serving
