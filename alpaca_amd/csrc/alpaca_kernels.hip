@@ -579,6 +579,27 @@ __global__ void finish_step_kernel(alpaca_case_state *state, int guard = 0) {
   if (!(state->time < state->t_end)) state->flags |= kFlagDone;
 }
 
+// finish_step_kernel followed by dt_kernel for the next step, both guarded,
+// in one launch (the same operations on the record in the same order): the
+// step boundary inside alpaca_advance.
+__global__ void finish_and_dt_kernel(alpaca_case_state *state, double cell_size) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (state->flags & kFlagDone) return;
+  state->time += 0.0 + state->dt;
+  state->steps += 1;
+  if (!(state->time < state->t_end)) {
+    state->flags |= kFlagDone;
+    return;
+  }
+  double const s = __longlong_as_double((long long)state->smax);
+  double const rate = exact::fdiv(s, cell_size);
+  double dt = rate == 0.0 ? 1.7976931348623157e308 : exact::fdiv(state->cfl, rate);
+  if (state->time + dt > state->t_end) dt = exact::smax(state->t_end - state->time, 0.0);
+  state->dt = dt;
+  state->smax = 0;
+  if (dt < 2.220446049250313080847263336181640625e-16 && dt > 0.0) state->flags |= 2;
+}
+
 __global__ void state_init_kernel(alpaca_case_state *state, double time, double t_end, double cfl) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   state->time = time;
@@ -1375,7 +1396,6 @@ int alpaca_advance(const alpaca_case_desc *desc, double *d_u, double *d_s1, doub
   // Steps once the record is done (its time reached t_end) do nothing: a run
   // can be advanced in chunks of steps with one read-back per chunk.
   for (int n = 0; n < nsteps; ++n) {
-    if (n > 0) hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(64), 0, s, d_state, desc->cell_size, 1);
     bool const signal = n + 1 < nsteps;
     if (desc->dim == 2) {
       stage_fused<2>(desc, 0, d_u, nullptr, d_s1, nullptr, d_state, 0, s, true);
@@ -1386,7 +1406,10 @@ int alpaca_advance(const alpaca_case_desc *desc, double *d_u, double *d_s1, doub
       stage_fused<3>(desc, 1, d_s1, d_u, d_s2, nullptr, d_state, 0, s, true);
       stage_fused<3>(desc, 2, d_s2, d_u, d_u, nullptr, d_state, signal, s, true);
     }
-    hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, s, d_state, 1);
+    if (signal)  // this step's end and the next step's dt
+      hipLaunchKernelGGL(finish_and_dt_kernel, dim3(1), dim3(64), 0, s, d_state, desc->cell_size);
+    else
+      hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, s, d_state, 1);
   }
   return hip_status(hipGetLastError());
 }
