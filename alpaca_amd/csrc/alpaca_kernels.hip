@@ -563,8 +563,12 @@ __global__ __launch_bounds__(kThreads) void exact_ops_kernel(const double *__res
     out[i] = exact::fdiv(a[i], b[i]);
   else if constexpr (OP == 1)
     out[i] = exact::weno_div(a[i], b[i]);
-  else
+  else if constexpr (OP == 2)
     out[i] = exact::dsqrt(a[i]);
+  else if constexpr (OP == 3)
+    out[i] = fast::fdiv(a[i], b[i]);
+  else
+    out[i] = fast::dsqrt(a[i]);
 }
 
 // FinishMacroTimestep (time_integrator.h:244-248).
@@ -1257,7 +1261,8 @@ int alpaca_right_hand_side_noalloc(const alpaca_case_desc *desc, int nblocks, co
 
 int alpaca_exact_fp64_ops(int op, const double *d_a, const double *d_b, double *d_out, int n,
                           void *stream) {
-  if (op < 0 || op > 2 || n < 0 || n > kMaxBlocks * 64 || !d_a || !d_out || (op < 2 && !d_b))
+  if (op < 0 || op > 4 || n < 0 || n > kMaxBlocks * 64 || !d_a || !d_out ||
+      ((op < 2 || op == 3) && !d_b))
     return ALPACA_ERR_INVALID;
   if (n == 0) return ALPACA_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -1266,8 +1271,12 @@ int alpaca_exact_fp64_ops(int op, const double *d_a, const double *d_b, double *
     hipLaunchKernelGGL(exact_ops_kernel<0>, grid, dim3(kThreads), 0, s, d_a, d_b, d_out, n);
   else if (op == 1)
     hipLaunchKernelGGL(exact_ops_kernel<1>, grid, dim3(kThreads), 0, s, d_a, d_b, d_out, n);
-  else
+  else if (op == 2)
     hipLaunchKernelGGL(exact_ops_kernel<2>, grid, dim3(kThreads), 0, s, d_a, d_a, d_out, n);
+  else if (op == 3)
+    hipLaunchKernelGGL(exact_ops_kernel<3>, grid, dim3(kThreads), 0, s, d_a, d_b, d_out, n);
+  else
+    hipLaunchKernelGGL(exact_ops_kernel<4>, grid, dim3(kThreads), 0, s, d_a, d_a, d_out, n);
   return hip_status(hipGetLastError());
 }
 

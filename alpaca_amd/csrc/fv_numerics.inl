@@ -28,7 +28,9 @@ __device__ __forceinline__ double nr_div64(double a, double b) {
 // exponents -200..200, including divisors just below a power of two), the
 // step leaves at most about 2^-48, and the correction (residual exact in an
 // FMA) squares that again, so the result is within 1 ulp of a/b for normal
-// operands; on those 16.7 M pairs it was correctly rounded every time.
+// operands.  Random pairs come out correctly rounded (all 16.7 M of them);
+// quotients within about 2^-90 of a rounding midpoint can round the other
+// way.
 __device__ __forceinline__ double fast_div64(double a, double b) {
   double r = __builtin_amdgcn_rcp(b);
   double const e = __builtin_fma(-b, r, 1.0);

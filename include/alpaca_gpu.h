@@ -227,12 +227,14 @@ int alpaca_copy_blocks_list(int dim, const double *d_src, double *d_dst, int cap
 int alpaca_right_hand_side_noalloc(const alpaca_case_desc *desc, int nblocks,
                                    const double *d_avg, double *d_rhs, void *stream);
 
-/* Verification hook, no counterpart in the reference: the exact form's FP64
- * operations as its face kernels evaluate them, elementwise over n values on
- * `stream`.  op 0: d_out = d_a / d_b (the division of physical quantities);
- * op 1: the WENO5 weight division d_a / d_b (its core for divisors in
- * [1e-12, 2^700)); op 2: d_out = sqrt(d_a) (d_b unused).  Tests compare the
- * results with the x86 reference build's correctly rounded operations. */
+/* Verification hook, no counterpart in the reference: the FP64 operations
+ * as the face kernels evaluate them, elementwise over n values on `stream`.
+ * Exact form: op 0: d_out = d_a / d_b (the division of physical
+ * quantities); op 1: the WENO5 weight division d_a / d_b (its core for
+ * divisors in [1e-12, 2^700)); op 2: d_out = sqrt(d_a) (d_b unused).
+ * FP64-with-FMA form: op 3: d_a / d_b; op 4: sqrt(d_a) (d_b unused).  Tests
+ * compare the results with the x86 reference build's correctly rounded
+ * operations (exact form: the same bits; FMA form: within 1 ulp). */
 int alpaca_exact_fp64_ops(int op, const double *d_a, const double *d_b, double *d_out, int n,
                           void *stream);
 

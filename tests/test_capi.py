@@ -145,10 +145,13 @@ def test_noalloc_and_exact_ops_refuse_before_any_device_call():
     bad = gpu.make_desc(3, [1, 1, 1], [0, 0, 0])
     bad.dim = 5
     assert lib.alpaca_right_hand_side_noalloc(ctypes.byref(bad), 1, p, q, None) == 1
-    assert lib.alpaca_exact_fp64_ops(3, p, p, q, 10, None) == 1
+    assert lib.alpaca_exact_fp64_ops(5, p, p, q, 10, None) == 1
+    assert lib.alpaca_exact_fp64_ops(-1, p, p, q, 10, None) == 1
     assert lib.alpaca_exact_fp64_ops(0, p, None, q, 10, None) == 1
+    assert lib.alpaca_exact_fp64_ops(3, p, None, q, 10, None) == 1
     assert lib.alpaca_exact_fp64_ops(0, p, p, q, -1, None) == 1
     assert lib.alpaca_exact_fp64_ops(2, p, None, q, 0, None) == 0
+    assert lib.alpaca_exact_fp64_ops(4, p, None, q, 0, None) == 0
 
 
 def test_pool_list_entries_refuse_before_any_device_call():

@@ -2,7 +2,8 @@
 rounded ones on constructed hard cases (results within 2^-100 of a rounding
 midpoint) and on random operands.  The reference's CPU build computes a / b
 and sqrt(x) correctly rounded, so the exact form can reproduce it bit for
-bit only if these agree everywhere the kernels divide."""
+bit only if these agree everywhere the kernels divide.  The FP64-with-FMA
+form's Newton-Raphson division and square root are held to 1 ulp."""
 import ctypes
 
 import numpy as np
@@ -13,7 +14,7 @@ import fp64_hard_cases as hard
 
 pytestmark = pytest.mark.gpu
 
-OP_DIV, OP_WENO_DIV, OP_SQRT = 0, 1, 2
+OP_DIV, OP_WENO_DIV, OP_SQRT, OP_FAST_DIV, OP_FAST_SQRT = 0, 1, 2, 3, 4
 N = 20000
 
 
@@ -84,3 +85,39 @@ def test_sqrt_hard_and_random_cases(torch):
                         np.round(rng.uniform(0.0, 10.0, N), 4)])
     got = _run(torch, OP_SQRT, x)
     assert _mismatch(got, np.sqrt(x)) == 0
+
+
+def _ulps(got, want):
+    """Distance in units in the last place (same-sign doubles)."""
+    g = got.view(np.int64)
+    w = want.view(np.int64)
+    return np.abs(g - w)
+
+
+def test_fast_division_and_sqrt_within_one_ulp(torch):
+    """The FP64-with-FMA form divides with one Newton step on the hardware
+    reciprocal and a residual correction, and takes square roots by one
+    coupled Newton step with a residual correction: within 1 ulp of the
+    correctly rounded result on operands of exponents -200..200, on divisors
+    just below a power of two and on near-midpoint quotients (in practice
+    correctly rounded; the count is printed)."""
+    print("[fp64 fast div/sqrt]", flush=True)
+    rng = np.random.default_rng(21)
+    a = rng.uniform(1.0, 2.0, N) * np.exp2(rng.integers(-200, 201, N)) * rng.choice([-1.0, 1.0], N)
+    b = rng.uniform(1.0, 2.0, N) * np.exp2(rng.integers(-200, 201, N))
+    ha, hb = hard.division_cases(N, seed=23, recip_hard=True, exp_range=(-30, 30))
+    na, nb = hard.division_cases(N, seed=29, recip_hard=False, exp_range=(-30, 30))
+    A = np.concatenate([a, ha, na])
+    B = np.concatenate([b, hb, nb])
+    q = _run(torch, OP_FAST_DIV, A, B)
+    d = _ulps(np.abs(q), np.abs(A / B))
+    assert np.all(np.sign(q) == np.sign(A / B))
+    print(f"  division: max {int(d.max())} ulp, {int(np.count_nonzero(d))} of {len(A)} not "
+          f"correctly rounded", flush=True)
+    assert d.max() <= 1
+    x = np.concatenate([hard.sqrt_cases(N, seed=31), np.exp(rng.uniform(-600, 600, N))])
+    r = _run(torch, OP_FAST_SQRT, x)
+    d = _ulps(r, np.sqrt(x))
+    print(f"  sqrt: max {int(d.max())} ulp, {int(np.count_nonzero(d))} of {len(x)} not "
+          f"correctly rounded", flush=True)
+    assert d.max() <= 1
